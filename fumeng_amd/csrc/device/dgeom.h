@@ -1,0 +1,363 @@
+/*
+ * Ray-primitive tests, BVH traversal and hit fill on the device.
+ *
+ * Arithmetic of the tests is written operation for operation like the
+ * reference (FMA contraction off) so that hit t and the winning primitive
+ * match the compiled FP32 reference:
+ *   src/geometry/triangle.cpp:13-94     watertight test; GetMaxAxis truncates
+ *                                       |d| through ::abs(int)
+ *   src/geometry/sphere.cpp:61-129      object-space quadratic, Ray::Contains
+ *   src/geometry/rectangle.cpp:35-85    halves (pa,pb,pc) then (pc,pd,pa)
+ *   include/engine/core/bbox.h:59-69    slab test with std::min/max semantics
+ *   src/accelerator/bvh_aggregate.cpp:81-123  closest / any hit
+ *
+ * Traversal order differs from the reference (near child first, explicit
+ * stack in LDS) so ties are resolved by an order-free rule equal to the
+ * reference's sequential one over leaf slots p (m_objList order):
+ * triangles and rectangles need t < t_max, spheres accept t == t_max, so at
+ * equal t a sphere beats a non-sphere, the sphere with the larger slot wins
+ * among spheres and the smaller slot wins among non-spheres.
+ */
+#ifndef FM_DEVICE_DGEOM_H
+#define FM_DEVICE_DGEOM_H
+
+#include "dmath.h"
+#include "../fm_layout.h"
+
+namespace fmd {
+
+struct Ray {
+    v3 o, d;
+    float tmin, tmax;
+};
+
+/* Per-ray constants of the watertight test (kx, ky, kz permutation and the
+ * shear); the values are the ones the reference computes per call. */
+struct RayPre {
+    int kx, ky, kz;
+    float sx, sy, sz;
+    v3 inv;          /* 1 / d per component */
+    bool fast;       /* no zero direction component: slab values cannot be NaN */
+};
+
+FMD int trunc_iabs(float x) { const int i = (int)x; return i < 0 ? -i : i; }
+
+FMD RayPre precompute(const Ray& r) {
+#pragma clang fp contract(off)
+    RayPre p;
+    const int a0 = trunc_iabs(r.d.x), a1 = trunc_iabs(r.d.y), a2 = trunc_iabs(r.d.z);
+    int az;
+    if (a0 > a1 && a0 > a2) az = 0;
+    else az = a1 > a2 ? 1 : 2;
+    p.kz = az;
+    p.kx = (az + 1) % 3;
+    p.ky = (p.kx + 1) % 3;
+    const float dx = comp(r.d, p.kx), dy = comp(r.d, p.ky), dz = comp(r.d, p.kz);
+    p.sx = -dx / dz;
+    p.sy = -dy / dz;
+    p.sz = 1.0f / dz;
+    p.inv = mk(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+    p.fast = r.d.x != 0.0f && r.d.y != 0.0f && r.d.z != 0.0f;
+    return p;
+}
+
+FMD v3 permute(v3 v, const RayPre& p) {
+    if (p.kz == 2) return v;
+    return mk(comp(v, p.kx), comp(v, p.ky), comp(v, p.kz));
+}
+
+/* Edge functions and t of the watertight test; returns false when rejected
+ * before the t range check.  Candidate t and barycentrics are outputs. */
+FMD bool tri_core(const Ray& r, const RayPre& rp, v3 v0, v3 v1, v3 v2, float& t, float& b0, float& b1, float& b2) {
+#pragma clang fp contract(off)
+    v0 = permute(v0 - r.o, rp);
+    v1 = permute(v1 - r.o, rp);
+    v2 = permute(v2 - r.o, rp);
+    v0.x += rp.sx * v0.z; v0.y += rp.sy * v0.z;
+    v1.x += rp.sx * v1.z; v1.y += rp.sy * v1.z;
+    v2.x += rp.sx * v2.z; v2.y += rp.sy * v2.z;
+    float e0 = v1.x * v2.y - v1.y * v2.x;
+    float e1 = v2.x * v0.y - v2.y * v0.x;
+    float e2 = v0.x * v1.y - v0.y * v1.x;
+    if (e0 == 0.0f || e1 == 0.0f || e2 == 0.0f) {
+        /* the reference's own FP64 recomputation of exactly-zero edges */
+        e0 = (float)((double)v1.x * (double)v2.y - (double)v1.y * (double)v2.x);
+        e1 = (float)((double)v2.x * (double)v0.y - (double)v2.y * (double)v0.x);
+        e2 = (float)((double)v0.x * (double)v1.y - (double)v0.y * (double)v1.x);
+    }
+    if ((e0 < 0 || e1 < 0 || e2 < 0) && (e0 > 0 || e1 > 0 || e2 > 0)) return false;
+    const float det = e0 + e1 + e2;
+    if (det == 0) return false;
+    const float z0 = v0.z * rp.sz, z1 = v1.z * rp.sz, z2 = v2.z * rp.sz;
+    const float inv_det = 1.0f / det;
+    t = (e0 * z0 + e1 * z1 + e2 * z2) * inv_det;
+    b0 = e0 * inv_det; b1 = e1 * inv_det; b2 = e2 * inv_det;
+    return true;
+}
+
+/* Sphere::IntersectSphere: smallest root with t_min <= t (the t_max side of
+ * Ray::Contains is applied by the caller). */
+FMD bool sphere_core(const Ray& r, const fm_dsphere& s, float tmax_incl, float& t) {
+#pragma clang fp contract(off)
+    const v3 o = xf_point(s.w2o, r.o);
+    const v3 d = xf_vec(s.w2o, r.d);
+    const float a = d.x * d.x + d.y * d.y + d.z * d.z;
+    const float b = 2.0f * (d.x * o.x + d.y * o.y + d.z * o.z);
+    const float c = (o.x * o.x + o.y * o.y + o.z * o.z) - s.radius * s.radius;
+    float delta = b * b - 4.0f * a * c;
+    if (delta < 0) return false;
+    delta = sqrtf(delta);
+    const float inv2a = 0.5f / a;
+    const float t0 = (-b - delta) * inv2a;
+    const float t1 = (-b + delta) * inv2a;
+    if (t0 >= r.tmin && t0 <= tmax_incl) { t = t0; return true; }
+    if (t1 >= r.tmin && t1 <= tmax_incl) { t = t1; return true; }
+    return false;
+}
+
+/* BBox::IsIntersect with the current t_max.  `fast` uses fminf/fmaxf, equal
+ * to std::min/max when no slab value is NaN. */
+template <bool FAST>
+FMD bool box_hit(const float* b, const Ray& r, const RayPre& rp, float tmax, float& tnear) {
+#pragma clang fp contract(off)
+    const float lx = rp.inv.x * (b[0] - r.o.x), ly = rp.inv.y * (b[1] - r.o.y), lz = rp.inv.z * (b[2] - r.o.z);
+    const float hx = rp.inv.x * (b[3] - r.o.x), hy = rp.inv.y * (b[4] - r.o.y), hz = rp.inv.z * (b[5] - r.o.z);
+    if (FAST) {
+        const float n = fmaxf(fmaxf(r.tmin, fminf(lx, hx)), fmaxf(fminf(ly, hy), fminf(lz, hz)));
+        const float f = fminf(fminf(tmax, fmaxf(lx, hx)), fminf(fmaxf(ly, hy), fmaxf(lz, hz)));
+        tnear = n;
+        return n <= f;
+    } else {
+        /* std::min(a, b) = (b < a) ? b : a ; std::max(a, b) = (a < b) ? b : a */
+        const float ax = (hx < lx) ? hx : lx, ay = (hy < ly) ? hy : ly, az = (hz < lz) ? hz : lz;
+        const float bx = (lx < hx) ? hx : lx, by = (ly < hy) ? hy : ly, bz = (lz < hz) ? hz : lz;
+        const float mx = (r.tmin < ax) ? ax : r.tmin, my = (r.tmin < ay) ? ay : r.tmin, mz = (r.tmin < az) ? az : r.tmin;
+        const float nx = (bx < tmax) ? bx : tmax, ny = (by < tmax) ? by : tmax, nz = (bz < tmax) ? bz : tmax;
+        const float n = mx > my ? (mx > mz ? mx : mz) : (my > mz ? my : mz);
+        const float f = nx < ny ? (nx < nz ? nx : nz) : (ny < nz ? ny : nz);
+        tnear = n;
+        return n <= f;
+    }
+}
+
+struct Hit {
+    float t;
+    int32_t slot;    /* -1 = miss */
+    int32_t kind;    /* FM_PRIM_* of the winner */
+    int32_t tie;     /* tie key of the winner (slot, or geometry id for "naive") */
+    int32_t half;    /* rectangle: 1 = (pa,pb,pc), 0 = (pc,pd,pa) */
+    float b0, b1, b2;
+};
+
+/* Leaf slot test for the closest hit with the order-free tie rule. */
+FMD void test_slot_closest(const fm_dscene& S, const Ray& r, const RayPre& rp, int32_t p, Hit& h, int32_t key) {
+    const float4* pr = reinterpret_cast<const float4*>(S.prims) + 3 * (size_t)p;
+    const float4 a = pr[0];
+    const int kind = __float_as_int(a.w);
+    const bool have = h.slot >= 0;
+    if (kind == FM_PRIM_SPHERE) {
+        const float4 b = pr[1];
+        const fm_dsphere& s = S.spheres[__float_as_int(b.w)];
+        float t;
+        if (!sphere_core(r, s, h.t, t)) return;  /* t_min <= t <= t_max */
+        if (t < h.t || !have || h.kind != FM_PRIM_SPHERE || key > h.tie) {
+            h.t = t; h.slot = p; h.kind = FM_PRIM_SPHERE; h.tie = key; h.half = 0;
+        }
+        return;
+    }
+    const float4 b = pr[1], c = pr[2];
+    float t, b0, b1, b2;
+    /* IntersectTriangle keeps t_min < t < t_max; Triangle::GetIntersect also
+     * rejects t <= 0 (triangle.cpp:221), Rectangle::GetIntersect does not */
+    const bool tri = kind == FM_PRIM_TRI;
+    auto accept = [&](float tt) {
+        if (!(tt > r.tmin) || (tri && !(tt > 0.0f))) return false;
+        if (tt < h.t) return true;
+        return tt == h.t && have && h.kind != FM_PRIM_SPHERE && key < h.tie;
+    };
+    if (tri) {
+        if (tri_core(r, rp, f4to3(a), f4to3(b), f4to3(c), t, b0, b1, b2) && accept(t)) {
+            h.t = t; h.slot = p; h.kind = FM_PRIM_TRI; h.tie = key; h.half = 0; h.b0 = b0; h.b1 = b1; h.b2 = b2;
+        }
+        return;
+    }
+    /* rectangle: (pa,pb,pc) first, (pc,pd,pa) only if the first is rejected */
+    if (tri_core(r, rp, f4to3(a), f4to3(b), f4to3(c), t, b0, b1, b2) && accept(t)) {
+        h.t = t; h.slot = p; h.kind = FM_PRIM_RECT; h.tie = key; h.half = 1; h.b0 = b0; h.b1 = b1; h.b2 = b2;
+        return;
+    }
+    const v3 pd = ld3(S.rects[__float_as_int(b.w)].pd);
+    if (tri_core(r, rp, f4to3(c), pd, f4to3(a), t, b0, b1, b2) && accept(t)) {
+        h.t = t; h.slot = p; h.kind = FM_PRIM_RECT; h.tie = key; h.half = 0; h.b0 = b0; h.b1 = b1; h.b2 = b2;
+    }
+}
+
+/* Any-hit slot test: Geometry::IsIntersect. */
+FMD bool test_slot_any(const fm_dscene& S, const Ray& r, const RayPre& rp, int32_t p) {
+    const float4* pr = reinterpret_cast<const float4*>(S.prims) + 3 * (size_t)p;
+    const float4 a = pr[0];
+    const int kind = __float_as_int(a.w);
+    const float4 b = pr[1];
+    float t, b0, b1, b2;
+    if (kind == FM_PRIM_SPHERE) {
+        return sphere_core(r, S.spheres[__float_as_int(b.w)], r.tmax, t);
+    }
+    const float4 c = pr[2];
+    auto inrange = [&](float tt) { return !(tt <= r.tmin || tt >= r.tmax); };
+    if (tri_core(r, rp, f4to3(a), f4to3(b), f4to3(c), t, b0, b1, b2) && inrange(t)) return true;
+    if (kind == FM_PRIM_RECT) {
+        const v3 pd = ld3(S.rects[__float_as_int(b.w)].pd);
+        if (tri_core(r, rp, f4to3(c), pd, f4to3(a), t, b0, b1, b2) && inrange(t)) return true;
+    }
+    return false;
+}
+
+/* Traversal with a per-thread stack in LDS: stack[level * stride + lane]. */
+template <bool FAST>
+FMD void closest_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& h, uint32_t* stk, int stride) {
+    float tn;
+    if (!box_hit<FAST>(S.root_box, r, rp, h.t, tn)) return;
+    int32_t ref = S.root_ref;
+    int sp = 0;
+    for (;;) {
+        if (ref >= 0) {
+            const float4* nd = reinterpret_cast<const float4*>(S.nodes + ref);
+            const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2];
+            const int4 cr = reinterpret_cast<const int4*>(nd)[3];
+            const float b0[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+            const float b1[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+            float t0, t1;
+            const bool h0 = box_hit<FAST>(b0, r, rp, h.t, t0);
+            const bool h1 = box_hit<FAST>(b1, r, rp, h.t, t1);
+            if (h0 && h1) {
+                const bool first0 = t0 <= t1;
+                stk[sp * stride] = (uint32_t)(first0 ? cr.y : cr.x);
+                sp++;
+                ref = first0 ? cr.x : cr.y;
+                continue;
+            }
+            if (h0) { ref = cr.x; continue; }
+            if (h1) { ref = cr.y; continue; }
+        } else {
+            const int32_t start = FM_LEAF_START(ref), cnt = FM_LEAF_COUNT(ref);
+            for (int k = 0; k < cnt; k++) {
+                const int32_t p = start + k;
+                const int32_t key = S.naive ? S.meta[4 * (size_t)p] : p;
+                test_slot_closest(S, r, rp, p, h, key);
+            }
+        }
+        if (sp == 0) break;
+        sp--;
+        ref = (int32_t)stk[sp * stride];
+    }
+}
+
+FMD Hit trace_closest(const fm_dscene& S, const Ray& r, uint32_t* stk, int stride) {
+    Hit h;
+    h.t = r.tmax; h.slot = -1; h.kind = -1; h.tie = 0; h.half = 0; h.b0 = h.b1 = h.b2 = 0.0f;
+    if (S.num_slots == 0) return h;
+    const RayPre rp = precompute(r);
+    if (rp.fast) closest_impl<true>(S, r, rp, h, stk, stride);
+    else closest_impl<false>(S, r, rp, h, stk, stride);
+    return h;
+}
+
+template <bool FAST>
+FMD bool any_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, uint32_t* stk, int stride) {
+    float tn;
+    if (!box_hit<FAST>(S.root_box, r, rp, r.tmax, tn)) return false;
+    int32_t ref = S.root_ref;
+    int sp = 0;
+    for (;;) {
+        if (ref >= 0) {
+            const float4* nd = reinterpret_cast<const float4*>(S.nodes + ref);
+            const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2];
+            const int4 cr = reinterpret_cast<const int4*>(nd)[3];
+            const float b0[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+            const float b1[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+            float t0, t1;
+            const bool h0 = box_hit<FAST>(b0, r, rp, r.tmax, t0);
+            const bool h1 = box_hit<FAST>(b1, r, rp, r.tmax, t1);
+            if (h0 && h1) { stk[sp * stride] = (uint32_t)cr.y; sp++; ref = cr.x; continue; }
+            if (h0) { ref = cr.x; continue; }
+            if (h1) { ref = cr.y; continue; }
+        } else {
+            const int32_t start = FM_LEAF_START(ref), cnt = FM_LEAF_COUNT(ref);
+            for (int k = 0; k < cnt; k++)
+                if (test_slot_any(S, r, rp, start + k)) return true;
+        }
+        if (sp == 0) break;
+        sp--;
+        ref = (int32_t)stk[sp * stride];
+    }
+    return false;
+}
+
+FMD bool trace_any(const fm_dscene& S, const Ray& r, uint32_t* stk, int stride) {
+    if (S.num_slots == 0) return false;
+    const RayPre rp = precompute(r);
+    return rp.fast ? any_impl<true>(S, r, rp, stk, stride) : any_impl<false>(S, r, rp, stk, stride);
+}
+
+/* Hit point as the geometry's GetIntersect fills it (triangle.cpp:211-248,
+ * rectangle.cpp:58-85, sphere.cpp:40-89). */
+struct Surf {
+    v3 pos, ng, ns, ss, wo;
+    float u, v;
+    int32_t geom, material, light;
+};
+
+FMD Surf fill_hit(const fm_dscene& S, const Ray& r, const Hit& h) {
+    Surf s;
+    const int4 me = reinterpret_cast<const int4*>(S.meta)[h.slot];
+    s.geom = me.x; s.material = me.y; s.light = me.z;
+    s.wo = -r.d;
+    const float4* pr = reinterpret_cast<const float4*>(S.prims) + 3 * (size_t)h.slot;
+    if (h.kind == FM_PRIM_TRI) {
+        s.pos = r.o + r.d * h.t;
+        const float4* sh = reinterpret_cast<const float4*>(S.tri_sh) + 4 * (size_t)h.slot;
+        const float4 s0 = sh[0], s1 = sh[1], s2 = sh[2], s3 = sh[3];
+        s.u = h.b0 * s0.w + h.b1 * s2.w + h.b2 * s3.y;
+        s.v = h.b0 * s1.w + h.b1 * s3.x + h.b2 * s3.z;
+        if (me.w & FM_META_HAS_NORMAL) {
+            s.ng = h.b0 * f4to3(s0) + h.b1 * f4to3(s1) + h.b2 * f4to3(s2);
+        } else {
+            const v3 v0 = f4to3(pr[0]), v1 = f4to3(pr[1]), v2 = f4to3(pr[2]);
+            s.ng = normalize(cross(v0 - v2, v1 - v2));
+        }
+        s.ns = s.ng;
+        s.ss = normalize(f4to3(pr[1]) - f4to3(pr[0]));
+    } else if (h.kind == FM_PRIM_RECT) {
+        const fm_drect& R = S.rects[__float_as_int(pr[1].w)];
+        s.pos = r.o + r.d * h.t;
+        s.ng = ld3(R.normal); s.ns = s.ng; s.ss = ld3(R.tangent);
+        if (h.half) { /* bc.x*(0,0) + bc.y*(0,1) + bc.z*(1,1) */
+            s.u = (h.b0 * 0.0f + h.b1 * 0.0f) + h.b2 * 1.0f;
+            s.v = (h.b0 * 0.0f + h.b1 * 1.0f) + h.b2 * 1.0f;
+        } else {      /* bc.x*(1,1) + bc.y*(0,1) + bc.z*(0,0) */
+            s.u = (h.b0 * 1.0f + h.b1 * 0.0f) + h.b2 * 0.0f;
+            s.v = (h.b0 * 1.0f + h.b1 * 1.0f) + h.b2 * 0.0f;
+        }
+    } else {
+        const fm_dsphere& sp = S.spheres[__float_as_int(pr[1].w)];
+        const v3 ol = xf_point(sp.w2o, r.o);
+        const v3 dl = xf_vec(sp.w2o, r.d);
+        const v3 pos = ol + dl * h.t;
+        float phi = (pos.x == 0.0f && pos.y == 0.0f) ? 0.0f : atan2f(pos.y, pos.x);
+        if (phi < 0) phi += 2.0f * FM_PI;
+        const float theta = acosf(clampf(pos.z / sp.radius, -1.0f, 1.0f));
+        const v3 gn = normalize(pos);
+        v3 aux = mk(0, 0, 1);
+        if (fabsf(gn.z - 1.0f) < FM_EPS || fabsf(gn.z + 1.0f) < FM_EPS) aux = mk(1, 0, 0);
+        s.u = phi * 0.5f * FM_INV_PI;
+        s.v = theta * FM_INV_PI;
+        s.pos = xf_point(sp.o2w, pos);
+        s.ng = xf_normal_inv(sp.w2o, gn);
+        s.ns = s.ng;
+        s.ss = xf_vec(sp.o2w, cross(gn, aux));
+    }
+    return s;
+}
+
+}  // namespace fmd
+#endif

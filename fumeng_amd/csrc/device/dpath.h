@@ -1,0 +1,299 @@
+/*
+ * Lights, camera and the per-sample path tracer on the device.  Follows
+ * (paths relative to the reference's src/engine/):
+ *   src/light/area_light.cpp:15-50             two-sided sampling, one-sided radiance/pdf
+ *   src/light/directional_light.cpp:11-31      delta light at world radius 1e6
+ *   src/light/env_light.cpp:38-71              env radiance / pdf / sampling
+ *   src/light/env_importance_sampler.h:67-95   2D piecewise-constant sampling
+ *   src/math/distribution_{1d,2d}.cpp          search (last cdf <= u), pdfs
+ *   src/geometry/{triangle,rectangle,sphere}.cpp  Sample / Pdf of emitters
+ *   src/camera/{pin_hole,thin_lens}_camera.cpp  primary rays
+ *   src/renderer/path_tracing.cpp:34-229       RenderPixel / MisLight / MisBSDF
+ */
+#ifndef FM_DEVICE_DPATH_H
+#define FM_DEVICE_DPATH_H
+
+#include "dbsdf.h"
+#include "dgeom.h"
+
+namespace fmd {
+
+struct LightSample { v3 ref_pos, light_pos, wi, radiance; float pdf, dist; };
+
+/* ---- env ---- */
+FMD v3 env_radiance(const fm_dscene& S, const fm_denv& E, v3 light_to_shd) {
+    const v3 d = m3_apply(E.w2l, -light_to_shd);
+    const float z = clampf(d.z, -1.0f, 1.0f);
+    const float theta = acosf(z);
+    float phi = atan2f(d.y, d.x);
+    if (phi < 0) phi += 2.0f * FM_PI;
+    const float u = phi * FM_INV_PI * 0.5f, v = theta * FM_INV_PI;
+    const fm_dimage& im = S.images[E.image];
+    return E.factor * image_sample(S, im, FM_SAMPLER_LINEAR, wrapf(FM_WRAP_REPEAT, u), wrapf(FM_WRAP_REPEAT, v));
+}
+/* Distribution2D::Pdf divides the conditional pdf by the marginal integral
+ * (distribution_2d.cpp:47-58), then EnvImportanceSampler::Pdf / (2 pi^2 sin). */
+FMD float env_pdf(const fm_dscene& S, const fm_denv& E, v3 dir) {
+    const v3 d = m3_apply(E.w2l, dir);
+    const float z = clampf(d.z, -1.0f, 1.0f);
+    const float theta = acosf(z);
+    float phi = atan2f(d.y, d.x);
+    if (phi < 0) phi += 2.0f * FM_PI;
+    const float u = phi * FM_INV_PI * 0.5f, v = theta * FM_INV_PI;
+    const float st = sinf(theta);
+    if (st == 0.0f) return 0.0f;
+    const int iu = min((int)(u * (float)E.W), E.W - 1);
+    const int iv = min((int)(v * (float)E.H), E.H - 1);
+    if (E.marg_sum < FM_EPS_PDF) return 0.0f;
+    const float* T = S.env_tables;
+    const float cond = T[E.off_cond_val + (size_t)iv * E.W + iu] / T[E.off_cond_sum + iv];
+    return cond / E.marg_sum / (2.0f * FM_PI * FM_PI * st);
+}
+/* last index with cdf[idx] <= u among [0, n-1] */
+FMD int cdf_search(const float* cdf, int n, float u) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (cdf[mid] <= u) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+FMD LightSample env_sample(const fm_dscene& S, const fm_denv& E, v3 pos, v3 s) {
+    const float* T = S.env_tables;
+    const float* mc = T + E.off_marg_cdf;
+    const int iv = cdf_search(mc, E.H, s.x);
+    const float vv = ((s.x - mc[iv]) / (mc[iv + 1] - mc[iv]) + (float)iv) / (float)E.H;
+    const float pm = T[E.off_marg_val + iv] / E.marg_sum;
+    const float* cc = T + E.off_cond_cdf + (size_t)iv * (E.W + 1);
+    const int iu = cdf_search(cc, E.W, s.y);
+    const float uu = ((s.y - cc[iu]) / (cc[iu + 1] - cc[iu]) + (float)iu) / (float)E.W;
+    const float pc = T[E.off_cond_val + (size_t)iv * E.W + iu] / T[E.off_cond_sum + iv];
+    const float p2 = pm * pc;
+    const float phi = uu * 2.0f * FM_PI, theta = vv * FM_PI;
+    float sp, cp, st, ct;
+    sincosf(phi, &sp, &cp);
+    sincosf(theta, &st, &ct);
+    LightSample o;
+    v3 dl;
+    if (st == 0.0f) { dl = splat(0.0f); o.pdf = 0.0f; }
+    else { o.pdf = p2 / (2.0f * FM_PI * FM_PI * st); dl = mk(st * cp, st * sp, ct); }
+    const v3 dw = m3_apply(E.l2w, dl);
+    o.ref_pos = pos; o.light_pos = pos + dw * 1.0e6f; o.wi = dw;
+    o.radiance = env_radiance(S, E, -dw);
+    o.dist = FM_REAL_MAX;
+    return o;
+}
+
+/* ---- area lights: sample a point, return solid-angle pdf ---- */
+FMD void sphere_point(const fm_dsphere& s, v3 pl, v3& pos, v3& ng) {
+    pos = xf_point(s.o2w, pl);
+    ng = xf_normal_inv(s.w2o, normalize(pl));
+}
+FMD LightSample light_sample(const fm_dscene& S, const fm_dlight& L, v3 pos, v3 s) {
+    LightSample o;
+    o.pdf = 0.0f; o.dist = 0.0f; o.radiance = splat(0.0f); o.wi = splat(0.0f); o.ref_pos = splat(0.0f); o.light_pos = splat(0.0f);
+    if (L.kind == FM_LIGHT_ENV) return env_sample(S, S.envs[L.aux], pos, s);
+    if (L.kind == FM_LIGHT_DIR) {
+        const v3 d = ld3(L.dir);
+        o.ref_pos = pos; o.light_pos = pos - d * 1.0e6f; o.wi = -d; o.radiance = ld3(L.radiance);
+        o.pdf = 1.0f; o.dist = FM_REAL_MAX;
+        return o;
+    }
+    v3 lp, ln;
+    float pa;
+    if (L.kind == FM_LIGHT_TRI) {
+        const float su = sqrtf(s.x);
+        const float b0 = 1.0f - su, b1 = s.y * su, b2 = 1.0f - b0 - b1;
+        const v3 v0 = ld3(L.p[0]), v1 = ld3(L.p[1]), v2 = ld3(L.p[2]);
+        lp = b0 * v0 + b1 * v1 + b2 * v2;
+        ln = L.has_normal ? b0 * ld3(L.n[0]) + b1 * ld3(L.n[1]) + b2 * ld3(L.n[2]) : normalize(cross(v0 - v2, v1 - v2));
+        pa = L.p[0][3];
+    } else if (L.kind == FM_LIGHT_RECT) {
+        const fm_drect& R = S.rects[L.aux];
+        const float su = sqrtf(s.x);
+        const float b0 = 1.0f - su, b1 = s.y * su, b2 = 1.0f - b0 - b1;
+        if (s.z < 0.5f) lp = b0 * ld3(R.pa) + b1 * ld3(R.pb) + b2 * ld3(R.pc);
+        else lp = b0 * ld3(R.pc) + b1 * ld3(R.pd) + b2 * ld3(R.pa);
+        ln = ld3(R.normal);
+        pa = R.tangent[3];
+    } else {
+        const fm_dsphere& sp = S.spheres[L.aux];
+        const v3 spl = xf_point(sp.w2o, pos);
+        const float dis = length(spl);
+        v3 pl;
+        if (dis < sp.radius) {
+            const float z = 1.0f - 2.0f * s.x;
+            const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+            float sn, cs;
+            sincosf(2.0f * FM_PI * s.y, &sn, &cs);
+            pl = mk(r * cs, r * sn, z) * sp.radius;
+            pa = sp.inv_area;
+        } else {
+            const float ctm = fminf(sp.radius / dis, 1.0f);
+            const float ct = ctm + s.x * (1.0f - ctm);
+            const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+            float sn, cs;
+            sincosf(2.0f * FM_PI * s.y, &sn, &cs);
+            const v3 d = mk(cs * st, sn * st, ct);
+            const float pdf_cone = 1.0f / (2.0f * FM_PI * (1.0f - ctm));
+            const v3 az = normalize(spl);
+            v3 axx;
+            if (fabsf(az.x) > fabsf(az.y)) axx = normalize(mk(-az.z, 0.0f, az.x));
+            else axx = normalize(mk(0.0f, az.z, -az.y));
+            const v3 ayy = cross(az, axx);
+            pl = sp.radius * (axx * d.x + ayy * d.y + az * d.z);
+            pa = pdf_cone / sp.rw2;
+        }
+        sphere_point(sp, pl, lp, ln);
+    }
+    const v3 d = lp - pos;
+    const float dist = length(d);
+    if (dist < FM_EPS) return o;
+    const v3 wi = normalize(d);
+    o.ref_pos = pos; o.light_pos = lp; o.wi = wi; o.radiance = ld3(L.radiance);
+    o.pdf = pa * dist * dist / absdot(ln, -wi);
+    o.dist = dist;
+    return o;
+}
+/* AreaLight::Pdf (area_light.cpp:40-50) with Geometry::Pdf(shd_pos, light_pos) */
+FMD float area_pdf(const fm_dscene& S, const fm_dlight& L, v3 shd, v3 lp, v3 ln, v3 l2s) {
+    if (dot(ln, l2s) < 0) return 0.0f;
+    float pa;
+    if (L.kind == FM_LIGHT_TRI) pa = L.p[0][3];
+    else if (L.kind == FM_LIGHT_RECT) pa = S.rects[L.aux].tangent[3];
+    else {
+        const fm_dsphere& sp = S.spheres[L.aux];
+        const float dis = length(xf_point(sp.w2o, shd));
+        if (dis < sp.radius) pa = sp.inv_area;
+        else pa = 1.0f / (2.0f * FM_PI * (1.0f - fminf(sp.radius / dis, 1.0f))) / sp.rw2;
+    }
+    const float dd = length(shd - lp);
+    return pa * dd * dd / absdot(ln, l2s);
+}
+
+/* ---- camera ---- */
+FMD Ray camera_ray(const fm_dcamera& c, float nx, float ny, float lx, float ly) {
+    const v3 film = mk((nx - 0.5f) * c.fw, (ny - 0.5f) * c.fh, c.fd);
+    Ray r;
+    r.tmin = 0.0f; r.tmax = FM_REAL_MAX;
+    if (c.type == FM_CAMERA_PIN_HOLE) {
+        const v3 fp = xf_point(c.c2w, film);
+        r.o = ld3(c.pos);
+        r.d = normalize(fp - r.o);
+        return r;
+    }
+    const v3 o = mk(c.lens * lx, c.lens * ly, 0.0f);
+    const v3 d = normalize(film - o);
+    r.o = xf_point(c.c2w, o);
+    r.d = xf_vec(c.c2w, d);
+    return r;
+}
+
+/* HitPoint::GenRay (hit_point.h:26-37) */
+FMD Ray gen_ray(v3 pos, v3 ng, v3 dir) {
+    Ray r;
+    r.o = dot(dir, ng) > 0 ? pos + ng * FM_EPS : pos - ng * FM_EPS;
+    r.d = dir; r.tmin = 0.0f; r.tmax = FM_REAL_MAX;
+    return r;
+}
+
+struct PathOut { v3 color, albedo, normal; uint32_t rays; };
+
+struct PathCfg { int depth, direct, rr_depth; float rr_coef; };
+
+/* PathTracer::RenderPixel for one camera ray. */
+FMD PathOut render_path(const fm_dscene& S, const PathCfg& cfg, Ray ray, Rng& rng, uint32_t* stk, int stride) {
+    PathOut out;
+    out.color = out.albedo = out.normal = splat(0.0f);
+    out.rays = 0;
+    v3 beta = splat(1.0f);
+    Ray r = ray;
+    for (int depth = 1; depth <= cfg.depth; depth++) {
+        const Hit h = trace_closest(S, r, stk, stride);
+        out.rays++;
+        if (h.slot < 0) {
+            if (depth == 1 && S.env >= 0) out.color = env_radiance(S, S.envs[S.env], -ray.d);
+            break;
+        }
+        const Surf sf = fill_hit(S, r, h);
+        if (depth == 1 && sf.light >= 0) {
+            const fm_dlight& L = S.lights[sf.light];
+            if (dot(sf.ng, sf.wo) > 0) out.color += beta * ld3(L.radiance);
+            break;
+        }
+        const Bsdf b = make_bsdf(S, sf.material, sf.ns, sf.ss, sf.wo, sf.u, sf.v);
+        if (depth == 1) { out.albedo = b.albedo(); out.normal = sf.ng; }
+        v3 direct = splat(0.0f);
+        for (int it = 0; it < cfg.direct; it++) {
+            /* MisLight for every light */
+            for (int li = 0; li < S.num_lights; li++) {
+                const fm_dlight& L = S.lights[li];
+                const LightSample ls = light_sample(S, L, sf.pos, rng.next3());
+                if (ls.pdf < FM_EPS_PDF || length(ls.radiance) < FM_EPS_PDF) continue;
+                Ray sh; sh.o = ls.ref_pos; sh.d = ls.wi; sh.tmin = FM_EPS; sh.tmax = ls.dist - FM_EPS;
+                out.rays++;
+                if (trace_any(S, sh, stk, stride)) continue;
+                const v3 f = b.eval(sf.wo, ls.wi);
+                if (length(f) < FM_EPS_PDF) continue;
+                const float bp = b.pdf(sf.wo, ls.wi);
+                if (bp < FM_EPS_PDF) continue;
+                const v3 ff = f * ls.radiance * absdot(ls.wi, sf.ng);
+                if (L.kind == FM_LIGHT_DIR) { direct += beta * (ff / ls.pdf); continue; }
+                const float w = (ls.pdf * ls.pdf) / (ls.pdf * ls.pdf + bp * bp);
+                direct += beta * (ff / ls.pdf * w);
+            }
+            /* MisBSDF */
+            BsdfSample bs = b.sample(sf.wo, rng.next3());
+            if (!(length(bs.f) < FM_EPS_PDF || bs.pdf < FM_EPS_PDF)) {
+                bs.wi = normalize(bs.wi);
+                const Ray mr = gen_ray(sf.pos, sf.ng, bs.wi);
+                const Hit lh = trace_closest(S, mr, stk, stride);
+                out.rays++;
+                if (lh.slot < 0) {
+                    if (S.env >= 0) {
+                        const fm_denv& E = S.envs[S.env];
+                        const v3 rad = env_radiance(S, E, -mr.d);
+                        const v3 f = bs.f * rad * absdot(mr.d, sf.ng);
+                        if (bs.delta) direct += beta * (f / bs.pdf);
+                        else {
+                            const float lp = env_pdf(S, E, mr.d);
+                            const float w = (bs.pdf * bs.pdf) / (bs.pdf * bs.pdf + lp * lp);
+                            direct += beta * (f / bs.pdf * w);
+                        }
+                    }
+                } else {
+                    const Surf ls = fill_hit(S, mr, lh);
+                    if (ls.light >= 0) {
+                        const fm_dlight& L = S.lights[ls.light];
+                        const v3 l2s = normalize(mr.o - ls.pos);
+                        const v3 rad = dot(ls.ng, l2s) > 0 ? ld3(L.radiance) : splat(0.0f);
+                        if (!(length(rad) < FM_EPS_PDF)) {
+                            const v3 f = bs.f * rad * absdot(sf.ng, mr.d);
+                            if (bs.delta) direct += beta * (f / bs.pdf);
+                            else {
+                                const float lp = area_pdf(S, L, mr.o, ls.pos, ls.ng, l2s);
+                                const float w = (bs.pdf * bs.pdf) / (bs.pdf * bs.pdf + lp * lp);
+                                direct += beta * (f / bs.pdf * w);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        out.color += direct / (float)cfg.direct;
+        /* continuation */
+        const BsdfSample cs = b.sample(sf.wo, rng.next3());
+        if (length(cs.f) < FM_EPS_PDF || cs.pdf < FM_EPS_PDF) break;
+        const float ac = absdot(sf.ng, cs.wi);
+        beta *= cs.f * ac / cs.pdf;
+        r = gen_ray(sf.pos, sf.ng, cs.wi);
+        if (cfg.depth > cfg.rr_depth) {
+            if (rng.next() > cfg.rr_coef) break;
+            beta = beta / cfg.rr_coef;
+        }
+    }
+    return out;
+}
+
+}  // namespace fmd
+#endif

@@ -1,0 +1,134 @@
+/*
+ * Device-resident scene layout, shared by the host flattener and the HIP
+ * kernels.  Plain structs, no pointers to host memory.
+ *
+ * HBM layout (all arrays 64-byte aligned, indexed by leaf slot p = position
+ * of a geometry in the reference BVH's m_objList, bvh_aggregate.cpp:43-55):
+ *   nodes    BVH2 node = both child boxes + child refs, 64 B (one line)
+ *   prims    3 x float4 per slot: triangle v0,v1,v2 (w: type, aux index);
+ *            rectangles store pa,pb,pc and pd in the rect table;
+ *            spheres store their centre/radius and index the sphere table
+ *   meta     int4 per slot {geometry id, material, light, flags}
+ *   tri_sh   4 x float4 per slot: vertex normals and uvs (shading only)
+ */
+#ifndef FM_LAYOUT_H
+#define FM_LAYOUT_H
+
+#include <stdint.h>
+
+#define FM_PRIM_TRI 0
+#define FM_PRIM_RECT 1
+#define FM_PRIM_SPHERE 2
+
+#define FM_META_HAS_NORMAL 1
+
+/* child ref: >= 0 inner node index; < 0 leaf: ~((start << 3) | count) */
+#define FM_LEAF_REF(start, count) (~(int32_t)(((uint32_t)(start) << 3) | (uint32_t)(count)))
+#define FM_REF_IS_LEAF(r) ((r) < 0)
+#define FM_LEAF_START(r) ((int32_t)(((uint32_t)(~(r))) >> 3))
+#define FM_LEAF_COUNT(r) ((int32_t)(((uint32_t)(~(r))) & 7u))
+
+#define FM_STACK_DEPTH 32
+
+typedef struct fm_dnode {
+    float b[12];     /* c0 min xyz, c0 max xyz, c1 min xyz, c1 max xyz */
+    int32_t child[2];
+    int32_t pad[2];
+} fm_dnode;          /* 64 bytes */
+
+typedef struct fm_drect {
+    float pa[4], pb[4], pc[4], pd[4]; /* xyz + pad */
+    float normal[4];                   /* w = area */
+    float tangent[4];                  /* w = (float)(1.0 / (double)area) */
+} fm_drect;          /* 96 bytes */
+
+typedef struct fm_dsphere {
+    float w2o[16];    /* column-major world_to_object */
+    float o2w[16];    /* column-major object_to_world */
+    float radius, radius_world, rw2, area;
+    float inv_area;   /* 1 / area (float division, Geometry::Pdf) */
+    float center[3];  /* object_to_world.ApplyToPoint(0) */
+} fm_dsphere;         /* 160 bytes */
+
+typedef struct fm_dtex {
+    int32_t type, image, sampler, wrap_u, wrap_v;
+    float value[3];
+} fm_dtex;            /* 32 bytes */
+
+typedef struct fm_dmaterial {
+    int32_t type, thin, has_image, pad;
+    fm_dtex tex[15];
+    float refl[4], refr[4];
+    float eta_i, eta_t, pad2[2];
+} fm_dmaterial;
+
+#define FM_LIGHT_TRI 0
+#define FM_LIGHT_RECT 1
+#define FM_LIGHT_SPHERE 2
+#define FM_LIGHT_ENV 3
+#define FM_LIGHT_DIR 4
+
+typedef struct fm_dlight {
+    int32_t kind;       /* FM_LIGHT_* */
+    int32_t aux;        /* rect / sphere table index, env index */
+    int32_t has_normal; /* triangle */
+    int32_t geom;       /* reference geometry index */
+    float radiance[4];
+    float p[3][4];      /* triangle v0 v1 v2 (w of p[0] = (float)(1.0/(double)area)) */
+    float n[3][4];      /* triangle vertex normals */
+    float dir[4];       /* directional: normalised direction */
+} fm_dlight;            /* 144 bytes */
+
+typedef struct fm_denv {
+    int32_t image, W, H, pad;
+    float factor, marg_sum, pad2[2];
+    float l2w[12];      /* 3 columns (xyz + pad) of light_to_world */
+    float w2l[12];
+    int64_t off_marg_cdf;   /* floats: H + 1 */
+    int64_t off_marg_val;   /* H */
+    int64_t off_cond_cdf;   /* H * (W + 1) */
+    int64_t off_cond_val;   /* H * W */
+    int64_t off_cond_sum;   /* H */
+    int64_t pad3;
+} fm_denv;
+
+typedef struct fm_dimage {
+    int32_t w, h, format, pad;   /* format: 0 float4 texels, 1 RGBA8 sRGB */
+    int64_t offset;              /* bytes into the texel buffer */
+} fm_dimage;
+
+typedef struct fm_dcamera {
+    int32_t type, pad[3];
+    float pos[4];
+    float fw, fh, fd, lens;
+    float c2w[16];               /* column-major camera_to_world */
+} fm_dcamera;
+
+/* Everything a kernel needs, passed by value. */
+typedef struct fm_dscene {
+    const fm_dnode* nodes;
+    const float* prims;          /* float4 x 3 per slot */
+    const int32_t* meta;         /* int4 per slot */
+    const float* tri_sh;         /* float4 x 4 per slot */
+    const fm_drect* rects;
+    const fm_dsphere* spheres;
+    const fm_dmaterial* materials;
+    const fm_dlight* lights;
+    const fm_denv* envs;
+    const fm_dimage* images;
+    const unsigned char* texels;
+    const float* env_tables;
+    const float* srgb_lut;       /* 256 entries */
+    int32_t root_ref;            /* node index or leaf ref; ignored if num_slots == 0 */
+    int32_t num_slots;
+    int32_t num_lights;
+    int32_t env_light;           /* light index of the first env light or -1 */
+    int32_t env;                 /* env table index or -1 */
+    int32_t naive;               /* brute force in object order */
+    int32_t num_materials;
+    int32_t pad;
+    float root_box[6];
+    fm_dcamera camera;
+} fm_dscene;
+
+#endif

@@ -1,0 +1,563 @@
+/*
+ * fm_scene_desc -> device layout.
+ *
+ * Scene semantics follow (paths relative to the reference's src/engine/):
+ *   include/engine/create/scene_loader.h:86-146   object order, default material
+ *   include/engine/core/render_object.h:25-33    emissive object -> AreaLight
+ *   include/engine/core/scene.h:21-25,46-56      lights: area lights, then JSON lights;
+ *                                                the env is the first EnvLight
+ *   src/geometry/triangle.cpp:114-134,301-310    world-space vertices / normals, bound
+ *   src/geometry/rectangle.cpp:20-32,142-151     normal, area, bound
+ *   src/geometry/sphere.cpp:19-26,211-218        radius_world, area, bound
+ *   src/math/transform.cpp:9-63                  TRS, inverse, ApplyToPoint/Normal
+ *   src/camera/pin_hole_camera.cpp:5-16          camera_to_world, film size
+ *   src/accelerator/bvh_aggregate.cpp:38-79,131-155  the builder (restated over
+ *                                                compact arrays, same topology)
+ *   src/light/env_importance_sampler.h:28-65     luminance x sin(theta) table
+ *   src/math/distribution_{1d,2d}.cpp            CDF tables (built here in FP64)
+ */
+#include "scene_build.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+namespace fmh {
+
+namespace {
+
+constexpr float kRealMax = std::numeric_limits<float>::max();
+constexpr float kRealMin = std::numeric_limits<float>::min();
+const float kPI = (float)3.14159265354;  /* common.h:53 */
+constexpr double kPId = 3.14159265354;
+
+struct F3 { float x, y, z; };
+inline F3 f3(float x, float y, float z) { return {x, y, z}; }
+inline F3 add(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline F3 sub(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline F3 mul(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline F3 cross(F3 a, F3 b) { return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y}; }
+inline float len(F3 a) { return std::sqrt(dot(a, a)); }
+inline F3 nrm(F3 a) { return mul(a, 1.0f / std::sqrt(dot(a, a))); }
+inline float comp(F3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+/* column-major 4x4, m[col][row] */
+struct M4 {
+    float m[4][4];
+    static M4 I() { M4 r{}; for (int i = 0; i < 4; i++) r.m[i][i] = 1.0f; return r; }
+};
+M4 mmul(const M4& a, const M4& b) {
+    M4 r;
+    for (int c = 0; c < 4; c++)
+        for (int k = 0; k < 4; k++)
+            r.m[c][k] = a.m[0][k] * b.m[c][0] + a.m[1][k] * b.m[c][1] + a.m[2][k] * b.m[c][2] + a.m[3][k] * b.m[c][3];
+    return r;
+}
+M4 minv(const M4& A) {
+    float a[4][4], cof[4][4], inv[4][4];
+    for (int r = 0; r < 4; r++) for (int c = 0; c < 4; c++) a[r][c] = A.m[c][r];
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) {
+            float s[3][3]; int rr = 0;
+            for (int i = 0; i < 4; i++) {
+                if (i == r) continue;
+                int cc = 0;
+                for (int j = 0; j < 4; j++) { if (j == c) continue; s[rr][cc++] = a[i][j]; }
+                rr++;
+            }
+            const float mnr = s[0][0] * (s[1][1] * s[2][2] - s[1][2] * s[2][1])
+                            - s[0][1] * (s[1][0] * s[2][2] - s[1][2] * s[2][0])
+                            + s[0][2] * (s[1][0] * s[2][1] - s[1][1] * s[2][0]);
+            cof[r][c] = ((r + c) & 1) ? -mnr : mnr;
+        }
+    const float det = a[0][0] * cof[0][0] + a[0][1] * cof[0][1] + a[0][2] * cof[0][2] + a[0][3] * cof[0][3];
+    if (det == 0.0f || !std::isfinite(det)) throw BuildError(FM_ERR_INVALID, "singular transform");
+    const float id = 1.0f / det;
+    for (int r = 0; r < 4; r++) for (int c = 0; c < 4; c++) inv[r][c] = cof[c][r] * id;
+    M4 o;
+    for (int r = 0; r < 4; r++) for (int c = 0; c < 4; c++) o.m[c][r] = inv[r][c];
+    return o;
+}
+M4 rot(float ang, F3 axis) {
+    const float c = std::cos(ang), s = std::sin(ang);
+    F3 ax = nrm(axis);
+    F3 t = mul(ax, 1.0f - c);
+    M4 r = M4::I();
+    r.m[0][0] = c + t.x * ax.x;        r.m[0][1] = t.x * ax.y + s * ax.z; r.m[0][2] = t.x * ax.z - s * ax.y;
+    r.m[1][0] = t.y * ax.x - s * ax.z; r.m[1][1] = c + t.y * ax.y;        r.m[1][2] = t.y * ax.z + s * ax.x;
+    r.m[2][0] = t.z * ax.x + s * ax.y; r.m[2][1] = t.z * ax.y - s * ax.x; r.m[2][2] = c + t.z * ax.z;
+    return r;
+}
+inline float deg2rad(float d) { return d / 180.0f * kPI; }
+
+struct Xf {
+    M4 mat, inv;
+    static Xf TRS(const float tr[3], const float ro[3], const float sc[3]) {
+        M4 r = mmul(mmul(rot(deg2rad(ro[2]), f3(0, 0, 1)), rot(deg2rad(ro[1]), f3(0, 1, 0))), rot(deg2rad(ro[0]), f3(1, 0, 0)));
+        M4 t = M4::I(); t.m[3][0] = tr[0]; t.m[3][1] = tr[1]; t.m[3][2] = tr[2];
+        M4 s = M4::I(); s.m[0][0] = sc[0]; s.m[1][1] = sc[1]; s.m[2][2] = sc[2];
+        Xf x;
+        x.mat = mmul(mmul(t, r), s);
+        x.inv = minv(x.mat);
+        return x;
+    }
+    F3 vec(F3 v) const {
+        return {mat.m[0][0] * v.x + mat.m[1][0] * v.y + mat.m[2][0] * v.z,
+                mat.m[0][1] * v.x + mat.m[1][1] * v.y + mat.m[2][1] * v.z,
+                mat.m[0][2] * v.x + mat.m[1][2] * v.y + mat.m[2][2] * v.z};
+    }
+    F3 point(F3 v) const {
+        float o[4];
+        for (int k = 0; k < 4; k++) o[k] = mat.m[0][k] * v.x + mat.m[1][k] * v.y + mat.m[2][k] * v.z + mat.m[3][k];
+        return {o[0] / o[3], o[1] / o[3], o[2] / o[3]};
+    }
+    F3 normal(F3 n) const {  /* normalize(transpose(inv) * n) */
+        return nrm(f3(inv.m[0][0] * n.x + inv.m[0][1] * n.y + inv.m[0][2] * n.z,
+                      inv.m[1][0] * n.x + inv.m[1][1] * n.y + inv.m[1][2] * n.z,
+                      inv.m[2][0] * n.x + inv.m[2][1] * n.y + inv.m[2][2] * n.z));
+    }
+};
+
+/* Reference BBox (bbox.h:11-57): default box is (REAL_MAX, REAL_MIN). */
+struct Box {
+    float mn[3] = {kRealMax, kRealMax, kRealMax};
+    float mx[3] = {kRealMin, kRealMin, kRealMin};
+    void grow(F3 p) {
+        for (int i = 0; i < 3; i++) {
+            const float v = comp(p, i);
+            mn[i] = std::min(v, mn[i]);
+            mx[i] = std::max(v, mx[i]);
+        }
+    }
+    void grow(const Box& b) {
+        for (int i = 0; i < 3; i++) { mn[i] = std::min(b.mn[i], mn[i]); mx[i] = std::max(b.mx[i], mx[i]); }
+    }
+    int max_span() const {
+        float best = 0.0f; int axis = 0;
+        for (int i = 0; i < 3; i++) { const float l = mx[i] - mn[i]; if (l > best) { best = l; axis = i; } }
+        return axis;
+    }
+};
+
+/* Geometry after baking, in reference object order. */
+struct Geo {
+    int32_t type;       /* FM_PRIM_* */
+    int32_t aux;        /* rect / sphere index, or triangle index */
+    int32_t material;
+    int32_t light;      /* area light index or -1 */
+};
+struct Tri {
+    F3 v[3], n[3];
+    float uv[3][2];
+    int has_normal;
+    float area;
+};
+
+struct BInfo { int32_t geom; Box box; };
+
+struct Builder {
+    std::vector<BInfo> infos;
+    std::vector<RefNode> nodes;
+    std::vector<int32_t> order;
+    int depth = 0;
+    /* bvh_aggregate.cpp:38-79 */
+    int64_t build(size_t start, size_t end, int level) {
+        depth = std::max(depth, level);
+        const size_t size = end - start;
+        if (size <= 5) {
+            Box box;
+            const int64_t s0 = (int64_t)order.size();
+            for (size_t i = start; i < end; i++) { order.push_back(infos[i].geom); box.grow(infos[i].box); }
+            RefNode n;
+            memcpy(n.mn, box.mn, sizeof n.mn); memcpy(n.mx, box.mx, sizeof n.mx);
+            n.leaf = true; n.a = s0; n.b = (int64_t)order.size();
+            nodes.push_back(n);
+            return (int64_t)nodes.size() - 1;
+        }
+        Box root;
+        for (size_t i = start; i < end; i++) root.grow(infos[i].box);
+        const int ax = root.max_span();
+        std::sort(infos.begin() + (long)start, infos.begin() + (long)end, [ax](const BInfo& a, const BInfo& b) {
+            return a.box.mn[ax] + a.box.mx[ax] < b.box.mn[ax] + b.box.mx[ax];
+        });
+        const int64_t id = (int64_t)nodes.size();
+        RefNode n;
+        memcpy(n.mn, root.mn, sizeof n.mn); memcpy(n.mx, root.mx, sizeof n.mx);
+        n.leaf = false; n.a = 0; n.b = 0;
+        nodes.push_back(n);
+        const size_t mid = (start + end) / 2;
+        const int64_t a = build(start, mid, level + 1);
+        const int64_t b = build(mid, end, level + 1);
+        nodes[(size_t)id].a = a;
+        nodes[(size_t)id].b = b;
+        return id;
+    }
+};
+
+void put4(float* dst, F3 v, float w) { dst[0] = v.x; dst[1] = v.y; dst[2] = v.z; dst[3] = w; }
+inline float ibits(int32_t i) { float f; memcpy(&f, &i, 4); return f; }
+
+/* Texture slot -> device record; constant textures keep their value. */
+fm_dtex dtex(const fm_texture_desc& t, int nimages) {
+    fm_dtex o{};
+    o.type = t.type; o.image = t.image; o.sampler = t.sampler; o.wrap_u = t.wrap_u; o.wrap_v = t.wrap_v;
+    o.value[0] = t.value[0]; o.value[1] = t.value[1]; o.value[2] = t.value[2];
+    if (t.type == FM_TEX_IMAGE && (t.image < 0 || t.image >= nimages)) throw BuildError(FM_ERR_INVALID, "texture image index out of range");
+    if (t.type != FM_TEX_IMAGE && t.type != FM_TEX_CONSTANT) throw BuildError(FM_ERR_INVALID, "texture type");
+    return o;
+}
+
+/* SrgbToLinear (utils.h:120-125) of x/255 in float, as Image::to_linear does. */
+float srgb_to_linear(float c) {
+    if (c <= 0.04045f) return c * 1.0f / 12.92f;
+    return std::pow((c + 0.055f) * 1.0f / 1.055f, 2.4f);
+}
+
+}  // namespace
+
+void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
+    if (d.abi_version != FM_ABI_VERSION) throw BuildError(FM_ERR_VERSION, "fm_scene_desc.abi_version mismatch");
+    if (d.num_images < 0 || d.num_materials < 0 || d.num_meshes < 0 || d.num_objects < 0 || d.num_lights < 0)
+        throw BuildError(FM_ERR_INVALID, "negative count in descriptor");
+    hs.renderer = d.renderer;
+    hs.camera_desc = d.camera;
+    hs.naive = d.accelerator == FM_ACCEL_NAIVE;
+    const int default_mat = d.num_materials;
+
+    /* ---- geometries in object order ---- */
+    std::vector<Geo> geos;
+    std::vector<Tri> tris;
+    std::vector<Box> bounds;
+    std::vector<fm_drect> rects;
+    std::vector<fm_dsphere> spheres;
+    std::vector<fm_dlight> area_lights;
+    for (int oi = 0; oi < d.num_objects; oi++) {
+        const fm_object_desc& o = d.objects[oi];
+        const int mat = (o.material >= 0 && o.material < d.num_materials) ? o.material : default_mat;
+        if (o.material >= d.num_materials) throw BuildError(FM_ERR_INVALID, "object material index out of range");
+        const F3 em = f3(o.emittance[0], o.emittance[1], o.emittance[2]);
+        /* IsBlack (utils.h:86-97): all zero or any negative channel */
+        const bool emissive = !((em.x == 0 && em.y == 0 && em.z == 0) || em.x < 0 || em.y < 0 || em.z < 0);
+        auto add_light = [&](fm_dlight L) -> int32_t {
+            L.radiance[0] = em.x; L.radiance[1] = em.y; L.radiance[2] = em.z;
+            L.geom = (int32_t)geos.size();
+            area_lights.push_back(L);
+            return (int32_t)area_lights.size() - 1;
+        };
+        if (o.shape == FM_SHAPE_TRIANGLE_MESH) {
+            if (o.mesh < 0 || o.mesh >= d.num_meshes) throw BuildError(FM_ERR_INVALID, "object mesh index out of range");
+            const fm_mesh_desc& m = d.meshes[o.mesh];
+            const Xf xf = Xf::TRS(o.translation, o.rotation, o.scale);
+            for (int64_t f = 0; f < m.num_triangles; f++) {
+                const int32_t* ix = m.indices + 9 * f;
+                Tri t{};
+                for (int c = 0; c < 3; c++) {
+                    const int32_t vi = ix[3 * c], ni = ix[3 * c + 1], ti = ix[3 * c + 2];
+                    if (vi < 0 || vi >= m.num_positions || ni >= m.num_normals || ti >= m.num_texcoords)
+                        throw BuildError(FM_ERR_INVALID, "mesh index out of range");
+                    const float* p = m.positions + 3 * (int64_t)vi;
+                    t.v[c] = xf.point(f3(p[0], p[1], p[2]));
+                    if (ti >= 0) { t.uv[c][0] = m.texcoords[2 * (int64_t)ti]; t.uv[c][1] = 1.0f - m.texcoords[2 * (int64_t)ti + 1]; }
+                }
+                t.has_normal = ix[1] >= 0 && ix[4] >= 0 && ix[7] >= 0;
+                if (t.has_normal)
+                    for (int c = 0; c < 3; c++) {
+                        const float* n = m.normals + 3 * (int64_t)ix[3 * c + 1];
+                        t.n[c] = xf.normal(f3(n[0], n[1], n[2]));
+                    }
+                t.area = len(cross(sub(t.v[1], t.v[0]), sub(t.v[2], t.v[0]))) * 0.5f;
+                Box b; b.grow(t.v[0]); b.grow(t.v[1]); b.grow(t.v[2]);
+                int32_t light = -1;
+                if (emissive) {
+                    fm_dlight L{};
+                    L.kind = FM_LIGHT_TRI; L.has_normal = t.has_normal;
+                    for (int c = 0; c < 3; c++) { put4(L.p[c], t.v[c], 0.0f); put4(L.n[c], t.n[c], 0.0f); }
+                    L.p[0][3] = (float)(1.0 / (double)t.area);  /* triangle.cpp:293 */
+                    light = add_light(L);
+                }
+                geos.push_back(Geo{FM_PRIM_TRI, (int32_t)tris.size(), mat, light});
+                tris.push_back(t);
+                bounds.push_back(b);
+            }
+        } else if (o.shape == FM_SHAPE_RECTANGLE) {
+            fm_drect r{};
+            const F3 pa = f3(o.pa[0], o.pa[1], o.pa[2]), pb = f3(o.pb[0], o.pb[1], o.pb[2]);
+            const F3 pc = f3(o.pc[0], o.pc[1], o.pc[2]), pd = f3(o.pd[0], o.pd[1], o.pd[2]);
+            F3 n = cross(sub(pb, pa), sub(pc, pa));
+            const float area = len(n);
+            n = nrm(n);
+            F3 aux = f3(0, 0, 1);
+            if (std::abs(n.z - 1.0f) < 1e-4f || std::abs(n.z + 1.0f) < 1e-4f) aux = f3(1, 0, 0);
+            put4(r.pa, pa, 0); put4(r.pb, pb, 0); put4(r.pc, pc, 0); put4(r.pd, pd, 0);
+            put4(r.normal, n, area);
+            put4(r.tangent, cross(n, aux), (float)(1.0 / (double)area));  /* rectangle.cpp:134 */
+            Box b; b.grow(pa); b.grow(pb); b.grow(pc); b.grow(pd);
+            int32_t light = -1;
+            if (emissive) { fm_dlight L{}; L.kind = FM_LIGHT_RECT; L.aux = (int32_t)rects.size(); light = add_light(L); }
+            geos.push_back(Geo{FM_PRIM_RECT, (int32_t)rects.size(), mat, light});
+            rects.push_back(r);
+            bounds.push_back(b);
+        } else if (o.shape == FM_SHAPE_SPHERE) {
+            const Xf xf = Xf::TRS(o.translation, o.rotation, o.scale);
+            fm_dsphere s{};
+            memcpy(s.w2o, xf.inv.m, sizeof s.w2o);
+            memcpy(s.o2w, xf.mat.m, sizeof s.o2w);
+            s.radius = o.radius;
+            s.radius_world = o.radius * len(xf.vec(f3(1, 0, 0)));
+            s.rw2 = s.radius_world * s.radius_world;
+            s.area = 4.0f * kPI * s.rw2;
+            s.inv_area = 1.0f / s.area;
+            const F3 c = xf.point(f3(0, 0, 0));
+            s.center[0] = c.x; s.center[1] = c.y; s.center[2] = c.z;
+            Box b;
+            b.mn[0] = c.x - s.radius_world; b.mn[1] = c.y - s.radius_world; b.mn[2] = c.z - s.radius_world;
+            b.mx[0] = c.x + s.radius_world; b.mx[1] = c.y + s.radius_world; b.mx[2] = c.z + s.radius_world;
+            int32_t light = -1;
+            if (emissive) { fm_dlight L{}; L.kind = FM_LIGHT_SPHERE; L.aux = (int32_t)spheres.size(); light = add_light(L); }
+            geos.push_back(Geo{FM_PRIM_SPHERE, (int32_t)spheres.size(), mat, light});
+            spheres.push_back(s);
+            bounds.push_back(b);
+        } else {
+            throw BuildError(FM_ERR_INVALID, "unknown shape type");
+        }
+    }
+    const size_t G = geos.size();
+    if (G > (size_t)1 << 28) throw BuildError(FM_ERR_UNSUPPORTED, "more than 2^28 geometries");
+    hs.num_geoms = (int64_t)G;
+
+    /* ---- reference BVH ---- */
+    Builder B;
+    B.infos.resize(G);
+    for (size_t i = 0; i < G; i++) { B.infos[i].geom = (int32_t)i; B.infos[i].box = bounds[i]; }
+    if (G > 0) B.build(0, G, 1);
+    hs.ref_nodes.swap(B.nodes);
+    hs.order.swap(B.order);
+    hs.bvh_depth = B.depth;
+    if (only_bvh) return;
+    if (hs.bvh_depth > FM_STACK_DEPTH) throw BuildError(FM_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
+
+    /* ---- flatten: interior nodes in DFS pre-order become device nodes ---- */
+    std::vector<int32_t> dev_id(hs.ref_nodes.size(), -1);
+    int32_t nint = 0;
+    for (size_t i = 0; i < hs.ref_nodes.size(); i++) if (!hs.ref_nodes[i].leaf) dev_id[i] = nint++;
+    auto ref_of = [&](int64_t i) -> int32_t {
+        const RefNode& n = hs.ref_nodes[(size_t)i];
+        if (n.leaf) return FM_LEAF_REF(n.a, n.b - n.a);
+        return dev_id[(size_t)i];
+    };
+    hs.nodes.assign((size_t)nint, fm_dnode{});
+    for (size_t i = 0; i < hs.ref_nodes.size(); i++) {
+        const RefNode& n = hs.ref_nodes[i];
+        if (n.leaf) continue;
+        fm_dnode& o = hs.nodes[(size_t)dev_id[i]];
+        const RefNode& a = hs.ref_nodes[(size_t)n.a];
+        const RefNode& b = hs.ref_nodes[(size_t)n.b];
+        for (int k = 0; k < 3; k++) { o.b[k] = a.mn[k]; o.b[3 + k] = a.mx[k]; o.b[6 + k] = b.mn[k]; o.b[9 + k] = b.mx[k]; }
+        o.child[0] = ref_of(n.a);
+        o.child[1] = ref_of(n.b);
+    }
+    if (G > 0) {
+        hs.root_ref = ref_of(0);
+        for (int k = 0; k < 3; k++) { hs.root_box[k] = hs.ref_nodes[0].mn[k]; hs.root_box[3 + k] = hs.ref_nodes[0].mx[k]; }
+    }
+
+    /* ---- slots ---- */
+    hs.prims.assign(G * 12, 0.0f);
+    hs.meta.assign(G * 4, 0);
+    hs.tri_sh.assign(G * 16, 0.0f);
+    for (size_t p = 0; p < G; p++) {
+        const int32_t g = hs.order[p];
+        const Geo& ge = geos[(size_t)g];
+        float* pr = &hs.prims[p * 12];
+        int32_t flags = 0;
+        if (ge.type == FM_PRIM_TRI) {
+            const Tri& t = tris[(size_t)ge.aux];
+            put4(pr, t.v[0], ibits(FM_PRIM_TRI)); put4(pr + 4, t.v[1], ibits(-1)); put4(pr + 8, t.v[2], 0.0f);
+            float* sh = &hs.tri_sh[p * 16];
+            put4(sh, t.n[0], t.uv[0][0]); put4(sh + 4, t.n[1], t.uv[0][1]); put4(sh + 8, t.n[2], t.uv[1][0]);
+            sh[12] = t.uv[1][1]; sh[13] = t.uv[2][0]; sh[14] = t.uv[2][1]; sh[15] = 0.0f;
+            if (t.has_normal) flags |= FM_META_HAS_NORMAL;
+        } else if (ge.type == FM_PRIM_RECT) {
+            const fm_drect& r = rects[(size_t)ge.aux];
+            memcpy(pr, r.pa, 16); memcpy(pr + 4, r.pb, 16); memcpy(pr + 8, r.pc, 16);
+            pr[3] = ibits(FM_PRIM_RECT); pr[7] = ibits(ge.aux);
+        } else {
+            const fm_dsphere& s = spheres[(size_t)ge.aux];
+            pr[0] = s.center[0]; pr[1] = s.center[1]; pr[2] = s.center[2]; pr[3] = ibits(FM_PRIM_SPHERE);
+            pr[4] = s.radius_world; pr[7] = ibits(ge.aux);
+        }
+        int32_t* me = &hs.meta[p * 4];
+        me[0] = g; me[1] = ge.material; me[2] = ge.light; me[3] = flags;
+    }
+    hs.rects.swap(rects);
+    hs.spheres.swap(spheres);
+
+    /* ---- images: float4 texels for .hdr, RGBA8 for 8-bit sRGB ---- */
+    size_t off = 0;
+    for (int i = 0; i < d.num_images; i++) {
+        const fm_image_desc& im = d.images[i];
+        if (im.width <= 0 || im.height <= 0 || !im.data) throw BuildError(FM_ERR_INVALID, "empty image");
+        fm_dimage di{};
+        di.w = im.width; di.h = im.height;
+        di.format = im.format == FM_IMAGE_U8_SRGB ? 1 : 0;
+        off = (off + 255) & ~(size_t)255;
+        di.offset = (int64_t)off;
+        const size_t n = (size_t)im.width * im.height;
+        const size_t bytes = n * (di.format ? 4 : 16);
+        hs.texels.resize(off + bytes);
+        unsigned char* dst = hs.texels.data() + off;
+        if (di.format) {
+            const unsigned char* s = (const unsigned char*)im.data;
+            for (size_t k = 0; k < n; k++) { dst[4 * k] = s[3 * k]; dst[4 * k + 1] = s[3 * k + 1]; dst[4 * k + 2] = s[3 * k + 2]; dst[4 * k + 3] = 255; }
+        } else {
+            const float* s = (const float*)im.data;
+            float* o = (float*)dst;
+            for (size_t k = 0; k < n; k++) { o[4 * k] = s[3 * k]; o[4 * k + 1] = s[3 * k + 1]; o[4 * k + 2] = s[3 * k + 2]; o[4 * k + 3] = 0.0f; }
+        }
+        off += bytes;
+        hs.images.push_back(di);
+    }
+    if (hs.texels.empty()) hs.texels.resize(256);
+    hs.srgb_lut.resize(256);
+    for (int k = 0; k < 256; k++) hs.srgb_lut[(size_t)k] = srgb_to_linear((float)k / 255.0f);
+
+    /* ---- materials (+ default_material = Lambert white) ---- */
+    for (int i = 0; i <= d.num_materials; i++) {
+        fm_dmaterial m{};
+        if (i == d.num_materials) {
+            m.type = FM_MAT_LAMBERT;
+            for (int k = 0; k < 15; k++) { m.tex[k].type = FM_TEX_CONSTANT; }
+            m.tex[0].value[0] = m.tex[0].value[1] = m.tex[0].value[2] = 1.0f;
+        } else {
+            const fm_material_desc& s = d.materials[i];
+            if (s.type != FM_MAT_LAMBERT && s.type != FM_MAT_DISNEY && s.type != FM_MAT_SPECULAR_TRANSMISSION)
+                throw BuildError(FM_ERR_INVALID, "material type");
+            m.type = s.type; m.thin = s.thin ? 1 : 0;
+            for (int k = 0; k < 15; k++) {
+                m.tex[k] = dtex(s.tex[k], d.num_images);
+                if (m.tex[k].type == FM_TEX_IMAGE) m.has_image = 1;
+            }
+            for (int c = 0; c < 3; c++) { m.refl[c] = s.reflect_color[c]; m.refr[c] = s.refract_color[c]; }
+            m.eta_i = s.eta_i; m.eta_t = s.eta_t;
+        }
+        hs.materials.push_back(m);
+    }
+
+    /* ---- lights: area lights in object order, then the JSON lights ---- */
+    hs.lights = area_lights;
+    for (int li = 0; li < d.num_lights; li++) {
+        const fm_light_desc& l = d.lights[li];
+        fm_dlight L{};
+        if (l.type == FM_LIGHT_DIRECTIONAL) {
+            L.kind = FM_LIGHT_DIR;
+            for (int c = 0; c < 3; c++) L.radiance[c] = l.radiance[c];
+            const F3 dn = nrm(f3(l.direction[0], l.direction[1], l.direction[2]));
+            put4(L.dir, dn, 0.0f);
+            hs.lights.push_back(L);
+            continue;
+        }
+        if (l.type != FM_LIGHT_ENV) throw BuildError(FM_ERR_INVALID, "light type");
+        if (l.image < 0 || l.image >= d.num_images) throw BuildError(FM_ERR_INVALID, "env image index out of range");
+        const fm_image_desc& im = d.images[l.image];
+        const int W = im.width, H = im.height;
+        fm_denv e{};
+        e.image = l.image; e.W = W; e.H = H; e.factor = l.factor;
+        const float zero[3] = {0, 0, 0}, one[3] = {1, 1, 1};
+        const Xf l2w = Xf::TRS(zero, l.rotation, one);
+        for (int c = 0; c < 3; c++)
+            for (int k = 0; k < 3; k++) { e.l2w[4 * c + k] = l2w.mat.m[c][k]; e.w2l[4 * c + k] = l2w.inv.m[c][k]; }
+        /* luminance x sin(theta) at texel centres through the bilinear
+         * sampler with repeat wrap (env_importance_sampler.h:37-53), FP64 */
+        auto texel = [&](int u, int v, double rgb[3]) {
+            const size_t k = (size_t)v * W + (size_t)u;
+            if (im.format == FM_IMAGE_U8_SRGB) {
+                const unsigned char* s = (const unsigned char*)im.data + 3 * k;
+                for (int c = 0; c < 3; c++) rgb[c] = hs.srgb_lut[s[c]];
+            } else {
+                const float* s = (const float*)im.data + 3 * k;
+                for (int c = 0; c < 3; c++) rgb[c] = s[c];
+            }
+        };
+        std::vector<double> lum((size_t)W * H);
+        double total = 0.0;
+        for (int j = 0; j < H; j++) {
+            const double v = ((double)j + 0.5) / H;
+            const double st = std::sin(kPId * v);
+            for (int i = 0; i < W; i++) {
+                const double u = ((double)i + 0.5) / W;
+                const double ru = u * W, rv = v * H;
+                const int u0 = (int)std::min(std::max(ru - 0.5, 0.0), (double)(W - 1));
+                const int u1 = std::min(u0 + 1, W - 1);
+                const double pu = std::min(std::max(ru - 0.5 - u0, 0.0), 1.0);
+                const int v0 = (int)std::min(std::max(rv - 0.5, 0.0), (double)(H - 1));
+                const int v1 = std::min(v0 + 1, H - 1);
+                const double pv = std::min(std::max(rv - 0.5 - v0, 0.0), 1.0);
+                double a[3], b[3], c3[3], d3[3], rgb[3];
+                texel(u0, v0, a); texel(u1, v0, b); texel(u0, v1, c3); texel(u1, v1, d3);
+                for (int c = 0; c < 3; c++) rgb[c] = (a[c] * (1 - pu) + b[c] * pu) * (1 - pv) + (c3[c] * (1 - pu) + d3[c] * pu) * pv;
+                const double L = (0.2126 * rgb[0] + 0.7152 * rgb[1] + 0.0722 * rgb[2]) * st;
+                lum[(size_t)j * W + i] = L;
+                total += L;
+            }
+        }
+        if (!(total > 0.0)) throw BuildError(FM_ERR_INVALID, "env map has zero luminance (invalid 1d distribution)");
+        const double scale = (double)W * (double)H / total;
+        std::vector<float>& T = hs.env_tables;
+        auto alloc = [&](size_t n) { const size_t o = T.size(); T.resize(o + n); return (int64_t)o; };
+        e.off_marg_cdf = alloc((size_t)H + 1);
+        e.off_marg_val = alloc((size_t)H);
+        e.off_cond_cdf = alloc((size_t)H * (W + 1));
+        e.off_cond_val = alloc((size_t)H * W);
+        e.off_cond_sum = alloc((size_t)H);
+        std::vector<double> rowsum((size_t)H), cdf((size_t)W + 1);
+        for (int j = 0; j < H; j++) {
+            cdf[0] = 0.0;
+            for (int i = 0; i < W; i++) cdf[(size_t)i + 1] = cdf[(size_t)i] + lum[(size_t)j * W + i] * scale / W;
+            const double s = cdf[(size_t)W];
+            rowsum[(size_t)j] = s;
+            for (int i = 0; i <= W; i++) T[(size_t)e.off_cond_cdf + (size_t)j * (W + 1) + i] = s > 0 ? (float)(cdf[(size_t)i] / s) : 0.0f;
+            for (int i = 0; i < W; i++) T[(size_t)e.off_cond_val + (size_t)j * W + i] = (float)(lum[(size_t)j * W + i] * scale);
+            T[(size_t)e.off_cond_sum + j] = (float)s;
+        }
+        std::vector<double> mc((size_t)H + 1);
+        mc[0] = 0.0;
+        for (int j = 0; j < H; j++) mc[(size_t)j + 1] = mc[(size_t)j] + rowsum[(size_t)j] / H;
+        const double ms = mc[(size_t)H];
+        for (int j = 0; j <= H; j++) T[(size_t)e.off_marg_cdf + j] = (float)(mc[(size_t)j] / ms);
+        for (int j = 0; j < H; j++) T[(size_t)e.off_marg_val + j] = (float)rowsum[(size_t)j];
+        e.marg_sum = (float)ms;
+        L.kind = FM_LIGHT_ENV;
+        L.aux = (int32_t)hs.envs.size();
+        if (hs.env_light < 0) { hs.env_light = (int32_t)hs.lights.size(); hs.env = L.aux; }
+        hs.envs.push_back(e);
+        hs.lights.push_back(L);
+    }
+    if (hs.env_tables.empty()) hs.env_tables.resize(16);
+
+    /* ---- camera (pin_hole_camera.cpp:5-16) ---- */
+    const fm_camera_desc& c = d.camera;
+    const F3 eye = f3(c.pos[0], c.pos[1], c.pos[2]);
+    const F3 at = f3(c.look_at[0], c.look_at[1], c.look_at[2]);
+    const F3 up = nrm(f3(c.up[0], c.up[1], c.up[2]));
+    const F3 fz = nrm(sub(at, eye));
+    const F3 sx = nrm(cross(up, fz));
+    const F3 uy = cross(fz, sx);
+    M4 look = M4::I();  /* lookAt under GLM_FORCE_LEFT_HANDED */
+    look.m[0][0] = sx.x; look.m[1][0] = sx.y; look.m[2][0] = sx.z;
+    look.m[0][1] = uy.x; look.m[1][1] = uy.y; look.m[2][1] = uy.z;
+    look.m[0][2] = fz.x; look.m[1][2] = fz.y; look.m[2][2] = fz.z;
+    look.m[3][0] = -dot(sx, eye); look.m[3][1] = -dot(uy, eye); look.m[3][2] = -dot(fz, eye);
+    const M4 c2w = minv(look);
+    fm_dcamera& dc = hs.camera;
+    dc.type = c.type;
+    dc.pos[0] = eye.x; dc.pos[1] = eye.y; dc.pos[2] = eye.z;
+    dc.fd = c.focal_distance;
+    dc.fh = 2.0f * dc.fd * std::tan(deg2rad(c.fov) / 2.0f);
+    dc.fw = c.aspect * dc.fh;
+    dc.lens = c.lens_radius;
+    memcpy(dc.c2w, c2w.m, sizeof dc.c2w);
+}
+
+}  // namespace fmh

@@ -1,0 +1,59 @@
+/* Host scene construction: fm_scene_desc -> flattened device arrays. */
+#ifndef FM_HOST_SCENE_BUILD_H
+#define FM_HOST_SCENE_BUILD_H
+
+#include "../../../include/fm_gpu.h"
+#include "../fm_layout.h"
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace fmh {
+
+struct BuildError : std::runtime_error {
+    int code;
+    BuildError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+/* Reference binary BVH node (bvh_aggregate.cpp:14-22). */
+struct RefNode {
+    float mn[3], mx[3];
+    bool leaf;
+    int64_t a, b;
+};
+
+/* Host copy of everything that goes to HBM, plus facts for fm_scene_info. */
+struct HostScene {
+    std::vector<RefNode> ref_nodes;
+    std::vector<int32_t> order;         /* geometry index per leaf slot */
+    int32_t bvh_depth = 0;
+    int64_t num_geoms = 0;
+
+    std::vector<fm_dnode> nodes;
+    std::vector<float> prims;            /* 12 floats per slot */
+    std::vector<int32_t> meta;           /* 4 ints per slot */
+    std::vector<float> tri_sh;           /* 16 floats per slot */
+    std::vector<fm_drect> rects;
+    std::vector<fm_dsphere> spheres;
+    std::vector<fm_dmaterial> materials;
+    std::vector<fm_dlight> lights;
+    std::vector<fm_denv> envs;
+    std::vector<fm_dimage> images;
+    std::vector<unsigned char> texels;
+    std::vector<float> env_tables;
+    std::vector<float> srgb_lut;
+    int32_t root_ref = 0;
+    float root_box[6] = {0, 0, 0, 0, 0, 0};
+    int32_t env_light = -1, env = -1;
+    int32_t naive = 0;
+    fm_dcamera camera{};
+    fm_renderer_desc renderer{};
+    fm_camera_desc camera_desc{};
+};
+
+/* Throws BuildError.  only_bvh stops after the reference BVH (host tests). */
+void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh = false);
+
+}  // namespace fmh
+#endif

@@ -1,0 +1,340 @@
+/*
+ * fm_gpu.h - C ABI of the MI355X path-tracing backend for Fumeng scenes.
+ *
+ * Plain C: fixed-width integers, floats and pointers only; no C++ types and
+ * no exceptions cross this boundary.  Every call returns FM_OK (0) or a
+ * negative FM_ERR_* code; fm_last_error() gives the message of the last
+ * failing call on a context (or of fm_init when ctx is NULL).
+ *
+ * Reference interfaces these entry points replace (paths relative to the
+ * reference repository, src/engine/ abbreviated to engine/):
+ *   scene construction  engine/include/engine/create/scene_loader.h:293-302
+ *                       (SceneLoader::Load -> ParseCamera/ParseRender/ParseScene)
+ *   integrator entry    engine/include/engine/create/renderer.h:8 (CreatePathTracer)
+ *                       engine/include/engine/core/renderer.h:84 (Renderer::DrawFrame)
+ *                       engine/src/renderer/path_tracing.cpp:34 (PathTracer::RenderPixel)
+ *   accelerator         engine/include/engine/core/aggregate.h:24-30
+ *                       (Aggregate::IsIntersect / GetIntersect)
+ *   camera              engine/include/engine/core/camera.h:47 (Camera::SampleRay)
+ *   BSDF                engine/include/engine/core/bsdf.h:50-99
+ *                       (BSDF::CalFunc / Pdf / SampleBSDF / GetAlbedo)
+ *   lights              engine/include/engine/core/light.h:37-66
+ *                       (Light::Sample / Pdf / GetRadiance), EnvLight::Pdf(dir)
+ */
+#ifndef FM_GPU_H
+#define FM_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FM_ABI_VERSION 1u
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    FM_OK = 0,
+    FM_ERR_INVALID = -1,     /* bad argument, size or descriptor */
+    FM_ERR_OOM = -2,         /* device or host allocation failed */
+    FM_ERR_HIP = -3,         /* a HIP runtime call failed */
+    FM_ERR_VERSION = -4,     /* descriptor abi_version mismatch */
+    FM_ERR_IO = -5,          /* file missing or unreadable */
+    FM_ERR_PARSE = -6,       /* malformed scene JSON / OBJ / image */
+    FM_ERR_UNSUPPORTED = -7, /* valid input this build does not handle */
+    FM_ERR_NO_DEVICE = -8    /* no HIP device / extension cannot run */
+};
+
+typedef struct fm_ctx fm_ctx;
+typedef struct fm_scene fm_scene;
+
+/* ---- scene descriptor (what SceneLoader extracts from JSON and files) --- */
+
+/* Texture descriptor: engine/include/engine/core/config.h:13-31 (TextureDesc),
+ * engine/include/engine/core/texture.h:11-68 (constant / image configs). */
+enum { FM_TEX_CONSTANT = 0, FM_TEX_IMAGE = 1 };
+enum { FM_WRAP_REPEAT = 0, FM_WRAP_CLAMP = 1 };
+enum { FM_SAMPLER_LINEAR = 0, FM_SAMPLER_NEAREST = 1 };
+typedef struct fm_texture_desc {
+    int32_t type;     /* FM_TEX_* */
+    int32_t image;    /* index into fm_scene_desc.images for FM_TEX_IMAGE */
+    int32_t sampler;  /* FM_SAMPLER_* (ImageTextureConfig.sampler, default linear) */
+    int32_t wrap_u;   /* FM_WRAP_* (TextureConfig.wrap_u, default repeat) */
+    int32_t wrap_v;
+    float value[3];   /* ConstantTextureConfig.value */
+} fm_texture_desc;
+
+/* Decoded image: Image::load_from_file (engine/src/image/image.cpp:127-148).
+ * Row-major, row 0 first: texel (u, v) is at v * width + u. */
+enum {
+    FM_IMAGE_F32_LINEAR = 0, /* 3 floats per texel, used as-is (.hdr path) */
+    FM_IMAGE_U8_SRGB = 1     /* 3 bytes per texel: x/255 then SrgbToLinear */
+};
+typedef struct fm_image_desc {
+    int32_t width;
+    int32_t height;
+    int32_t format;   /* FM_IMAGE_* */
+    int32_t pad_;
+    const void* data;
+} fm_image_desc;
+
+/* Materials: engine/include/engine/core/material.h:10-76 and
+ * engine/include/engine/create/scene_loader.h:177-200. */
+enum { FM_MAT_LAMBERT = 0, FM_MAT_DISNEY = 1, FM_MAT_SPECULAR_TRANSMISSION = 2 };
+enum {
+    FM_D_BASECOLOR = 0, FM_D_METALLIC, FM_D_SPECULAR, FM_D_SPECULAR_TINT,
+    FM_D_ROUGHNESS, FM_D_ANISOTROPIC, FM_D_SHEEN, FM_D_SHEEN_TINT,
+    FM_D_CLEARCOAT, FM_D_CLEARCOAT_GLOSS, FM_D_SPEC_TRANS,
+    FM_D_SPEC_TRANS_ROUGHNESS, FM_D_DIFF_TRANS, FM_D_FLATNESS, FM_D_IOR,
+    FM_D_NPARAM = 15
+};
+typedef struct fm_material_desc {
+    int32_t type;              /* FM_MAT_* */
+    int32_t thin;              /* Disney `thin` */
+    fm_texture_desc tex[FM_D_NPARAM]; /* Lambert uses tex[0] = basecolor */
+    float reflect_color[3];    /* specular_transmission */
+    float refract_color[3];
+    float eta_i;
+    float eta_t;
+} fm_material_desc;
+
+/* Triangle mesh after OBJ parsing (tinyobj attrib_t + index_t triples).
+ * indices holds 9 int32 per triangle: (v, vn, vt) for corners 0, 1, 2;
+ * vn / vt are -1 when absent (engine/src/geometry/triangle.cpp:104-197). */
+typedef struct fm_mesh_desc {
+    int64_t num_positions;  const float* positions;  /* xyz */
+    int64_t num_normals;    const float* normals;    /* xyz */
+    int64_t num_texcoords;  const float* texcoords;  /* uv */
+    int64_t num_triangles;  const int32_t* indices;
+} fm_mesh_desc;
+
+/* One JSON render_object: engine/include/engine/create/scene_loader.h:202-252.
+ * A triangle mesh expands into one geometry per face, in face order. */
+enum { FM_SHAPE_TRIANGLE_MESH = 0, FM_SHAPE_SPHERE = 1, FM_SHAPE_RECTANGLE = 2 };
+typedef struct fm_object_desc {
+    int32_t shape;          /* FM_SHAPE_* */
+    int32_t material;       /* index into materials; -1 = default_material */
+    int32_t mesh;           /* index into meshes (triangle mesh) */
+    float radius;           /* sphere */
+    float translation[3];   /* mesh and sphere (Transform TRS, degrees) */
+    float rotation[3];
+    float scale[3];
+    float pa[3], pb[3], pc[3], pd[3];  /* rectangle */
+    float emittance[3];     /* AreaLight radiance when not IsBlack */
+} fm_object_desc;
+
+/* Lights from the JSON `lights` array: scene_loader.h:148-175. */
+enum { FM_LIGHT_ENV = 0, FM_LIGHT_DIRECTIONAL = 1 };
+typedef struct fm_light_desc {
+    int32_t type;          /* FM_LIGHT_* */
+    int32_t image;         /* env: index into images */
+    float rotation[3];     /* env: degrees */
+    float factor;          /* env */
+    float radiance[3];     /* directional */
+    float direction[3];    /* directional (the loader normalises it) */
+} fm_light_desc;
+
+/* Camera: engine/include/engine/core/camera.h:9-41. */
+enum { FM_CAMERA_PIN_HOLE = 0, FM_CAMERA_THIN_LENS = 1 };
+typedef struct fm_camera_desc {
+    int32_t type;          /* FM_CAMERA_* */
+    float pos[3];
+    float look_at[3];
+    float up[3];
+    float fov;             /* degrees, vertical */
+    float aspect;
+    float focal_distance;
+    float lens_radius;     /* thin lens */
+} fm_camera_desc;
+
+/* Renderer block: engine/include/engine/core/renderer.h:17-60. */
+typedef struct fm_renderer_desc {
+    int32_t width, height, spp, thread_count;
+    int32_t depth, direct_loop, rr_depth;
+    float rr_coef;
+    char output[256];      /* output prefix, already joined to the scene dir */
+} fm_renderer_desc;
+
+enum { FM_ACCEL_BVH = 0, FM_ACCEL_NAIVE = 1 };
+typedef struct fm_scene_desc {
+    uint32_t abi_version;  /* FM_ABI_VERSION */
+    int32_t accelerator;   /* FM_ACCEL_*; both build the reference BVH topology */
+    fm_camera_desc camera;
+    fm_renderer_desc renderer;
+    int32_t num_images;     const fm_image_desc* images;
+    int32_t num_materials;  const fm_material_desc* materials;
+    int32_t num_meshes;     const fm_mesh_desc* meshes;
+    int32_t num_objects;    const fm_object_desc* objects;
+    int32_t num_lights;     const fm_light_desc* lights;
+} fm_scene_desc;
+
+/* ---- context and scene -------------------------------------------------- */
+
+/* One context per device; it owns a HIP stream (default priority), the
+ * device film and scratch buffers.  Not thread-safe: one host thread per ctx. */
+int fm_init(int device, fm_ctx** out);
+void fm_shutdown(fm_ctx* ctx);
+const char* fm_last_error(const fm_ctx* ctx);
+
+/* Builds the scene exactly as SceneLoader::ParseScene does (object order,
+ * default material, area lights, reference BVH topology, env CDF tables) and
+ * uploads it to HBM.  The descriptor is only read during the call. */
+int fm_scene_create(fm_ctx* ctx, const fm_scene_desc* desc, fm_scene** out);
+/* Parses a Fumeng scene JSON (and the OBJ/.hdr/.tga files it names) with the
+ * reference loader semantics, then fm_scene_create.  scene_loader.h:293-302 */
+int fm_scene_load(fm_ctx* ctx, const char* json_path, fm_scene** out);
+void fm_scene_destroy(fm_scene* scene);
+
+/* Scene facts for callers and tests. */
+typedef struct fm_scene_info {
+    int64_t num_geometries;   /* reference RenderObjects (one per mesh face) */
+    int64_t num_bvh_nodes;    /* reference binary BVH nodes */
+    int32_t bvh_depth;
+    int32_t num_lights;       /* area lights first, then JSON lights */
+    int32_t num_materials;    /* including default_material */
+    int32_t has_env;
+    int64_t device_bytes;     /* scene bytes resident in HBM */
+    fm_renderer_desc renderer;
+    fm_camera_desc camera;
+} fm_scene_info;
+int fm_scene_get_info(const fm_scene* scene, fm_scene_info* out);
+
+/* ---- rendering ------------------------------------------------------------ */
+
+enum { FM_MODE_MEGAKERNEL = 0, FM_MODE_WAVEFRONT = 1 };
+typedef struct fm_render_params {
+    int32_t width, height, spp, depth;   /* <= 0: take the scene's renderer block */
+    int32_t direct_loop;                 /* <= 0: scene value */
+    int32_t rr_depth;                    /* < 0: scene value */
+    float rr_coef;                       /* <= 0: scene value */
+    int32_t mode;                        /* FM_MODE_* */
+    uint64_t seed;                       /* Philox key; equal seeds replay paths */
+    int32_t sample_offset;               /* first sample index (progressive passes) */
+    int32_t block_size;                  /* 0 = tuned default */
+    int32_t grid_blocks;                 /* 0 = tuned default */
+    int32_t chunk;                       /* samples per work item, 0 = default */
+    int32_t accumulate;                  /* 1: add to the existing device film */
+    int32_t pad_;
+} fm_render_params;
+
+/* Film, row 0 = top (the reference stores pixel row j at height-1-j,
+ * core/renderer.h:98).  Values are means over spp like RenderTarget.
+ * Any pointer may be NULL; sumsq receives the per-pixel sum of squared
+ * per-sample colour for variance estimates. */
+typedef struct fm_film {
+    float* color;
+    float* albedo;
+    float* normal;
+    double* color_sum;   /* optional raw FP64 sums (before / spp) */
+    double* sumsq;
+} fm_film;
+
+typedef struct fm_render_stats {
+    double kernel_ms;        /* device time of the render launches (HIP events) */
+    int64_t paths;
+    int64_t rays;            /* closest-hit + any-hit queries */
+    int64_t node_visits;     /* only when counting is enabled */
+    int64_t prim_tests;
+} fm_render_stats;
+
+/* Renders into the context's device film (FP64 sums) and, when `film` has
+ * non-NULL fields, copies the means back.  Blocking. */
+int fm_render(fm_ctx* ctx, fm_scene* scene, const fm_render_params* params,
+              fm_film* film, fm_render_stats* stats);
+/* Copies the current device film (means over the samples rendered so far). */
+int fm_film_read(fm_ctx* ctx, fm_film* film);
+/* Enables the node / primitive counters in later renders (0 = off). */
+int fm_set_counting(fm_ctx* ctx, int enabled);
+
+/* ---- parity hooks (batched queries against the uploaded scene) ----------- */
+
+typedef struct fm_ray {
+    float ori[3];
+    float t_min;
+    float dir[3];
+    float t_max;
+} fm_ray;
+
+typedef struct fm_hit {
+    int32_t geom;       /* reference geometry index (object order), -1 = miss */
+    float t;
+    float pos[3];
+    float ng[3];
+    float ns[3];
+    float ss[3];
+    float wo[3];
+    float uv[2];
+    int32_t material;
+    int32_t light;      /* area light index of the object, -1 if none */
+} fm_hit;
+
+/* Closest hit (any_hit = 0, Scene::GetIntersect) or occlusion (any_hit = 1,
+ * Scene::IsIntersect: only geom >= 0 / -1 is meaningful). Host arrays. */
+int fm_trace(fm_ctx* ctx, fm_scene* scene, const fm_ray* rays, int64_t n,
+             int any_hit, fm_hit* hits);
+
+/* Camera::SampleRay for n (ndc.x, ndc.y, lens.x, lens.y) quadruples. */
+int fm_camera_rays(fm_ctx* ctx, fm_scene* scene, const float* ndc_lens,
+                   int64_t n, fm_ray* rays);
+
+/* BSDF query on a given hit frame; material is a scene material index. */
+typedef struct fm_bsdf_query {
+    int32_t material;
+    int32_t pad_;
+    float ng[3], ns[3], ss[3];
+    float wo[3];        /* HitPoint::wo_r_w */
+    float uv[2];
+    float wi[3];        /* direction for CalFunc / Pdf */
+    float sample[3];    /* for SampleBSDF */
+} fm_bsdf_query;
+typedef struct fm_bsdf_result {
+    float f[3];         /* BSDF::CalFunc(wo, wi) */
+    float pdf;          /* BSDF::Pdf(wo, wi) */
+    float sf[3];        /* SampleBSDF: f */
+    float spdf;         /* SampleBSDF: pdf */
+    float swi[3];       /* SampleBSDF: wi_w */
+    int32_t sdelta;     /* SampleBSDF: is_delta */
+    float albedo[3];    /* GetAlbedo */
+    float pad_;
+} fm_bsdf_result;
+int fm_bsdf_eval(fm_ctx* ctx, fm_scene* scene, const fm_bsdf_query* q,
+                 int64_t n, fm_bsdf_result* out);
+
+/* Light::Sample at a shading point, and the light pdfs (solid angle). */
+typedef struct fm_light_query {
+    int32_t light;       /* scene light index */
+    int32_t pad_;
+    float pos[3];        /* shading point (HitPoint::pos) */
+    float sample[3];
+} fm_light_query;
+typedef struct fm_light_result {
+    float ref_pos[3], light_pos[3], wi[3], radiance[3];
+    float pdf, dist;
+} fm_light_result;
+int fm_light_sample(fm_ctx* ctx, fm_scene* scene, const fm_light_query* q,
+                    int64_t n, fm_light_result* out);
+/* EnvLight::Pdf(dir) and EnvLight::GetRadiance(-dir) of the scene's env
+ * light for n directions (3 floats each): out = pdf, rgb. */
+int fm_env_eval(fm_ctx* ctx, fm_scene* scene, const float* dirs, int64_t n,
+                float* pdf, float* rgb);
+
+/* ---- host-only helpers (no device needed) ---------------------------------- */
+
+/* Reference BVH topology of a descriptor, built on the host with the
+ * reference builder semantics (bvh_aggregate.cpp:38-79): per node
+ * {min xyz, max xyz} boxes, is_leaf, idx_a, idx_b; and the leaf-order list of
+ * geometry indices.  Pass NULL arrays to query the sizes. */
+int fm_host_build_bvh(const fm_scene_desc* desc, int64_t* num_nodes,
+                      float* boxes, int32_t* is_leaf, int64_t* idx_a,
+                      int64_t* idx_b, int64_t* num_geoms, int64_t* order);
+/* Parses a scene JSON into a descriptor held by an opaque handle. */
+typedef struct fm_loaded_desc fm_loaded_desc;
+int fm_desc_load(const char* json_path, fm_loaded_desc** out,
+                 const fm_scene_desc** desc, char* err, int64_t err_len);
+void fm_desc_free(fm_loaded_desc* d);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FM_GPU_H */
