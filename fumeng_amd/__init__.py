@@ -1,0 +1,192 @@
+"""fumeng_amd: MI355X path-tracing backend for Fumeng scenes.
+
+The product is the C ABI in include/fm_gpu.h (libfumeng_gpu.so, HIP kernels
+for gfx950).  This module binds it with ctypes and mirrors the reference's
+driver interface: SceneLoader(path).load() / .draw_frame()
+(create/scene_loader.h:293-307).  There is no CPU fallback: without the built
+library or a GPU every call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi as A
+from .scene import SceneDesc  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfumeng_gpu.so")
+_lib = None
+
+
+class FumengError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("fumeng error %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib():
+    """The loaded C ABI library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FumengError(A.FM_ERR_NO_DEVICE, "libfumeng_gpu.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in A.EXPORTS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc, ctx=None):
+    if rc != A.FM_OK:
+        msg = lib().fm_last_error(ctx)
+        raise FumengError(rc, msg.decode() if msg else "")
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None else None
+
+
+class Context:
+    """One HIP device: stream, device film, scratch (fm_init / fm_shutdown)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(lib().fm_init(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().fm_shutdown(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def scene(self, desc):
+        """Upload a SceneDesc (fm_scene_create)."""
+        d, keep = desc.to_c()
+        h = C.c_void_p()
+        _check(lib().fm_scene_create(self.h, C.byref(d), C.byref(h)), self.h)
+        return Scene(self, h)
+
+    def load(self, json_path):
+        """Parse a scene JSON with the reference loader semantics (fm_scene_load)."""
+        h = C.c_void_p()
+        _check(lib().fm_scene_load(self.h, json_path.encode(), C.byref(h)), self.h)
+        return Scene(self, h)
+
+
+class Scene:
+    def __init__(self, ctx, h):
+        self.ctx, self.h = ctx, h
+
+    def close(self):
+        if self.h:
+            lib().fm_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        i = A.fm_scene_info()
+        _check(lib().fm_scene_get_info(self.h, C.byref(i)))
+        return i
+
+    def render(self, width=0, height=0, spp=0, depth=0, seed=1, rr_depth=-1, rr_coef=0.0, direct_loop=0,
+               mode=A.FM_MODE_MEGAKERNEL, sample_offset=0, block_size=0, grid_blocks=0, chunk=0, accumulate=False,
+               read=True):
+        """Returns (color, albedo, normal, sumsq, stats); arrays are (H, W, 3), row 0 = top."""
+        p = A.fm_render_params(width=width, height=height, spp=spp, depth=depth, direct_loop=direct_loop,
+                               rr_depth=rr_depth, rr_coef=rr_coef, mode=mode, seed=seed, sample_offset=sample_offset,
+                               block_size=block_size, grid_blocks=grid_blocks, chunk=chunk,
+                               accumulate=1 if accumulate else 0)
+        info = self.info()
+        W = width or info.renderer.width
+        H = height or info.renderer.height
+        st = A.fm_render_stats()
+        film = A.fm_film()
+        out = None
+        if read:
+            out = [np.zeros((H, W, 3), np.float32) for _ in range(3)] + [np.zeros((H, W, 3), np.float64)]
+            film.color, film.albedo, film.normal, film.sumsq = (o.ctypes.data for o in out)
+        _check(lib().fm_render(self.ctx.h, self.h, C.byref(p), C.byref(film), C.byref(st)), self.ctx.h)
+        if read:
+            return out[0], out[1], out[2], out[3], st
+        return None, None, None, None, st
+
+    def trace(self, rays, any_hit=False):
+        """rays: (N, 8) float32 [ox oy oz tmin dx dy dz tmax]; returns fm_hit records."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        out = np.zeros(rays.shape[0], dtype=np.dtype(A.fm_hit))
+        _check(lib().fm_trace(self.ctx.h, self.h, _ptr(rays), rays.shape[0], 1 if any_hit else 0, _ptr(out)),
+               self.ctx.h)
+        return out
+
+    def camera_rays(self, ndc_lens):
+        q = np.ascontiguousarray(ndc_lens, dtype=np.float32).reshape(-1, 4)
+        out = np.zeros(q.shape[0], dtype=np.dtype(A.fm_ray))
+        _check(lib().fm_camera_rays(self.ctx.h, self.h, _ptr(q), q.shape[0], _ptr(out)), self.ctx.h)
+        return out
+
+    def bsdf(self, queries):
+        q = np.ascontiguousarray(queries, dtype=np.dtype(A.fm_bsdf_query))
+        out = np.zeros(q.shape[0], dtype=np.dtype(A.fm_bsdf_result))
+        _check(lib().fm_bsdf_eval(self.ctx.h, self.h, _ptr(q), q.shape[0], _ptr(out)), self.ctx.h)
+        return out
+
+    def light_sample(self, queries):
+        q = np.ascontiguousarray(queries, dtype=np.dtype(A.fm_light_query))
+        out = np.zeros(q.shape[0], dtype=np.dtype(A.fm_light_result))
+        _check(lib().fm_light_sample(self.ctx.h, self.h, _ptr(q), q.shape[0], _ptr(out)), self.ctx.h)
+        return out
+
+    def env_eval(self, dirs):
+        d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+        pdf = np.zeros(d.shape[0], np.float32)
+        rgb = np.zeros((d.shape[0], 3), np.float32)
+        _check(lib().fm_env_eval(self.ctx.h, self.h, _ptr(d), d.shape[0], _ptr(pdf), _ptr(rgb)), self.ctx.h)
+        return pdf, rgb
+
+
+def host_bvh(desc):
+    """Reference BVH topology built on the host (no GPU needed)."""
+    d, keep = desc.to_c()
+    nn, ng = C.c_int64(), C.c_int64()
+    _check(lib().fm_host_build_bvh(C.byref(d), C.byref(nn), None, None, None, None, C.byref(ng), None))
+    boxes = np.zeros((nn.value, 6), np.float32)
+    leaf = np.zeros(nn.value, np.int32)
+    a = np.zeros(nn.value, np.int64)
+    b = np.zeros(nn.value, np.int64)
+    order = np.zeros(ng.value, np.int64)
+    _check(lib().fm_host_build_bvh(C.byref(d), C.byref(nn), _ptr(boxes), _ptr(leaf), _ptr(a), _ptr(b),
+                                   C.byref(ng), _ptr(order)))
+    return boxes, leaf, a, b, order
+
+
+class SceneLoader:
+    """Mirror of the reference's SceneLoader: load(path) then draw_frame()
+    renders with the scene's renderer block and writes the output files
+    through the fumeng_gpu front end's conventions."""
+
+    def __init__(self, device=0):
+        self.ctx = Context(device)
+        self.scene = None
+
+    def load(self, path):
+        self.scene = self.ctx.load(os.path.abspath(path))
+        return self
+
+    def draw_frame(self, seed=1):
+        color, albedo, normal, _, st = self.scene.render(seed=seed)
+        return color, albedo, normal, st

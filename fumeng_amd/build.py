@@ -1,0 +1,76 @@
+"""Builds the MI355X path-tracing library (libfumeng_gpu.so), the
+CMD_EXE-compatible front end (bin/fumeng_gpu) and, when asked, the CPU oracle.
+
+Host C++ is compiled with g++, HIP kernels with hipcc for gfx950; everything is
+written in-tree so that the built files travel with the source tree.
+"""
+import concurrent.futures
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_obj")
+LIB = os.path.join(HERE, "libfumeng_gpu.so")
+CLI = os.path.join(HERE, "bin", "fumeng_gpu")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+ARCH = "gfx950"
+
+HIP_FLAGS = ["-O3", "--offload-arch=" + ARCH, "-std=c++17", "-fPIC",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
+             "-I" + os.path.join(ROCM, "include"), "-D__HIP_PLATFORM_AMD__"]
+
+HOST_SOURCES = ["api.cpp", "host/loader.cpp", "host/scene_build.cpp"]
+HIP_SOURCES = ["kernels.hip"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        raise RuntimeError("command failed: %s\n%s" % (" ".join(cmd), out))
+    return out
+
+
+def _compile(src):
+    path = os.path.join(CSRC, src)
+    obj = os.path.join(OBJ, src.replace("/", "_") + ".o")
+    if src.endswith(".hip"):
+        cmd = [os.path.join(ROCM, "bin", "hipcc")] + HIP_FLAGS + ["-c", path, "-o", obj]
+    else:
+        cmd = ["g++"] + CXX_FLAGS + ["-c", path, "-o", obj]
+    _run(cmd)
+    return obj
+
+
+def build_library(verbose=False):
+    os.makedirs(OBJ, exist_ok=True)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=4) as ex:
+        objs = list(ex.map(_compile, HOST_SOURCES + HIP_SOURCES))
+    tmp = LIB + ".tmp"
+    _run([os.path.join(ROCM, "bin", "hipcc"), "-shared", "-fPIC", "--offload-arch=" + ARCH,
+          "-o", tmp] + objs + ["-lpthread"])
+    os.replace(tmp, LIB)
+    os.makedirs(os.path.dirname(CLI), exist_ok=True)
+    _run(["g++"] + CXX_FLAGS + [os.path.join(CSRC, "cli", "main.cpp"), "-o", CLI,
+          "-L" + HERE, "-lfumeng_gpu", "-Wl,-rpath,$ORIGIN/.."])
+    if verbose:
+        print("built", LIB, "and", CLI)
+    return LIB
+
+
+def build_oracle(verbose=False):
+    out = _run(["make", "-C", os.path.join(ROOT, "oracle"), "-s"])
+    if verbose and out.strip():
+        print(out)
+    return os.path.join(ROOT, "oracle", "libfm_oracle.so")
+
+
+if __name__ == "__main__":
+    build_library(verbose=True)
+    if "--oracle" in sys.argv:
+        build_oracle(verbose=True)

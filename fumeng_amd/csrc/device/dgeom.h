@@ -212,15 +212,19 @@ FMD bool test_slot_any(const fm_dscene& S, const Ray& r, const RayPre& rp, int32
     return false;
 }
 
+/* Work counters of the counting variant (node visits, primitive tests). */
+struct Cnt { uint32_t nodes = 0, prims = 0; };
+
 /* Traversal with a per-thread stack in LDS: stack[level * stride + lane]. */
-template <bool FAST>
-FMD void closest_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& h, uint32_t* stk, int stride) {
+template <bool FAST, bool COUNT>
+FMD void closest_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& h, uint32_t* stk, int stride, Cnt& cn) {
     float tn;
     if (!box_hit<FAST>(S.root_box, r, rp, h.t, tn)) return;
     int32_t ref = S.root_ref;
     int sp = 0;
     for (;;) {
         if (ref >= 0) {
+            if (COUNT) cn.nodes++;
             const float4* nd = reinterpret_cast<const float4*>(S.nodes + ref);
             const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2];
             const int4 cr = reinterpret_cast<const int4*>(nd)[3];
@@ -240,6 +244,7 @@ FMD void closest_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& h
             if (h1) { ref = cr.y; continue; }
         } else {
             const int32_t start = FM_LEAF_START(ref), cnt = FM_LEAF_COUNT(ref);
+            if (COUNT) cn.prims += (uint32_t)cnt;
             for (int k = 0; k < cnt; k++) {
                 const int32_t p = start + k;
                 const int32_t key = S.naive ? S.meta[4 * (size_t)p] : p;
@@ -252,24 +257,28 @@ FMD void closest_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& h
     }
 }
 
-FMD Hit trace_closest(const fm_dscene& S, const Ray& r, uint32_t* stk, int stride) {
+template <bool COUNT = false>
+FMD Hit trace_closest(const fm_dscene& S, const Ray& r, uint32_t* stk, int stride, Cnt* cn = nullptr) {
     Hit h;
     h.t = r.tmax; h.slot = -1; h.kind = -1; h.tie = 0; h.half = 0; h.b0 = h.b1 = h.b2 = 0.0f;
     if (S.num_slots == 0) return h;
     const RayPre rp = precompute(r);
-    if (rp.fast) closest_impl<true>(S, r, rp, h, stk, stride);
-    else closest_impl<false>(S, r, rp, h, stk, stride);
+    Cnt dummy;
+    Cnt& c = COUNT ? *cn : dummy;
+    if (rp.fast) closest_impl<true, COUNT>(S, r, rp, h, stk, stride, c);
+    else closest_impl<false, COUNT>(S, r, rp, h, stk, stride, c);
     return h;
 }
 
-template <bool FAST>
-FMD bool any_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, uint32_t* stk, int stride) {
+template <bool FAST, bool COUNT>
+FMD bool any_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, uint32_t* stk, int stride, Cnt& cn) {
     float tn;
     if (!box_hit<FAST>(S.root_box, r, rp, r.tmax, tn)) return false;
     int32_t ref = S.root_ref;
     int sp = 0;
     for (;;) {
         if (ref >= 0) {
+            if (COUNT) cn.nodes++;
             const float4* nd = reinterpret_cast<const float4*>(S.nodes + ref);
             const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2];
             const int4 cr = reinterpret_cast<const int4*>(nd)[3];
@@ -283,8 +292,10 @@ FMD bool any_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, uint32_t* 
             if (h1) { ref = cr.y; continue; }
         } else {
             const int32_t start = FM_LEAF_START(ref), cnt = FM_LEAF_COUNT(ref);
-            for (int k = 0; k < cnt; k++)
+            for (int k = 0; k < cnt; k++) {
+                if (COUNT) cn.prims++;
                 if (test_slot_any(S, r, rp, start + k)) return true;
+            }
         }
         if (sp == 0) break;
         sp--;
@@ -293,10 +304,13 @@ FMD bool any_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, uint32_t* 
     return false;
 }
 
-FMD bool trace_any(const fm_dscene& S, const Ray& r, uint32_t* stk, int stride) {
+template <bool COUNT = false>
+FMD bool trace_any(const fm_dscene& S, const Ray& r, uint32_t* stk, int stride, Cnt* cn = nullptr) {
     if (S.num_slots == 0) return false;
     const RayPre rp = precompute(r);
-    return rp.fast ? any_impl<true>(S, r, rp, stk, stride) : any_impl<false>(S, r, rp, stk, stride);
+    Cnt dummy;
+    Cnt& c = COUNT ? *cn : dummy;
+    return rp.fast ? any_impl<true, COUNT>(S, r, rp, stk, stride, c) : any_impl<false, COUNT>(S, r, rp, stk, stride, c);
 }
 
 /* Hit point as the geometry's GetIntersect fills it (triangle.cpp:211-248,

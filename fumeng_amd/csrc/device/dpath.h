@@ -202,14 +202,16 @@ struct PathOut { v3 color, albedo, normal; uint32_t rays; };
 struct PathCfg { int depth, direct, rr_depth; float rr_coef; };
 
 /* PathTracer::RenderPixel for one camera ray. */
-FMD PathOut render_path(const fm_dscene& S, const PathCfg& cfg, Ray ray, Rng& rng, uint32_t* stk, int stride) {
+template <bool COUNT = false>
+FMD PathOut render_path(const fm_dscene& S, const PathCfg& cfg, Ray ray, Rng& rng, uint32_t* stk, int stride,
+                        Cnt* cn = nullptr) {
     PathOut out;
     out.color = out.albedo = out.normal = splat(0.0f);
     out.rays = 0;
     v3 beta = splat(1.0f);
     Ray r = ray;
     for (int depth = 1; depth <= cfg.depth; depth++) {
-        const Hit h = trace_closest(S, r, stk, stride);
+        const Hit h = trace_closest<COUNT>(S, r, stk, stride, cn);
         out.rays++;
         if (h.slot < 0) {
             if (depth == 1 && S.env >= 0) out.color = env_radiance(S, S.envs[S.env], -ray.d);
@@ -232,7 +234,7 @@ FMD PathOut render_path(const fm_dscene& S, const PathCfg& cfg, Ray ray, Rng& rn
                 if (ls.pdf < FM_EPS_PDF || length(ls.radiance) < FM_EPS_PDF) continue;
                 Ray sh; sh.o = ls.ref_pos; sh.d = ls.wi; sh.tmin = FM_EPS; sh.tmax = ls.dist - FM_EPS;
                 out.rays++;
-                if (trace_any(S, sh, stk, stride)) continue;
+                if (trace_any<COUNT>(S, sh, stk, stride, cn)) continue;
                 const v3 f = b.eval(sf.wo, ls.wi);
                 if (length(f) < FM_EPS_PDF) continue;
                 const float bp = b.pdf(sf.wo, ls.wi);
@@ -247,7 +249,7 @@ FMD PathOut render_path(const fm_dscene& S, const PathCfg& cfg, Ray ray, Rng& rn
             if (!(length(bs.f) < FM_EPS_PDF || bs.pdf < FM_EPS_PDF)) {
                 bs.wi = normalize(bs.wi);
                 const Ray mr = gen_ray(sf.pos, sf.ng, bs.wi);
-                const Hit lh = trace_closest(S, mr, stk, stride);
+                const Hit lh = trace_closest<COUNT>(S, mr, stk, stride, cn);
                 out.rays++;
                 if (lh.slot < 0) {
                     if (S.env >= 0) {

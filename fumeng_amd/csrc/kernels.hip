@@ -117,11 +117,15 @@ __global__ void k_env(fm_dscene S, const float* __restrict__ dirs, int64_t n, fl
     rgb[3 * i] = c.x; rgb[3 * i + 1] = c.y; rgb[3 * i + 2] = c.z;
 }
 
+template <bool COUNT>
 __global__ void __launch_bounds__(256) k_render_mega(fm_dscene S, fm::RenderArgs A) {
     uint32_t* stk = lane_stack();
     const int stride = blockDim.x;
     const int64_t npix = (int64_t)A.W * A.H;
     unsigned long long rays = 0, paths = 0;
+    Cnt cn;
+    PathCfg cfg;
+    cfg.depth = A.cfg_raw.depth; cfg.direct = A.cfg_raw.direct; cfg.rr_depth = A.cfg_raw.rr_depth; cfg.rr_coef = A.cfg_raw.rr_coef;
     for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < A.total_items; w += (int64_t)gridDim.x * blockDim.x) {
         const int64_t chunk = w / npix;
         const int64_t pix = w - chunk * npix;
@@ -138,7 +142,7 @@ __global__ void __launch_bounds__(256) k_render_mega(fm_dscene S, fm::RenderArgs
             const float lx = rng.next(), ly = rng.next();
             const float px = ((float)i + fx) / (float)A.W, py = ((float)j + fy) / (float)A.H;
             const Ray cr = camera_ray(S.camera, px, py, lx, ly);
-            const PathOut po = render_path(S, A.cfg, cr, rng, stk, stride);
+            const PathOut po = render_path<COUNT>(S, cfg, cr, rng, stk, stride, &cn);
             cs += po.color; as += po.albedo; ns += po.normal;
             q0 += po.color.x * po.color.x; q1 += po.color.y * po.color.y; q2 += po.color.z * po.color.z;
             rays += po.rays;
@@ -162,6 +166,10 @@ __global__ void __launch_bounds__(256) k_render_mega(fm_dscene S, fm::RenderArgs
     }
     wave_add_u64(A.counters + 0, paths);
     wave_add_u64(A.counters + 1, rays);
+    if (COUNT) {
+        wave_add_u64(A.counters + 2, cn.nodes);
+        wave_add_u64(A.counters + 3, cn.prims);
+    }
 }
 
 __global__ void k_zero_f64(double* p, int64_t n) {
@@ -201,9 +209,15 @@ hipError_t launch_zero_f64(double* p, int64_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_f64, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, n);
     return hipGetLastError();
 }
-hipError_t launch_render_mega(const fm_dscene& S, const RenderArgs& A, int block, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_render_mega, dim3(grid), dim3(block), stack_bytes(block), st, S, A);
+hipError_t launch_render_mega(const fm_dscene& S, const RenderArgs& A, int block, int grid, bool count, hipStream_t st) {
+    if (count) hipLaunchKernelGGL(k_render_mega<true>, dim3(grid), dim3(block), stack_bytes(block), st, S, A);
+    else hipLaunchKernelGGL(k_render_mega<false>, dim3(grid), dim3(block), stack_bytes(block), st, S, A);
     return hipGetLastError();
+}
+int mega_blocks_per_cu(int block) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_render_mega<false>, block, stack_bytes(block)) != hipSuccess) return 0;
+    return n;
 }
 
 }  // namespace fm

@@ -1,0 +1,203 @@
+"""Seeded procedural scenes for the benchmark configurations.
+
+box_scene       open 2x2x2 box, 5 Lambert walls, 0.5x0.5 ceiling light,
+                Lambert + dielectric spheres (512x512, 64 spp, depth 8, rr 3)
+displaced_icosphere / icosphere_scene
+                geodesic icosphere at frequency f (20 f^2 triangles) displaced
+                by 6-octave value-noise fbm, ground plane, 64 random Disney
+                materials, analytic sky + sun HDR env, thin-lens camera
+                (f = 224: 1,003,520 triangles, 1920x1080, 256 spp, depth 12)
+All values are float32 and fully determined by the seed.
+"""
+import numpy as np
+
+from .scene import SceneDesc, rgbe_decode, rgbe_encode
+
+
+def box_scene(width=512, height=512, spp=64, depth=8, rr_depth=3, rr_coef=0.85, accelerator="bvh"):
+    s = SceneDesc()
+    s.camera = {"type": "pin_hole", "pos": [0.0, -3.75, 0.0], "look_at": [0.0, 0.0, 0.0], "up": [0.0, 0.0, 1.0],
+                "fov": 40.0, "aspect": 1.0, "focal_distance": 1.0, "lens_radius": 0.0}
+    s.renderer = {"width": width, "height": height, "spp": spp, "thread_count": 8, "depth": depth, "direct_loop": 1,
+                  "rr_depth": rr_depth, "rr_coef": rr_coef, "output": "box"}
+    s.accelerator = accelerator
+    white = s.add_lambert(0.73, name="white")
+    red = s.add_lambert([0.65, 0.05, 0.05], name="red")
+    green = s.add_lambert([0.12, 0.45, 0.15], name="green")
+    glass = s.add_dielectric(1.0, 1.5, name="glass")
+    # normals point into the box: normalize((pb - pa) x (pc - pa))
+    s.add_rectangle((-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1), white)   # floor, +z
+    s.add_rectangle((-1, -1, 1), (-1, 1, 1), (1, 1, 1), (1, -1, 1), white)       # ceiling, -z
+    s.add_rectangle((-1, 1, -1), (1, 1, -1), (1, 1, 1), (-1, 1, 1), white)       # back, -y
+    s.add_rectangle((1, -1, -1), (1, -1, 1), (1, 1, 1), (1, 1, -1), red)         # x = +1, -x (image left)
+    s.add_rectangle((-1, -1, -1), (-1, 1, -1), (-1, 1, 1), (-1, -1, 1), green)   # x = -1, +x (image right)
+    z = 1.0 - 1e-3
+    s.add_rectangle((-0.25, -0.25, z), (-0.25, 0.25, z), (0.25, 0.25, z), (0.25, -0.25, z), white,
+                    emittance=[17.0, 12.0, 4.0])
+    s.add_sphere(0.3, translation=(0.4, 0.3, -0.7), material=white)
+    s.add_sphere(0.3, translation=(-0.4, -0.2, -0.7), material=glass)
+    return s
+
+
+# ---- noise -------------------------------------------------------------------
+def _hash_u32(*xs):
+    h = np.uint32(0x9E3779B9)
+    out = np.full(np.broadcast(*xs).shape, h, dtype=np.uint32)
+    for x in xs:
+        v = np.asarray(x).astype(np.int64).astype(np.uint32)
+        out = (out ^ v) * np.uint32(0x85EBCA6B)
+        out ^= out >> np.uint32(13)
+        out = out * np.uint32(0xC2B2AE35)
+        out ^= out >> np.uint32(16)
+    return out
+
+
+def value_noise(p, seed, octave):
+    """3D lattice value noise in [-1, 1] with smoothstep interpolation."""
+    f = np.floor(p)
+    i = f.astype(np.int64)
+    t = p - f
+    t = t * t * (3.0 - 2.0 * t)
+    acc = 0.0
+    for dx in (0, 1):
+        wx = t[:, 0] if dx else 1.0 - t[:, 0]
+        for dy in (0, 1):
+            wy = t[:, 1] if dy else 1.0 - t[:, 1]
+            for dz in (0, 1):
+                wz = t[:, 2] if dz else 1.0 - t[:, 2]
+                h = _hash_u32(i[:, 0] + dx, i[:, 1] + dy, i[:, 2] + dz, np.int64(octave), np.int64(seed))
+                val = (h.astype(np.float64) / 4294967295.0) * 2.0 - 1.0
+                acc = acc + wx * wy * wz * val
+    return acc
+
+
+def fbm(p, seed, octaves=6, lacunarity=2.0, gain=0.5, base_freq=2.0):
+    amp, freq, total, norm = 1.0, base_freq, 0.0, 0.0
+    for o in range(octaves):
+        total = total + amp * value_noise(p * freq, seed, o)
+        norm += amp
+        amp *= gain
+        freq *= lacunarity
+    return total / norm
+
+
+# ---- icosphere -----------------------------------------------------------------
+def _icosahedron():
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], dtype=np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+                  [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11],
+                  [6, 2, 10], [8, 6, 7], [9, 8, 1]], dtype=np.int64)
+    return v, f
+
+
+def geodesic_sphere(freq):
+    """Class-I geodesic subdivision: 20 * freq^2 triangles, shared vertices."""
+    v, faces = _icosahedron()
+    # lattice points (i, j) with i + j <= freq on each face: p = (A (f-i-j) + B i + C j) / f
+    ii, jj = np.meshgrid(np.arange(freq + 1), np.arange(freq + 1), indexing="ij")
+    m = (ii + jj) <= freq
+    ii, jj = ii[m], jj[m]
+    kk = freq - ii - jj
+    A, B, Cc = v[faces[:, 0]], v[faces[:, 1]], v[faces[:, 2]]
+    pts = (A[:, None, :] * kk[None, :, None] + B[:, None, :] * ii[None, :, None] + Cc[:, None, :] * jj[None, :, None])
+    # integer key of a lattice point: the barycentric weights attached to global corner ids
+    # (points on shared edges / corners get the same key from every face)
+    npf = ii.size
+    w = np.stack([np.broadcast_to(kk, (20, npf)), np.broadcast_to(ii, (20, npf)), np.broadcast_to(jj, (20, npf))], -1)
+    cid = np.broadcast_to(faces[:, None, :], (20, npf, 3))
+    wk = np.where(w > 0, cid, 12)            # corner id, 12 = unused
+    wv = np.where(w > 0, w, 0)
+    order = np.argsort(wk, axis=-1)
+    wk = np.take_along_axis(wk, order, -1)
+    wv = np.take_along_axis(wv, order, -1)
+    key = ((wk[..., 0] * 13 + wk[..., 1]) * 13 + wk[..., 2]) * (freq + 1) ** 2 + wv[..., 0] * (freq + 1) + wv[..., 1]
+    key = key.reshape(-1)
+    uniq, first, inv = np.unique(key, return_index=True, return_inverse=True)
+    verts = pts.reshape(-1, 3)[first] / freq
+    verts /= np.linalg.norm(verts, axis=1, keepdims=True)
+    # triangles of the lattice
+    lid = -np.ones((freq + 1, freq + 1), dtype=np.int64)
+    lid[ii, jj] = np.arange(npf)
+    tris = []
+    for i in range(freq):
+        for j in range(freq - i):
+            tris.append((lid[i, j], lid[i + 1, j], lid[i, j + 1]))
+            if i + j < freq - 1:
+                tris.append((lid[i + 1, j], lid[i + 1, j + 1], lid[i, j + 1]))
+    tris = np.array(tris, dtype=np.int64)
+    all_tris = (tris[None, :, :] + (np.arange(20) * npf)[:, None, None]).reshape(-1, 3)
+    return verts, inv[all_tris].astype(np.int32)
+
+
+def displaced_icosphere(freq, seed, amplitude=0.15, radius=1.0):
+    verts, tris = geodesic_sphere(freq)
+    d = fbm(verts * 1.0, seed)
+    d = d / max(1e-9, np.abs(d).max())
+    pos = verts * (radius + amplitude * d)[:, None]
+    return pos.astype(np.float32), tris
+
+
+def sky_env(width, height, seed, sun_theta_deg=45.0, sun_phi_deg=60.0, sun_sigma_deg=1.0, sun_peak=5e4, sigma=0.3):
+    """z-up lat-long float RGB: sky gradient + Gaussian sun lobe, log-normal
+    texel noise, quantised through RGBE like a .hdr file."""
+    rng = np.random.default_rng(seed)
+    v = (np.arange(height) + 0.5) / height
+    u = (np.arange(width) + 0.5) / width
+    th = np.pi * v[:, None]
+    ph = 2 * np.pi * u[None, :]
+    x, y, z = np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th) * np.ones_like(ph)
+    zen = np.array([0.25, 0.45, 0.95]) * 0.9
+    hor = np.array([0.95, 0.95, 1.0]) * 1.2
+    gnd = np.array([0.35, 0.30, 0.25]) * 0.4
+    zc = np.clip(z, 0, 1)[..., None]
+    sky = np.where(z[..., None] >= 0, hor * (1 - np.sqrt(zc)) + zen * np.sqrt(zc), gnd)
+    st, sp = np.radians(sun_theta_deg), np.radians(sun_phi_deg)
+    sd = np.array([np.sin(st) * np.cos(sp), np.sin(st) * np.sin(sp), np.cos(st)])
+    cosang = np.clip(x * sd[0] + y * sd[1] + z * sd[2], -1, 1)
+    ang = np.arccos(cosang)
+    sun = sun_peak * np.exp(-0.5 * (ang / np.radians(sun_sigma_deg)) ** 2)
+    img = sky + sun[..., None] * np.array([1.0, 0.95, 0.88])
+    img = img * np.exp(sigma * rng.standard_normal((height, width)))[..., None]
+    return rgbe_decode(rgbe_encode(img.astype(np.float32)))
+
+
+def random_disney(s, n, seed, prefix="d"):
+    rng = np.random.default_rng(seed)
+    ids = []
+    for k in range(n):
+        r = rng.random(8)
+        ids.append(s.add_disney(name="%s%d" % (prefix, k), basecolor=[float(r[0]), float(r[1]), float(r[2])],
+                                roughness=float(r[3]), metallic=float(r[4]), specular=float(r[5]),
+                                clearcoat=float(r[6]), sheen=float(r[7])))
+    return ids
+
+
+def icosphere_scene(freq=224, width=1920, height=1080, spp=256, depth=12, seed=2, env_size=(2048, 1024),
+                    n_materials=64):
+    s = SceneDesc()
+    s.camera = {"type": "thin_lens", "pos": [0.0, -4.0, 0.3], "look_at": [0.0, 0.0, 0.0], "up": [0.0, 0.0, 1.0],
+                "fov": 40.0, "aspect": 16.0 / 9.0, "focal_distance": 3.0, "lens_radius": 0.05}
+    # rr keys absent in the JSON: defaults rr_depth 10, rr_coef 0.85
+    s.renderer = {"width": width, "height": height, "spp": spp, "thread_count": 8, "depth": depth, "direct_loop": 1,
+                  "rr_depth": 10, "rr_coef": 0.85, "output": "icosphere"}
+    mats = random_disney(s, n_materials, seed)
+    ground_mat = s.add_lambert(0.4, name="ground")
+    pos, tris = displaced_icosphere(freq, seed)
+    tid = np.arange(tris.shape[0])
+    mid = (_hash_u32(tid, np.int64(seed)) % np.uint32(n_materials)).astype(np.int64)
+    for k in range(n_materials):
+        sel = tris[mid == k]
+        if sel.shape[0] == 0:
+            continue
+        used, local = np.unique(sel.reshape(-1), return_inverse=True)
+        m = s.add_mesh(pos[used], local.reshape(-1, 3))
+        s.add_mesh_object(m, material=mats[k])
+    g = s.add_mesh(np.array([[-10, -10, -1.2], [10, -10, -1.2], [10, 10, -1.2], [-10, 10, -1.2]], np.float32),
+                   np.array([[0, 1, 2], [0, 2, 3]], np.int32))
+    s.add_mesh_object(g, material=ground_mat)
+    env = s.add_image(sky_env(env_size[0], env_size[1], seed))
+    s.add_env(env)
+    return s

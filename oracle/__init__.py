@@ -1,0 +1,137 @@
+"""ORACLE (test infrastructure only): ctypes binding of libfm_oracle.so, the
+CPU restatement of the reference path tracer.  Only tests/, smoke() and
+bench.py's cpu_baseline leg may load this, as the checker or CPU baseline."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from fumeng_amd import _abi as A
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfm_oracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-C", _HERE, "-s"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        V, P, I64 = C.c_void_p, C.c_void_p, C.c_int64
+        sig = {
+            "or_scene_create": (V, [C.POINTER(A.fm_scene_desc), C.c_int, C.c_char_p, I64]),
+            "or_scene_destroy": (None, [V]),
+            "or_scene_info": (C.c_int, [V, C.POINTER(I64), C.POINTER(I64), C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+            "or_bvh_export": (C.c_int, [V, P, P, P, P, P]),
+            "or_trace": (C.c_int, [V, P, I64, C.c_int, C.c_int, P]),
+            "or_bsdf_eval": (C.c_int, [V, P, I64, P]),
+            "or_light_sample": (C.c_int, [V, P, I64, P]),
+            "or_env_eval": (C.c_int, [V, P, I64, P, P]),
+            "or_camera_rays": (C.c_int, [V, P, I64, P]),
+            "or_render": (C.c_int, [V, C.POINTER(A.fm_render_params), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    P, P, P, P]),
+            "or_philox": (None, [P, P, P]),
+            "or_abi_version": (C.c_int, []),
+        }
+        for n, (r, a) in sig.items():
+            f = getattr(L, n)
+            f.restype, f.argtypes = r, a
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data if a is not None else None
+
+
+class OracleScene:
+    """Reference scene built on the CPU, FP32 (USE_FLOAT) or FP64."""
+
+    def __init__(self, desc, fp64=False):
+        d, keep = desc.to_c()
+        err = C.create_string_buffer(512)
+        self.h = lib().or_scene_create(C.byref(d), 1 if fp64 else 0, err, 512)
+        if not self.h:
+            raise RuntimeError("oracle scene: " + err.value.decode())
+        self.fp64 = fp64
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_scene_destroy(self.h)
+            self.h = None
+
+    def info(self):
+        ng, nn = C.c_int64(), C.c_int64()
+        depth, nl, env = C.c_int32(), C.c_int32(), C.c_int32()
+        lib().or_scene_info(self.h, C.byref(ng), C.byref(nn), C.byref(depth), C.byref(nl), C.byref(env))
+        return {"geoms": ng.value, "nodes": nn.value, "depth": depth.value, "lights": nl.value, "env": env.value}
+
+    def bvh(self):
+        i = self.info()
+        boxes = np.zeros((i["nodes"], 6), np.float32)
+        leaf = np.zeros(i["nodes"], np.int32)
+        a = np.zeros(i["nodes"], np.int64)
+        b = np.zeros(i["nodes"], np.int64)
+        order = np.zeros(i["geoms"], np.int64)
+        assert lib().or_bvh_export(self.h, _p(boxes), _p(leaf), _p(a), _p(b), _p(order)) == 0
+        return boxes, leaf, a, b, order
+
+    def trace(self, rays, any_hit=False, brute=False):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        out = np.zeros(rays.shape[0], dtype=np.dtype(A.fm_hit))
+        lib().or_trace(self.h, _p(rays), rays.shape[0], 1 if any_hit else 0, 1 if brute else 0, _p(out))
+        return out
+
+    def camera_rays(self, ndc_lens):
+        q = np.ascontiguousarray(ndc_lens, dtype=np.float32).reshape(-1, 4)
+        out = np.zeros(q.shape[0], dtype=np.dtype(A.fm_ray))
+        lib().or_camera_rays(self.h, _p(q), q.shape[0], _p(out))
+        return out
+
+    def bsdf(self, queries):
+        q = np.ascontiguousarray(queries, dtype=np.dtype(A.fm_bsdf_query))
+        out = np.zeros(q.shape[0], dtype=np.dtype(A.fm_bsdf_result))
+        lib().or_bsdf_eval(self.h, _p(q), q.shape[0], _p(out))
+        return out
+
+    def light_sample(self, queries):
+        q = np.ascontiguousarray(queries, dtype=np.dtype(A.fm_light_query))
+        out = np.zeros(q.shape[0], dtype=np.dtype(A.fm_light_result))
+        lib().or_light_sample(self.h, _p(q), q.shape[0], _p(out))
+        return out
+
+    def env_eval(self, dirs):
+        d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+        pdf = np.zeros(d.shape[0], np.float32)
+        rgb = np.zeros((d.shape[0], 3), np.float32)
+        if lib().or_env_eval(self.h, _p(d), d.shape[0], _p(pdf), _p(rgb)) != 0:
+            raise RuntimeError("scene has no env light")
+        return pdf, rgb
+
+    def render(self, x0=0, y0=0, w=None, h=None, spp=0, seed=1, depth=0, rr_depth=-1, width=0, height=0,
+               sample_offset=0, threads=1):
+        """Renders the sub-rectangle [x0, x0+w) x [y0, y0+h) of the image
+        (row 0 = top); returns float64 (h, w, 3) color, albedo, normal, sumsq."""
+        p = A.fm_render_params(width=width, height=height, spp=spp, depth=depth, rr_depth=rr_depth, seed=seed,
+                               sample_offset=sample_offset)
+        if w is None or h is None:
+            raise ValueError("w and h are required")
+        outs = [np.zeros((h, w, 3), np.float64) for _ in range(4)]
+        lib().or_render(self.h, C.byref(p), x0, y0, w, h, threads, *[_p(o) for o in outs])
+        return outs
+
+
+def philox(ctr, key):
+    c = np.asarray(ctr, np.uint32)
+    k = np.asarray(key, np.uint32)
+    o = np.zeros(4, np.uint32)
+    lib().or_philox(_p(c), _p(k), _p(o))
+    return o

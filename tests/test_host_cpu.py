@@ -1,0 +1,195 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports
+every declared symbol, the ctypes mirror has the C layouts, the scene JSON
+loader follows the reference loader's semantics, and the product's restated
+BVH builder reproduces the reference topology."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import fumeng_amd
+import oracle
+from fumeng_amd import _abi as A
+from fumeng_amd import scenes
+from fumeng_amd.scene import SceneDesc, rgbe_decode, rgbe_encode
+
+from helpers import mesh_scene
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "fm_gpu.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fm_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = C.CDLL(fumeng_amd.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 18
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(A.EXPORTS), set(names) ^ set(A.EXPORTS)
+
+
+def test_struct_layouts_match_c(tmp_path):
+    structs = ["fm_texture_desc", "fm_image_desc", "fm_material_desc", "fm_mesh_desc", "fm_object_desc",
+               "fm_light_desc", "fm_camera_desc", "fm_renderer_desc", "fm_scene_desc", "fm_scene_info",
+               "fm_render_params", "fm_film", "fm_render_stats", "fm_ray", "fm_hit", "fm_bsdf_query",
+               "fm_bsdf_result", "fm_light_query", "fm_light_result"]
+    src = ['#include "%s"' % HEADER, "#include <stdio.h>", "#include <stddef.h>", "int main(void){"]
+    for s in structs:
+        src.append('printf("%s %%zu\\n", sizeof(%s));' % (s, s))
+        for f, _ in getattr(A, s)._fields_:
+            src.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (s, f, s, f))
+    src.append("return 0;}")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", str(c), "-o", str(exe)], check=True)
+    out = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines())
+    for s in structs:
+        T = getattr(A, s)
+        assert int(out[s]) == C.sizeof(T), s
+        for f, _ in T._fields_:
+            assert int(out[s + "." + f]) == getattr(T, f).offset, s + "." + f
+
+
+def test_init_reports_missing_device_or_succeeds():
+    h = C.c_void_p()
+    rc = fumeng_amd.lib().fm_init(0, C.byref(h))
+    assert rc in (A.FM_OK, A.FM_ERR_NO_DEVICE)
+    if rc == A.FM_OK:
+        fumeng_amd.lib().fm_shutdown(h)
+    else:
+        assert b"device" in fumeng_amd.lib().fm_last_error(None)
+
+
+@pytest.mark.parametrize("freq", [6, 14])
+def test_host_builder_equals_reference_topology(freq):
+    s = mesh_scene(freq=freq)
+    ours = fumeng_amd.host_bvh(s)
+    ref = oracle.OracleScene(s).bvh()
+    for a, b in zip(ours, ref):
+        assert np.array_equal(a, b)
+
+
+def _load_desc(path):
+    h = C.c_void_p()
+    dp = C.POINTER(A.fm_scene_desc)()
+    err = C.create_string_buffer(512)
+    rc = fumeng_amd.lib().fm_desc_load(path.encode(), C.byref(h), C.byref(dp), err, 512)
+    return rc, h, dp, err.value.decode()
+
+
+def test_json_round_trip_reproduces_descriptor(tmp_path):
+    s = mesh_scene(freq=5)
+    s.images.append(np.clip(np.random.default_rng(1).random((8, 16, 3)) * 3, 0, 3).astype(np.float32))
+    s.images[-1] = rgbe_decode(rgbe_encode(s.images[-1]))
+    s.add_env(len(s.images) - 1, rotation=(0, 0, 30), factor=1.5)
+    tex = np.random.default_rng(2).integers(0, 256, (4, 5, 3), dtype=np.uint8)
+    ti = s.add_image(tex)
+    s.add_lambert({"type": "image", "image": ti, "sampler": "nearest", "wrap_u": "clamp", "wrap_v": "repeat"})
+    path = s.write(str(tmp_path))
+    rc, h, dp, err = _load_desc(path)
+    assert rc == A.FM_OK, err
+    d = dp.contents
+    try:
+        assert d.num_objects == len(s.objects)
+        assert d.num_materials == len(s.materials)
+        assert d.num_lights == 1 and d.lights[0].type == A.FM_LIGHT_ENV
+        assert abs(d.lights[0].factor - 1.5) < 1e-7
+        # the mesh re-read from OBJ is bit-identical (9 significant digits round-trip float32)
+        m = d.meshes[0]
+        pos = np.ctypeslib.as_array(C.cast(m.positions, C.POINTER(C.c_float)), (m.num_positions * 3,))
+        assert np.array_equal(pos, s.meshes[0]["positions"].ravel())
+        # images are numbered in load order: material textures first, then the env
+        assert d.num_images == 2 and d.lights[0].image == 1
+        im = d.images[1]
+        assert im.format == A.FM_IMAGE_F32_LINEAR
+        px = np.ctypeslib.as_array(C.cast(im.data, C.POINTER(C.c_float)), (im.width * im.height * 3,))
+        assert np.array_equal(px, s.images[0].ravel())   # the .hdr decodes to the same texels
+        im = d.images[0]
+        assert im.format == A.FM_IMAGE_U8_SRGB
+        px = np.ctypeslib.as_array(C.cast(im.data, C.POINTER(C.c_uint8)), (im.width * im.height * 3,))
+        assert np.array_equal(px, tex.ravel())
+        last = d.materials[d.num_materials - 1]
+        assert last.tex[0].type == A.FM_TEX_IMAGE and last.tex[0].sampler == A.FM_SAMPLER_NEAREST
+        assert last.tex[0].wrap_u == A.FM_WRAP_CLAMP and last.tex[0].wrap_v == A.FM_WRAP_REPEAT
+        # same BVH topology from the parsed descriptor as from the in-memory one
+        nn, ng = C.c_int64(), C.c_int64()
+        assert fumeng_amd.lib().fm_host_build_bvh(dp, C.byref(nn), None, None, None, None, C.byref(ng), None) == 0
+        assert ng.value == s.num_geometries()
+    finally:
+        fumeng_amd.lib().fm_desc_free(h)
+
+
+def test_loader_defaults_and_error_semantics(tmp_path):
+    """config.h LoadValue: missing keys keep defaults, wrong types log and keep
+    the default, vec3 broadcasts a single number; missing camera / renderer /
+    scene blocks are errors (scene_loader.h:62,83,144)."""
+    doc = """{
+      "camera": {"type": "pin_hole", "pos": [0, -3, 0], "fov": [45], "up": [2]},
+      "renderer": {"width": 8, "spp": "many"},
+      "scene": {
+        "materials": [{"type": "disney", "name": "d", "basecolor": {"type": "constant", "value": [0.5]}},
+                      {"type": "mystery", "name": "m"}],
+        "render_objects": [{"shape": {"type": "sphere", "radius": 0.5}, "material": "d", "emittance": 2},
+                           {"shape": {"type": "rectangle", "pa": [0,0,0], "pb": [1,0,0], "pc": [1,1,0], "pd": [0,1,0]},
+                            "material": "nope"},
+                           {"shape": {"type": "cone"}, "material": "m"}],
+        "lights": [{"type": "directional", "radiance": [1]}]
+      }
+    }"""
+    p = tmp_path / "s.json"
+    p.write_text(doc)
+    rc, h, dp, err = _load_desc(str(p))
+    assert rc == A.FM_OK, err
+    d = dp.contents
+    try:
+        assert d.camera.fov == 60.0                 # array given for a real: default kept
+        assert list(d.camera.up) == [2.0, 2.0, 2.0]  # 1 number broadcasts
+        assert d.renderer.width == 8 and d.renderer.height == 1 and d.renderer.spp == 10
+        assert d.renderer.depth == 10 and d.renderer.rr_depth == 10 and abs(d.renderer.rr_coef - 0.85) < 1e-7
+        assert d.num_objects == 2                   # the unknown shape adds no geometry
+        assert list(d.objects[0].emittance) == [0.0, 0.0, 0.0]   # bare number: not an array, default kept
+        assert d.objects[1].material == -1          # unknown name -> default_material
+        m = d.materials[0]
+        assert m.type == A.FM_MAT_DISNEY and list(m.tex[0].value) == [0.5, 0.5, 0.5]
+        assert abs(m.tex[A.DISNEY_PARAMS.index("specular")].value[0] - 0.5) < 1e-7
+        assert abs(m.tex[A.DISNEY_PARAMS.index("ior")].value[0] - 1.5) < 1e-7
+        assert d.lights[0].type == A.FM_LIGHT_DIRECTIONAL and list(d.lights[0].direction) == [1.0, 0.0, 0.0]
+    finally:
+        fumeng_amd.lib().fm_desc_free(h)
+    for broken in ['{"renderer": {}, "scene": {}}', '{"camera": {}, "scene": {}}', '{"camera": {}, "renderer": {}}',
+                   '{"camera": {"type": "fisheye"}, "renderer": {}, "scene": {}}', '{"camera": ']:
+        p.write_text(broken)
+        rc, h, dp, err = _load_desc(str(p))
+        assert rc in (A.FM_ERR_PARSE,), (broken, rc, err)
+    rc, h, dp, err = _load_desc(str(tmp_path / "missing.json"))
+    assert rc == A.FM_ERR_IO
+
+
+def test_rgbe_roundtrip_matches_radiance_decoding():
+    img = np.array([[[1.0, 0.5, 0.25], [5e4, 3.0, 0.0]], [[0.0, 0.0, 0.0], [1e-3, 2e-3, 4e-3]]], np.float32)
+    dec = rgbe_decode(rgbe_encode(img))
+    assert np.all(dec[1, 0] == 0)
+    # one shared exponent: every channel is within 1/128 of the pixel's largest channel
+    err = np.abs(dec - img)
+    assert np.all(err <= img.max(axis=2, keepdims=True) / 128.0)
+    assert np.all(dec <= img)   # mantissas truncate
+
+
+def test_geodesic_sphere_counts_and_closure():
+    v, t = scenes.geodesic_sphere(7)
+    assert t.shape[0] == 20 * 49 and v.shape[0] == 10 * 49 + 2
+    # closed 2-manifold: every edge shared by exactly two triangles
+    e = np.sort(np.concatenate([t[:, [0, 1]], t[:, [1, 2]], t[:, [2, 0]]]), axis=1)
+    _, counts = np.unique(e, axis=0, return_counts=True)
+    assert np.all(counts == 2)
