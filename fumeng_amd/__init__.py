@@ -8,6 +8,7 @@ library or a GPU every call raises.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -57,9 +58,12 @@ class Context:
         h = C.c_void_p()
         _check(lib().fm_init(device, C.byref(h)))
         self.h = h
+        self._scenes = weakref.WeakSet()
 
     def close(self):
         if self.h:
+            for s in list(self._scenes):   # fm_shutdown releases them
+                s.h = None
             lib().fm_shutdown(self.h)
             self.h = None
 
@@ -74,13 +78,17 @@ class Context:
         d, keep = desc.to_c()
         h = C.c_void_p()
         _check(lib().fm_scene_create(self.h, C.byref(d), C.byref(h)), self.h)
-        return Scene(self, h)
+        return self._track(Scene(self, h))
+
+    def _track(self, scene):
+        self._scenes.add(scene)
+        return scene
 
     def load(self, json_path):
         """Parse a scene JSON with the reference loader semantics (fm_scene_load)."""
         h = C.c_void_p()
         _check(lib().fm_scene_load(self.h, json_path.encode(), C.byref(h)), self.h)
-        return Scene(self, h)
+        return self._track(Scene(self, h))
 
 
 class Scene:
@@ -88,9 +96,9 @@ class Scene:
         self.ctx, self.h = ctx, h
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             lib().fm_scene_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -65,6 +66,7 @@ struct fm_ctx {
     int film_w = 0, film_h = 0;
     int64_t film_spp = 0;     /* samples accumulated into the device film */
     int counting = 0;
+    std::set<fm_scene*> scenes;  /* live scenes, released by fm_shutdown */
 };
 
 struct fm_scene {
@@ -198,6 +200,8 @@ void fm_shutdown(fm_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (fm_scene* s : ctx->scenes) delete s;   /* shutdown releases the context's scenes */
+    ctx->scenes.clear();
     ctx->film.release();
     ctx->counters.release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -218,6 +222,7 @@ int fm_scene_create(fm_ctx* ctx, const fm_scene_desc* desc, fm_scene** out) {
         fmh::build_host_scene(*desc, s->hs);
         upload_scene(s.get());
         *out = s.release();
+        ctx->scenes.insert(*out);
         return FM_OK;
     });
 }
@@ -235,6 +240,7 @@ void fm_scene_destroy(fm_scene* s) {
     if (!s) return;
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
+    s->ctx->scenes.erase(s);
     delete s;
 }
 

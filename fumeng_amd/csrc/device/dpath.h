@@ -92,8 +92,8 @@ FMD void sphere_point(const fm_dsphere& s, v3 pl, v3& pos, v3& ng) {
 FMD LightSample light_sample(const fm_dscene& S, const fm_dlight& L, v3 pos, v3 s) {
     LightSample o;
     o.pdf = 0.0f; o.dist = 0.0f; o.radiance = splat(0.0f); o.wi = splat(0.0f); o.ref_pos = splat(0.0f); o.light_pos = splat(0.0f);
-    if (L.kind == FM_LIGHT_ENV) return env_sample(S, S.envs[L.aux], pos, s);
-    if (L.kind == FM_LIGHT_DIR) {
+    if (L.kind == FM_DLIGHT_ENV) return env_sample(S, S.envs[L.aux], pos, s);
+    if (L.kind == FM_DLIGHT_DIR) {
         const v3 d = ld3(L.dir);
         o.ref_pos = pos; o.light_pos = pos - d * 1.0e6f; o.wi = -d; o.radiance = ld3(L.radiance);
         o.pdf = 1.0f; o.dist = FM_REAL_MAX;
@@ -101,14 +101,14 @@ FMD LightSample light_sample(const fm_dscene& S, const fm_dlight& L, v3 pos, v3 
     }
     v3 lp, ln;
     float pa;
-    if (L.kind == FM_LIGHT_TRI) {
+    if (L.kind == FM_DLIGHT_TRI) {
         const float su = sqrtf(s.x);
         const float b0 = 1.0f - su, b1 = s.y * su, b2 = 1.0f - b0 - b1;
         const v3 v0 = ld3(L.p[0]), v1 = ld3(L.p[1]), v2 = ld3(L.p[2]);
         lp = b0 * v0 + b1 * v1 + b2 * v2;
         ln = L.has_normal ? b0 * ld3(L.n[0]) + b1 * ld3(L.n[1]) + b2 * ld3(L.n[2]) : normalize(cross(v0 - v2, v1 - v2));
         pa = L.p[0][3];
-    } else if (L.kind == FM_LIGHT_RECT) {
+    } else if (L.kind == FM_DLIGHT_RECT) {
         const fm_drect& R = S.rects[L.aux];
         const float su = sqrtf(s.x);
         const float b0 = 1.0f - su, b1 = s.y * su, b2 = 1.0f - b0 - b1;
@@ -159,8 +159,8 @@ FMD LightSample light_sample(const fm_dscene& S, const fm_dlight& L, v3 pos, v3 
 FMD float area_pdf(const fm_dscene& S, const fm_dlight& L, v3 shd, v3 lp, v3 ln, v3 l2s) {
     if (dot(ln, l2s) < 0) return 0.0f;
     float pa;
-    if (L.kind == FM_LIGHT_TRI) pa = L.p[0][3];
-    else if (L.kind == FM_LIGHT_RECT) pa = S.rects[L.aux].tangent[3];
+    if (L.kind == FM_DLIGHT_TRI) pa = L.p[0][3];
+    else if (L.kind == FM_DLIGHT_RECT) pa = S.rects[L.aux].tangent[3];
     else {
         const fm_dsphere& sp = S.spheres[L.aux];
         const float dis = length(xf_point(sp.w2o, shd));
@@ -201,7 +201,17 @@ struct PathOut { v3 color, albedo, normal; uint32_t rays; };
 
 struct PathCfg { int depth, direct, rr_depth; float rr_coef; };
 
-/* PathTracer::RenderPixel for one camera ray. */
+/* PathTracer::RenderPixel for one camera ray, written as a state machine
+ * with one closest-hit and one any-hit site (register pressure):
+ *   EXTEND  closest hit of the path ray: miss / emitter / new vertex
+ *   SHADOW  any hit of a light sample's shadow ray (MisLight)
+ *   MIS     closest hit of the BSDF-sampled direct-light ray (MisBSDF)
+ * The random numbers are drawn in the reference's order: per vertex
+ * 3 per light, 3 for MisBSDF (per direct_loop pass), 3 for the continuation,
+ * 1 for Russian roulette.  A light sample's BSDF value is evaluated before
+ * its shadow ray (it draws no random numbers, so the result is the same). */
+enum { PH_EXTEND = 0, PH_SHADOW = 1, PH_MIS = 2 };
+
 template <bool COUNT = false>
 FMD PathOut render_path(const fm_dscene& S, const PathCfg& cfg, Ray ray, Rng& rng, uint32_t* stk, int stride,
                         Cnt* cn = nullptr) {
@@ -209,90 +219,115 @@ FMD PathOut render_path(const fm_dscene& S, const PathCfg& cfg, Ray ray, Rng& rn
     out.color = out.albedo = out.normal = splat(0.0f);
     out.rays = 0;
     v3 beta = splat(1.0f);
-    Ray r = ray;
-    for (int depth = 1; depth <= cfg.depth; depth++) {
-        const Hit h = trace_closest<COUNT>(S, r, stk, stride, cn);
+    v3 direct = splat(0.0f);
+    v3 pend = splat(0.0f);           /* MisLight contribution waiting for its shadow ray */
+    Ray tr = ray;
+    int phase = PH_EXTEND;
+    int depth = 1, li = 0, pass = 0;
+    Surf sf;
+    BsdfSample ms;                   /* MisBSDF sample waiting for its ray */
+    for (;;) {
+        Hit h;
+        bool occluded = false;
+        if (phase == PH_SHADOW) occluded = trace_any<COUNT>(S, tr, stk, stride, cn);
+        else h = trace_closest<COUNT>(S, tr, stk, stride, cn);
         out.rays++;
-        if (h.slot < 0) {
-            if (depth == 1 && S.env >= 0) out.color = env_radiance(S, S.envs[S.env], -ray.d);
-            break;
-        }
-        const Surf sf = fill_hit(S, r, h);
-        if (depth == 1 && sf.light >= 0) {
-            const fm_dlight& L = S.lights[sf.light];
-            if (dot(sf.ng, sf.wo) > 0) out.color += beta * ld3(L.radiance);
-            break;
-        }
-        const Bsdf b = make_bsdf(S, sf.material, sf.ns, sf.ss, sf.wo, sf.u, sf.v);
-        if (depth == 1) { out.albedo = b.albedo(); out.normal = sf.ng; }
-        v3 direct = splat(0.0f);
-        for (int it = 0; it < cfg.direct; it++) {
-            /* MisLight for every light */
-            for (int li = 0; li < S.num_lights; li++) {
-                const fm_dlight& L = S.lights[li];
-                const LightSample ls = light_sample(S, L, sf.pos, rng.next3());
-                if (ls.pdf < FM_EPS_PDF || length(ls.radiance) < FM_EPS_PDF) continue;
-                Ray sh; sh.o = ls.ref_pos; sh.d = ls.wi; sh.tmin = FM_EPS; sh.tmax = ls.dist - FM_EPS;
-                out.rays++;
-                if (trace_any<COUNT>(S, sh, stk, stride, cn)) continue;
-                const v3 f = b.eval(sf.wo, ls.wi);
-                if (length(f) < FM_EPS_PDF) continue;
-                const float bp = b.pdf(sf.wo, ls.wi);
-                if (bp < FM_EPS_PDF) continue;
-                const v3 ff = f * ls.radiance * absdot(ls.wi, sf.ng);
-                if (L.kind == FM_LIGHT_DIR) { direct += beta * (ff / ls.pdf); continue; }
-                const float w = (ls.pdf * ls.pdf) / (ls.pdf * ls.pdf + bp * bp);
-                direct += beta * (ff / ls.pdf * w);
+        if (phase == PH_EXTEND) {
+            if (h.slot < 0) {
+                if (depth == 1 && S.env >= 0) out.color = env_radiance(S, S.envs[S.env], -ray.d);
+                break;
             }
-            /* MisBSDF */
-            BsdfSample bs = b.sample(sf.wo, rng.next3());
-            if (!(length(bs.f) < FM_EPS_PDF || bs.pdf < FM_EPS_PDF)) {
-                bs.wi = normalize(bs.wi);
-                const Ray mr = gen_ray(sf.pos, sf.ng, bs.wi);
-                const Hit lh = trace_closest<COUNT>(S, mr, stk, stride, cn);
-                out.rays++;
-                if (lh.slot < 0) {
-                    if (S.env >= 0) {
-                        const fm_denv& E = S.envs[S.env];
-                        const v3 rad = env_radiance(S, E, -mr.d);
-                        const v3 f = bs.f * rad * absdot(mr.d, sf.ng);
-                        if (bs.delta) direct += beta * (f / bs.pdf);
-                        else {
-                            const float lp = env_pdf(S, E, mr.d);
-                            const float w = (bs.pdf * bs.pdf) / (bs.pdf * bs.pdf + lp * lp);
-                            direct += beta * (f / bs.pdf * w);
-                        }
+            sf = fill_hit(S, tr, h);
+            if (depth == 1 && sf.light >= 0) {
+                if (dot(sf.ng, sf.wo) > 0) out.color += beta * ld3(S.lights[sf.light].radiance);
+                break;
+            }
+            direct = splat(0.0f);
+            pass = 0; li = 0;
+        } else if (phase == PH_SHADOW) {
+            if (!occluded) direct += pend;
+            li++;
+        } else {
+            if (h.slot < 0) {
+                if (S.env >= 0) {
+                    const fm_denv& E = S.envs[S.env];
+                    const v3 f = ms.f * env_radiance(S, E, -tr.d) * absdot(tr.d, sf.ng);
+                    if (ms.delta) direct += beta * (f / ms.pdf);
+                    else {
+                        const float lp = env_pdf(S, E, tr.d);
+                        direct += beta * (f / ms.pdf * ((ms.pdf * ms.pdf) / (ms.pdf * ms.pdf + lp * lp)));
                     }
-                } else {
-                    const Surf ls = fill_hit(S, mr, lh);
-                    if (ls.light >= 0) {
-                        const fm_dlight& L = S.lights[ls.light];
-                        const v3 l2s = normalize(mr.o - ls.pos);
-                        const v3 rad = dot(ls.ng, l2s) > 0 ? ld3(L.radiance) : splat(0.0f);
-                        if (!(length(rad) < FM_EPS_PDF)) {
-                            const v3 f = bs.f * rad * absdot(sf.ng, mr.d);
-                            if (bs.delta) direct += beta * (f / bs.pdf);
-                            else {
-                                const float lp = area_pdf(S, L, mr.o, ls.pos, ls.ng, l2s);
-                                const float w = (bs.pdf * bs.pdf) / (bs.pdf * bs.pdf + lp * lp);
-                                direct += beta * (f / bs.pdf * w);
-                            }
+                }
+            } else {
+                const Surf ls = fill_hit(S, tr, h);
+                if (ls.light >= 0) {
+                    const fm_dlight& L = S.lights[ls.light];
+                    const v3 l2s = normalize(tr.o - ls.pos);
+                    const v3 rad = dot(ls.ng, l2s) > 0 ? ld3(L.radiance) : splat(0.0f);
+                    if (!(length(rad) < FM_EPS_PDF)) {
+                        const v3 f = ms.f * rad * absdot(sf.ng, tr.d);
+                        if (ms.delta) direct += beta * (f / ms.pdf);
+                        else {
+                            const float lp = area_pdf(S, L, tr.o, ls.pos, ls.ng, l2s);
+                            direct += beta * (f / ms.pdf * ((ms.pdf * ms.pdf) / (ms.pdf * ms.pdf + lp * lp)));
                         }
                     }
                 }
             }
+            pass++; li = 0;
         }
+        /* the BSDF record is rebuilt after every trace instead of being kept
+         * live across the traversal (it is ~50 floats for Disney) */
+        const Bsdf b = make_bsdf(S, sf.material, sf.ns, sf.ss, sf.wo, sf.u, sf.v);
+        if (phase == PH_EXTEND && depth == 1) { out.albedo = b.albedo(); out.normal = sf.ng; }
+        /* next ray of this vertex: light samples, then the MisBSDF ray */
+        bool traced = false;
+        while (pass < cfg.direct && !traced) {
+            while (li < S.num_lights) {
+                const fm_dlight& L = S.lights[li];
+                const LightSample ls = light_sample(S, L, sf.pos, rng.next3());
+                if (!(ls.pdf < FM_EPS_PDF || length(ls.radiance) < FM_EPS_PDF)) {
+                    const v3 f = b.eval(sf.wo, ls.wi);
+                    if (!(length(f) < FM_EPS_PDF)) {
+                        const float bp = b.pdf(sf.wo, ls.wi);
+                        if (!(bp < FM_EPS_PDF)) {
+                            const v3 ff = f * ls.radiance * absdot(ls.wi, sf.ng);
+                            pend = L.kind == FM_DLIGHT_DIR ? beta * (ff / ls.pdf)
+                                                          : beta * (ff / ls.pdf * ((ls.pdf * ls.pdf) / (ls.pdf * ls.pdf + bp * bp)));
+                            tr.o = ls.ref_pos; tr.d = ls.wi; tr.tmin = FM_EPS; tr.tmax = ls.dist - FM_EPS;
+                            phase = PH_SHADOW;
+                            traced = true;
+                            break;
+                        }
+                    }
+                }
+                li++;
+            }
+            if (traced) break;
+            ms = b.sample(sf.wo, rng.next3());
+            if (!(length(ms.f) < FM_EPS_PDF || ms.pdf < FM_EPS_PDF)) {
+                ms.wi = normalize(ms.wi);
+                tr = gen_ray(sf.pos, sf.ng, ms.wi);
+                phase = PH_MIS;
+                traced = true;
+                break;
+            }
+            pass++; li = 0;
+        }
+        if (traced) continue;
+        /* vertex done: continuation (path_tracing.cpp:93-113) */
         out.color += direct / (float)cfg.direct;
-        /* continuation */
         const BsdfSample cs = b.sample(sf.wo, rng.next3());
         if (length(cs.f) < FM_EPS_PDF || cs.pdf < FM_EPS_PDF) break;
-        const float ac = absdot(sf.ng, cs.wi);
-        beta *= cs.f * ac / cs.pdf;
-        r = gen_ray(sf.pos, sf.ng, cs.wi);
+        beta *= cs.f * absdot(sf.ng, cs.wi) / cs.pdf;
+        tr = gen_ray(sf.pos, sf.ng, cs.wi);
         if (cfg.depth > cfg.rr_depth) {
             if (rng.next() > cfg.rr_coef) break;
             beta = beta / cfg.rr_coef;
         }
+        depth++;
+        if (depth > cfg.depth) break;
+        phase = PH_EXTEND;
     }
     return out;
 }

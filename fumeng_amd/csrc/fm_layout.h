@@ -62,14 +62,14 @@ typedef struct fm_dmaterial {
     float eta_i, eta_t, pad2[2];
 } fm_dmaterial;
 
-#define FM_LIGHT_TRI 0
-#define FM_LIGHT_RECT 1
-#define FM_LIGHT_SPHERE 2
-#define FM_LIGHT_ENV 3
-#define FM_LIGHT_DIR 4
+#define FM_DLIGHT_TRI 0
+#define FM_DLIGHT_RECT 1
+#define FM_DLIGHT_SPHERE 2
+#define FM_DLIGHT_ENV 3
+#define FM_DLIGHT_DIR 4
 
 typedef struct fm_dlight {
-    int32_t kind;       /* FM_LIGHT_* */
+    int32_t kind;       /* FM_DLIGHT_* */
     int32_t aux;        /* rect / sphere table index, env index */
     int32_t has_normal; /* triangle */
     int32_t geom;       /* reference geometry index */

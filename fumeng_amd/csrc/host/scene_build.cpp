@@ -73,8 +73,7 @@ M4 minv(const M4& A) {
             cof[r][c] = ((r + c) & 1) ? -mnr : mnr;
         }
     const float det = a[0][0] * cof[0][0] + a[0][1] * cof[0][1] + a[0][2] * cof[0][2] + a[0][3] * cof[0][3];
-    if (det == 0.0f || !std::isfinite(det)) throw BuildError(FM_ERR_INVALID, "singular transform");
-    const float id = 1.0f / det;
+    const float id = 1.0f / det;  /* singular: inf / NaN entries, as glm::inverse gives */
     for (int r = 0; r < 4; r++) for (int c = 0; c < 4; c++) inv[r][c] = cof[c][r] * id;
     M4 o;
     for (int r = 0; r < 4; r++) for (int c = 0; c < 4; c++) o.m[c][r] = inv[r][c];
@@ -157,6 +156,14 @@ struct Tri {
 
 struct BInfo { int32_t geom; Box box; };
 
+void push_tight(std::vector<float>& out, const F3* p, int n) {
+    float mn[3] = {p[0].x, p[0].y, p[0].z}, mx[3] = {p[0].x, p[0].y, p[0].z};
+    for (int i = 1; i < n; i++)
+        for (int k = 0; k < 3; k++) { mn[k] = std::min(mn[k], comp(p[i], k)); mx[k] = std::max(mx[k], comp(p[i], k)); }
+    for (int k = 0; k < 3; k++) out.push_back(mn[k]);
+    for (int k = 0; k < 3; k++) out.push_back(mx[k]);
+}
+
 struct Builder {
     std::vector<BInfo> infos;
     std::vector<RefNode> nodes;
@@ -229,7 +236,8 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
     /* ---- geometries in object order ---- */
     std::vector<Geo> geos;
     std::vector<Tri> tris;
-    std::vector<Box> bounds;
+    std::vector<Box> bounds;          /* reference WorldBound (REAL_MIN quirk): drives the topology */
+    std::vector<float> tight;         /* true bounds, 6 per geometry: stored in the device nodes */
     std::vector<fm_drect> rects;
     std::vector<fm_dsphere> spheres;
     std::vector<fm_dlight> area_lights;
@@ -269,10 +277,11 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
                     }
                 t.area = len(cross(sub(t.v[1], t.v[0]), sub(t.v[2], t.v[0]))) * 0.5f;
                 Box b; b.grow(t.v[0]); b.grow(t.v[1]); b.grow(t.v[2]);
+                push_tight(tight, t.v, 3);
                 int32_t light = -1;
                 if (emissive) {
                     fm_dlight L{};
-                    L.kind = FM_LIGHT_TRI; L.has_normal = t.has_normal;
+                    L.kind = FM_DLIGHT_TRI; L.has_normal = t.has_normal;
                     for (int c = 0; c < 3; c++) { put4(L.p[c], t.v[c], 0.0f); put4(L.n[c], t.n[c], 0.0f); }
                     L.p[0][3] = (float)(1.0 / (double)t.area);  /* triangle.cpp:293 */
                     light = add_light(L);
@@ -294,8 +303,9 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
             put4(r.normal, n, area);
             put4(r.tangent, cross(n, aux), (float)(1.0 / (double)area));  /* rectangle.cpp:134 */
             Box b; b.grow(pa); b.grow(pb); b.grow(pc); b.grow(pd);
+            { const F3 ps[4] = {pa, pb, pc, pd}; push_tight(tight, ps, 4); }
             int32_t light = -1;
-            if (emissive) { fm_dlight L{}; L.kind = FM_LIGHT_RECT; L.aux = (int32_t)rects.size(); light = add_light(L); }
+            if (emissive) { fm_dlight L{}; L.kind = FM_DLIGHT_RECT; L.aux = (int32_t)rects.size(); light = add_light(L); }
             geos.push_back(Geo{FM_PRIM_RECT, (int32_t)rects.size(), mat, light});
             rects.push_back(r);
             bounds.push_back(b);
@@ -314,8 +324,10 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
             Box b;
             b.mn[0] = c.x - s.radius_world; b.mn[1] = c.y - s.radius_world; b.mn[2] = c.z - s.radius_world;
             b.mx[0] = c.x + s.radius_world; b.mx[1] = c.y + s.radius_world; b.mx[2] = c.z + s.radius_world;
+            for (int k = 0; k < 3; k++) tight.push_back(b.mn[k]);
+            for (int k = 0; k < 3; k++) tight.push_back(b.mx[k]);
             int32_t light = -1;
-            if (emissive) { fm_dlight L{}; L.kind = FM_LIGHT_SPHERE; L.aux = (int32_t)spheres.size(); light = add_light(L); }
+            if (emissive) { fm_dlight L{}; L.kind = FM_DLIGHT_SPHERE; L.aux = (int32_t)spheres.size(); light = add_light(L); }
             geos.push_back(Geo{FM_PRIM_SPHERE, (int32_t)spheres.size(), mat, light});
             spheres.push_back(s);
             bounds.push_back(b);
@@ -338,6 +350,32 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
     if (only_bvh) return;
     if (hs.bvh_depth > FM_STACK_DEPTH) throw BuildError(FM_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
 
+    /* ---- tight node boxes ----
+     * The reference's boxes start from max = REAL_MIN (bbox.h:16-19), so every
+     * box on the negative side of an axis reaches the origin; they fix the
+     * split topology above but prune poorly.  The device nodes keep that
+     * topology with boxes recomputed from the primitives (padded by 2 ulps).
+     * The closest hit does not depend on which boxes prune (the tie rule is
+     * order-free), so the results are unchanged except for rays that graze a
+     * box boundary exactly. */
+    std::vector<float> nb(hs.ref_nodes.size() * 6);
+    for (size_t ii = hs.ref_nodes.size(); ii-- > 0;) {
+        const RefNode& n = hs.ref_nodes[ii];
+        float* o = &nb[ii * 6];
+        for (int k = 0; k < 3; k++) { o[k] = std::numeric_limits<float>::infinity(); o[3 + k] = -std::numeric_limits<float>::infinity(); }
+        auto merge = [&](const float* q) {
+            for (int k = 0; k < 3; k++) { o[k] = std::min(o[k], q[k]); o[3 + k] = std::max(o[3 + k], q[3 + k]); }
+        };
+        if (n.leaf) for (int64_t j = n.a; j < n.b; j++) merge(&tight[(size_t)hs.order[(size_t)j] * 6]);
+        else { merge(&nb[(size_t)n.a * 6]); merge(&nb[(size_t)n.b * 6]); }
+    }
+    auto pad = [](float v, bool up) {
+        const float inf = std::numeric_limits<float>::infinity();
+        return std::nextafter(std::nextafter(v, up ? inf : -inf), up ? inf : -inf);
+    };
+    for (size_t ii = 0; ii < hs.ref_nodes.size(); ii++)
+        for (int k = 0; k < 3; k++) { nb[ii * 6 + k] = pad(nb[ii * 6 + k], false); nb[ii * 6 + 3 + k] = pad(nb[ii * 6 + 3 + k], true); }
+
     /* ---- flatten: interior nodes in DFS pre-order become device nodes ---- */
     std::vector<int32_t> dev_id(hs.ref_nodes.size(), -1);
     int32_t nint = 0;
@@ -352,15 +390,15 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
         const RefNode& n = hs.ref_nodes[i];
         if (n.leaf) continue;
         fm_dnode& o = hs.nodes[(size_t)dev_id[i]];
-        const RefNode& a = hs.ref_nodes[(size_t)n.a];
-        const RefNode& b = hs.ref_nodes[(size_t)n.b];
-        for (int k = 0; k < 3; k++) { o.b[k] = a.mn[k]; o.b[3 + k] = a.mx[k]; o.b[6 + k] = b.mn[k]; o.b[9 + k] = b.mx[k]; }
+        const float* a = &nb[(size_t)n.a * 6];
+        const float* b = &nb[(size_t)n.b * 6];
+        for (int k = 0; k < 6; k++) { o.b[k] = a[k]; o.b[6 + k] = b[k]; }
         o.child[0] = ref_of(n.a);
         o.child[1] = ref_of(n.b);
     }
     if (G > 0) {
         hs.root_ref = ref_of(0);
-        for (int k = 0; k < 3; k++) { hs.root_box[k] = hs.ref_nodes[0].mn[k]; hs.root_box[3 + k] = hs.ref_nodes[0].mx[k]; }
+        for (int k = 0; k < 6; k++) hs.root_box[k] = nb[(size_t)k];
     }
 
     /* ---- slots ---- */
@@ -451,7 +489,7 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
         const fm_light_desc& l = d.lights[li];
         fm_dlight L{};
         if (l.type == FM_LIGHT_DIRECTIONAL) {
-            L.kind = FM_LIGHT_DIR;
+            L.kind = FM_DLIGHT_DIR;
             for (int c = 0; c < 3; c++) L.radiance[c] = l.radiance[c];
             const F3 dn = nrm(f3(l.direction[0], l.direction[1], l.direction[2]));
             put4(L.dir, dn, 0.0f);
@@ -528,7 +566,7 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
         for (int j = 0; j <= H; j++) T[(size_t)e.off_marg_cdf + j] = (float)(mc[(size_t)j] / ms);
         for (int j = 0; j < H; j++) T[(size_t)e.off_marg_val + j] = (float)rowsum[(size_t)j];
         e.marg_sum = (float)ms;
-        L.kind = FM_LIGHT_ENV;
+        L.kind = FM_DLIGHT_ENV;
         L.aux = (int32_t)hs.envs.size();
         if (hs.env_light < 0) { hs.env_light = (int32_t)hs.lights.size(); hs.env = L.aux; }
         hs.envs.push_back(e);

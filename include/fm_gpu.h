@@ -172,7 +172,9 @@ typedef struct fm_scene_desc {
 /* ---- context and scene -------------------------------------------------- */
 
 /* One context per device; it owns a HIP stream (default priority), the
- * device film and scratch buffers.  Not thread-safe: one host thread per ctx. */
+ * device film and scratch buffers.  Not thread-safe: one host thread per ctx.
+ * fm_shutdown also releases every scene still alive on the context; such
+ * scene handles must not be used (or destroyed) afterwards. */
 int fm_init(int device, fm_ctx** out);
 void fm_shutdown(fm_ctx* ctx);
 const char* fm_last_error(const fm_ctx* ctx);

@@ -44,7 +44,8 @@ def test_closest_hit_parity(mesh):
     diff = hit & (hg["geom"] != ho["geom"])
     assert diff.sum() <= max(2, hit.sum() // 2000)                # only exact-t ties may differ
     for k in ("pos", "ng", "ns", "ss", "wo"):
-        assert np.all(_rel(hg[k][hit & ~diff], ho[k][hit & ~diff], 1e-6) < 1e-5), k
+        a, b = hg[k][hit & ~diff], ho[k][hit & ~diff]
+        assert np.all(np.abs(a - b) <= 1e-5 * np.maximum(1.0, np.abs(b))), k
     assert np.all(np.abs(hg["uv"][hit & ~diff] - ho["uv"][hit & ~diff]) < 1e-5)
     assert np.array_equal(hg["material"][hit & ~diff], ho["material"][hit & ~diff])
     assert np.array_equal(hg["light"][hit & ~diff], ho["light"][hit & ~diff])
@@ -96,28 +97,40 @@ def test_camera_rays_parity(gpu_ctx):
         assert np.all(np.abs(rg["dir"] - ro["dir"]) < 1e-6)
 
 
-def _check_bsdf(rg, ro, tol=1e-4):
-    for k, floor in (("f", 1e-4), ("pdf", 1e-4)):
-        bad = _rel(rg[k], ro[k], floor) > tol
-        assert bad.mean() < 2e-3, (k, bad.mean())
-    same = ro["sdelta"] == rg["sdelta"]
-    assert same.all()
-    ok = _rel(rg["swi"], ro["swi"], 1e-3) < 1e-3
-    assert ok.all(axis=1).mean() > 0.995
-    close = ok.all(axis=1)
-    for k, floor in (("sf", 1e-4), ("spdf", 1e-4)):
-        bad = _rel(rg[k][close], ro[k][close], floor) > tol
-        assert bad.mean() < 2e-3, (k, bad.mean())
-    assert np.all(_rel(rg["albedo"], ro["albedo"], 1e-6) < 1e-6)
+def _check_bsdf(rg, ro32, ro64, oracle64, q, tol=1e-4):
+    """GPU vs FP64 oracle, allowing 4x the FP32 oracle's own distance to FP64
+    (the reference formulas lose precision near the refraction half-vector
+    and at grazing angles; the GPU must stay inside that FP32 envelope)."""
+    def bad(g, a32, a64, floor):
+        env = np.maximum(tol, 4 * _rel(a32, a64, floor))
+        return _rel(g, a64, floor) > env
+    assert not bad(rg["f"], ro32["f"], ro64["f"], 1e-4).any(), "f"
+    assert not bad(rg["pdf"], ro32["pdf"], ro64["pdf"], 1e-4).any(), "pdf"
+    assert np.array_equal(rg["sdelta"], ro32["sdelta"])
+    # sampled directions: same lobe and direction up to rounding
+    same = np.all(np.abs(rg["swi"] - ro32["swi"]) < 1e-4, axis=1)
+    assert same.mean() > 0.999, same.mean()
+    # sampled f / pdf against the FP64 oracle evaluated at the GPU's direction
+    q2 = q.copy()
+    q2["wi"] = rg["swi"]
+    e64, e32 = oracle64.bsdf(q2), None
+    ok = (rg["spdf"] > 0) & same & ~rg["sdelta"].astype(bool)
+    e32 = ro32
+    assert not bad(rg["sf"][ok], ro32["sf"][ok], e64["f"][ok], 1e-4).any(), "sf"
+    assert not bad(rg["spdf"][ok], ro32["spdf"][ok], e64["pdf"][ok], 1e-4).any(), "spdf"
+    assert np.all(_rel(rg["albedo"], ro32["albedo"], 1e-6) < 1e-6)
+
+
+def _bsdf_case(g, o32, o64, q):
+    _check_bsdf(g.bsdf(q), o32.bsdf(q), o64.bsdf(q), o64, q)
 
 
 def test_lambert_and_dielectric_bsdf_parity(gpu_ctx):
     s = scenes.box_scene()
-    g, o = gpu_ctx.scene(s), oracle.OracleScene(s)
+    g, o32, o64 = gpu_ctx.scene(s), oracle.OracleScene(s), oracle.OracleScene(s, fp64=True)
     rng = np.random.default_rng(26)
     for mat in (0, 1, 3, -1):
-        q = bsdf_queries(rng, 4096, mat)
-        _check_bsdf(g.bsdf(q), o.bsdf(q))
+        _bsdf_case(g, o32, o64, bsdf_queries(rng, 4096, mat))
 
 
 @pytest.mark.parametrize("seed", [0, 1, 2])
@@ -134,10 +147,9 @@ def test_disney_bsdf_parity(gpu_ctx, seed):
             p.update(diffTrans=float(rng.random()), flatness=float(rng.random()))
         p["roughness"] = max(p["roughness"], 0.05)
         mats.append(s.add_disney(thin=(k % 4 == 3), basecolor=rng.random(3).tolist(), **p))
-    g, o = gpu_ctx.scene(s), oracle.OracleScene(s)
+    g, o32, o64 = gpu_ctx.scene(s), oracle.OracleScene(s), oracle.OracleScene(s, fp64=True)
     for m in mats:
-        q = bsdf_queries(rng, 4096, m)
-        _check_bsdf(g.bsdf(q), o.bsdf(q))
+        _bsdf_case(g, o32, o64, bsdf_queries(rng, 4096, m))
 
 
 def test_light_sample_parity(gpu_ctx):
