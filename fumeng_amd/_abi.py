@@ -25,7 +25,7 @@ FM_D_NPARAM = 15
 FM_SHAPE_TRIANGLE_MESH, FM_SHAPE_SPHERE, FM_SHAPE_RECTANGLE = 0, 1, 2
 FM_LIGHT_ENV, FM_LIGHT_DIRECTIONAL = 0, 1
 FM_CAMERA_PIN_HOLE, FM_CAMERA_THIN_LENS = 0, 1
-FM_ACCEL_BVH, FM_ACCEL_NAIVE = 0, 1
+FM_ACCEL_BVH, FM_ACCEL_NAIVE, FM_ACCEL_BVH_REFERENCE_TREE = 0, 1, 2
 FM_MODE_MEGAKERNEL, FM_MODE_WAVEFRONT = 0, 1
 
 F3 = C.c_float * 3

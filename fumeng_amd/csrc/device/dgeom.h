@@ -11,9 +11,10 @@
  *   include/engine/core/bbox.h:59-69    slab test with std::min/max semantics
  *   src/accelerator/bvh_aggregate.cpp:81-123  closest / any hit
  *
- * Traversal order differs from the reference (near child first, explicit
- * stack in LDS) so ties are resolved by an order-free rule equal to the
- * reference's sequential one over leaf slots p (m_objList order):
+ * Traversal order and tree differ from the reference (near child first,
+ * explicit stack in LDS, binned-SAH tree) so ties are resolved by an
+ * order-free rule equal to the reference's sequential one over its leaf
+ * slots p (m_objList order), p being stored with each primitive:
  * triangles and rectangles need t < t_max, spheres accept t == t_max, so at
  * equal t a sphere beats a non-sphere, the sphere with the larger slot wins
  * among spheres and the smaller slot wins among non-spheres.
@@ -150,12 +151,13 @@ struct Hit {
 };
 
 /* Leaf slot test for the closest hit with the order-free tie rule. */
-FMD void test_slot_closest(const fm_dscene& S, const Ray& r, const RayPre& rp, int32_t p, Hit& h, int32_t key) {
+FMD void test_slot_closest(const fm_dscene& S, const Ray& r, const RayPre& rp, int32_t p, Hit& h) {
     const float4* pr = reinterpret_cast<const float4*>(S.prims) + 3 * (size_t)p;
     const float4 a = pr[0];
     const int kind = __float_as_int(a.w);
     const bool have = h.slot >= 0;
     if (kind == FM_PRIM_SPHERE) {
+        const int32_t key = __float_as_int(pr[2].w);
         const float4 b = pr[1];
         const fm_dsphere& s = S.spheres[__float_as_int(b.w)];
         float t;
@@ -166,6 +168,7 @@ FMD void test_slot_closest(const fm_dscene& S, const Ray& r, const RayPre& rp, i
         return;
     }
     const float4 b = pr[1], c = pr[2];
+    const int32_t key = __float_as_int(c.w);   /* reference slot (or object index for "naive") */
     float t, b0, b1, b2;
     /* IntersectTriangle keeps t_min < t < t_max; Triangle::GetIntersect also
      * rejects t <= 0 (triangle.cpp:221), Rectangle::GetIntersect does not */
@@ -245,11 +248,7 @@ FMD void closest_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& h
         } else {
             const int32_t start = FM_LEAF_START(ref), cnt = FM_LEAF_COUNT(ref);
             if (COUNT) cn.prims += (uint32_t)cnt;
-            for (int k = 0; k < cnt; k++) {
-                const int32_t p = start + k;
-                const int32_t key = S.naive ? S.meta[4 * (size_t)p] : p;
-                test_slot_closest(S, r, rp, p, h, key);
-            }
+            for (int k = 0; k < cnt; k++) test_slot_closest(S, r, rp, start + k, h);
         }
         if (sp == 0) break;
         sp--;

@@ -222,6 +222,117 @@ float srgb_to_linear(float c) {
     return std::pow((c + 0.055f) * 1.0f / 1.055f, 2.4f);
 }
 
+/* Binned SAH builder over true bounds (16 bins, three axes).  Leaves hold at
+ * most 7 primitives; when the remaining depth budget gets tight it switches
+ * to median splits so that the tree always fits FM_STACK_DEPTH levels. */
+struct SahBuilder {
+    struct N { bool leaf; int64_t a, b; };
+    const std::vector<float>& bb;
+    int64_t n;
+    std::vector<int32_t> order;
+    std::vector<float> cen;
+    std::vector<N> nodes;
+    SahBuilder(const std::vector<float>& b, int64_t count) : bb(b), n(count) {
+        order.resize((size_t)n);
+        cen.resize((size_t)n * 3);
+        for (int64_t i = 0; i < n; i++) {
+            order[(size_t)i] = (int32_t)i;
+            for (int k = 0; k < 3; k++) cen[(size_t)i * 3 + k] = 0.5f * (bb[(size_t)i * 6 + k] + bb[(size_t)i * 6 + 3 + k]);
+        }
+    }
+    static float area(const float* mn, const float* mx) {
+        const float dx = std::max(0.0f, mx[0] - mn[0]), dy = std::max(0.0f, mx[1] - mn[1]), dz = std::max(0.0f, mx[2] - mn[2]);
+        return 2.0f * (dx * dy + dy * dz + dz * dx);
+    }
+    void build() { nodes.reserve((size_t)(2 * n / 3 + 8)); rec(0, n, 1); }
+    int64_t rec(int64_t s0, int64_t s1, int depth) {
+        const int64_t id = (int64_t)nodes.size();
+        nodes.push_back(N{true, s0, s1});
+        const int64_t cnt = s1 - s0;
+        if (cnt <= 2) return id;
+        float cmn[3] = {INFINITY, INFINITY, INFINITY}, cmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        float pmn[3] = {INFINITY, INFINITY, INFINITY}, pmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int64_t i = s0; i < s1; i++) {
+            const int32_t g = order[(size_t)i];
+            for (int k = 0; k < 3; k++) {
+                const float c = cen[(size_t)g * 3 + k];
+                cmn[k] = std::min(cmn[k], c); cmx[k] = std::max(cmx[k], c);
+                pmn[k] = std::min(pmn[k], bb[(size_t)g * 6 + k]); pmx[k] = std::max(pmx[k], bb[(size_t)g * 6 + 3 + k]);
+            }
+        }
+        /* levels a median split would still need below this node */
+        int need = 0;
+        for (int64_t m = (cnt + 6) / 7; m > 1; m = (m + 1) / 2) need++;
+        const bool force_median = depth + need + 1 >= FM_STACK_DEPTH;
+        int best_axis = -1;
+        int best_bin = -1;
+        float best_cost = INFINITY;
+        constexpr int B = 16;
+        if (!force_median) {
+            for (int ax = 0; ax < 3; ax++) {
+                const float ext = cmx[ax] - cmn[ax];
+                if (!(ext > 0.0f)) continue;
+                int bc[B] = {0};
+                float bmn[B][3], bmx[B][3];
+                for (int b = 0; b < B; b++) for (int k = 0; k < 3; k++) { bmn[b][k] = INFINITY; bmx[b][k] = -INFINITY; }
+                const float sc = B / ext;
+                for (int64_t i = s0; i < s1; i++) {
+                    const int32_t g = order[(size_t)i];
+                    int b = (int)((cen[(size_t)g * 3 + ax] - cmn[ax]) * sc);
+                    b = std::min(std::max(b, 0), B - 1);
+                    bc[b]++;
+                    for (int k = 0; k < 3; k++) { bmn[b][k] = std::min(bmn[b][k], bb[(size_t)g * 6 + k]); bmx[b][k] = std::max(bmx[b][k], bb[(size_t)g * 6 + 3 + k]); }
+                }
+                float ra[B], rmn[3] = {INFINITY, INFINITY, INFINITY}, rmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+                int rc[B], racc = 0;
+                for (int b = B - 1; b > 0; b--) {
+                    racc += bc[b];
+                    for (int k = 0; k < 3; k++) { rmn[k] = std::min(rmn[k], bmn[b][k]); rmx[k] = std::max(rmx[k], bmx[b][k]); }
+                    rc[b] = racc; ra[b] = area(rmn, rmx);
+                }
+                float lmn[3] = {INFINITY, INFINITY, INFINITY}, lmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+                int lacc = 0;
+                for (int b = 0; b < B - 1; b++) {
+                    lacc += bc[b];
+                    for (int k = 0; k < 3; k++) { lmn[k] = std::min(lmn[k], bmn[b][k]); lmx[k] = std::max(lmx[k], bmx[b][k]); }
+                    if (lacc == 0 || rc[b + 1] == 0) continue;
+                    const float cost = area(lmn, lmx) * (float)lacc + ra[b + 1] * (float)rc[b + 1];
+                    if (cost < best_cost) { best_cost = cost; best_axis = ax; best_bin = b; }
+                }
+            }
+        }
+        const float parent_area = area(pmn, pmx);
+        /* SAH with traversal cost 1 and intersection cost 1 per primitive */
+        if (!force_median && cnt <= 7 && (best_axis < 0 || 1.0f + best_cost / std::max(parent_area, 1e-30f) >= (float)cnt)) return id;
+        int64_t mid;
+        if (force_median || best_axis < 0) {
+            int ax = 0;
+            for (int k = 1; k < 3; k++) if (cmx[k] - cmn[k] > cmx[ax] - cmn[ax]) ax = k;
+            mid = (s0 + s1) / 2;
+            std::nth_element(order.begin() + s0, order.begin() + mid, order.begin() + s1, [&](int32_t x, int32_t y) {
+                const float cx = cen[(size_t)x * 3 + ax], cy = cen[(size_t)y * 3 + ax];
+                return cx < cy || (cx == cy && x < y);
+            });
+        } else {
+            const float ext = cmx[best_axis] - cmn[best_axis];
+            const float sc = B / ext;
+            const int ax = best_axis;
+            auto it = std::partition(order.begin() + s0, order.begin() + s1, [&](int32_t g) {
+                int b = (int)((cen[(size_t)g * 3 + ax] - cmn[ax]) * sc);
+                b = std::min(std::max(b, 0), B - 1);
+                return b <= best_bin;
+            });
+            mid = it - order.begin();
+        }
+        const int64_t l = rec(s0, mid, depth + 1);
+        const int64_t r = rec(mid, s1, depth + 1);
+        nodes[(size_t)id].leaf = false;
+        nodes[(size_t)id].a = l;
+        nodes[(size_t)id].b = r;
+        return id;
+    }
+};
+
 }  // namespace
 
 void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
@@ -350,63 +461,96 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
     if (only_bvh) return;
     if (hs.bvh_depth > FM_STACK_DEPTH) throw BuildError(FM_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
 
-    /* ---- tight node boxes ----
-     * The reference's boxes start from max = REAL_MIN (bbox.h:16-19), so every
-     * box on the negative side of an axis reaches the origin; they fix the
-     * split topology above but prune poorly.  The device nodes keep that
-     * topology with boxes recomputed from the primitives (padded by 2 ulps).
-     * The closest hit does not depend on which boxes prune (the tie rule is
-     * order-free), so the results are unchanged except for rays that graze a
-     * box boundary exactly. */
-    std::vector<float> nb(hs.ref_nodes.size() * 6);
-    for (size_t ii = hs.ref_nodes.size(); ii-- > 0;) {
-        const RefNode& n = hs.ref_nodes[ii];
-        float* o = &nb[ii * 6];
+    /* ---- traversal tree ----
+     * The reference builder's topology (above) fixes the tie order: the slot
+     * of a geometry in its m_objList.  Its boxes start from max = REAL_MIN
+     * (bbox.h:16-19) and its median splits then pick poor axes on the
+     * negative side of the origin, so by default the device traverses a
+     * binned-SAH tree over the true bounds instead and resolves equal-t hits
+     * by each geometry's reference slot (stored with the primitive).  The
+     * closest hit is independent of the tree apart from rays that graze a
+     * box boundary exactly.  FM_ACCEL_BVH_REFERENCE_TREE traverses the
+     * reference topology itself (with true bounds). */
+    std::vector<int32_t> ref_pos(G);
+    for (size_t p = 0; p < G; p++) ref_pos[(size_t)hs.order[p]] = (int32_t)p;
+    std::vector<int32_t> slot_geom;   /* geometry per device slot */
+    std::vector<float> node_boxes;    /* 6 per tree node */
+    struct TNode { bool leaf; int64_t a, b; };
+    std::vector<TNode> tn;
+    if (d.accelerator == FM_ACCEL_BVH_REFERENCE_TREE) {
+        slot_geom = hs.order;
+        tn.resize(hs.ref_nodes.size());
+        for (size_t i = 0; i < hs.ref_nodes.size(); i++) tn[i] = TNode{hs.ref_nodes[i].leaf, hs.ref_nodes[i].a, hs.ref_nodes[i].b};
+    } else {
+        SahBuilder sb(tight, (int64_t)G);
+        if (G > 0) sb.build();
+        slot_geom.swap(sb.order);
+        tn.resize(sb.nodes.size());
+        for (size_t i = 0; i < sb.nodes.size(); i++) tn[i] = TNode{sb.nodes[i].leaf, sb.nodes[i].a, sb.nodes[i].b};
+    }
+    /* true bounds bottom-up (children have larger indices), padded by 2 ulps */
+    node_boxes.assign(tn.size() * 6, 0.0f);
+    for (size_t ii = tn.size(); ii-- > 0;) {
+        float* o = &node_boxes[ii * 6];
         for (int k = 0; k < 3; k++) { o[k] = std::numeric_limits<float>::infinity(); o[3 + k] = -std::numeric_limits<float>::infinity(); }
         auto merge = [&](const float* q) {
             for (int k = 0; k < 3; k++) { o[k] = std::min(o[k], q[k]); o[3 + k] = std::max(o[3 + k], q[3 + k]); }
         };
-        if (n.leaf) for (int64_t j = n.a; j < n.b; j++) merge(&tight[(size_t)hs.order[(size_t)j] * 6]);
-        else { merge(&nb[(size_t)n.a * 6]); merge(&nb[(size_t)n.b * 6]); }
+        if (tn[ii].leaf) for (int64_t j = tn[ii].a; j < tn[ii].b; j++) merge(&tight[(size_t)slot_geom[(size_t)j] * 6]);
+        else { merge(&node_boxes[(size_t)tn[ii].a * 6]); merge(&node_boxes[(size_t)tn[ii].b * 6]); }
     }
-    auto pad = [](float v, bool up) {
-        const float inf = std::numeric_limits<float>::infinity();
-        return std::nextafter(std::nextafter(v, up ? inf : -inf), up ? inf : -inf);
-    };
-    for (size_t ii = 0; ii < hs.ref_nodes.size(); ii++)
-        for (int k = 0; k < 3; k++) { nb[ii * 6 + k] = pad(nb[ii * 6 + k], false); nb[ii * 6 + 3 + k] = pad(nb[ii * 6 + 3 + k], true); }
+    for (size_t ii = 0; ii < tn.size(); ii++)
+        for (int k = 0; k < 3; k++) {
+            float& lo = node_boxes[ii * 6 + k];
+            float& hi = node_boxes[ii * 6 + 3 + k];
+            lo = std::nextafter(std::nextafter(lo, -INFINITY), -INFINITY);
+            hi = std::nextafter(std::nextafter(hi, INFINITY), INFINITY);
+        }
+    int tree_depth = 0;
+    {
+        std::vector<int> dep(tn.size(), 1);
+        for (size_t ii = 0; ii < tn.size(); ii++) {
+            tree_depth = std::max(tree_depth, dep[ii]);
+            if (!tn[ii].leaf) { dep[(size_t)tn[ii].a] = dep[ii] + 1; dep[(size_t)tn[ii].b] = dep[ii] + 1; }
+        }
+    }
+    if (tree_depth > FM_STACK_DEPTH) throw BuildError(FM_ERR_UNSUPPORTED, "traversal tree deeper than the LDS stack");
+    for (const TNode& t : tn)
+        if (t.leaf && t.b - t.a > 7) throw BuildError(FM_ERR_UNSUPPORTED, "leaf larger than 7 primitives");
+    hs.trav_depth = tree_depth;
 
-    /* ---- flatten: interior nodes in DFS pre-order become device nodes ---- */
-    std::vector<int32_t> dev_id(hs.ref_nodes.size(), -1);
+    /* ---- flatten: interior nodes in index order become device nodes ---- */
+    std::vector<int32_t> dev_id(tn.size(), -1);
     int32_t nint = 0;
-    for (size_t i = 0; i < hs.ref_nodes.size(); i++) if (!hs.ref_nodes[i].leaf) dev_id[i] = nint++;
+    for (size_t i = 0; i < tn.size(); i++) if (!tn[i].leaf) dev_id[i] = nint++;
     auto ref_of = [&](int64_t i) -> int32_t {
-        const RefNode& n = hs.ref_nodes[(size_t)i];
+        const TNode& n = tn[(size_t)i];
         if (n.leaf) return FM_LEAF_REF(n.a, n.b - n.a);
         return dev_id[(size_t)i];
     };
     hs.nodes.assign((size_t)nint, fm_dnode{});
-    for (size_t i = 0; i < hs.ref_nodes.size(); i++) {
-        const RefNode& n = hs.ref_nodes[i];
+    for (size_t i = 0; i < tn.size(); i++) {
+        const TNode& n = tn[i];
         if (n.leaf) continue;
         fm_dnode& o = hs.nodes[(size_t)dev_id[i]];
-        const float* a = &nb[(size_t)n.a * 6];
-        const float* b = &nb[(size_t)n.b * 6];
+        const float* a = &node_boxes[(size_t)n.a * 6];
+        const float* b = &node_boxes[(size_t)n.b * 6];
         for (int k = 0; k < 6; k++) { o.b[k] = a[k]; o.b[6 + k] = b[k]; }
         o.child[0] = ref_of(n.a);
         o.child[1] = ref_of(n.b);
     }
     if (G > 0) {
         hs.root_ref = ref_of(0);
-        for (int k = 0; k < 6; k++) hs.root_box[k] = nb[(size_t)k];
+        for (int k = 0; k < 6; k++) hs.root_box[k] = node_boxes[(size_t)k];
     }
 
-    /* ---- slots ---- */
+    /* ---- slots (w of the third float4 = tie key: reference slot, or the
+     * geometry index for the brute-force "naive" accelerator) ---- */
     hs.prims.assign(G * 12, 0.0f);
     hs.meta.assign(G * 4, 0);
     hs.tri_sh.assign(G * 16, 0.0f);
     for (size_t p = 0; p < G; p++) {
-        const int32_t g = hs.order[p];
+        const int32_t g = slot_geom[p];
         const Geo& ge = geos[(size_t)g];
         float* pr = &hs.prims[p * 12];
         int32_t flags = 0;
@@ -426,6 +570,7 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
             pr[0] = s.center[0]; pr[1] = s.center[1]; pr[2] = s.center[2]; pr[3] = ibits(FM_PRIM_SPHERE);
             pr[4] = s.radius_world; pr[7] = ibits(ge.aux);
         }
+        pr[11] = ibits(hs.naive ? g : ref_pos[(size_t)g]);
         int32_t* me = &hs.meta[p * 4];
         me[0] = g; me[1] = ge.material; me[2] = ge.light; me[3] = flags;
     }

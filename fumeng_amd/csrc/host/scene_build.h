@@ -27,7 +27,8 @@ struct RefNode {
 struct HostScene {
     std::vector<RefNode> ref_nodes;
     std::vector<int32_t> order;         /* geometry index per leaf slot */
-    int32_t bvh_depth = 0;
+    int32_t bvh_depth = 0;          /* reference tree */
+    int32_t trav_depth = 0;         /* traversal tree */
     int64_t num_geoms = 0;
 
     std::vector<fm_dnode> nodes;

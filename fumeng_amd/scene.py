@@ -133,7 +133,8 @@ class SceneDesc:
         keep = []
         d = A.fm_scene_desc()
         d.abi_version = A.FM_ABI_VERSION
-        d.accelerator = A.FM_ACCEL_NAIVE if self.accelerator == "naive" else A.FM_ACCEL_BVH
+        d.accelerator = {"naive": A.FM_ACCEL_NAIVE, "reference_tree": A.FM_ACCEL_BVH_REFERENCE_TREE}.get(
+            self.accelerator, A.FM_ACCEL_BVH)
         c = self.camera
         d.camera.type = A.FM_CAMERA_THIN_LENS if c["type"] == "thin_lens" else A.FM_CAMERA_PIN_HOLE
         for k in ("pos", "look_at", "up"):

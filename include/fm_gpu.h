@@ -156,10 +156,14 @@ typedef struct fm_renderer_desc {
     char output[256];      /* output prefix, already joined to the scene dir */
 } fm_renderer_desc;
 
-enum { FM_ACCEL_BVH = 0, FM_ACCEL_NAIVE = 1 };
+/* All accelerators return the reference's closest hit: equal-t hits are
+ * resolved by the reference BVH's leaf order (or object order for "naive").
+ * FM_ACCEL_BVH traverses a binned-SAH tree; FM_ACCEL_BVH_REFERENCE_TREE
+ * traverses the reference builder's own topology (bvh_aggregate.cpp:38-79). */
+enum { FM_ACCEL_BVH = 0, FM_ACCEL_NAIVE = 1, FM_ACCEL_BVH_REFERENCE_TREE = 2 };
 typedef struct fm_scene_desc {
     uint32_t abi_version;  /* FM_ABI_VERSION */
-    int32_t accelerator;   /* FM_ACCEL_*; both build the reference BVH topology */
+    int32_t accelerator;   /* FM_ACCEL_* */
     fm_camera_desc camera;
     fm_renderer_desc renderer;
     int32_t num_images;     const fm_image_desc* images;

@@ -75,6 +75,22 @@ def test_trace_axis_aligned_and_degenerate_rays(mesh):
     assert np.array_equal(hg["t"], ho["t"])
 
 
+def test_traversal_tree_does_not_change_the_closest_hit(gpu_ctx):
+    """The SAH traversal tree and the reference builder's own tree return the
+    same geometry and t: ties follow the reference leaf order in both."""
+    s = mesh_scene(freq=20)
+    g1 = gpu_ctx.scene(s)
+    s.accelerator = "reference_tree"
+    g2 = gpu_ctx.scene(s)
+    rng = np.random.default_rng(30)
+    r = scene_rays(rng, 20000)
+    # many exact ties: rays aimed at shared mesh vertices and edges
+    h1, h2 = g1.trace(r), g2.trace(r)
+    assert np.array_equal(h1["t"], h2["t"])
+    assert np.array_equal(h1["geom"], h2["geom"])
+    assert np.array_equal(g1.trace(r, True)["geom"], g2.trace(r, True)["geom"])
+
+
 def test_naive_accelerator_parity(gpu_ctx):
     s = mesh_scene(freq=10, accelerator="naive")
     g, o = gpu_ctx.scene(s), oracle.OracleScene(s)
