@@ -9,8 +9,9 @@
  *   src/material/disney.cpp:11-663           Disney lobes, weights, sampling
  *   src/material/material_common.cpp:7-125   GTR1/GTR2, Smith, refraction
  * The microfacet terms are rewritten without acos/atan2/tan: for a unit
- * vector w, cos(phi)=x/rho, sin(phi)=y/rho, sin^2(theta)=1-z^2,
- * tan^2(theta)=(1-z^2)/z^2, which is algebraically identical.
+ * vector w, cos(phi)=x/rho, sin(phi)=y/rho, sin^2(theta)=x^2+y^2=rho^2,
+ * tan^2(theta)=rho^2/z^2, which is algebraically identical (and, near the
+ * pole, more accurate than the FP32 reference's acos round trip).
  */
 #ifndef FM_DEVICE_DBSDF_H
 #define FM_DEVICE_DBSDF_H
@@ -89,7 +90,7 @@ FMD float gtr2(v3 wh, float ax, float ay) {
     float c2 = 1.0f, s2 = 0.0f;
     if (rho2 > 0.0f) { c2 = wh.x * wh.x / rho2; s2 = wh.y * wh.y / rho2; }
     const float z = clampf(wh.z, -1.0f, 1.0f);
-    const float st2 = fmaxf(0.0f, 1.0f - z * z);
+    const float st2 = rho2;   /* sin^2(theta) of a unit vector: no 1 - z^2 cancellation near the pole */
     const float den = st2 * (c2 / (ax * ax) + s2 / (ay * ay)) + z * z;
     return 1.0f / (FM_PI * ax * ay * den * den);
 }
@@ -108,7 +109,7 @@ FMD v3 sample_gtr2(float ax, float ay, float sx, float sy) {
 FMD float gtr1(v3 wh, float alpha) {
     const float z = clampf(wh.z, -1.0f, 1.0f);
     const float a2 = alpha * alpha;
-    const float den = 2.0f * FM_PI * logf(alpha) * (a2 * z * z + (1.0f - z * z));
+    const float den = 2.0f * FM_PI * logf(alpha) * (a2 * z * z + (wh.x * wh.x + wh.y * wh.y));
     return (a2 - 1.0f) / den;
 }
 FMD v3 sample_gtr1(float alpha, float sx, float sy) {
@@ -121,7 +122,7 @@ FMD v3 sample_gtr1(float alpha, float sx, float sy) {
 /* SmithGGXAnisotropic passes alpha_y for both axes: lambda = (-1 + sqrt(1 + ay^2 tan^2)) / 2 */
 FMD float smith_g(v3 wo, v3 wi, float ay) {
     const float zo = clampf(wo.z, -1.0f, 1.0f), zi = clampf(wi.z, -1.0f, 1.0f);
-    const float t2o = (1.0f - zo * zo) / (zo * zo), t2i = (1.0f - zi * zi) / (zi * zi);
+    const float t2o = (wo.x * wo.x + wo.y * wo.y) / (zo * zo), t2i = (wi.x * wi.x + wi.y * wi.y) / (zi * zi);
     const float a2 = ay * ay;
     const float li = -0.5f + 0.5f * sqrtf(1.0f + a2 * t2i) + 1.0f;
     const float lo = -0.5f + 0.5f * sqrtf(1.0f + a2 * t2o) + 1.0f;
@@ -129,7 +130,7 @@ FMD float smith_g(v3 wo, v3 wi, float ay) {
 }
 FMD float smith_sep(v3 w, float alpha) {
     const float z = clampf(w.z, -1.0f, 1.0f);
-    const float tt = sqrtf(fmaxf(0.0f, 1.0f - z * z)) / fabsf(z);
+    const float tt = sqrtf(w.x * w.x + w.y * w.y) / fabsf(z);
     if (tt < FM_EPS_PDF) return 1.0f;
     const float den = tt * alpha;
     return 2.0f / (1.0f + sqrtf(1.0f + den * den));
