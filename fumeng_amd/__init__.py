@@ -183,9 +183,9 @@ def host_bvh(desc):
 
 
 class SceneLoader:
-    """Mirror of the reference's SceneLoader: load(path) then draw_frame()
-    renders with the scene's renderer block and writes the output files
-    through the fumeng_gpu front end's conventions."""
+    """Mirror of the reference's SceneLoader (create/scene_loader.h:278-307):
+    load(path) then draw_frame() renders with the scene's renderer block and
+    returns the mean colour, albedo and normal films (row 0 at the top)."""
 
     def __init__(self, device=0):
         self.ctx = Context(device)

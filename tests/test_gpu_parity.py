@@ -4,7 +4,8 @@ same seeded inputs.
 Bars (stated per test):
   traversal   closest-hit geometry ids and hit t bit-identical, any-hit
               identical; rays with |d.z| < 1e-2 (where the reference's
-              truncated shear axis can give false FP32 hits, Q29) are logged
+              truncated shear axis, triangle.cpp:13-18, can give false FP32
+              hits) are logged
   hit fill    pos / normals / uv within 1e-5 x max(1, |value|)
   BSDF        f / pdf / sampled direction and values: error against the FP64
               oracle no worse than the FP32 oracle's (median, p99, max); the
@@ -39,7 +40,7 @@ def test_closest_hit_parity(mesh):
     rng = np.random.default_rng(21)
     r = scene_rays(rng, 20000)
     hg, ho = g.trace(r), o.trace(r)
-    # Q29: the reference shears along z for every |d| < 1 (truncating abs), so
+    # The reference shears along z for every |d| < 1 (truncating abs), so
     # rays with tiny |d.z| can get false hits from its FP32 test; which of
     # those it reports depends on the boxes it traverses.  Such rays are
     # logged and excluded; all others must match exactly.

@@ -220,7 +220,7 @@ def test_env_sampling_constant_map_and_pdf_quirk():
     """Constant env: Sample() returns the piecewise-constant uv pdf over the
     sin(theta) Jacobian (1/(4 pi) on average); the reference's
     Distribution2D::Pdf divides by the row integral twice, so Pdf(dir) is
-    1/(2 pi^2 sin theta) for a constant map (Q30 in the semantics register)."""
+    1/(2 pi^2 sin theta) for a constant map (distribution_2d.cpp:42-58)."""
     s = SceneDesc()
     img = s.add_image(np.full((32, 64, 3), 2.0, np.float32))
     s.add_env(img)
@@ -270,8 +270,8 @@ def test_env_sample_follows_luminance():
 def test_directional_light_closed_form_render():
     """A Lambert quad lit by a directional light with no env: every path is
     camera -> quad -> shadow ray, so each pixel equals a/pi * E * cos.  The
-    camera looks down z so that the truncated-axis triangle test (Q29) is
-    accurate for its rays."""
+    camera looks down z so that the truncated-axis triangle test
+    (triangle.cpp:13-18) is accurate for its rays."""
     s = SceneDesc()
     s.camera = {"type": "pin_hole", "pos": [0, 0, 3], "look_at": [0, 0, 0], "up": [0, 1, 0], "fov": 20.0,
                 "aspect": 1.0, "focal_distance": 1.0, "lens_radius": 0.0}
@@ -291,7 +291,7 @@ def test_truncated_max_axis_semantics():
     reference's includes this is ::abs(int) (checked by compiling the same
     call here), so |d| < 1 truncates to 0 and the shear axis is z.  A ray
     along +y onto the plane y = 0 is then solved in a steep shear and its t
-    is visibly less accurate than with the true max axis (Q29)."""
+    is visibly less accurate than with the true max axis."""
     s = SceneDesc()
     s.add_rectangle((-5, 0, -5), (-5, 0, 5), (5, 0, 5), (5, 0, -5))
     o = oracle.OracleScene(s)
