@@ -84,6 +84,7 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--mode", choices=["megakernel", "wavefront"], default="megakernel")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -109,7 +110,9 @@ def main():
     log("scene built and uploaded in %.1f s: %d BVH nodes, depth %d, %.1f MB in HBM"
         % (time.perf_counter() - t0, info.num_bvh_nodes, info.bvh_depth, info.device_bytes / 1e6))
 
-    kw = dict(seed=1, block_size=args.block, grid_blocks=args.grid, chunk=args.chunk, read=False)
+    from fumeng_amd import _abi
+    mode = _abi.FM_MODE_WAVEFRONT if args.mode == "wavefront" else _abi.FM_MODE_MEGAKERNEL
+    kw = dict(seed=1, block_size=args.block, grid_blocks=args.grid, chunk=args.chunk, read=False, mode=mode)
     for i in range(args.warmup):
         _, _, _, _, st = scene.render(**kw)
         log("warmup %d: %.1f ms" % (i, st.kernel_ms))
@@ -133,14 +136,14 @@ def main():
     roofline = None
     try:
         fumeng_amd.lib().fm_set_counting(ctx.h, 1)
-        _, _, _, _, cst = scene.render(seed=2, spp=1, read=False)
+        _, _, _, _, cst = scene.render(seed=2, spp=1, read=False, mode=mode)
         fumeng_amd.lib().fm_set_counting(ctx.h, 0)
         if cst.node_visits > 0 and cst.rays > 0:
             bytes_per_ray = (cst.node_visits * 64.0 + cst.prim_tests * 48.0) / cst.rays
             launch_bytes = bytes_per_ray * rays / args.steps
             achieved = launch_bytes / (kernel_ms / args.steps * 1e-3) / 1e9
             default_run = (args.freq, args.spp, args.block, args.grid, args.chunk) == (224, 256, 0, 0, 0)
-            traffic = profiled_traffic("k_render_mega<false>") if default_run else None
+            traffic = profiled_traffic("k_render_mega<false>") if default_run and args.mode == "megakernel" else None
             roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                         "bytes_per_ray": bytes_per_ray, "kernel_ms": kernel_ms / args.steps}
@@ -158,8 +161,8 @@ def main():
            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scene)",
            "config": {"workload": "displaced icosphere %d tris + ground, 64 Disney materials, 2048x1024 HDR env, "
-                                  "thin lens, %dx%d, %d spp, depth 12, megakernel" % (20 * args.freq ** 2, W, H, spp),
-                      "mode": "megakernel", "width": W, "height": H, "spp": spp, "depth": 12},
+                                  "thin lens, %dx%d, %d spp, depth 12, %s" % (20 * args.freq ** 2, W, H, spp, args.mode),
+                      "mode": args.mode, "width": W, "height": H, "spp": spp, "depth": 12},
            "mrays_per_s": mrays, "paths": paths, "rays": rays,
            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out))

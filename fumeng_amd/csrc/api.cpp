@@ -55,6 +55,55 @@ struct DevBuf {
 
 }  // namespace
 
+/* Wavefront path state and queues, sized for the largest pass seen so far. */
+struct WfBuffers {
+    DevBuf ray_o, ray_d, hit_a, hit_b, st_beta, st_color, st_flags, out_alb, out_nrm;
+    DevBuf mis_f, mis_beta, mis_ng, sh_o, sh_d, sh_c, sh_vis;
+    DevBuf q_ext[2], q_sh[2], q_cls[2], counts;
+    /* capacities in elements: paths, ray slots, MisBSDF records, shadow records, iterations */
+    int64_t cap_p = 0, cap_r = 0, cap_pd = 0, cap_pdl = 0, cap_it = 0;
+    void ensure(int64_t P, int D, int NL, int depth) {
+        const int64_t PD = P * D, R = P + PD, PDL = PD * NL, IT = depth + 2;
+        /* a group's capacity is 0 while it is being reallocated, so an
+         * allocation failure never leaves a smaller buffer behind a larger one */
+        if (P > cap_p) {
+            cap_p = 0;
+            for (DevBuf* b : {&st_beta, &st_color, &out_alb, &out_nrm}) b->alloc((size_t)P * 16);
+            st_flags.alloc((size_t)P * 4);
+            for (DevBuf& b : q_cls) b.alloc((size_t)P * fm::WQ_NCLS * 4);
+            cap_p = P;
+        }
+        if (R > cap_r) {
+            cap_r = 0;
+            for (DevBuf* b : {&ray_o, &ray_d, &hit_a}) b->alloc((size_t)R * 16);
+            hit_b.alloc((size_t)R * 8);
+            for (DevBuf& b : q_ext) b.alloc((size_t)R * 4);
+            cap_r = R;
+        }
+        if (PD > cap_pd) {
+            cap_pd = 0;
+            for (DevBuf* b : {&mis_f, &mis_beta, &mis_ng}) b->alloc((size_t)PD * 16);
+            cap_pd = PD;
+        }
+        if (PDL > cap_pdl) {
+            cap_pdl = 0;
+            for (DevBuf* b : {&sh_o, &sh_d, &sh_c}) b->alloc((size_t)PDL * 16);
+            sh_vis.alloc((size_t)PDL);
+            for (DevBuf& b : q_sh) b.alloc((size_t)PDL * 4);
+            cap_pdl = PDL;
+        }
+        if (IT > cap_it) { cap_it = 0; counts.alloc((size_t)IT * fm::WQ_N * 4); cap_it = IT; }
+    }
+    int64_t bytes() const {
+        int64_t n = 0;
+        for (const DevBuf* b : {&ray_o, &ray_d, &hit_a, &hit_b, &st_beta, &st_color, &st_flags, &out_alb, &out_nrm, &mis_f,
+                                &mis_beta, &mis_ng, &sh_o, &sh_d, &sh_c, &sh_vis, &q_ext[0], &q_ext[1], &q_sh[0], &q_sh[1],
+                                &q_cls[0], &q_cls[1], &counts})
+            n += (int64_t)b->n;
+        return n;
+    }
+};
+
 struct fm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -67,6 +116,7 @@ struct fm_ctx {
     int64_t film_spp = 0;     /* samples accumulated into the device film */
     int counting = 0;
     std::set<fm_scene*> scenes;  /* live scenes, released by fm_shutdown */
+    WfBuffers wf;
 };
 
 struct fm_scene {
@@ -274,7 +324,7 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
         const int spp = p->spp > 0 ? p->spp : rd.spp;
         if (W <= 0 || H <= 0 || spp <= 0) return fail(ctx, FM_ERR_INVALID, "width, height and spp must be positive");
         if ((int64_t)W * H > ((int64_t)1 << 31)) return fail(ctx, FM_ERR_INVALID, "image too large");
-        if (p->mode != FM_MODE_MEGAKERNEL) return fail(ctx, FM_ERR_UNSUPPORTED, "only FM_MODE_MEGAKERNEL is built");
+        if (p->mode != FM_MODE_MEGAKERNEL && p->mode != FM_MODE_WAVEFRONT) return fail(ctx, FM_ERR_INVALID, "unknown render mode");
         HIPCHECK(hipSetDevice(ctx->device));
         fm::RenderArgs A{};
         A.W = W; A.H = H; A.spp = spp;
@@ -285,8 +335,13 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
         if (A.cfg_raw.direct <= 0) return fail(ctx, FM_ERR_INVALID, "direct_loop must be positive");
         A.seed = p->seed;
         A.sample_offset = p->sample_offset;
-        A.chunk = p->chunk > 0 ? std::min(p->chunk, spp) : std::min(spp, 16);
         const int64_t npix = (int64_t)W * H;
+        const bool wave = p->mode == FM_MODE_WAVEFRONT;
+        /* megakernel: samples per work item; wavefront: samples per pass
+         * (about 8 M paths in flight by default) */
+        if (p->chunk > 0) A.chunk = std::min(p->chunk, spp);
+        else if (wave) A.chunk = (int)std::max<int64_t>(1, std::min<int64_t>(spp, ((int64_t)1 << 23) / npix));
+        else A.chunk = std::min(spp, 16);
         const int64_t nchunks = (spp + A.chunk - 1) / A.chunk;
         A.total_items = npix * nchunks;
         /* film */
@@ -307,17 +362,60 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
         HIPCHECK(fm::launch_zero_f64((double*)A.counters, 4, ctx->stream));  /* a fill kernel, not a memset node */
         const int block = p->block_size > 0 ? p->block_size : 256;
         if (block % 64 != 0 || block > 256) return fail(ctx, FM_ERR_INVALID, "block_size must be a multiple of 64, at most 256");
-        int grid = p->grid_blocks;
-        if (grid <= 0) {
-            int per_cu = fm::mega_blocks_per_cu(block);
-            if (per_cu <= 0) per_cu = 1;
-            grid = ctx->num_cus * per_cu * 4;
+        if (!wave) {
+            int grid = p->grid_blocks;
+            if (grid <= 0) {
+                int per_cu = fm::mega_blocks_per_cu(block);
+                if (per_cu <= 0) per_cu = 1;
+                grid = ctx->num_cus * per_cu * 4;
+            }
+            const int64_t need = (A.total_items + block - 1) / block;
+            if (grid > need) grid = (int)need;
+            HIPCHECK(hipEventRecord(ctx->ev0, ctx->stream));
+            HIPCHECK(fm::launch_render_mega(s->d, A, block, grid, ctx->counting != 0, ctx->stream));
+            HIPCHECK(hipEventRecord(ctx->ev1, ctx->stream));
+        } else {
+            const int D = A.cfg_raw.direct, NL = s->d.num_lights;
+            const int64_t Pmax = npix * A.chunk;
+            if (Pmax * (1 + D) >= ((int64_t)1 << 31) || Pmax * D * std::max(NL, 1) >= ((int64_t)1 << 31))
+                return fail(ctx, FM_ERR_INVALID, "wavefront pass too large: lower chunk (samples per pass)");
+            const int depth_it = std::max(A.cfg_raw.depth, 1);
+            ctx->wf.ensure(Pmax, D, NL, depth_it);
+            fm::WfArgs W_{};
+            W_.W = W; W_.H = H; W_.D = D; W_.NL = NL;
+            W_.depth = A.cfg_raw.depth; W_.rr_depth = A.cfg_raw.rr_depth; W_.rr_coef = A.cfg_raw.rr_coef;
+            W_.seed = A.seed;
+            W_.n_iter = depth_it + 1;
+            WfBuffers& B = ctx->wf;
+            W_.ray_o = B.ray_o.as<float>(); W_.ray_d = B.ray_d.as<float>();
+            W_.hit_a = B.hit_a.as<float>(); W_.hit_b = B.hit_b.as<float>();
+            W_.st_beta = B.st_beta.as<float>(); W_.st_color = B.st_color.as<float>(); W_.st_flags = B.st_flags.as<int>();
+            W_.out_alb = B.out_alb.as<float>(); W_.out_nrm = B.out_nrm.as<float>();
+            W_.mis_f = B.mis_f.as<float>(); W_.mis_beta = B.mis_beta.as<float>(); W_.mis_ng = B.mis_ng.as<float>();
+            W_.sh_o = B.sh_o.as<float>(); W_.sh_d = B.sh_d.as<float>(); W_.sh_c = B.sh_c.as<float>();
+            W_.sh_vis = B.sh_vis.as<unsigned char>();
+            for (int b = 0; b < 2; b++) {
+                W_.q_ext[b] = B.q_ext[b].as<int>(); W_.q_sh[b] = B.q_sh[b].as<int>(); W_.q_cls[b] = B.q_cls[b].as<int>();
+            }
+            W_.counts = B.counts.as<unsigned>();
+            W_.color_sum = A.color_sum; W_.albedo_sum = A.albedo_sum; W_.normal_sum = A.normal_sum; W_.sumsq = A.sumsq;
+            W_.counters = A.counters;
+            fm::WfLaunch L{};
+            L.block_trace = block; L.block_shade = block;
+            L.grid_trace = p->grid_blocks; L.grid_shade = p->grid_blocks;
+            for (const fm_dmaterial& m : s->hs.materials)
+                if (m.type >= 0 && m.type + 1 < fm::WQ_NCLS) L.has_class[m.type + 1] = true;
+            fm::wavefront_grids(ctx->num_cus, std::max(1, s->hs.trav_depth), L);
+            HIPCHECK(hipEventRecord(ctx->ev0, ctx->stream));
+            for (int64_t c = 0; c < nchunks; c++) {
+                const int kp = (int)std::min<int64_t>(A.chunk, spp - c * A.chunk);
+                W_.k = kp;
+                W_.P = npix * kp;
+                W_.sample_base = A.sample_offset + (int)(c * A.chunk);
+                HIPCHECK(fm::launch_wavefront_pass(s->d, W_, L, ctx->counting != 0, ctx->stream));
+            }
+            HIPCHECK(hipEventRecord(ctx->ev1, ctx->stream));
         }
-        const int64_t need = (A.total_items + block - 1) / block;
-        if (grid > need) grid = (int)need;
-        HIPCHECK(hipEventRecord(ctx->ev0, ctx->stream));
-        HIPCHECK(fm::launch_render_mega(s->d, A, block, grid, ctx->counting != 0, ctx->stream));
-        HIPCHECK(hipEventRecord(ctx->ev1, ctx->stream));
         HIPCHECK(hipStreamSynchronize(ctx->stream));
         ctx->film_spp += spp;
         if (stats) {

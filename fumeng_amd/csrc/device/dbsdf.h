@@ -362,16 +362,20 @@ struct Bsdf {
 };
 
 /* Material::CreateBSDF for a hit: frame + texture lookups + derived terms
- * (disney.cpp:375-461, 627-643). */
+ * (disney.cpp:375-461, 627-643).  TYPE >= 0 states the material type at
+ * compile time (a kernel that only shades one material class), so the other
+ * classes' code drops out of it. */
+template <int TYPE = -1>
 FMD Bsdf make_bsdf(const fm_dscene& S, int mat, v3 ns, v3 ss, v3 wo_w, float u, float v) {
     Bsdf b;
     const fm_dmaterial& M = S.materials[mat];
-    b.type = M.type;
+    const int type = TYPE >= 0 ? TYPE : M.type;
+    b.type = type;
     b.zs = ns;
     b.xs = normalize(ss);
     b.ys = cross(b.zs, b.xs);
-    if (M.type == FM_MAT_LAMBERT) { b.base = tex_sample(S, M.tex[0], u, v); return b; }
-    if (M.type == FM_MAT_SPECULAR_TRANSMISSION) {
+    if (type == FM_MAT_LAMBERT) { b.base = tex_sample(S, M.tex[0], u, v); return b; }
+    if (type == FM_MAT_SPECULAR_TRANSMISSION) {
         b.refl = ld3(M.refl); b.refr = ld3(M.refr); b.eta_i = M.eta_i; b.eta_t = M.eta_t; b.base = b.refl;
         return b;
     }

@@ -91,6 +91,11 @@ struct Rng {
         k0 = (uint32_t)seed; k1 = (uint32_t)(seed >> 32);
         pixel = pix; sample = smp; stream = strm; dim = 0;
     }
+    /* continue a sequence at dimension d (a path resumed by another kernel) */
+    FMD void seek(uint32_t d) {
+        dim = d;
+        if (d & 3u) philox(d >> 2, sample, pixel, stream, k0, k1, buf);
+    }
     FMD float next() {
         if ((dim & 3u) == 0u) philox(dim >> 2, sample, pixel, stream, k0, k1, buf);
         const uint32_t lane = dim & 3u;

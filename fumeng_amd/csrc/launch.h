@@ -23,6 +23,63 @@ struct RenderArgs {
     unsigned long long* counters;   /* [0] paths, [1] rays, [2] node visits, [3] primitive tests */
 };
 
+/* Wavefront form.  Per-iteration queue counters, [iteration][WQ_*]. */
+enum { WQ_EXT = 0, WQ_SH = 1, WQ_CLS = 2, WQ_NCLS = 4, WQ_N = WQ_CLS + WQ_NCLS };
+/* shading classes: WQ_CLS + c, c = 0 path end (miss, emitter seen by the
+ * camera, or only deferred direct light left), 1 + material type */
+enum { WC_END = 0, WC_LAMBERT = 1, WC_DISNEY = 2, WC_SPEC = 3 };
+
+/* Device buffers and parameters of one pass of the wavefront tracer: P paths
+ * (k samples of every pixel, path p = j * W * H + pixel), D direct-light
+ * passes per vertex, NL lights.  Ray slot p is path p's continuation ray,
+ * ray slot P + p * D + pass its MisBSDF ray; shadow record (p * D + pass) *
+ * NL + light. */
+struct WfArgs {
+    int W, H, k, D, NL, depth, rr_depth, iter;
+    int n_iter;        /* bounce iterations per pass: max(depth, 1) + 1 */
+    float rr_coef;
+    int sample_base;
+    int64_t P;
+    uint64_t seed;
+    float* ray_o;      /* float4 [P (1 + D)]: origin, t_min */
+    float* ray_d;      /* float4 [P (1 + D)]: direction, t_max */
+    float* hit_a;      /* float4 [P (1 + D)]: t, slot, kind | half << 8, b0 */
+    float* hit_b;      /* float2 [P (1 + D)]: b1, b2 */
+    float* st_beta;    /* float4 [P]: throughput, depth */
+    float* st_color;   /* float4 [P]: radiance, next RNG dimension */
+    int* st_flags;     /* [P] */
+    float* out_alb;    /* float4 [P] */
+    float* out_nrm;    /* float4 [P] */
+    float* mis_f;      /* float4 [P D]: BSDF value, pdf */
+    float* mis_beta;   /* float4 [P D]: throughput at the vertex, delta */
+    float* mis_ng;     /* float4 [P D]: geometric normal, valid */
+    float* sh_o;       /* float4 [P D NL]: origin, t_min */
+    float* sh_d;       /* float4 [P D NL]: direction, t_max */
+    float* sh_c;       /* float4 [P D NL]: contribution if unoccluded */
+    unsigned char* sh_vis;  /* [P D NL]: 0 none, 1 occluded / pending, 2 visible */
+    int* q_ext[2];     /* ray queues, ping-pong by iteration */
+    int* q_sh[2];      /* shadow queues */
+    int* q_cls[2];     /* WQ_NCLS segments of P path ids */
+    unsigned* counts;  /* [(n_iter + 1) * WQ_N] */
+    double* color_sum;
+    double* albedo_sum;
+    double* normal_sum;
+    double* sumsq;
+    unsigned long long* counters;
+};
+
+struct WfLaunch {
+    int block_trace, grid_trace;   /* extend / shadow */
+    int block_shade, grid_shade;
+    int stack_levels;
+    bool has_class[WQ_NCLS];
+};
+
+/* Kernel sequence of one pass (camera, n_iter bounce iterations, film). */
+hipError_t launch_wavefront_pass(const fm_dscene& S, WfArgs A, const WfLaunch& L, bool count, hipStream_t st);
+/* Persistent grid sizes for the wavefront kernels on this device. */
+void wavefront_grids(int num_cus, int stack_levels, WfLaunch& L);
+
 hipError_t launch_trace(const fm_dscene& S, const fm_ray* rays, int64_t n, int any, fm_hit* out, hipStream_t st);
 hipError_t launch_camera(const fm_dscene& S, const float* nl, int64_t n, fm_ray* out, hipStream_t st);
 hipError_t launch_bsdf(const fm_dscene& S, const fm_bsdf_query* q, int64_t n, fm_bsdf_result* out, hipStream_t st);

@@ -1,0 +1,420 @@
+/*
+ * Wavefront form of the path tracer for gfx950.
+ *
+ * The estimator is PathTracer::RenderPixel (path_tracing.cpp:34-229) exactly
+ * as the megakernel runs it, with the same random-number dimensions, split
+ * into one kernel per stage of a bounce so that each stage runs at the
+ * occupancy its own register count allows:
+ *
+ *   k_wf_camera    camera rays of P paths (k samples of every pixel)
+ *   k_wf_extend    closest hit of every queued ray (continuation and MisBSDF
+ *                  rays); sorts the paths whose continuation ray it traced
+ *                  into per-material-class shading queues
+ *   k_wf_shadow    any hit of every queued shadow ray (MisLight)
+ *   k_wf_shade<C>  per class: first adds the previous vertex's direct light
+ *                  (shadow results, MisBSDF hit) to the path radiance, then
+ *                  shades the new vertex: light samples -> shadow queue,
+ *                  MisBSDF sample -> ray queue, continuation + RR -> ray queue
+ *   k_wf_film      sums the k samples of each pixel in sample order (FP32) and
+ *                  adds them to the FP64 film
+ *
+ * Queues are compacted with one atomic per wave (ballot + mbcnt).  Queue
+ * lengths live on the device; every kernel is a persistent grid-stride loop
+ * that reads its length there, so the host issues a fixed sequence of
+ * max(depth, 1) + 1 iterations per pass without reading anything back (the
+ * last one only adds the direct light of the deepest vertices).
+ *
+ * Traversal stacks are in LDS with stack_levels entries per lane (the
+ * traversal tree's depth), level-major like the megakernel's.
+ */
+#include "launch.h"
+#include "device/dpath.h"
+
+using namespace fmd;
+using fm::WfArgs;
+
+namespace {
+
+enum { F_PENDING = 1, F_CONT = 2 };
+
+__device__ __forceinline__ uint32_t* wf_stack() {
+    extern __shared__ uint32_t fm_wf_lds[];
+    return fm_wf_lds + threadIdx.x;
+}
+
+__device__ __forceinline__ unsigned lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+/* Append `value` to queue q (length *count) for every active lane with pred
+ * set: one atomic per wave, slots in lane order. */
+__device__ __forceinline__ void wave_push(unsigned* count, int* q, bool pred, int value) {
+    const unsigned long long m = __ballot(pred);
+    if (m == 0ull) return;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned base = 0;
+    if ((int)lane_id() == leader) base = atomicAdd(count, (unsigned)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (pred) {
+        const unsigned below = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        q[base + below] = value;
+    }
+}
+
+__device__ __forceinline__ void wave_add_u64(unsigned long long* dst, unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(dst, v);
+}
+
+FMD float4 f4(v3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+FMD v3 xyz(float4 a) { return mk(a.x, a.y, a.z); }
+
+FMD Ray load_ray(const WfArgs& A, int64_t r) {
+    const float4 o = reinterpret_cast<const float4*>(A.ray_o)[r];
+    const float4 d = reinterpret_cast<const float4*>(A.ray_d)[r];
+    Ray ray;
+    ray.o = xyz(o); ray.tmin = o.w;
+    ray.d = xyz(d); ray.tmax = d.w;
+    return ray;
+}
+FMD void store_ray(const WfArgs& A, int64_t r, const Ray& ray) {
+    reinterpret_cast<float4*>(A.ray_o)[r] = f4(ray.o, ray.tmin);
+    reinterpret_cast<float4*>(A.ray_d)[r] = f4(ray.d, ray.tmax);
+}
+FMD Hit load_hit(const WfArgs& A, int64_t r) {
+    const float4 a = reinterpret_cast<const float4*>(A.hit_a)[r];
+    const float2 b = reinterpret_cast<const float2*>(A.hit_b)[r];
+    Hit h;
+    h.t = a.x; h.slot = __float_as_int(a.y);
+    const int kh = __float_as_int(a.z);
+    h.kind = (kh << 24) >> 24; h.half = kh >> 8; h.tie = 0;
+    h.b0 = a.w; h.b1 = b.x; h.b2 = b.y;
+    return h;
+}
+
+FMD int64_t grid_id() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+FMD int64_t grid_stride() { return (int64_t)gridDim.x * blockDim.x; }
+FMD unsigned* cnt(const WfArgs& A, int iter, int q) { return A.counts + (size_t)iter * fm::WQ_N + q; }
+
+/* ---- camera ------------------------------------------------------------- */
+
+__global__ void __launch_bounds__(256) k_wf_camera(fm_dscene S, WfArgs A) {
+    const int64_t npix = (int64_t)A.W * A.H;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        for (int k = 0; k < (A.n_iter + 1) * fm::WQ_N; k++) A.counts[k] = 0u;
+        A.counts[fm::WQ_EXT] = (unsigned)A.P;
+        atomicAdd(A.counters + 0, (unsigned long long)A.P);
+    }
+    for (int64_t p = grid_id(); p < A.P; p += grid_stride()) {
+        const int64_t j = p / npix;
+        const int64_t pix = p - j * npix;
+        const int row = (int)(pix / A.W), i = (int)(pix - (int64_t)row * A.W);
+        const int jj = A.H - 1 - row;
+        Rng rng;
+        rng.init(A.seed, (uint32_t)pix, (uint32_t)(A.sample_base + j), 0u);
+        const float fx = rng.next(), fy = rng.next();
+        const float lx = rng.next(), ly = rng.next();
+        const float px = ((float)i + fx) / (float)A.W, py = ((float)jj + fy) / (float)A.H;
+        const Ray cr = camera_ray(S.camera, px, py, lx, ly);
+        store_ray(A, p, cr);
+        reinterpret_cast<float4*>(A.st_beta)[p] = make_float4(1.0f, 1.0f, 1.0f, __int_as_float(1));
+        reinterpret_cast<float4*>(A.st_color)[p] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(rng.dim));
+        A.st_flags[p] = F_CONT;
+        reinterpret_cast<float4*>(A.out_alb)[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        reinterpret_cast<float4*>(A.out_nrm)[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        A.q_ext[0][p] = (int)p;
+    }
+}
+
+/* ---- extend: closest hit + classification -------------------------------- */
+
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_wf_extend(fm_dscene S, WfArgs A) {
+    uint32_t* stk = wf_stack();
+    const int stride = blockDim.x;
+    const int it = A.iter;
+    const unsigned n = *cnt(A, it, fm::WQ_EXT);
+    const int* q = A.q_ext[it & 1];
+    int* qc = A.q_cls[it & 1];
+    Cnt cn;
+    for (int64_t i = grid_id(); i < (int64_t)n; i += grid_stride()) {
+        const int r = q[i];
+        const Ray tr = load_ray(A, r);
+        const Hit h = trace_closest<COUNT>(S, tr, stk, stride, &cn);
+        reinterpret_cast<float4*>(A.hit_a)[r] =
+            make_float4(h.t, __int_as_float(h.slot), __int_as_float((h.kind & 0xff) | (h.half << 8)), h.b0);
+        reinterpret_cast<float2*>(A.hit_b)[r] = make_float2(h.b1, h.b2);
+        int cls = -1;
+        if (r < A.P) {
+            cls = fm::WC_END;
+            if (h.slot >= 0) {
+                const int4 me = reinterpret_cast<const int4*>(S.meta)[h.slot];
+                const int depth = __float_as_int(reinterpret_cast<const float4*>(A.st_beta)[r].w);
+                if (!(depth == 1 && me.z >= 0)) cls = 1 + S.materials[me.y].type;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < fm::WQ_NCLS; c++)
+            wave_push(cnt(A, it, fm::WQ_CLS + c), qc + (size_t)c * A.P, cls == c, r);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(A.counters + 1, (unsigned long long)n);
+    if (COUNT) {
+        wave_add_u64(A.counters + 2, cn.nodes);
+        wave_add_u64(A.counters + 3, cn.prims);
+    }
+}
+
+/* ---- shadow: any hit ------------------------------------------------------ */
+
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_wf_shadow(fm_dscene S, WfArgs A) {
+    uint32_t* stk = wf_stack();
+    const int stride = blockDim.x;
+    const int it = A.iter;
+    const unsigned n = *cnt(A, it, fm::WQ_SH);
+    const int* q = A.q_sh[it & 1];
+    Cnt cn;
+    for (int64_t i = grid_id(); i < (int64_t)n; i += grid_stride()) {
+        const int k = q[i];
+        const float4 o = reinterpret_cast<const float4*>(A.sh_o)[k];
+        const float4 d = reinterpret_cast<const float4*>(A.sh_d)[k];
+        Ray tr;
+        tr.o = xyz(o); tr.tmin = o.w; tr.d = xyz(d); tr.tmax = d.w;
+        const bool occluded = trace_any<COUNT>(S, tr, stk, stride, &cn);
+        A.sh_vis[k] = occluded ? 1 : 2;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(A.counters + 1, (unsigned long long)n);
+    if (COUNT) {
+        wave_add_u64(A.counters + 2, cn.nodes);
+        wave_add_u64(A.counters + 3, cn.prims);
+    }
+}
+
+/* ---- shade ---------------------------------------------------------------- */
+
+/* Direct light of the previous vertex, in the megakernel's order: per pass
+ * the light samples whose shadow rays were unoccluded, then the MisBSDF
+ * contribution (path_tracing.cpp:84-92, 122-229). */
+FMD v3 resolve_direct(const fm_dscene& S, const WfArgs& A, int64_t p) {
+    v3 direct = splat(0.0f);
+    for (int pass = 0; pass < A.D; pass++) {
+        for (int li = 0; li < A.NL; li++) {
+            const int64_t k = ((int64_t)p * A.D + pass) * A.NL + li;
+            if (A.sh_vis[k] == 2) direct += xyz(reinterpret_cast<const float4*>(A.sh_c)[k]);
+        }
+        const int64_t m = (int64_t)p * A.D + pass;
+        const float4 ng4 = reinterpret_cast<const float4*>(A.mis_ng)[m];
+        if (ng4.w == 0.0f) continue;
+        const float4 f4v = reinterpret_cast<const float4*>(A.mis_f)[m];
+        const float4 b4 = reinterpret_cast<const float4*>(A.mis_beta)[m];
+        const v3 msf = xyz(f4v), beta = xyz(b4), ng = xyz(ng4);
+        const float mpdf = f4v.w;
+        const bool delta = b4.w != 0.0f;
+        const Ray tr = load_ray(A, A.P + m);
+        const Hit h = load_hit(A, A.P + m);
+        if (h.slot < 0) {
+            if (S.env >= 0) {
+                const fm_denv& E = S.envs[S.env];
+                const v3 f = msf * env_radiance(S, E, -tr.d) * absdot(tr.d, ng);
+                if (delta) direct += beta * (f / mpdf);
+                else {
+                    const float lp = env_pdf(S, E, tr.d);
+                    direct += beta * (f / mpdf * ((mpdf * mpdf) / (mpdf * mpdf + lp * lp)));
+                }
+            }
+        } else {
+            const Surf ls = fill_hit(S, tr, h);
+            if (ls.light >= 0) {
+                const fm_dlight& L = S.lights[ls.light];
+                const v3 l2s = normalize(tr.o - ls.pos);
+                const v3 rad = dot(ls.ng, l2s) > 0 ? ld3(L.radiance) : splat(0.0f);
+                if (!(length(rad) < FM_EPS_PDF)) {
+                    const v3 f = msf * rad * absdot(ng, tr.d);
+                    if (delta) direct += beta * (f / mpdf);
+                    else {
+                        const float lp = area_pdf(S, L, tr.o, ls.pos, ls.ng, l2s);
+                        direct += beta * (f / mpdf * ((mpdf * mpdf) / (mpdf * mpdf + lp * lp)));
+                    }
+                }
+            }
+        }
+    }
+    return direct;
+}
+
+template <int C>
+__global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
+    const int it = A.iter, nx = it + 1;
+    const unsigned n = *cnt(A, it, fm::WQ_CLS + C);
+    const int* q = A.q_cls[it & 1] + (size_t)C * A.P;
+    const int64_t npix = (int64_t)A.W * A.H;
+    for (int64_t i = grid_id(); i < (int64_t)n; i += grid_stride()) {
+        const int p = q[i];
+        const float4 b4 = reinterpret_cast<const float4*>(A.st_beta)[p];
+        const float4 c4 = reinterpret_cast<const float4*>(A.st_color)[p];
+        const int fl = A.st_flags[p];
+        v3 beta = xyz(b4), color = xyz(c4);
+        int depth = __float_as_int(b4.w);
+        const uint32_t dim = __float_as_uint(c4.w);
+        if (fl & F_PENDING) color += resolve_direct(S, A, p) / (float)A.D;
+        bool any = false, term = true;
+        uint32_t dim_out = dim;
+        if constexpr (C == fm::WC_END) {
+            if (fl & F_CONT) {
+                const Ray tr = load_ray(A, p);
+                const Hit h = load_hit(A, p);
+                if (h.slot < 0) {
+                    if (depth == 1 && S.env >= 0) color = env_radiance(S, S.envs[S.env], -tr.d);
+                } else {
+                    const Surf sf = fill_hit(S, tr, h);
+                    if (dot(sf.ng, sf.wo) > 0) color += beta * ld3(S.lights[sf.light].radiance);
+                }
+            }
+        } else {
+            const Ray tr = load_ray(A, p);
+            const Hit h = load_hit(A, p);
+            const Surf sf = fill_hit(S, tr, h);
+            const Bsdf b = make_bsdf<C - 1>(S, sf.material, sf.ns, sf.ss, sf.wo, sf.u, sf.v);
+            if (depth == 1) {
+                reinterpret_cast<float4*>(A.out_alb)[p] = f4(b.albedo(), 0.0f);
+                reinterpret_cast<float4*>(A.out_nrm)[p] = f4(sf.ng, 0.0f);
+            }
+            const int64_t j = p / npix;
+            Rng rng;
+            rng.init(A.seed, (uint32_t)(p - j * npix), (uint32_t)(A.sample_base + j), 0u);
+            rng.seek(dim);
+            for (int pass = 0; pass < A.D; pass++) {
+                for (int li = 0; li < A.NL; li++) {
+                    const int64_t k = ((int64_t)p * A.D + pass) * A.NL + li;
+                    const fm_dlight& L = S.lights[li];
+                    const LightSample ls = light_sample(S, L, sf.pos, rng.next3());
+                    bool ok = false;
+                    if (!(ls.pdf < FM_EPS_PDF || length(ls.radiance) < FM_EPS_PDF)) {
+                        const v3 f = b.eval(sf.wo, ls.wi);
+                        if (!(length(f) < FM_EPS_PDF)) {
+                            const float bp = b.pdf(sf.wo, ls.wi);
+                            if (!(bp < FM_EPS_PDF)) {
+                                const v3 ff = f * ls.radiance * absdot(ls.wi, sf.ng);
+                                const v3 pend = L.kind == FM_DLIGHT_DIR ? beta * (ff / ls.pdf)
+                                                                       : beta * (ff / ls.pdf * ((ls.pdf * ls.pdf) / (ls.pdf * ls.pdf + bp * bp)));
+                                reinterpret_cast<float4*>(A.sh_o)[k] = f4(ls.ref_pos, FM_EPS);
+                                reinterpret_cast<float4*>(A.sh_d)[k] = f4(ls.wi, ls.dist - FM_EPS);
+                                reinterpret_cast<float4*>(A.sh_c)[k] = f4(pend, 0.0f);
+                                ok = true;
+                            }
+                        }
+                    }
+                    A.sh_vis[k] = ok ? 1 : 0;
+                    wave_push(cnt(A, nx, fm::WQ_SH), A.q_sh[nx & 1], ok, (int)k);
+                    any = any || ok;
+                }
+                const int64_t m = (int64_t)p * A.D + pass;
+                BsdfSample ms = b.sample(sf.wo, rng.next3());
+                const bool ok = !(length(ms.f) < FM_EPS_PDF || ms.pdf < FM_EPS_PDF);
+                if (ok) {
+                    ms.wi = normalize(ms.wi);
+                    store_ray(A, A.P + m, gen_ray(sf.pos, sf.ng, ms.wi));
+                    reinterpret_cast<float4*>(A.mis_f)[m] = f4(ms.f, ms.pdf);
+                    reinterpret_cast<float4*>(A.mis_beta)[m] = f4(beta, ms.delta ? 1.0f : 0.0f);
+                    reinterpret_cast<float4*>(A.mis_ng)[m] = f4(sf.ng, 1.0f);
+                } else {
+                    reinterpret_cast<float4*>(A.mis_ng)[m] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                }
+                wave_push(cnt(A, nx, fm::WQ_EXT), A.q_ext[nx & 1], ok, (int)(A.P + m));
+                any = any || ok;
+            }
+            /* continuation (path_tracing.cpp:93-113) */
+            const BsdfSample cs = b.sample(sf.wo, rng.next3());
+            if (!(length(cs.f) < FM_EPS_PDF || cs.pdf < FM_EPS_PDF)) {
+                beta *= cs.f * absdot(sf.ng, cs.wi) / cs.pdf;
+                const Ray nr = gen_ray(sf.pos, sf.ng, cs.wi);
+                term = false;
+                if (A.depth > A.rr_depth) {
+                    if (rng.next() > A.rr_coef) term = true;
+                    else beta = beta / A.rr_coef;
+                }
+                if (!term) {
+                    depth++;
+                    if (depth > A.depth) term = true;
+                }
+                if (!term) store_ray(A, p, nr);
+            }
+            dim_out = rng.dim;
+        }
+        wave_push(cnt(A, nx, fm::WQ_EXT), A.q_ext[nx & 1], !term, p);
+        wave_push(cnt(A, nx, fm::WQ_CLS + fm::WC_END), A.q_cls[nx & 1] + (size_t)fm::WC_END * A.P, term && any, p);
+        reinterpret_cast<float4*>(A.st_beta)[p] = f4(beta, __int_as_float(depth));
+        reinterpret_cast<float4*>(A.st_color)[p] = f4(color, __uint_as_float(dim_out));
+        A.st_flags[p] = (any ? F_PENDING : 0) | (term ? 0 : F_CONT);
+    }
+}
+
+/* ---- film ----------------------------------------------------------------- */
+
+__global__ void __launch_bounds__(256) k_wf_film(WfArgs A) {
+    const int64_t npix = (int64_t)A.W * A.H;
+    for (int64_t pix = grid_id(); pix < npix; pix += grid_stride()) {
+        v3 cs = splat(0.0f), as = splat(0.0f), ns = splat(0.0f);
+        float q0 = 0.0f, q1 = 0.0f, q2 = 0.0f;
+        for (int j = 0; j < A.k; j++) {
+            const int64_t p = (int64_t)j * npix + pix;
+            const v3 c = xyz(reinterpret_cast<const float4*>(A.st_color)[p]);
+            cs += c;
+            as += xyz(reinterpret_cast<const float4*>(A.out_alb)[p]);
+            ns += xyz(reinterpret_cast<const float4*>(A.out_nrm)[p]);
+            q0 += c.x * c.x; q1 += c.y * c.y; q2 += c.z * c.z;
+        }
+        double* c = A.color_sum + 3 * pix;
+        double* a = A.albedo_sum + 3 * pix;
+        double* nn = A.normal_sum + 3 * pix;
+        double* q = A.sumsq + 3 * pix;
+        c[0] += cs.x; c[1] += cs.y; c[2] += cs.z;
+        a[0] += as.x; a[1] += as.y; a[2] += as.z;
+        nn[0] += ns.x; nn[1] += ns.y; nn[2] += ns.z;
+        q[0] += q0; q[1] += q1; q[2] += q2;
+    }
+}
+
+template <typename K>
+int occupancy(K kern, int block, size_t lds) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, block, lds) != hipSuccess || n <= 0) return 1;
+    return n;
+}
+
+}  // namespace
+
+namespace fm {
+
+void wavefront_grids(int num_cus, int stack_levels, WfLaunch& L) {
+    L.stack_levels = stack_levels;
+    if (L.block_trace <= 0) L.block_trace = 256;
+    if (L.block_shade <= 0) L.block_shade = 256;
+    const size_t lds = (size_t)stack_levels * L.block_trace * sizeof(uint32_t);
+    if (L.grid_trace <= 0) L.grid_trace = num_cus * occupancy(k_wf_extend<false>, L.block_trace, lds);
+    if (L.grid_shade <= 0) L.grid_shade = num_cus * occupancy(k_wf_shade<WC_DISNEY>, L.block_shade, 0);
+}
+
+hipError_t launch_wavefront_pass(const fm_dscene& S, WfArgs A, const WfLaunch& L, bool count, hipStream_t st) {
+    const size_t lds = (size_t)L.stack_levels * L.block_trace * sizeof(uint32_t);
+    const int64_t npix = (int64_t)A.W * A.H;
+    const int cam_grid = (int)std::min<int64_t>((A.P + 255) / 256, (int64_t)L.grid_shade * 4);
+    hipLaunchKernelGGL(k_wf_camera, dim3(std::max(cam_grid, 1)), dim3(256), 0, st, S, A);
+    for (int it = 0; it < A.n_iter; it++) {
+        A.iter = it;
+        if (count) {
+            hipLaunchKernelGGL(k_wf_extend<true>, dim3(L.grid_trace), dim3(L.block_trace), lds, st, S, A);
+            hipLaunchKernelGGL(k_wf_shadow<true>, dim3(L.grid_trace), dim3(L.block_trace), lds, st, S, A);
+        } else {
+            hipLaunchKernelGGL(k_wf_extend<false>, dim3(L.grid_trace), dim3(L.block_trace), lds, st, S, A);
+            hipLaunchKernelGGL(k_wf_shadow<false>, dim3(L.grid_trace), dim3(L.block_trace), lds, st, S, A);
+        }
+        hipLaunchKernelGGL(k_wf_shade<WC_END>, dim3(L.grid_shade), dim3(L.block_shade), 0, st, S, A);
+        if (L.has_class[WC_LAMBERT]) hipLaunchKernelGGL(k_wf_shade<WC_LAMBERT>, dim3(L.grid_shade), dim3(L.block_shade), 0, st, S, A);
+        if (L.has_class[WC_DISNEY]) hipLaunchKernelGGL(k_wf_shade<WC_DISNEY>, dim3(L.grid_shade), dim3(L.block_shade), 0, st, S, A);
+        if (L.has_class[WC_SPEC]) hipLaunchKernelGGL(k_wf_shade<WC_SPEC>, dim3(L.grid_shade), dim3(L.block_shade), 0, st, S, A);
+    }
+    const int film_grid = (int)std::min<int64_t>((npix + 255) / 256, (int64_t)L.grid_shade * 4);
+    hipLaunchKernelGGL(k_wf_film, dim3(std::max(film_grid, 1)), dim3(256), 0, st, A);
+    return hipGetLastError();
+}
+
+}  // namespace fm

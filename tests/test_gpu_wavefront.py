@@ -1,0 +1,111 @@
+"""GPU: the wavefront form of the path tracer.
+
+It runs the megakernel's estimator with the same random-number dimensions,
+so, given the same samples per pass (chunk), a pixel gets the same paths in
+both forms.  Bars:
+  film        per pixel |wavefront - megakernel| <= 1e-5 x max(1, |value|)
+              on all but a few pixels (a path may take another branch if the
+              two kernels round one operation differently), and the image
+              means agree to 1e-4
+  counters    same number of paths; ray counts within 0.5 %
+  oracle      common random numbers against the FP32 oracle, as for the
+              megakernel
+"""
+import numpy as np
+import pytest
+
+import oracle
+from fumeng_amd import _abi as A
+from fumeng_amd import scenes
+from helpers import mesh_scene, unit
+
+pytestmark = pytest.mark.gpu
+
+WF = A.FM_MODE_WAVEFRONT
+MK = A.FM_MODE_MEGAKERNEL
+
+
+def _same_film(a, b, name, frac=0.995):
+    for x, y, what in zip(a[:3], b[:3], ("color", "albedo", "normal")):
+        close = np.abs(x - y) <= 1e-5 * np.maximum(1.0, np.abs(x))
+        print("%s %s: pixels equal to 1e-5: %.5f, means %.6g %.6g" % (name, what, close.mean(), x.mean(), y.mean()))
+        assert close.mean() >= frac, (name, what)
+        assert abs(x.mean() - y.mean()) <= 1e-4 * max(1.0, abs(x.mean())), (name, what)
+
+
+def _both(g, **kw):
+    mk = g.render(mode=MK, **kw)
+    wf = g.render(mode=WF, **kw)
+    return mk, wf
+
+
+def test_wavefront_matches_megakernel_box(gpu_ctx):
+    """Lambert walls, dielectric and Lambert spheres, one area light, RR at
+    every vertex (depth 8 > rr_depth 3)."""
+    s = scenes.box_scene(width=32, height=32, spp=8)
+    g = gpu_ctx.scene(s)
+    mk, wf = _both(g, seed=5, chunk=4)
+    _same_film(mk, wf, "box")
+    assert wf[4].paths == mk[4].paths == 32 * 32 * 8
+    assert abs(wf[4].rays - mk[4].rays) <= 5e-3 * mk[4].rays
+
+
+def test_wavefront_matches_megakernel_disney_env_two_lights(gpu_ctx):
+    """Disney mesh + box + env map + directional light, two direct-light
+    passes per vertex: exercises every shading class, several shadow records
+    and MisBSDF rays per vertex, env MIS and delta lights."""
+    s = mesh_scene(freq=10)
+    env = s.add_image(scenes.sky_env(64, 32, 3))
+    s.add_env(env, rotation=(0, 0, 30), factor=0.5)
+    s.add_directional(2.0, unit([0.2, 0.3, -1.0]))
+    s.renderer.update(width=40, height=32, spp=6, depth=6, rr_depth=2, direct_loop=2)
+    g = gpu_ctx.scene(s)
+    assert g.info().num_lights == 3
+    mk, wf = _both(g, seed=11, chunk=3)
+    _same_film(mk, wf, "disney")
+    assert abs(wf[4].rays - mk[4].rays) <= 5e-3 * mk[4].rays
+
+
+def test_wavefront_uneven_passes_and_shallow_depths(gpu_ctx):
+    """spp not a multiple of the pass size (passes of 2, 2, 1) and depths 1
+    and 2 (the deferred direct light of the last vertex still lands)."""
+    s = mesh_scene(freq=8)
+    g = gpu_ctx.scene(s)
+    for depth in (1, 2):
+        mk, wf = _both(g, seed=13, spp=5, chunk=2, depth=depth, width=24, height=20)
+        _same_film(mk, wf, "depth%d" % depth)
+        assert wf[4].paths == 24 * 20 * 5
+
+
+def test_wavefront_matches_oracle_with_common_random_numbers(gpu_ctx):
+    s = scenes.box_scene(width=32, height=32, spp=16)
+    g = gpu_ctx.scene(s)
+    cg, ag, ng, qg, st = g.render(seed=7, mode=WF)
+    co, ao, no, qo = oracle.OracleScene(s).render(0, 0, 32, 32, spp=16, seed=7, threads=8)
+    close = np.abs(cg - co) <= 1e-3 * np.maximum(1.0, np.abs(co))
+    print("pixels equal to 1e-3:", close.mean(), "mean", cg.mean(), co.mean())
+    assert close.mean() > 0.9
+    assert abs(cg.mean() - co.mean()) < 0.02 * co.mean()
+    assert np.mean(np.abs(ag - ao) < 1e-5) > 0.95
+
+
+def test_wavefront_pass_size_and_launch_shape(gpu_ctx):
+    """Different pass sizes group the FP32 per-pass sums differently, so the
+    films agree to rounding; block / grid shapes change nothing."""
+    s = scenes.box_scene(width=24, height=24, spp=8)
+    g = gpu_ctx.scene(s)
+    a = g.render(seed=3, chunk=8, mode=WF)[0]
+    b = g.render(seed=3, chunk=1, mode=WF, block_size=64, grid_blocks=7)[0]
+    c = g.render(seed=3, chunk=8, mode=WF, block_size=128, grid_blocks=3)[0]
+    assert np.array_equal(a, c)
+    np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-6)
+
+
+def test_wavefront_empty_scene(gpu_ctx):
+    from fumeng_amd.scene import SceneDesc
+    s = SceneDesc()
+    s.renderer.update(width=4, height=4, spp=2)
+    g = gpu_ctx.scene(s)
+    c, a, n, q, st = g.render(mode=WF)
+    assert np.all(c == 0) and np.all(a == 0)
+    assert st.paths == 32
