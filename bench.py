@@ -1,4 +1,5 @@
-"""Benchmark: Mpaths/s of the plain FP32 megakernel on the displaced
+"""Benchmark: Mpaths/s of the FP32 path tracer (wavefront form by default,
+the megakernel timed beside it) on the displaced
 icosphere scene (1,003,520 triangles + ground, 64 Disney materials,
 2048x1024 HDR env, thin lens, 1920x1080, 256 spp, max depth 12).
 
@@ -84,7 +85,9 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
-    ap.add_argument("--mode", choices=["megakernel", "wavefront"], default="megakernel")
+    ap.add_argument("--mode", choices=["megakernel", "wavefront"], default="wavefront")
+    ap.add_argument("--no-compare", dest="compare", action="store_false",
+                    help="skip the one-step run of the other form")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -112,41 +115,59 @@ def main():
 
     from fumeng_amd import _abi
     mode = _abi.FM_MODE_WAVEFRONT if args.mode == "wavefront" else _abi.FM_MODE_MEGAKERNEL
-    kw = dict(seed=1, block_size=args.block, grid_blocks=args.grid, chunk=args.chunk, read=False, mode=mode)
+    kw = dict(seed=1, block_size=args.block, grid_blocks=args.grid, chunk=args.chunk, read=False, mode=mode,
+              time_trace=True)
     for i in range(args.warmup):
         _, _, _, _, st = scene.render(**kw)
         log("warmup %d: %.1f ms" % (i, st.kernel_ms))
-    kernel_ms, rays, paths = 0.0, 0, 0
+    kernel_ms, trace_ms, rays, paths = 0.0, 0.0, 0, 0
     t0 = time.perf_counter()
     for i in range(args.steps):
         _, _, _, _, st = scene.render(**kw)
         kernel_ms += st.kernel_ms
+        trace_ms += st.trace_ms
         rays += st.rays
         paths += st.paths
-        log("step %d: %.1f ms, %.1f Mrays" % (i, st.kernel_ms, st.rays / 1e6))
+        log("step %d: %.1f ms (traversal launches %.1f ms), %.1f Mrays" % (i, st.kernel_ms, st.trace_ms, st.rays / 1e6))
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed * 1e3 / args.steps
     mpaths = W * H * spp * args.steps / elapsed / 1e6
     mrays = rays / elapsed / 1e6
 
-    # Roofline of the render kernel: algorithmic bytes per ray = the node and
-    # triangle records a BVH2 traversal must fetch (64 B per visited node,
-    # 48 B per tested triangle), counted by the kernel's counting variant on a
-    # 1-spp pass; see the design notes for the accounting.
+    # The other form on the same frame (one step): the wavefront runs the
+    # megakernel's estimator with the same random numbers, so the two films
+    # have equal error and their throughputs compare directly.
+    other = None
+    if args.compare:
+        omode = _abi.FM_MODE_MEGAKERNEL if mode == _abi.FM_MODE_WAVEFRONT else _abi.FM_MODE_WAVEFRONT
+        t1 = time.perf_counter()
+        _, _, _, _, ost = scene.render(seed=1, read=False, mode=omode)
+        odt = time.perf_counter() - t1
+        other = {"mode": "megakernel" if omode == _abi.FM_MODE_MEGAKERNEL else "wavefront",
+                 "mpaths_per_s": W * H * spp / odt / 1e6, "mrays_per_s": ost.rays / odt / 1e6,
+                 "ms_per_step": odt * 1e3, "rays": ost.rays}
+        log("%s: %.1f ms, %.1f Mpaths/s" % (other["mode"], odt * 1e3, other["mpaths_per_s"]))
+
+    # Roofline of the traversal kernel (the wavefront's trace launches, or the
+    # whole megakernel): algorithmic bytes per ray = the node and primitive
+    # records a BVH2 traversal must fetch (64 B per visited node, 48 B per
+    # tested primitive), counted by the counting variant of the same kernels
+    # on a 1-spp pass, times the rays of the timed steps, over the summed
+    # duration of those launches (HIP events around each launch).
     roofline = None
     try:
         fumeng_amd.lib().fm_set_counting(ctx.h, 1)
         _, _, _, _, cst = scene.render(seed=2, spp=1, read=False, mode=mode)
         fumeng_amd.lib().fm_set_counting(ctx.h, 0)
-        if cst.node_visits > 0 and cst.rays > 0:
+        if cst.node_visits > 0 and cst.rays > 0 and trace_ms > 0:
             bytes_per_ray = (cst.node_visits * 64.0 + cst.prim_tests * 48.0) / cst.rays
-            launch_bytes = bytes_per_ray * rays / args.steps
-            achieved = launch_bytes / (kernel_ms / args.steps * 1e-3) / 1e9
+            achieved = bytes_per_ray * rays / (trace_ms * 1e-3) / 1e9
             default_run = (args.freq, args.spp, args.block, args.grid, args.chunk) == (224, 256, 0, 0, 0)
-            traffic = profiled_traffic("k_render_mega<false>") if default_run and args.mode == "megakernel" else None
+            kname = "k_wf_trace<false>" if mode == _abi.FM_MODE_WAVEFRONT else "k_render_mega<false>"
+            traffic = profiled_traffic(kname) if default_run else None
             roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                        "bytes_per_ray": bytes_per_ray, "kernel_ms": kernel_ms / args.steps}
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kname,
+                        "bytes_per_ray": bytes_per_ray, "kernel_ms_per_step": trace_ms / args.steps}
     except Exception as e:  # counting is diagnostic; the throughput stands without it
         log("counting pass failed: %s" % e)
 
@@ -163,7 +184,7 @@ def main():
            "config": {"workload": "displaced icosphere %d tris + ground, 64 Disney materials, 2048x1024 HDR env, "
                                   "thin lens, %dx%d, %d spp, depth 12, %s" % (20 * args.freq ** 2, W, H, spp, args.mode),
                       "mode": args.mode, "width": W, "height": H, "spp": spp, "depth": 12},
-           "mrays_per_s": mrays, "paths": paths, "rays": rays,
+           "mrays_per_s": mrays, "paths": paths, "rays": rays, "other_form": other,
            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out))
     scene.close()

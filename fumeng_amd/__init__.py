@@ -113,12 +113,13 @@ class Scene:
 
     def render(self, width=0, height=0, spp=0, depth=0, seed=1, rr_depth=-1, rr_coef=0.0, direct_loop=0,
                mode=A.FM_MODE_MEGAKERNEL, sample_offset=0, block_size=0, grid_blocks=0, chunk=0, accumulate=False,
-               read=True):
+               read=True, time_trace=False):
         """Returns (color, albedo, normal, sumsq, stats); arrays are (H, W, 3), row 0 = top."""
         p = A.fm_render_params(width=width, height=height, spp=spp, depth=depth, direct_loop=direct_loop,
                                rr_depth=rr_depth, rr_coef=rr_coef, mode=mode, seed=seed, sample_offset=sample_offset,
                                block_size=block_size, grid_blocks=grid_blocks, chunk=chunk,
-                               accumulate=1 if accumulate else 0)
+                               accumulate=1 if accumulate else 0,
+                               flags=A.FM_RENDER_TIME_TRACE if time_trace else 0)
         info = self.info()
         W = width or info.renderer.width
         H = height or info.renderer.height

@@ -27,6 +27,7 @@ FM_LIGHT_ENV, FM_LIGHT_DIRECTIONAL = 0, 1
 FM_CAMERA_PIN_HOLE, FM_CAMERA_THIN_LENS = 0, 1
 FM_ACCEL_BVH, FM_ACCEL_NAIVE, FM_ACCEL_BVH_REFERENCE_TREE = 0, 1, 2
 FM_MODE_MEGAKERNEL, FM_MODE_WAVEFRONT = 0, 1
+FM_RENDER_TIME_TRACE = 1
 
 F3 = C.c_float * 3
 
@@ -95,7 +96,7 @@ class fm_render_params(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("depth", C.c_int32),
                 ("direct_loop", C.c_int32), ("rr_depth", C.c_int32), ("rr_coef", C.c_float), ("mode", C.c_int32),
                 ("seed", C.c_uint64), ("sample_offset", C.c_int32), ("block_size", C.c_int32),
-                ("grid_blocks", C.c_int32), ("chunk", C.c_int32), ("accumulate", C.c_int32), ("pad_", C.c_int32)]
+                ("grid_blocks", C.c_int32), ("chunk", C.c_int32), ("accumulate", C.c_int32), ("flags", C.c_int32)]
 
 
 class fm_film(C.Structure):
@@ -105,7 +106,7 @@ class fm_film(C.Structure):
 
 class fm_render_stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("paths", C.c_int64), ("rays", C.c_int64),
-                ("node_visits", C.c_int64), ("prim_tests", C.c_int64)]
+                ("node_visits", C.c_int64), ("prim_tests", C.c_int64), ("trace_ms", C.c_double)]
 
 
 class fm_ray(C.Structure):

@@ -117,6 +117,7 @@ struct fm_ctx {
     int counting = 0;
     std::set<fm_scene*> scenes;  /* live scenes, released by fm_shutdown */
     WfBuffers wf;
+    std::vector<hipEvent_t> trace_ev;  /* pairs around wavefront trace launches */
 };
 
 struct fm_scene {
@@ -256,6 +257,8 @@ void fm_shutdown(fm_ctx* ctx) {
     ctx->counters.release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    for (hipEvent_t e : ctx->trace_ev) (void)hipEventDestroy(e);
+    ctx->trace_ev.clear();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -337,10 +340,12 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
         A.sample_offset = p->sample_offset;
         const int64_t npix = (int64_t)W * H;
         const bool wave = p->mode == FM_MODE_WAVEFRONT;
+        double trace_ms = 0.0;
         /* megakernel: samples per work item; wavefront: samples per pass
-         * (about 8 M paths in flight by default) */
+         * (about 32 M paths in flight by default, ~11 GB of path state for
+         * one light) */
         if (p->chunk > 0) A.chunk = std::min(p->chunk, spp);
-        else if (wave) A.chunk = (int)std::max<int64_t>(1, std::min<int64_t>(spp, ((int64_t)1 << 23) / npix));
+        else if (wave) A.chunk = (int)std::max<int64_t>(1, std::min<int64_t>(spp, ((int64_t)1 << 25) / npix));
         else A.chunk = std::min(spp, 16);
         const int64_t nchunks = (spp + A.chunk - 1) / A.chunk;
         A.total_items = npix * nchunks;
@@ -406,15 +411,29 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
             for (const fm_dmaterial& m : s->hs.materials)
                 if (m.type >= 0 && m.type + 1 < fm::WQ_NCLS) L.has_class[m.type + 1] = true;
             fm::wavefront_grids(ctx->num_cus, std::max(1, s->hs.trav_depth), L);
+            const bool timed = (p->flags & FM_RENDER_TIME_TRACE) != 0;
+            const size_t nev = timed ? (size_t)nchunks * 2 * W_.n_iter : 0;
+            while (ctx->trace_ev.size() < nev) {
+                hipEvent_t e;
+                HIPCHECK(hipEventCreate(&e));
+                ctx->trace_ev.push_back(e);
+            }
             HIPCHECK(hipEventRecord(ctx->ev0, ctx->stream));
             for (int64_t c = 0; c < nchunks; c++) {
                 const int kp = (int)std::min<int64_t>(A.chunk, spp - c * A.chunk);
                 W_.k = kp;
                 W_.P = npix * kp;
                 W_.sample_base = A.sample_offset + (int)(c * A.chunk);
-                HIPCHECK(fm::launch_wavefront_pass(s->d, W_, L, ctx->counting != 0, ctx->stream));
+                hipEvent_t* ev = timed ? &ctx->trace_ev[(size_t)c * 2 * W_.n_iter] : nullptr;
+                HIPCHECK(fm::launch_wavefront_pass(s->d, W_, L, ctx->counting != 0, ctx->stream, ev));
             }
             HIPCHECK(hipEventRecord(ctx->ev1, ctx->stream));
+            HIPCHECK(hipStreamSynchronize(ctx->stream));
+            for (size_t e = 0; e + 1 < nev; e += 2) {
+                float ms = 0.0f;
+                HIPCHECK(hipEventElapsedTime(&ms, ctx->trace_ev[e], ctx->trace_ev[e + 1]));
+                trace_ms += ms;
+            }
         }
         HIPCHECK(hipStreamSynchronize(ctx->stream));
         ctx->film_spp += spp;
@@ -429,6 +448,7 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
             stats->rays = (int64_t)c[1];
             stats->node_visits = (int64_t)c[2];
             stats->prim_tests = (int64_t)c[3];
+            if (p->flags & FM_RENDER_TIME_TRACE) stats->trace_ms = wave ? trace_ms : ms;
         }
         if (film && (film->color || film->albedo || film->normal || film->color_sum || film->sumsq)) return fm_film_read(ctx, film);
         return FM_OK;

@@ -75,8 +75,11 @@ struct WfLaunch {
     bool has_class[WQ_NCLS];
 };
 
-/* Kernel sequence of one pass (camera, n_iter bounce iterations, film). */
-hipError_t launch_wavefront_pass(const fm_dscene& S, WfArgs A, const WfLaunch& L, bool count, hipStream_t st);
+/* Kernel sequence of one pass (camera, n_iter bounce iterations, film).  If
+ * ev is given, events ev[2 it] / ev[2 it + 1] bracket iteration it's trace
+ * kernel. */
+hipError_t launch_wavefront_pass(const fm_dscene& S, WfArgs A, const WfLaunch& L, bool count, hipStream_t st,
+                                 hipEvent_t* ev = nullptr);
 /* Persistent grid sizes for the wavefront kernels on this device. */
 void wavefront_grids(int num_cus, int stack_levels, WfLaunch& L);
 

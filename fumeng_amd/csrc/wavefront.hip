@@ -7,10 +7,10 @@
  * occupancy its own register count allows:
  *
  *   k_wf_camera    camera rays of P paths (k samples of every pixel)
- *   k_wf_extend    closest hit of every queued ray (continuation and MisBSDF
- *                  rays); sorts the paths whose continuation ray it traced
- *                  into per-material-class shading queues
- *   k_wf_shadow    any hit of every queued shadow ray (MisLight)
+ *   k_wf_trace     closest hit of every queued ray (continuation and MisBSDF
+ *                  rays), sorting the paths whose continuation ray it traced
+ *                  into per-material-class shading queues; and in the same
+ *                  launch the any hit of every queued shadow ray (MisLight)
  *   k_wf_shade<C>  per class: first adds the previous vertex's direct light
  *                  (shadow results, MisBSDF hit) to the path radiance, then
  *                  shades the new vertex: light samples -> shadow queue,
@@ -124,64 +124,55 @@ __global__ void __launch_bounds__(256) k_wf_camera(fm_dscene S, WfArgs A) {
     }
 }
 
-/* ---- extend: closest hit + classification -------------------------------- */
+/* ---- trace: closest hits + classification, and shadow any hits ----------- */
 
+/* One launch per iteration covers both queues: items [0, n_ext) are ray
+ * slots for the closest hit, [n_ext, n_ext + n_sh) shadow records for the
+ * any hit, so the two share the machine (and the latency of a short tail
+ * iteration is paid once). */
 template <bool COUNT>
-__global__ void __launch_bounds__(256) k_wf_extend(fm_dscene S, WfArgs A) {
+__global__ void __launch_bounds__(256) k_wf_trace(fm_dscene S, WfArgs A) {
     uint32_t* stk = wf_stack();
     const int stride = blockDim.x;
     const int it = A.iter;
-    const unsigned n = *cnt(A, it, fm::WQ_EXT);
+    const unsigned n_ext = *cnt(A, it, fm::WQ_EXT);
+    const unsigned n_sh = *cnt(A, it, fm::WQ_SH);
+    const int64_t total = (int64_t)n_ext + n_sh;
     const int* q = A.q_ext[it & 1];
+    const int* qs = A.q_sh[it & 1];
     int* qc = A.q_cls[it & 1];
     Cnt cn;
-    for (int64_t i = grid_id(); i < (int64_t)n; i += grid_stride()) {
-        const int r = q[i];
-        const Ray tr = load_ray(A, r);
-        const Hit h = trace_closest<COUNT>(S, tr, stk, stride, &cn);
-        reinterpret_cast<float4*>(A.hit_a)[r] =
-            make_float4(h.t, __int_as_float(h.slot), __int_as_float((h.kind & 0xff) | (h.half << 8)), h.b0);
-        reinterpret_cast<float2*>(A.hit_b)[r] = make_float2(h.b1, h.b2);
-        int cls = -1;
-        if (r < A.P) {
-            cls = fm::WC_END;
-            if (h.slot >= 0) {
-                const int4 me = reinterpret_cast<const int4*>(S.meta)[h.slot];
-                const int depth = __float_as_int(reinterpret_cast<const float4*>(A.st_beta)[r].w);
-                if (!(depth == 1 && me.z >= 0)) cls = 1 + S.materials[me.y].type;
+    for (int64_t i = grid_id(); i < total; i += grid_stride()) {
+        if (i < (int64_t)n_ext) {
+            const int r = q[i];
+            const Ray tr = load_ray(A, r);
+            const Hit h = trace_closest<COUNT>(S, tr, stk, stride, &cn);
+            reinterpret_cast<float4*>(A.hit_a)[r] =
+                make_float4(h.t, __int_as_float(h.slot), __int_as_float((h.kind & 0xff) | (h.half << 8)), h.b0);
+            reinterpret_cast<float2*>(A.hit_b)[r] = make_float2(h.b1, h.b2);
+            int cls = -1;
+            if (r < A.P) {
+                cls = fm::WC_END;
+                if (h.slot >= 0) {
+                    const int4 me = reinterpret_cast<const int4*>(S.meta)[h.slot];
+                    const int depth = __float_as_int(reinterpret_cast<const float4*>(A.st_beta)[r].w);
+                    if (!(depth == 1 && me.z >= 0)) cls = 1 + S.materials[me.y].type;
+                }
             }
-        }
 #pragma unroll
-        for (int c = 0; c < fm::WQ_NCLS; c++)
-            wave_push(cnt(A, it, fm::WQ_CLS + c), qc + (size_t)c * A.P, cls == c, r);
+            for (int c = 0; c < fm::WQ_NCLS; c++)
+                wave_push(cnt(A, it, fm::WQ_CLS + c), qc + (size_t)c * A.P, cls == c, r);
+        } else {
+            const int k = qs[i - (int64_t)n_ext];
+            const float4 o = reinterpret_cast<const float4*>(A.sh_o)[k];
+            const float4 d = reinterpret_cast<const float4*>(A.sh_d)[k];
+            Ray tr;
+            tr.o = xyz(o); tr.tmin = o.w; tr.d = xyz(d); tr.tmax = d.w;
+            const bool occluded = trace_any<COUNT>(S, tr, stk, stride, &cn);
+            A.sh_vis[k] = occluded ? 1 : 2;
+        }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(A.counters + 1, (unsigned long long)n);
-    if (COUNT) {
-        wave_add_u64(A.counters + 2, cn.nodes);
-        wave_add_u64(A.counters + 3, cn.prims);
-    }
-}
-
-/* ---- shadow: any hit ------------------------------------------------------ */
-
-template <bool COUNT>
-__global__ void __launch_bounds__(256) k_wf_shadow(fm_dscene S, WfArgs A) {
-    uint32_t* stk = wf_stack();
-    const int stride = blockDim.x;
-    const int it = A.iter;
-    const unsigned n = *cnt(A, it, fm::WQ_SH);
-    const int* q = A.q_sh[it & 1];
-    Cnt cn;
-    for (int64_t i = grid_id(); i < (int64_t)n; i += grid_stride()) {
-        const int k = q[i];
-        const float4 o = reinterpret_cast<const float4*>(A.sh_o)[k];
-        const float4 d = reinterpret_cast<const float4*>(A.sh_d)[k];
-        Ray tr;
-        tr.o = xyz(o); tr.tmin = o.w; tr.d = xyz(d); tr.tmax = d.w;
-        const bool occluded = trace_any<COUNT>(S, tr, stk, stride, &cn);
-        A.sh_vis[k] = occluded ? 1 : 2;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(A.counters + 1, (unsigned long long)n);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(A.counters + 1, (unsigned long long)total);
     if (COUNT) {
         wave_add_u64(A.counters + 2, cn.nodes);
         wave_add_u64(A.counters + 3, cn.prims);
@@ -389,24 +380,22 @@ void wavefront_grids(int num_cus, int stack_levels, WfLaunch& L) {
     if (L.block_trace <= 0) L.block_trace = 256;
     if (L.block_shade <= 0) L.block_shade = 256;
     const size_t lds = (size_t)stack_levels * L.block_trace * sizeof(uint32_t);
-    if (L.grid_trace <= 0) L.grid_trace = num_cus * occupancy(k_wf_extend<false>, L.block_trace, lds);
+    if (L.grid_trace <= 0) L.grid_trace = num_cus * occupancy(k_wf_trace<false>, L.block_trace, lds);
     if (L.grid_shade <= 0) L.grid_shade = num_cus * occupancy(k_wf_shade<WC_DISNEY>, L.block_shade, 0);
 }
 
-hipError_t launch_wavefront_pass(const fm_dscene& S, WfArgs A, const WfLaunch& L, bool count, hipStream_t st) {
+hipError_t launch_wavefront_pass(const fm_dscene& S, WfArgs A, const WfLaunch& L, bool count, hipStream_t st,
+                                 hipEvent_t* ev) {
     const size_t lds = (size_t)L.stack_levels * L.block_trace * sizeof(uint32_t);
     const int64_t npix = (int64_t)A.W * A.H;
     const int cam_grid = (int)std::min<int64_t>((A.P + 255) / 256, (int64_t)L.grid_shade * 4);
     hipLaunchKernelGGL(k_wf_camera, dim3(std::max(cam_grid, 1)), dim3(256), 0, st, S, A);
     for (int it = 0; it < A.n_iter; it++) {
         A.iter = it;
-        if (count) {
-            hipLaunchKernelGGL(k_wf_extend<true>, dim3(L.grid_trace), dim3(L.block_trace), lds, st, S, A);
-            hipLaunchKernelGGL(k_wf_shadow<true>, dim3(L.grid_trace), dim3(L.block_trace), lds, st, S, A);
-        } else {
-            hipLaunchKernelGGL(k_wf_extend<false>, dim3(L.grid_trace), dim3(L.block_trace), lds, st, S, A);
-            hipLaunchKernelGGL(k_wf_shadow<false>, dim3(L.grid_trace), dim3(L.block_trace), lds, st, S, A);
-        }
+        if (ev) hipEventRecord(ev[2 * it], st);
+        if (count) hipLaunchKernelGGL(k_wf_trace<true>, dim3(L.grid_trace), dim3(L.block_trace), lds, st, S, A);
+        else hipLaunchKernelGGL(k_wf_trace<false>, dim3(L.grid_trace), dim3(L.block_trace), lds, st, S, A);
+        if (ev) hipEventRecord(ev[2 * it + 1], st);
         hipLaunchKernelGGL(k_wf_shade<WC_END>, dim3(L.grid_shade), dim3(L.block_shade), 0, st, S, A);
         if (L.has_class[WC_LAMBERT]) hipLaunchKernelGGL(k_wf_shade<WC_LAMBERT>, dim3(L.grid_shade), dim3(L.block_shade), 0, st, S, A);
         if (L.has_class[WC_DISNEY]) hipLaunchKernelGGL(k_wf_shade<WC_DISNEY>, dim3(L.grid_shade), dim3(L.block_shade), 0, st, S, A);

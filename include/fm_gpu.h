@@ -209,6 +209,8 @@ int fm_scene_get_info(const fm_scene* scene, fm_scene_info* out);
 /* ---- rendering ------------------------------------------------------------ */
 
 enum { FM_MODE_MEGAKERNEL = 0, FM_MODE_WAVEFRONT = 1 };
+/* fm_render_params.flags */
+enum { FM_RENDER_TIME_TRACE = 1 };   /* time each traversal launch (fm_render_stats.trace_ms) */
 typedef struct fm_render_params {
     int32_t width, height, spp, depth;   /* <= 0: take the scene's renderer block */
     int32_t direct_loop;                 /* <= 0: scene value */
@@ -219,9 +221,10 @@ typedef struct fm_render_params {
     int32_t sample_offset;               /* first sample index (progressive passes) */
     int32_t block_size;                  /* 0 = tuned default */
     int32_t grid_blocks;                 /* 0 = tuned default */
-    int32_t chunk;                       /* samples per work item, 0 = default */
+    int32_t chunk;                       /* megakernel: samples per work item; wavefront:
+                                            samples of every pixel per pass; 0 = default */
     int32_t accumulate;                  /* 1: add to the existing device film */
-    int32_t pad_;
+    int32_t flags;                       /* FM_RENDER_* */
 } fm_render_params;
 
 /* Film, row 0 = top (the reference stores pixel row j at height-1-j,
@@ -242,6 +245,9 @@ typedef struct fm_render_stats {
     int64_t rays;            /* closest-hit + any-hit queries */
     int64_t node_visits;     /* only when counting is enabled */
     int64_t prim_tests;
+    double trace_ms;         /* with FM_RENDER_TIME_TRACE: summed device time of the
+                                traversal launches (wavefront trace kernels; the
+                                whole megakernel) */
 } fm_render_stats;
 
 /* Renders into the context's device film (FP64 sums) and, when `film` has

@@ -109,3 +109,17 @@ def test_wavefront_empty_scene(gpu_ctx):
     c, a, n, q, st = g.render(mode=WF)
     assert np.all(c == 0) and np.all(a == 0)
     assert st.paths == 32
+
+
+def test_trace_timing_flag(gpu_ctx):
+    """FM_RENDER_TIME_TRACE sums the device time of the traversal launches;
+    it changes nothing in the image."""
+    s = scenes.box_scene(width=16, height=16, spp=4)
+    g = gpu_ctx.scene(s)
+    a = g.render(seed=9, mode=WF)
+    b = g.render(seed=9, mode=WF, time_trace=True)
+    assert np.array_equal(a[0], b[0])
+    assert a[4].trace_ms == 0.0
+    assert 0.0 < b[4].trace_ms <= b[4].kernel_ms
+    m = g.render(seed=9, mode=MK, time_trace=True)[4]
+    assert m.trace_ms == m.kernel_ms
