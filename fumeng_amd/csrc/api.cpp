@@ -341,12 +341,17 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
         const int64_t npix = (int64_t)W * H;
         const bool wave = p->mode == FM_MODE_WAVEFRONT;
         double trace_ms = 0.0;
-        /* megakernel: samples per work item; wavefront: samples per pass
-         * (about 32 M paths in flight by default, ~11 GB of path state for
-         * one light) */
+        /* megakernel: samples per work item.  wavefront: samples of every
+         * pixel per pass; by default about 64 M paths in flight, within a
+         * 24 GB budget for path state, rays, hit records and queues (about
+         * 330 B per path with one light and one direct-light pass). */
         if (p->chunk > 0) A.chunk = std::min(p->chunk, spp);
-        else if (wave) A.chunk = (int)std::max<int64_t>(1, std::min<int64_t>(spp, ((int64_t)1 << 25) / npix));
-        else A.chunk = std::min(spp, 16);
+        else if (wave) {
+            const int64_t D = std::max(1, A.cfg_raw.direct), NL = std::max(1, s->d.num_lights);
+            const int64_t per_path = 100 + 64 * (1 + D) + 48 * D + 57 * D * NL;
+            const int64_t target = std::min<int64_t>((int64_t)1 << 26, (int64_t)24e9 / per_path);
+            A.chunk = (int)std::max<int64_t>(1, std::min<int64_t>(spp, target / npix));
+        } else A.chunk = std::min(spp, 16);
         const int64_t nchunks = (spp + A.chunk - 1) / A.chunk;
         A.total_items = npix * nchunks;
         /* film */

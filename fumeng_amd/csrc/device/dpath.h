@@ -58,14 +58,29 @@ FMD int cdf_search(const float* cdf, int n, float u) {
     }
     return lo;
 }
+/* The same index as cdf_search, starting from a guide table: for u in
+ * [g / G, (g + 1) / G) the answer lies between guide[g] and guide[g + 1]. */
+FMD int cdf_search_guided(const float* cdf, int n, float u, const int* guide, int G) {
+    int g = (int)(u * (float)G);
+    g = g < 0 ? 0 : (g > G - 1 ? G - 1 : g);
+    int lo = guide[g], hi = guide[g + 1] + 1;
+    if (hi > n) hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (cdf[mid] <= u) lo = mid; else hi = mid;
+    }
+    return lo;
+}
 FMD LightSample env_sample(const fm_dscene& S, const fm_denv& E, v3 pos, v3 s) {
     const float* T = S.env_tables;
     const float* mc = T + E.off_marg_cdf;
-    const int iv = cdf_search(mc, E.H, s.x);
+    const int* mg = reinterpret_cast<const int*>(T + E.off_marg_guide);
+    const int iv = cdf_search_guided(mc, E.H, s.x, mg, E.guide_m);
     const float vv = ((s.x - mc[iv]) / (mc[iv + 1] - mc[iv]) + (float)iv) / (float)E.H;
     const float pm = T[E.off_marg_val + iv] / E.marg_sum;
     const float* cc = T + E.off_cond_cdf + (size_t)iv * (E.W + 1);
-    const int iu = cdf_search(cc, E.W, s.y);
+    const int* cg = reinterpret_cast<const int*>(T + E.off_cond_guide) + (size_t)iv * (E.guide_c + 1);
+    const int iu = cdf_search_guided(cc, E.W, s.y, cg, E.guide_c);
     const float uu = ((s.y - cc[iu]) / (cc[iu + 1] - cc[iu]) + (float)iu) / (float)E.W;
     const float pc = T[E.off_cond_val + (size_t)iv * E.W + iu] / T[E.off_cond_sum + iv];
     const float p2 = pm * pc;

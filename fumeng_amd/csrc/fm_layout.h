@@ -89,7 +89,9 @@ typedef struct fm_denv {
     int64_t off_cond_cdf;   /* H * (W + 1) */
     int64_t off_cond_val;   /* H * W */
     int64_t off_cond_sum;   /* H */
-    int64_t pad3;
+    int64_t off_marg_guide; /* int32 bits, guide_m + 1: last row with cdf <= g / guide_m */
+    int64_t off_cond_guide; /* int32 bits, H * (guide_c + 1): the same per row */
+    int32_t guide_m, guide_c;  /* powers of two */
 } fm_denv;
 
 typedef struct fm_dimage {

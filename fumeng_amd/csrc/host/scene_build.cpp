@@ -711,6 +711,29 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
         for (int j = 0; j <= H; j++) T[(size_t)e.off_marg_cdf + j] = (float)(mc[(size_t)j] / ms);
         for (int j = 0; j < H; j++) T[(size_t)e.off_marg_val + j] = (float)rowsum[(size_t)j];
         e.marg_sum = (float)ms;
+        /* guide tables for the device search: entry g = last index with
+         * cdf <= g / G over the stored FP32 cdf (G a power of two, so g / G
+         * and u * G are exact); a query u then only searches between the
+         * entries of floor(u G) and floor(u G) + 1 and finds the same index */
+        auto last_le = [](const float* cdf, int n, float u) {
+            int lo = 0, hi = n;
+            while (hi - lo > 1) { const int mid = (lo + hi) >> 1; if (cdf[mid] <= u) lo = mid; else hi = mid; }
+            return lo;
+        };
+        auto pow2_at_least = [](int n, int cap) { int g = 1; while (g < n && g < cap) g <<= 1; return g; };
+        e.guide_m = pow2_at_least(H, 4096);
+        e.guide_c = pow2_at_least(W / 8, 512);
+        e.off_marg_guide = alloc((size_t)e.guide_m + 1);
+        e.off_cond_guide = alloc((size_t)H * (e.guide_c + 1));
+        for (int g = 0; g <= e.guide_m; g++) {
+            const int32_t ix = last_le(&T[(size_t)e.off_marg_cdf], H, (float)g / (float)e.guide_m);
+            std::memcpy(&T[(size_t)e.off_marg_guide + g], &ix, 4);
+        }
+        for (int j = 0; j < H; j++)
+            for (int g = 0; g <= e.guide_c; g++) {
+                const int32_t ix = last_le(&T[(size_t)e.off_cond_cdf + (size_t)j * (W + 1)], W, (float)g / (float)e.guide_c);
+                std::memcpy(&T[(size_t)e.off_cond_guide + (size_t)j * (e.guide_c + 1) + g], &ix, 4);
+            }
         L.kind = FM_DLIGHT_ENV;
         L.aux = (int32_t)hs.envs.size();
         if (hs.env_light < 0) { hs.env_light = (int32_t)hs.lights.size(); hs.env = L.aux; }

@@ -70,7 +70,8 @@ struct WfArgs {
 
 struct WfLaunch {
     int block_trace, grid_trace;   /* extend / shadow */
-    int block_shade, grid_shade;
+    int block_shade, grid_shade;   /* grid_shade > 0 overrides every class */
+    int grid_class[WQ_NCLS];       /* persistent grid per shading class (its own occupancy) */
     int stack_levels;
     bool has_class[WQ_NCLS];
 };

@@ -381,7 +381,12 @@ void wavefront_grids(int num_cus, int stack_levels, WfLaunch& L) {
     if (L.block_shade <= 0) L.block_shade = 256;
     const size_t lds = (size_t)stack_levels * L.block_trace * sizeof(uint32_t);
     if (L.grid_trace <= 0) L.grid_trace = num_cus * occupancy(k_wf_trace<false>, L.block_trace, lds);
-    if (L.grid_shade <= 0) L.grid_shade = num_cus * occupancy(k_wf_shade<WC_DISNEY>, L.block_shade, 0);
+    const int g = L.grid_shade;
+    L.grid_class[WC_END] = g > 0 ? g : num_cus * occupancy(k_wf_shade<WC_END>, L.block_shade, 0);
+    L.grid_class[WC_LAMBERT] = g > 0 ? g : num_cus * occupancy(k_wf_shade<WC_LAMBERT>, L.block_shade, 0);
+    L.grid_class[WC_DISNEY] = g > 0 ? g : num_cus * occupancy(k_wf_shade<WC_DISNEY>, L.block_shade, 0);
+    L.grid_class[WC_SPEC] = g > 0 ? g : num_cus * occupancy(k_wf_shade<WC_SPEC>, L.block_shade, 0);
+    if (L.grid_shade <= 0) L.grid_shade = L.grid_class[WC_END];
 }
 
 hipError_t launch_wavefront_pass(const fm_dscene& S, WfArgs A, const WfLaunch& L, bool count, hipStream_t st,
@@ -396,10 +401,10 @@ hipError_t launch_wavefront_pass(const fm_dscene& S, WfArgs A, const WfLaunch& L
         if (count) hipLaunchKernelGGL(k_wf_trace<true>, dim3(L.grid_trace), dim3(L.block_trace), lds, st, S, A);
         else hipLaunchKernelGGL(k_wf_trace<false>, dim3(L.grid_trace), dim3(L.block_trace), lds, st, S, A);
         if (ev) hipEventRecord(ev[2 * it + 1], st);
-        hipLaunchKernelGGL(k_wf_shade<WC_END>, dim3(L.grid_shade), dim3(L.block_shade), 0, st, S, A);
-        if (L.has_class[WC_LAMBERT]) hipLaunchKernelGGL(k_wf_shade<WC_LAMBERT>, dim3(L.grid_shade), dim3(L.block_shade), 0, st, S, A);
-        if (L.has_class[WC_DISNEY]) hipLaunchKernelGGL(k_wf_shade<WC_DISNEY>, dim3(L.grid_shade), dim3(L.block_shade), 0, st, S, A);
-        if (L.has_class[WC_SPEC]) hipLaunchKernelGGL(k_wf_shade<WC_SPEC>, dim3(L.grid_shade), dim3(L.block_shade), 0, st, S, A);
+        hipLaunchKernelGGL(k_wf_shade<WC_END>, dim3(L.grid_class[WC_END]), dim3(L.block_shade), 0, st, S, A);
+        if (L.has_class[WC_LAMBERT]) hipLaunchKernelGGL(k_wf_shade<WC_LAMBERT>, dim3(L.grid_class[WC_LAMBERT]), dim3(L.block_shade), 0, st, S, A);
+        if (L.has_class[WC_DISNEY]) hipLaunchKernelGGL(k_wf_shade<WC_DISNEY>, dim3(L.grid_class[WC_DISNEY]), dim3(L.block_shade), 0, st, S, A);
+        if (L.has_class[WC_SPEC]) hipLaunchKernelGGL(k_wf_shade<WC_SPEC>, dim3(L.grid_class[WC_SPEC]), dim3(L.block_shade), 0, st, S, A);
     }
     const int film_grid = (int)std::min<int64_t>((npix + 255) / 256, (int64_t)L.grid_shade * 4);
     hipLaunchKernelGGL(k_wf_film, dim3(std::max(film_grid, 1)), dim3(256), 0, st, A);
