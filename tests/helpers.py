@@ -73,3 +73,59 @@ def bsdf_queries(rng, n, material, grazing=False):
     q["uv"] = rng.random((n, 2))
     q["sample"] = rng.random((n, 3)) * 0.999999
     return q
+
+
+def convergence_scene(name):
+    """Scenes of the FP64 image-convergence fixtures (tests/golden)."""
+    if name == "box32":
+        return scenes.box_scene(width=32, height=32, spp=64)
+    if name == "ico48x27":
+        return scenes.icosphere_scene(freq=24, width=48, height=27, spp=64, env_size=(256, 128))
+    raise KeyError(name)
+
+
+def scene_signature(s):
+    """A number that changes when the generated geometry, images, materials
+    or renderer settings change."""
+    acc = 0.0
+    for k, m in enumerate(s.meshes):
+        for key in sorted(m):
+            v = m[key]
+            if isinstance(v, np.ndarray) and v.size:
+                acc += float(np.sum(v.astype(np.float64) * (1.0 + 1e-3 * (k % 97))))
+    for im in s.images:
+        acc += float(np.sum(im.astype(np.float64))) * 1e-3
+    acc += float(len(s.objects)) * 7.0 + float(len(s.materials)) * 11.0 + float(len(s.lights)) * 13.0
+    for key in ("width", "height", "depth", "rr_depth", "direct_loop"):
+        acc += float(s.renderer.get(key, 0)) * 0.5
+    return acc
+
+
+def load_reference(name, prec="fp64"):
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "%s_%s.npz" % (prec, name))
+    with np.load(path) as f:
+        return {k: f[k] for k in f.files}
+
+
+def convergence_stats(mean, sumsq, n, ref):
+    """relMSE = mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2);
+    per-pixel z = (x - r) / sqrt(s_x^2 / n + s_r^2 / n_r) with both sample
+    variances from the sums of squares (channels with zero variance on both
+    sides and equal means count as z = 0)."""
+    nr = float(ref["spp"])
+    x = np.asarray(mean, np.float64)
+    q = np.asarray(sumsq, np.float64)
+    # the reference estimator has no NaN guard: a path whose direction has an
+    # exact zero component can pick up a NaN hit; such pixels are excluded
+    finite = np.all(np.isfinite(x), axis=-1) & np.all(np.isfinite(ref["mean"]), axis=-1)
+    x, q = x[finite], q[finite]
+    r, rq = ref["mean"][finite], ref["sumsq"][finite]
+    vx = np.maximum(q / n - x * x, 0.0) * n / max(n - 1.0, 1.0)
+    vr = np.maximum(rq / nr - r * r, 0.0) * nr / (nr - 1.0)
+    se = np.sqrt(vx / n + vr / nr)
+    diff = x - r
+    z = np.where(se > 0, diff / np.where(se > 0, se, 1.0), np.where(np.abs(diff) <= 1e-12, 0.0, np.inf))
+    rel = float(np.mean(diff * diff / (r * r + 1e-2)))
+    return {"relmse": rel, "mean_z": float(np.mean(z[np.isfinite(z)])), "frac_z3": float(np.mean(np.abs(z) > 3.0)),
+            "max_abs_z": float(np.max(np.abs(z))), "nonfinite_pixels": int(np.sum(~finite)), "z": z}
