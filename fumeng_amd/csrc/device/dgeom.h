@@ -174,6 +174,12 @@ FMD void test_slot_closest(const fm_dscene& S, const Ray& r, const RayPre& rp, i
      * rejects t <= 0 (triangle.cpp:221), Rectangle::GetIntersect does not */
     const bool tri = kind == FM_PRIM_TRI;
     auto accept = [&](float tt) {
+        /* A NaN t (a direction with an exact zero component sheared along
+         * it) passes every test the reference writes as a rejection
+         * (t <= t_min, t >= t_max, t > hit.t, t <= 0), and the NaN it then
+         * puts into the ray's t_max makes every later box test fail: the
+         * NaN hit stands.  Here no finite t can beat a NaN closest t. */
+        if (tt != tt) return true;
         if (!(tt > r.tmin) || (tri && !(tt > 0.0f))) return false;
         if (tt < h.t) return true;
         return tt == h.t && have && h.kind != FM_PRIM_SPHERE && key < h.tie;
