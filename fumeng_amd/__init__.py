@@ -113,13 +113,13 @@ class Scene:
 
     def render(self, width=0, height=0, spp=0, depth=0, seed=1, rr_depth=-1, rr_coef=0.0, direct_loop=0,
                mode=A.FM_MODE_MEGAKERNEL, sample_offset=0, block_size=0, grid_blocks=0, chunk=0, accumulate=False,
-               read=True, time_trace=False):
+               read=True, time_trace=False, graph=False):
         """Returns (color, albedo, normal, sumsq, stats); arrays are (H, W, 3), row 0 = top."""
         p = A.fm_render_params(width=width, height=height, spp=spp, depth=depth, direct_loop=direct_loop,
                                rr_depth=rr_depth, rr_coef=rr_coef, mode=mode, seed=seed, sample_offset=sample_offset,
                                block_size=block_size, grid_blocks=grid_blocks, chunk=chunk,
                                accumulate=1 if accumulate else 0,
-                               flags=A.FM_RENDER_TIME_TRACE if time_trace else 0)
+                               flags=(A.FM_RENDER_TIME_TRACE if time_trace else 0) | (A.FM_RENDER_GRAPH if graph else 0))
         info = self.info()
         W = width or info.renderer.width
         H = height or info.renderer.height
@@ -133,6 +133,42 @@ class Scene:
         if read:
             return out[0], out[1], out[2], out[3], st
         return None, None, None, None, st
+
+    def render_batch(self, jobs, direct_loop=0, rr_depth=-1, rr_coef=0.0, block_size=0, grid_blocks=0, chunk=0,
+                     graph=True, read=True):
+        """Renders a list of jobs in one wavefront run (fm_render_batch).  A
+        job is a dict with optional keys camera (a JSON camera dict; absent =
+        the scene's camera), width, height, spp, depth, seed, sample_offset.
+        Returns (films, done_ms, stats); films[i] = (color, albedo, normal,
+        sumsq) as (H, W, 3) arrays, or None when read is False."""
+        from .scene import camera_to_c
+        info = self.info()
+        n = len(jobs)
+        cj = (A.fm_job * max(1, n))()
+        films = []
+        for i, j in enumerate(jobs):
+            x = cj[i]
+            if j.get("camera") is not None:
+                camera_to_c(j["camera"], x.camera)
+            else:
+                x.scene_camera = 1
+            x.width, x.height = int(j.get("width", 0)), int(j.get("height", 0))
+            x.spp, x.depth = int(j.get("spp", 0)), int(j.get("depth", 0))
+            x.seed, x.sample_offset = int(j.get("seed", 1)), int(j.get("sample_offset", 0))
+            if read:
+                W = x.width if x.width > 0 else info.renderer.width
+                H = x.height if x.height > 0 else info.renderer.height
+                f = [np.zeros((H, W, 3), np.float32) for _ in range(3)] + [np.zeros((H, W, 3), np.float64)]
+                x.color, x.albedo, x.normal, x.sumsq = (o.ctypes.data for o in f)
+                films.append(tuple(f))
+            else:
+                films.append(None)
+        p = A.fm_render_params(direct_loop=direct_loop, rr_depth=rr_depth, rr_coef=rr_coef, block_size=block_size,
+                               grid_blocks=grid_blocks, chunk=chunk, flags=A.FM_RENDER_GRAPH if graph else 0)
+        done = (C.c_double * max(1, n))()
+        st = A.fm_batch_stats()
+        _check(lib().fm_render_batch(self.ctx.h, self.h, cj, n, C.byref(p), done, C.byref(st)), self.ctx.h)
+        return films, np.array(done[:n]), st
 
     def trace(self, rays, any_hit=False):
         """rays: (N, 8) float32 [ox oy oz tmin dx dy dz tmax]; returns fm_hit records."""

@@ -28,6 +28,7 @@ FM_CAMERA_PIN_HOLE, FM_CAMERA_THIN_LENS = 0, 1
 FM_ACCEL_BVH, FM_ACCEL_NAIVE, FM_ACCEL_BVH_REFERENCE_TREE = 0, 1, 2
 FM_MODE_MEGAKERNEL, FM_MODE_WAVEFRONT = 0, 1
 FM_RENDER_TIME_TRACE = 1
+FM_RENDER_GRAPH = 2
 
 F3 = C.c_float * 3
 
@@ -109,6 +110,17 @@ class fm_render_stats(C.Structure):
                 ("node_visits", C.c_int64), ("prim_tests", C.c_int64), ("trace_ms", C.c_double)]
 
 
+class fm_job(C.Structure):
+    _fields_ = [("camera", fm_camera_desc), ("scene_camera", C.c_int32), ("width", C.c_int32), ("height", C.c_int32),
+                ("spp", C.c_int32), ("depth", C.c_int32), ("sample_offset", C.c_int32), ("seed", C.c_uint64),
+                ("color", C.c_void_p), ("albedo", C.c_void_p), ("normal", C.c_void_p), ("sumsq", C.c_void_p)]
+
+
+class fm_batch_stats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("paths", C.c_int64), ("rays", C.c_int64), ("passes", C.c_int32),
+                ("units", C.c_int32)]
+
+
 class fm_ray(C.Structure):
     _fields_ = [("ori", F3), ("t_min", C.c_float), ("dir", F3), ("t_max", C.c_float)]
 
@@ -154,6 +166,8 @@ EXPORTS = {
     "fm_render": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(fm_render_params), C.POINTER(fm_film),
                             C.POINTER(fm_render_stats)]),
     "fm_film_read": (C.c_int, [C.c_void_p, C.POINTER(fm_film)]),
+    "fm_render_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(fm_job), C.c_int, C.POINTER(fm_render_params),
+                                  C.POINTER(C.c_double), C.POINTER(fm_batch_stats)]),
     "fm_set_counting": (C.c_int, [C.c_void_p, C.c_int]),
     "fm_trace": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p]),
     "fm_camera_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),

@@ -117,7 +117,10 @@ struct fm_ctx {
     int counting = 0;
     std::set<fm_scene*> scenes;  /* live scenes, released by fm_shutdown */
     WfBuffers wf;
+    DevBuf wf_passes, wf_units, wf_cams, wf_pass_idx;   /* wavefront tables of the last run */
+    DevBuf batch_film;                                  /* FP64 sums of fm_render_batch jobs */
     std::vector<hipEvent_t> trace_ev;  /* pairs around wavefront trace launches */
+    std::vector<hipEvent_t> pass_ev;   /* end of each wavefront pass */
 };
 
 struct fm_scene {
@@ -214,6 +217,177 @@ void run_hook(fm_ctx* ctx, const In* in, int64_t n_in, Out* out, int64_t n_out, 
     HIPCHECK(launch(din.as<In>(), dout.as<Out>()));
     HIPCHECK(hipMemcpyAsync(out, dout.p, (size_t)n_out * sizeof(Out), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
+}
+
+}  // namespace
+
+namespace {
+
+/* One render of a wavefront run: camera, resolution, samples, depth, seed
+ * and the device film (12 W H doubles) its sums go to. */
+struct WfJob {
+    fm_dcamera cam;
+    int W, H, spp, depth, sample_offset;
+    uint64_t seed;
+    double* film;
+};
+
+struct WfRunOut {
+    double kernel_ms = 0.0, trace_ms = 0.0;
+    int passes = 0, units = 0;
+    std::vector<double> job_done_ms;
+};
+
+hipEvent_t* event_pool(std::vector<hipEvent_t>& pool, size_t n) {
+    while (pool.size() < n) {
+        hipEvent_t e;
+        HIPCHECK(hipEventCreate(&e));
+        pool.push_back(e);
+    }
+    return pool.data();
+}
+
+/* Wavefront run: packs the jobs' samples into passes of about `budget`
+ * paths (jobs in order, at most one unit per job per pass, `chunk` > 0 caps
+ * the samples of a unit), uploads the pass / unit / camera tables and runs
+ * every pass, directly or (FM_RENDER_GRAPH) as one captured pass graph
+ * replayed per pass. */
+void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int rr_depth, float rr_coef, int block,
+            int grid, int chunk, int flags, WfRunOut& R) {
+    const int NL = s->d.num_lights;
+    const int64_t per_path = 100 + 64 * (1 + (int64_t)D) + 48 * (int64_t)D + 57 * (int64_t)D * std::max(NL, 1);
+    const int64_t budget = std::max<int64_t>(1, std::min<int64_t>((int64_t)1 << 26, (int64_t)24e9 / per_path));
+    std::vector<fm::WfUnit> units;
+    std::vector<fm::WfPass> passes;
+    std::vector<int> last_pass(jobs.size(), -1);
+    std::vector<int64_t> remaining(jobs.size()), next(jobs.size());
+    int max_depth = 1;
+    for (size_t j = 0; j < jobs.size(); j++) {
+        remaining[j] = jobs[j].spp;
+        next[j] = jobs[j].sample_offset;
+        max_depth = std::max(max_depth, jobs[j].depth);
+    }
+    int64_t cap = 0, max_pix = 0;
+    for (;;) {
+        fm::WfPass ps{};
+        ps.unit0 = (int)units.size();
+        int64_t used = 0, pix = 0;
+        for (size_t j = 0; j < jobs.size(); j++) {
+            if (remaining[j] <= 0) continue;
+            const int64_t npix = (int64_t)jobs[j].W * jobs[j].H;
+            int64_t k = std::min<int64_t>(remaining[j], chunk > 0 ? (int64_t)chunk : std::max<int64_t>(1, budget / npix));
+            if (ps.n_units > 0) {
+                const int64_t fit = (budget - used) / npix;
+                if (fit <= 0) continue;
+                k = std::min(k, fit);
+            }
+            fm::WfUnit u{};
+            u.path_base = used; u.pix_base = pix; u.npix = npix;
+            u.W = jobs[j].W; u.H = jobs[j].H; u.k = (int)k; u.sample_base = (int)next[j];
+            u.depth = jobs[j].depth; u.cam = (int)j; u.seed = jobs[j].seed; u.film = jobs[j].film;
+            units.push_back(u);
+            ps.n_units++;
+            used += k * npix; pix += npix;
+            remaining[j] -= k; next[j] += k;
+            if (remaining[j] == 0) last_pass[j] = (int)passes.size();
+            if (used >= budget) break;
+        }
+        if (ps.n_units == 0) break;
+        ps.P = used; ps.total_pix = pix;
+        passes.push_back(ps);
+        cap = std::max(cap, used); max_pix = std::max(max_pix, pix);
+    }
+    if (passes.empty()) return;
+    if (cap * (1 + D) >= ((int64_t)1 << 31) || cap * D * std::max(NL, 1) >= ((int64_t)1 << 31) ||
+        units.size() >= ((size_t)1 << 23))
+        throw std::invalid_argument("wavefront pass too large: lower chunk (samples per pass)");
+    ctx->wf.ensure(cap, D, NL, max_depth);
+    std::vector<fm_dcamera> cams(jobs.size());
+    for (size_t j = 0; j < jobs.size(); j++) cams[j] = jobs[j].cam;
+    hipStream_t st = ctx->stream;
+    ctx->wf_passes.upload(passes, st);
+    ctx->wf_units.upload(units, st);
+    ctx->wf_cams.upload(cams, st);
+    if (!ctx->wf_pass_idx.p) ctx->wf_pass_idx.alloc(64);
+
+    fm::WfArgs W_{};
+    W_.D = D; W_.NL = NL; W_.rr_depth = rr_depth; W_.rr_coef = rr_coef;
+    W_.n_iter = max_depth + 1;
+    W_.P = cap;
+    W_.passes = ctx->wf_passes.as<fm::WfPass>(); W_.units = ctx->wf_units.as<fm::WfUnit>();
+    W_.cams = ctx->wf_cams.as<fm_dcamera>(); W_.pass_idx = ctx->wf_pass_idx.as<int>();
+    WfBuffers& B = ctx->wf;
+    W_.ray_o = B.ray_o.as<float>(); W_.ray_d = B.ray_d.as<float>();
+    W_.hit_a = B.hit_a.as<float>(); W_.hit_b = B.hit_b.as<float>();
+    W_.st_beta = B.st_beta.as<float>(); W_.st_color = B.st_color.as<float>(); W_.st_flags = B.st_flags.as<int>();
+    W_.out_alb = B.out_alb.as<float>(); W_.out_nrm = B.out_nrm.as<float>();
+    W_.mis_f = B.mis_f.as<float>(); W_.mis_beta = B.mis_beta.as<float>(); W_.mis_ng = B.mis_ng.as<float>();
+    W_.sh_o = B.sh_o.as<float>(); W_.sh_d = B.sh_d.as<float>(); W_.sh_c = B.sh_c.as<float>();
+    W_.sh_vis = B.sh_vis.as<unsigned char>();
+    for (int b = 0; b < 2; b++) {
+        W_.q_ext[b] = B.q_ext[b].as<int>(); W_.q_sh[b] = B.q_sh[b].as<int>(); W_.q_cls[b] = B.q_cls[b].as<int>();
+    }
+    W_.counts = B.counts.as<unsigned>();
+    W_.counters = ctx->counters.as<unsigned long long>();
+    fm::WfLaunch L{};
+    L.max_paths = cap; L.max_pix = max_pix;
+    L.block_trace = block; L.block_shade = block;
+    L.grid_trace = grid; L.grid_shade = grid;
+    for (const fm_dmaterial& m : s->hs.materials)
+        if (m.type >= 0 && m.type + 1 < fm::WQ_NCLS) L.has_class[m.type + 1] = true;
+    fm::wavefront_grids(ctx->num_cus, std::max(1, s->hs.trav_depth), L);
+
+    const bool count = ctx->counting != 0;
+    const bool timed = (flags & FM_RENDER_TIME_TRACE) != 0;
+    const bool graph = (flags & FM_RENDER_GRAPH) != 0 && !timed;
+    const size_t npass = passes.size();
+    hipEvent_t* tev = timed ? event_pool(ctx->trace_ev, npass * 2 * (size_t)W_.n_iter) : nullptr;
+    hipEvent_t* pev = event_pool(ctx->pass_ev, npass);
+    HIPCHECK(fm::launch_set_pass(W_.pass_idx, 0, st));
+    hipGraphExec_t exec = nullptr;
+    if (graph) {
+        hipGraph_t g = nullptr;
+        HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const hipError_t e1 = fm::launch_wavefront_pass(s->d, W_, L, count, st, nullptr);
+        const hipError_t e2 = hipStreamEndCapture(st, &g);
+        if (e1 != hipSuccess || e2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            throw HipError(std::string("graph capture: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+        }
+        const hipError_t e3 = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e3 != hipSuccess) throw HipError(std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
+    }
+    HIPCHECK(hipEventRecord(ctx->ev0, st));
+    hipError_t err = hipSuccess;
+    for (size_t c = 0; c < npass && err == hipSuccess; c++) {
+        if (graph) err = hipGraphLaunch(exec, st);
+        else err = fm::launch_wavefront_pass(s->d, W_, L, count, st, tev ? tev + c * 2 * W_.n_iter : nullptr);
+        if (err == hipSuccess) err = hipEventRecord(pev[c], st);
+    }
+    if (err == hipSuccess) err = hipEventRecord(ctx->ev1, st);
+    const hipError_t es = hipStreamSynchronize(st);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (err != hipSuccess) throw HipError(std::string("wavefront launch: ") + hipGetErrorString(err));
+    if (es != hipSuccess) throw HipError(std::string("wavefront run: ") + hipGetErrorString(es));
+    float ms = 0.0f;
+    HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    R.kernel_ms = ms;
+    if (tev)
+        for (size_t e = 0; e < npass * 2 * (size_t)W_.n_iter; e += 2) {
+            float t = 0.0f;
+            HIPCHECK(hipEventElapsedTime(&t, tev[e], tev[e + 1]));
+            R.trace_ms += t;
+        }
+    R.passes = (int)npass;
+    R.units = (int)units.size();
+    R.job_done_ms.assign(jobs.size(), 0.0);
+    for (size_t j = 0; j < jobs.size(); j++)
+        if (last_pass[j] >= 0) {
+            float t = 0.0f;
+            HIPCHECK(hipEventElapsedTime(&t, ctx->ev0, pev[last_pass[j]]));
+            R.job_done_ms[j] = t;
+        }
 }
 
 }  // namespace
@@ -341,17 +515,9 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
         const int64_t npix = (int64_t)W * H;
         const bool wave = p->mode == FM_MODE_WAVEFRONT;
         double trace_ms = 0.0;
-        /* megakernel: samples per work item.  wavefront: samples of every
-         * pixel per pass; by default about 64 M paths in flight, within a
-         * 24 GB budget for path state, rays, hit records and queues (about
-         * 330 B per path with one light and one direct-light pass). */
-        if (p->chunk > 0) A.chunk = std::min(p->chunk, spp);
-        else if (wave) {
-            const int64_t D = std::max(1, A.cfg_raw.direct), NL = std::max(1, s->d.num_lights);
-            const int64_t per_path = 100 + 64 * (1 + D) + 48 * D + 57 * D * NL;
-            const int64_t target = std::min<int64_t>((int64_t)1 << 26, (int64_t)24e9 / per_path);
-            A.chunk = (int)std::max<int64_t>(1, std::min<int64_t>(spp, target / npix));
-        } else A.chunk = std::min(spp, 16);
+        /* megakernel: samples per work item (the wavefront sizes its passes
+         * from a memory budget unless chunk caps the samples per pass) */
+        A.chunk = p->chunk > 0 ? std::min(p->chunk, spp) : std::min(spp, 16);
         const int64_t nchunks = (spp + A.chunk - 1) / A.chunk;
         A.total_items = npix * nchunks;
         /* film */
@@ -385,60 +551,14 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
             HIPCHECK(fm::launch_render_mega(s->d, A, block, grid, ctx->counting != 0, ctx->stream));
             HIPCHECK(hipEventRecord(ctx->ev1, ctx->stream));
         } else {
-            const int D = A.cfg_raw.direct, NL = s->d.num_lights;
-            const int64_t Pmax = npix * A.chunk;
-            if (Pmax * (1 + D) >= ((int64_t)1 << 31) || Pmax * D * std::max(NL, 1) >= ((int64_t)1 << 31))
-                return fail(ctx, FM_ERR_INVALID, "wavefront pass too large: lower chunk (samples per pass)");
-            const int depth_it = std::max(A.cfg_raw.depth, 1);
-            ctx->wf.ensure(Pmax, D, NL, depth_it);
-            fm::WfArgs W_{};
-            W_.W = W; W_.H = H; W_.D = D; W_.NL = NL;
-            W_.depth = A.cfg_raw.depth; W_.rr_depth = A.cfg_raw.rr_depth; W_.rr_coef = A.cfg_raw.rr_coef;
-            W_.seed = A.seed;
-            W_.n_iter = depth_it + 1;
-            WfBuffers& B = ctx->wf;
-            W_.ray_o = B.ray_o.as<float>(); W_.ray_d = B.ray_d.as<float>();
-            W_.hit_a = B.hit_a.as<float>(); W_.hit_b = B.hit_b.as<float>();
-            W_.st_beta = B.st_beta.as<float>(); W_.st_color = B.st_color.as<float>(); W_.st_flags = B.st_flags.as<int>();
-            W_.out_alb = B.out_alb.as<float>(); W_.out_nrm = B.out_nrm.as<float>();
-            W_.mis_f = B.mis_f.as<float>(); W_.mis_beta = B.mis_beta.as<float>(); W_.mis_ng = B.mis_ng.as<float>();
-            W_.sh_o = B.sh_o.as<float>(); W_.sh_d = B.sh_d.as<float>(); W_.sh_c = B.sh_c.as<float>();
-            W_.sh_vis = B.sh_vis.as<unsigned char>();
-            for (int b = 0; b < 2; b++) {
-                W_.q_ext[b] = B.q_ext[b].as<int>(); W_.q_sh[b] = B.q_sh[b].as<int>(); W_.q_cls[b] = B.q_cls[b].as<int>();
-            }
-            W_.counts = B.counts.as<unsigned>();
-            W_.color_sum = A.color_sum; W_.albedo_sum = A.albedo_sum; W_.normal_sum = A.normal_sum; W_.sumsq = A.sumsq;
-            W_.counters = A.counters;
-            fm::WfLaunch L{};
-            L.block_trace = block; L.block_shade = block;
-            L.grid_trace = p->grid_blocks; L.grid_shade = p->grid_blocks;
-            for (const fm_dmaterial& m : s->hs.materials)
-                if (m.type >= 0 && m.type + 1 < fm::WQ_NCLS) L.has_class[m.type + 1] = true;
-            fm::wavefront_grids(ctx->num_cus, std::max(1, s->hs.trav_depth), L);
-            const bool timed = (p->flags & FM_RENDER_TIME_TRACE) != 0;
-            const size_t nev = timed ? (size_t)nchunks * 2 * W_.n_iter : 0;
-            while (ctx->trace_ev.size() < nev) {
-                hipEvent_t e;
-                HIPCHECK(hipEventCreate(&e));
-                ctx->trace_ev.push_back(e);
-            }
-            HIPCHECK(hipEventRecord(ctx->ev0, ctx->stream));
-            for (int64_t c = 0; c < nchunks; c++) {
-                const int kp = (int)std::min<int64_t>(A.chunk, spp - c * A.chunk);
-                W_.k = kp;
-                W_.P = npix * kp;
-                W_.sample_base = A.sample_offset + (int)(c * A.chunk);
-                hipEvent_t* ev = timed ? &ctx->trace_ev[(size_t)c * 2 * W_.n_iter] : nullptr;
-                HIPCHECK(fm::launch_wavefront_pass(s->d, W_, L, ctx->counting != 0, ctx->stream, ev));
-            }
-            HIPCHECK(hipEventRecord(ctx->ev1, ctx->stream));
-            HIPCHECK(hipStreamSynchronize(ctx->stream));
-            for (size_t e = 0; e + 1 < nev; e += 2) {
-                float ms = 0.0f;
-                HIPCHECK(hipEventElapsedTime(&ms, ctx->trace_ev[e], ctx->trace_ev[e + 1]));
-                trace_ms += ms;
-            }
+            WfJob job{};
+            job.cam = s->d.camera;
+            job.W = W; job.H = H; job.spp = spp; job.depth = std::max(A.cfg_raw.depth, 1);
+            job.sample_offset = A.sample_offset; job.seed = A.seed; job.film = base;
+            WfRunOut R;
+            wf_run(ctx, s, {job}, A.cfg_raw.direct, A.cfg_raw.rr_depth, A.cfg_raw.rr_coef, block, p->grid_blocks,
+                   p->chunk > 0 ? std::min(p->chunk, spp) : 0, p->flags, R);
+            trace_ms = R.trace_ms;
         }
         HIPCHECK(hipStreamSynchronize(ctx->stream));
         ctx->film_spp += spp;
@@ -476,6 +596,76 @@ int fm_film_read(fm_ctx* ctx, fm_film* film) {
             if (film->normal) film->normal[k] = (float)(h[(size_t)(2 * n + k)] * inv);
             if (film->color_sum) film->color_sum[k] = h[(size_t)k];
             if (film->sumsq) film->sumsq[k] = h[(size_t)(3 * n + k)];
+        }
+        return FM_OK;
+    });
+}
+
+int fm_render_batch(fm_ctx* ctx, fm_scene* s, const fm_job* jobs, int n, const fm_render_params* common,
+                    double* job_done_ms, fm_batch_stats* stats) {
+    if (!ctx || !s || n < 0 || (n > 0 && !jobs)) return fail(ctx, FM_ERR_INVALID, "bad argument");
+    return guarded(ctx, [&]() -> int {
+        const fm_renderer_desc& rd = s->hs.renderer;
+        fm_render_params cp{};
+        if (common) cp = *common;
+        const int D = cp.direct_loop > 0 ? cp.direct_loop : rd.direct_loop;
+        const int rr_depth = cp.rr_depth >= 0 ? cp.rr_depth : rd.rr_depth;
+        const float rr_coef = cp.rr_coef > 0 ? cp.rr_coef : rd.rr_coef;
+        const int block = cp.block_size > 0 ? cp.block_size : 256;
+        if (D <= 0) return fail(ctx, FM_ERR_INVALID, "direct_loop must be positive");
+        if (block % 64 != 0 || block > 256) return fail(ctx, FM_ERR_INVALID, "block_size must be a multiple of 64, at most 256");
+        HIPCHECK(hipSetDevice(ctx->device));
+        std::vector<WfJob> wj((size_t)n);
+        std::vector<int64_t> off((size_t)n + 1, 0);
+        for (int i = 0; i < n; i++) {
+            const fm_job& J = jobs[i];
+            WfJob& w = wj[(size_t)i];
+            w.W = J.width > 0 ? J.width : rd.width;
+            w.H = J.height > 0 ? J.height : rd.height;
+            w.spp = J.spp > 0 ? J.spp : rd.spp;
+            w.depth = J.depth > 0 ? J.depth : rd.depth;
+            if (w.W <= 0 || w.H <= 0 || w.spp <= 0) return fail(ctx, FM_ERR_INVALID, "job " + std::to_string(i) + ": width, height and spp must be positive");
+            if ((int64_t)w.W * w.H > ((int64_t)1 << 30)) return fail(ctx, FM_ERR_INVALID, "job image too large");
+            w.sample_offset = J.sample_offset;
+            w.seed = J.seed;
+            if (J.scene_camera) w.cam = s->d.camera;
+            else fmh::make_camera(J.camera, w.cam);
+            off[(size_t)i + 1] = off[(size_t)i] + (int64_t)w.W * w.H * 12;
+        }
+        if (n == 0) return FM_OK;
+        ctx->batch_film.alloc((size_t)off[(size_t)n] * sizeof(double));
+        double* film = ctx->batch_film.as<double>();
+        for (int i = 0; i < n; i++) wj[(size_t)i].film = film + off[(size_t)i];
+        HIPCHECK(fm::launch_zero_f64(film, off[(size_t)n], ctx->stream));
+        HIPCHECK(fm::launch_zero_f64((double*)ctx->counters.p, 4, ctx->stream));
+        WfRunOut R;
+        wf_run(ctx, s, wj, D, rr_depth, rr_coef, block, cp.grid_blocks, cp.chunk, cp.flags, R);
+        if (job_done_ms) for (int i = 0; i < n; i++) job_done_ms[i] = R.job_done_ms[(size_t)i];
+        if (stats) {
+            memset(stats, 0, sizeof(*stats));
+            unsigned long long c[4];
+            HIPCHECK(hipMemcpy(c, ctx->counters.p, sizeof c, hipMemcpyDeviceToHost));
+            stats->kernel_ms = R.kernel_ms;
+            stats->paths = (int64_t)c[0];
+            stats->rays = (int64_t)c[1];
+            stats->passes = R.passes;
+            stats->units = R.units;
+        }
+        /* copy back what the jobs asked for: means over spp, row 0 = top */
+        std::vector<double> h;
+        for (int i = 0; i < n; i++) {
+            const fm_job& J = jobs[i];
+            if (!J.color && !J.albedo && !J.normal && !J.sumsq) continue;
+            const int64_t m = (int64_t)wj[(size_t)i].W * wj[(size_t)i].H * 3;
+            h.resize((size_t)m * 4);
+            HIPCHECK(hipMemcpy(h.data(), film + off[(size_t)i], (size_t)m * 4 * sizeof(double), hipMemcpyDeviceToHost));
+            const double inv = 1.0 / (double)wj[(size_t)i].spp;
+            for (int64_t k = 0; k < m; k++) {
+                if (J.color) J.color[k] = (float)(h[(size_t)k] * inv);
+                if (J.albedo) J.albedo[k] = (float)(h[(size_t)(m + k)] * inv);
+                if (J.normal) J.normal[k] = (float)(h[(size_t)(2 * m + k)] * inv);
+                if (J.sumsq) J.sumsq[k] = h[(size_t)(3 * m + k)];
+            }
         }
         return FM_OK;
     });

@@ -335,6 +335,32 @@ struct SahBuilder {
 
 }  // namespace
 
+/* PinHoleCamera / ThinLensCamera constructors (pin_hole_camera.cpp:5-16,
+ * thin_lens_camera.cpp:14-21): camera_to_world = inverse(lookAtLH), film
+ * size from the vertical fov and the JSON aspect. */
+void make_camera(const fm_camera_desc& c, fm_dcamera& dc) {
+    const F3 eye = f3(c.pos[0], c.pos[1], c.pos[2]);
+    const F3 at = f3(c.look_at[0], c.look_at[1], c.look_at[2]);
+    const F3 up = nrm(f3(c.up[0], c.up[1], c.up[2]));
+    const F3 fz = nrm(sub(at, eye));
+    const F3 sx = nrm(cross(up, fz));
+    const F3 uy = cross(fz, sx);
+    M4 look = M4::I();  /* lookAt under GLM_FORCE_LEFT_HANDED */
+    look.m[0][0] = sx.x; look.m[1][0] = sx.y; look.m[2][0] = sx.z;
+    look.m[0][1] = uy.x; look.m[1][1] = uy.y; look.m[2][1] = uy.z;
+    look.m[0][2] = fz.x; look.m[1][2] = fz.y; look.m[2][2] = fz.z;
+    look.m[3][0] = -dot(sx, eye); look.m[3][1] = -dot(uy, eye); look.m[3][2] = -dot(fz, eye);
+    const M4 c2w = minv(look);
+    dc = fm_dcamera{};
+    dc.type = c.type;
+    dc.pos[0] = eye.x; dc.pos[1] = eye.y; dc.pos[2] = eye.z;
+    dc.fd = c.focal_distance;
+    dc.fh = 2.0f * dc.fd * std::tan(deg2rad(c.fov) / 2.0f);
+    dc.fw = c.aspect * dc.fh;
+    dc.lens = c.lens_radius;
+    memcpy(dc.c2w, c2w.m, sizeof dc.c2w);
+}
+
 void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
     if (d.abi_version != FM_ABI_VERSION) throw BuildError(FM_ERR_VERSION, "fm_scene_desc.abi_version mismatch");
     if (d.num_images < 0 || d.num_materials < 0 || d.num_meshes < 0 || d.num_objects < 0 || d.num_lights < 0)
@@ -743,27 +769,7 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
     if (hs.env_tables.empty()) hs.env_tables.resize(16);
 
     /* ---- camera (pin_hole_camera.cpp:5-16) ---- */
-    const fm_camera_desc& c = d.camera;
-    const F3 eye = f3(c.pos[0], c.pos[1], c.pos[2]);
-    const F3 at = f3(c.look_at[0], c.look_at[1], c.look_at[2]);
-    const F3 up = nrm(f3(c.up[0], c.up[1], c.up[2]));
-    const F3 fz = nrm(sub(at, eye));
-    const F3 sx = nrm(cross(up, fz));
-    const F3 uy = cross(fz, sx);
-    M4 look = M4::I();  /* lookAt under GLM_FORCE_LEFT_HANDED */
-    look.m[0][0] = sx.x; look.m[1][0] = sx.y; look.m[2][0] = sx.z;
-    look.m[0][1] = uy.x; look.m[1][1] = uy.y; look.m[2][1] = uy.z;
-    look.m[0][2] = fz.x; look.m[1][2] = fz.y; look.m[2][2] = fz.z;
-    look.m[3][0] = -dot(sx, eye); look.m[3][1] = -dot(uy, eye); look.m[3][2] = -dot(fz, eye);
-    const M4 c2w = minv(look);
-    fm_dcamera& dc = hs.camera;
-    dc.type = c.type;
-    dc.pos[0] = eye.x; dc.pos[1] = eye.y; dc.pos[2] = eye.z;
-    dc.fd = c.focal_distance;
-    dc.fh = 2.0f * dc.fd * std::tan(deg2rad(c.fov) / 2.0f);
-    dc.fw = c.aspect * dc.fh;
-    dc.lens = c.lens_radius;
-    memcpy(dc.c2w, c2w.m, sizeof dc.c2w);
+    make_camera(d.camera, hs.camera);
 }
 
 }  // namespace fmh

@@ -53,6 +53,9 @@ struct HostScene {
     fm_camera_desc camera_desc{};
 };
 
+/* Device camera record for a camera description (per-job cameras). */
+void make_camera(const fm_camera_desc& c, fm_dcamera& dc);
+
 /* Throws BuildError.  only_bvh stops after the reference BVH (host tests). */
 void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh = false);
 

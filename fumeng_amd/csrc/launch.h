@@ -29,25 +29,43 @@ enum { WQ_EXT = 0, WQ_SH = 1, WQ_CLS = 2, WQ_NCLS = 4, WQ_N = WQ_CLS + WQ_NCLS }
  * camera, or only deferred direct light left), 1 + material type */
 enum { WC_END = 0, WC_LAMBERT = 1, WC_DISNEY = 2, WC_SPEC = 3 };
 
-/* Device buffers and parameters of one pass of the wavefront tracer: P paths
- * (k samples of every pixel, path p = j * W * H + pixel), D direct-light
- * passes per vertex, NL lights.  Ray slot p is path p's continuation ray,
- * ray slot P + p * D + pass its MisBSDF ray; shadow record (p * D + pass) *
- * NL + light. */
-struct WfArgs {
-    int W, H, k, D, NL, depth, rr_depth, iter;
-    int n_iter;        /* bounce iterations per pass: max(depth, 1) + 1 */
-    float rr_coef;
-    int sample_base;
-    int64_t P;
+/* A unit of a pass: k samples of every pixel of one job (paths
+ * path_base + j * npix + pixel, j < k; samples sample_base + j).  `film`
+ * holds the job's FP64 sums: colour at film, albedo at film + 3 npix,
+ * normal at film + 6 npix, sum of squares at film + 9 npix. */
+struct WfUnit {
+    int64_t path_base, pix_base, npix;
+    int W, H, k, sample_base, depth, cam;
     uint64_t seed;
+    double* film;
+};
+/* A pass: P paths over units [unit0, unit0 + n_units), total_pix pixels. */
+struct WfPass {
+    int64_t P, total_pix;
+    int unit0, n_units;
+};
+
+/* Device buffers and parameters of the wavefront tracer.  Buffers are sized
+ * for cap paths; ray slot p < cap is path p's continuation ray, ray slot
+ * cap + p * D + pass its MisBSDF ray; shadow record (p * D + pass) * NL +
+ * light.  The pass being run is passes[*pass_idx]; every kernel reads it
+ * there, so one kernel sequence (and one captured graph) serves every pass. */
+struct WfArgs {
+    int D, NL, rr_depth, iter;
+    int n_iter;        /* bounce iterations per pass: max(depth, 1) + 1 over the batch */
+    float rr_coef;
+    int64_t P;         /* capacity in paths */
+    const WfPass* passes;
+    const WfUnit* units;
+    const fm_dcamera* cams;
+    int* pass_idx;
     float* ray_o;      /* float4 [P (1 + D)]: origin, t_min */
     float* ray_d;      /* float4 [P (1 + D)]: direction, t_max */
     float* hit_a;      /* float4 [P (1 + D)]: t, slot, kind | half << 8, b0 */
     float* hit_b;      /* float2 [P (1 + D)]: b1, b2 */
     float* st_beta;    /* float4 [P]: throughput, depth */
     float* st_color;   /* float4 [P]: radiance, next RNG dimension */
-    int* st_flags;     /* [P] */
+    int* st_flags;     /* [P]: F_* bits | unit index << 8 */
     float* out_alb;    /* float4 [P] */
     float* out_nrm;    /* float4 [P] */
     float* mis_f;      /* float4 [P D]: BSDF value, pdf */
@@ -61,26 +79,26 @@ struct WfArgs {
     int* q_sh[2];      /* shadow queues */
     int* q_cls[2];     /* WQ_NCLS segments of P path ids */
     unsigned* counts;  /* [(n_iter + 1) * WQ_N] */
-    double* color_sum;
-    double* albedo_sum;
-    double* normal_sum;
-    double* sumsq;
     unsigned long long* counters;
 };
 
 struct WfLaunch {
-    int block_trace, grid_trace;   /* extend / shadow */
+    int64_t max_paths, max_pix;    /* largest pass (grid sizes of camera / film) */
+    int block_trace, grid_trace;   /* trace kernel */
     int block_shade, grid_shade;   /* grid_shade > 0 overrides every class */
     int grid_class[WQ_NCLS];       /* persistent grid per shading class (its own occupancy) */
     int stack_levels;
     bool has_class[WQ_NCLS];
 };
 
-/* Kernel sequence of one pass (camera, n_iter bounce iterations, film).  If
- * ev is given, events ev[2 it] / ev[2 it + 1] bracket iteration it's trace
- * kernel. */
+/* Kernel sequence of one pass (camera, n_iter bounce iterations, film, pass
+ * counter + 1), grid sizes from L.  If ev is given,
+ * events ev[2 it] / ev[2 it + 1] bracket iteration it's trace kernel (not
+ * under graph capture). */
 hipError_t launch_wavefront_pass(const fm_dscene& S, WfArgs A, const WfLaunch& L, bool count, hipStream_t st,
                                  hipEvent_t* ev = nullptr);
+/* *pass_idx = v (a kernel, so it can precede a graph launch). */
+hipError_t launch_set_pass(int* pass_idx, int v, hipStream_t st);
 /* Persistent grid sizes for the wavefront kernels on this device. */
 void wavefront_grids(int num_cus, int stack_levels, WfLaunch& L);
 

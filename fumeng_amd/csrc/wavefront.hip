@@ -16,7 +16,13 @@
  *                  shades the new vertex: light samples -> shadow queue,
  *                  MisBSDF sample -> ray queue, continuation + RR -> ray queue
  *   k_wf_film      sums the k samples of each pixel in sample order (FP32) and
- *                  adds them to the FP64 film
+ *                  adds them to the job's FP64 film
+ *
+ * A pass holds one or more units (k samples of every pixel of one job, a
+ * job being a camera, resolution, spp, depth and seed over the scene); the
+ * pass and unit tables live on the device and the current pass index too,
+ * so the same kernel sequence runs every pass, directly or as a replayed
+ * HIP graph.
  *
  * Queues are compacted with one atomic per wave (ballot + mbcnt).  Queue
  * lengths live on the device; every kernel is a persistent grid-stride loop
@@ -96,28 +102,50 @@ FMD unsigned* cnt(const WfArgs& A, int iter, int q) { return A.counts + (size_t)
 
 /* ---- camera ------------------------------------------------------------- */
 
+/* Index of the unit of the current pass that holds path p (units are in
+ * path order). */
+FMD int find_unit_path(const WfArgs& A, const fm::WfPass& ps, int64_t p) {
+    int lo = ps.unit0, hi = ps.unit0 + ps.n_units;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (A.units[mid].path_base <= p) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+FMD int find_unit_pix(const WfArgs& A, const fm::WfPass& ps, int64_t q) {
+    int lo = ps.unit0, hi = ps.unit0 + ps.n_units;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (A.units[mid].pix_base <= q) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
 __global__ void __launch_bounds__(256) k_wf_camera(fm_dscene S, WfArgs A) {
-    const int64_t npix = (int64_t)A.W * A.H;
+    const fm::WfPass ps = A.passes[*A.pass_idx];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         for (int k = 0; k < (A.n_iter + 1) * fm::WQ_N; k++) A.counts[k] = 0u;
-        A.counts[fm::WQ_EXT] = (unsigned)A.P;
-        atomicAdd(A.counters + 0, (unsigned long long)A.P);
+        A.counts[fm::WQ_EXT] = (unsigned)ps.P;
+        atomicAdd(A.counters + 0, (unsigned long long)ps.P);
     }
-    for (int64_t p = grid_id(); p < A.P; p += grid_stride()) {
-        const int64_t j = p / npix;
-        const int64_t pix = p - j * npix;
-        const int row = (int)(pix / A.W), i = (int)(pix - (int64_t)row * A.W);
-        const int jj = A.H - 1 - row;
+    for (int64_t p = grid_id(); p < ps.P; p += grid_stride()) {
+        const int ui = find_unit_path(A, ps, p);
+        const fm::WfUnit& U = A.units[ui];
+        const int64_t rel = p - U.path_base;
+        const int64_t j = rel / U.npix;
+        const int64_t pix = rel - j * U.npix;
+        const int row = (int)(pix / U.W), i = (int)(pix - (int64_t)row * U.W);
+        const int jj = U.H - 1 - row;
         Rng rng;
-        rng.init(A.seed, (uint32_t)pix, (uint32_t)(A.sample_base + j), 0u);
+        rng.init(U.seed, (uint32_t)pix, (uint32_t)(U.sample_base + j), 0u);
         const float fx = rng.next(), fy = rng.next();
         const float lx = rng.next(), ly = rng.next();
-        const float px = ((float)i + fx) / (float)A.W, py = ((float)jj + fy) / (float)A.H;
-        const Ray cr = camera_ray(S.camera, px, py, lx, ly);
+        const float px = ((float)i + fx) / (float)U.W, py = ((float)jj + fy) / (float)U.H;
+        const Ray cr = camera_ray(A.cams[U.cam], px, py, lx, ly);
         store_ray(A, p, cr);
         reinterpret_cast<float4*>(A.st_beta)[p] = make_float4(1.0f, 1.0f, 1.0f, __int_as_float(1));
         reinterpret_cast<float4*>(A.st_color)[p] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(rng.dim));
-        A.st_flags[p] = F_CONT;
+        A.st_flags[p] = F_CONT | (ui << 8);
         reinterpret_cast<float4*>(A.out_alb)[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         reinterpret_cast<float4*>(A.out_nrm)[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         A.q_ext[0][p] = (int)p;
@@ -236,7 +264,6 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
     const int it = A.iter, nx = it + 1;
     const unsigned n = *cnt(A, it, fm::WQ_CLS + C);
     const int* q = A.q_cls[it & 1] + (size_t)C * A.P;
-    const int64_t npix = (int64_t)A.W * A.H;
     for (int64_t i = grid_id(); i < (int64_t)n; i += grid_stride()) {
         const int p = q[i];
         const float4 b4 = reinterpret_cast<const float4*>(A.st_beta)[p];
@@ -268,9 +295,11 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
                 reinterpret_cast<float4*>(A.out_alb)[p] = f4(b.albedo(), 0.0f);
                 reinterpret_cast<float4*>(A.out_nrm)[p] = f4(sf.ng, 0.0f);
             }
-            const int64_t j = p / npix;
+            const fm::WfUnit& U = A.units[fl >> 8];
+            const int64_t rel = p - U.path_base;
+            const int64_t j = rel / U.npix;
             Rng rng;
-            rng.init(A.seed, (uint32_t)(p - j * npix), (uint32_t)(A.sample_base + j), 0u);
+            rng.init(U.seed, (uint32_t)(rel - j * U.npix), (uint32_t)(U.sample_base + j), 0u);
             rng.seek(dim);
             for (int pass = 0; pass < A.D; pass++) {
                 for (int li = 0; li < A.NL; li++) {
@@ -318,13 +347,13 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
                 beta *= cs.f * absdot(sf.ng, cs.wi) / cs.pdf;
                 const Ray nr = gen_ray(sf.pos, sf.ng, cs.wi);
                 term = false;
-                if (A.depth > A.rr_depth) {
+                if (U.depth > A.rr_depth) {
                     if (rng.next() > A.rr_coef) term = true;
                     else beta = beta / A.rr_coef;
                 }
                 if (!term) {
                     depth++;
-                    if (depth > A.depth) term = true;
+                    if (depth > U.depth) term = true;
                 }
                 if (!term) store_ray(A, p, nr);
             }
@@ -334,34 +363,44 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
         wave_push(cnt(A, nx, fm::WQ_CLS + fm::WC_END), A.q_cls[nx & 1] + (size_t)fm::WC_END * A.P, term && any, p);
         reinterpret_cast<float4*>(A.st_beta)[p] = f4(beta, __int_as_float(depth));
         reinterpret_cast<float4*>(A.st_color)[p] = f4(color, __uint_as_float(dim_out));
-        A.st_flags[p] = (any ? F_PENDING : 0) | (term ? 0 : F_CONT);
+        A.st_flags[p] = (fl & ~0xff) | (any ? F_PENDING : 0) | (term ? 0 : F_CONT);
     }
 }
 
 /* ---- film ----------------------------------------------------------------- */
 
 __global__ void __launch_bounds__(256) k_wf_film(WfArgs A) {
-    const int64_t npix = (int64_t)A.W * A.H;
-    for (int64_t pix = grid_id(); pix < npix; pix += grid_stride()) {
+    const fm::WfPass ps = A.passes[*A.pass_idx];
+    for (int64_t q = grid_id(); q < ps.total_pix; q += grid_stride()) {
+        const fm::WfUnit& U = A.units[find_unit_pix(A, ps, q)];
+        const int64_t pix = q - U.pix_base;
         v3 cs = splat(0.0f), as = splat(0.0f), ns = splat(0.0f);
         float q0 = 0.0f, q1 = 0.0f, q2 = 0.0f;
-        for (int j = 0; j < A.k; j++) {
-            const int64_t p = (int64_t)j * npix + pix;
+        for (int j = 0; j < U.k; j++) {
+            const int64_t p = U.path_base + (int64_t)j * U.npix + pix;
             const v3 c = xyz(reinterpret_cast<const float4*>(A.st_color)[p]);
             cs += c;
             as += xyz(reinterpret_cast<const float4*>(A.out_alb)[p]);
             ns += xyz(reinterpret_cast<const float4*>(A.out_nrm)[p]);
             q0 += c.x * c.x; q1 += c.y * c.y; q2 += c.z * c.z;
         }
-        double* c = A.color_sum + 3 * pix;
-        double* a = A.albedo_sum + 3 * pix;
-        double* nn = A.normal_sum + 3 * pix;
-        double* q = A.sumsq + 3 * pix;
+        double* c = U.film + 3 * pix;
+        double* a = U.film + 3 * U.npix + 3 * pix;
+        double* nn = U.film + 6 * U.npix + 3 * pix;
+        double* qq = U.film + 9 * U.npix + 3 * pix;
         c[0] += cs.x; c[1] += cs.y; c[2] += cs.z;
         a[0] += as.x; a[1] += as.y; a[2] += as.z;
         nn[0] += ns.x; nn[1] += ns.y; nn[2] += ns.z;
-        q[0] += q0; q[1] += q1; q[2] += q2;
+        qq[0] += q0; qq[1] += q1; qq[2] += q2;
     }
+}
+
+/* Pass counter: next pass (last kernel of a pass) / set (before the first). */
+__global__ void k_wf_next_pass(int* pass_idx) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *pass_idx += 1;
+}
+__global__ void k_wf_set_pass(int* pass_idx, int v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *pass_idx = v;
 }
 
 template <typename K>
@@ -392,8 +431,7 @@ void wavefront_grids(int num_cus, int stack_levels, WfLaunch& L) {
 hipError_t launch_wavefront_pass(const fm_dscene& S, WfArgs A, const WfLaunch& L, bool count, hipStream_t st,
                                  hipEvent_t* ev) {
     const size_t lds = (size_t)L.stack_levels * L.block_trace * sizeof(uint32_t);
-    const int64_t npix = (int64_t)A.W * A.H;
-    const int cam_grid = (int)std::min<int64_t>((A.P + 255) / 256, (int64_t)L.grid_shade * 4);
+    const int cam_grid = (int)std::min<int64_t>((L.max_paths + 255) / 256, (int64_t)L.grid_shade * 4);
     hipLaunchKernelGGL(k_wf_camera, dim3(std::max(cam_grid, 1)), dim3(256), 0, st, S, A);
     for (int it = 0; it < A.n_iter; it++) {
         A.iter = it;
@@ -406,8 +444,14 @@ hipError_t launch_wavefront_pass(const fm_dscene& S, WfArgs A, const WfLaunch& L
         if (L.has_class[WC_DISNEY]) hipLaunchKernelGGL(k_wf_shade<WC_DISNEY>, dim3(L.grid_class[WC_DISNEY]), dim3(L.block_shade), 0, st, S, A);
         if (L.has_class[WC_SPEC]) hipLaunchKernelGGL(k_wf_shade<WC_SPEC>, dim3(L.grid_class[WC_SPEC]), dim3(L.block_shade), 0, st, S, A);
     }
-    const int film_grid = (int)std::min<int64_t>((npix + 255) / 256, (int64_t)L.grid_shade * 4);
+    const int film_grid = (int)std::min<int64_t>((L.max_pix + 255) / 256, (int64_t)L.grid_shade * 4);
     hipLaunchKernelGGL(k_wf_film, dim3(std::max(film_grid, 1)), dim3(256), 0, st, A);
+    hipLaunchKernelGGL(k_wf_next_pass, dim3(1), dim3(64), 0, st, A.pass_idx);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_pass(int* pass_idx, int v, hipStream_t st) {
+    hipLaunchKernelGGL(k_wf_set_pass, dim3(1), dim3(64), 0, st, pass_idx, v);
     return hipGetLastError();
 }
 

@@ -24,6 +24,16 @@ DISNEY_DEFAULTS = {k: 0.0 for k in A.DISNEY_PARAMS}
 DISNEY_DEFAULTS.update({"specular": 0.5, "ior": 1.5})  # core/material.h:36-76; basecolor black
 
 
+def camera_to_c(c, out):
+    """Fills an fm_camera_desc from a camera dict (the JSON camera block)."""
+    out.type = A.FM_CAMERA_THIN_LENS if c["type"] == "thin_lens" else A.FM_CAMERA_PIN_HOLE
+    for k in ("pos", "look_at", "up"):
+        getattr(out, k)[:] = [float(x) for x in c[k]]
+    out.fov, out.aspect = c["fov"], c["aspect"]
+    out.focal_distance, out.lens_radius = c["focal_distance"], c.get("lens_radius", 0.0)
+    return out
+
+
 class SceneDesc:
     """Objects are kept in reference order; a triangle mesh expands to one
     geometry per face (triangle.cpp:312-329)."""
@@ -135,12 +145,7 @@ class SceneDesc:
         d.abi_version = A.FM_ABI_VERSION
         d.accelerator = {"naive": A.FM_ACCEL_NAIVE, "reference_tree": A.FM_ACCEL_BVH_REFERENCE_TREE}.get(
             self.accelerator, A.FM_ACCEL_BVH)
-        c = self.camera
-        d.camera.type = A.FM_CAMERA_THIN_LENS if c["type"] == "thin_lens" else A.FM_CAMERA_PIN_HOLE
-        for k in ("pos", "look_at", "up"):
-            getattr(d.camera, k)[:] = [float(x) for x in c[k]]
-        d.camera.fov, d.camera.aspect = c["fov"], c["aspect"]
-        d.camera.focal_distance, d.camera.lens_radius = c["focal_distance"], c.get("lens_radius", 0.0)
+        camera_to_c(self.camera, d.camera)
         r = self.renderer
         for k in ("width", "height", "spp", "thread_count", "depth", "direct_loop", "rr_depth"):
             setattr(d.renderer, k, int(r[k]))

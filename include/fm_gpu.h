@@ -210,7 +210,11 @@ int fm_scene_get_info(const fm_scene* scene, fm_scene_info* out);
 
 enum { FM_MODE_MEGAKERNEL = 0, FM_MODE_WAVEFRONT = 1 };
 /* fm_render_params.flags */
-enum { FM_RENDER_TIME_TRACE = 1 };   /* time each traversal launch (fm_render_stats.trace_ms) */
+enum {
+    FM_RENDER_TIME_TRACE = 1,   /* time each traversal launch (fm_render_stats.trace_ms) */
+    FM_RENDER_GRAPH = 2         /* wavefront: capture one pass as a HIP graph, replay it per pass
+                                   (ignored with FM_RENDER_TIME_TRACE) */
+};
 typedef struct fm_render_params {
     int32_t width, height, spp, depth;   /* <= 0: take the scene's renderer block */
     int32_t direct_loop;                 /* <= 0: scene value */
@@ -256,6 +260,41 @@ int fm_render(fm_ctx* ctx, fm_scene* scene, const fm_render_params* params,
               fm_film* film, fm_render_stats* stats);
 /* Copies the current device film (means over the samples rendered so far). */
 int fm_film_read(fm_ctx* ctx, fm_film* film);
+
+/* One render of a batch: the scene seen through its own camera at its own
+ * resolution, samples, depth and seed.  Fields <= 0 take the scene's
+ * renderer block.  Outputs are optional host arrays of width * height * 3:
+ * means over spp (row 0 = top) and the per-pixel sum of squares. */
+typedef struct fm_job {
+    fm_camera_desc camera;    /* ignored when scene_camera != 0 */
+    int32_t scene_camera;     /* 1: the scene's camera */
+    int32_t width, height, spp, depth;
+    int32_t sample_offset;
+    uint64_t seed;
+    float* color;
+    float* albedo;
+    float* normal;
+    double* sumsq;
+} fm_job;
+
+typedef struct fm_batch_stats {
+    double kernel_ms;         /* device time of the whole batch */
+    int64_t paths, rays;
+    int32_t passes, units;    /* wavefront passes; units = (job, sample range) pieces */
+} fm_batch_stats;
+
+/* Renders n jobs in one sequence of wavefront passes: each pass packs
+ * samples of several jobs into one set of path queues (jobs in order, about
+ * 64 M paths per pass), so renders of uneven size, spp and depth share the
+ * device.  `common` (may be NULL) gives direct_loop, rr_depth, rr_coef,
+ * block_size, grid_blocks, chunk (max samples of a job per pass) and flags
+ * (FM_RENDER_GRAPH replays one captured pass graph).  job_done_ms (may be
+ * NULL, n entries) receives each job's completion time: device time from
+ * the start of the batch to the end of its last pass.  The replacement of
+ * the reference's one-render-per-process DrawFrame (core/renderer.h:84-144)
+ * for many renders of one scene; it has no single counterpart there. */
+int fm_render_batch(fm_ctx* ctx, fm_scene* scene, const fm_job* jobs, int n, const fm_render_params* common,
+                    double* job_done_ms, fm_batch_stats* stats);
 /* Enables the node / primitive counters in later renders (0 = off). */
 int fm_set_counting(fm_ctx* ctx, int enabled);
 
