@@ -46,6 +46,29 @@ def cpu_baseline(desc, W, H, spp, seconds_target=15.0):
             "sample": "%dx%d centred crop of the same frame at 16 spp (%d paths, %.1f s)" % (side, side, paths, dt)}
 
 
+def batch_leg(fumeng_amd, ctx):
+    """64 uneven renders of the box scene (cameras on a radius-4 sphere
+    looking at the origin, r x r with r ~ U{128..1024}, spp ~ U{16..256},
+    depth ~ U{4..16}, about 4.1e9 paths) in one wavefront run with the
+    captured pass graph; aggregate Mpaths/s and per-job completion times."""
+    import numpy as np
+    from fumeng_amd import scenes
+    jobs = scenes.batch_jobs(64, seed=4)
+    box = ctx.scene(scenes.box_scene())
+    box.render_batch([dict(j, spp=1) for j in jobs[:2]], read=False)   # warm-up (code objects)
+    t0 = time.perf_counter()
+    _, done, st = box.render_batch(jobs, read=False, graph=True)
+    wall = time.perf_counter() - t0
+    box.close()
+    res = {"workload": "64 uneven jobs of the box scene", "jobs": len(jobs), "paths": st.paths, "rays": st.rays,
+           "seconds": wall, "mpaths_per_s": st.paths / wall / 1e6, "mrays_per_s": st.rays / wall / 1e6,
+           "kernel_ms": st.kernel_ms, "passes": st.passes, "units": st.units,
+           "job_done_ms_p50": float(np.percentile(done, 50)), "job_done_ms_p99": float(np.percentile(done, 99))}
+    log("batch: %d jobs, %.3g paths in %.2f s = %.1f Mpaths/s, %d passes, p50 %.0f ms, p99 %.0f ms"
+        % (len(jobs), st.paths, wall, res["mpaths_per_s"], st.passes, res["job_done_ms_p50"], res["job_done_ms_p99"]))
+    return res
+
+
 def profiled_traffic(kernel):
     """HBM bytes per launch of `kernel` on the default workload, from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by a
@@ -86,6 +109,8 @@ def main():
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--mode", choices=["megakernel", "wavefront"], default="wavefront")
+    ap.add_argument("--no-batch", dest="batch", action="store_false",
+                    help="skip the batched-jobs leg (64 uneven renders of the box scene)")
     ap.add_argument("--no-compare", dest="compare", action="store_false",
                     help="skip the one-step run of the other form")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -171,6 +196,13 @@ def main():
     except Exception as e:  # counting is diagnostic; the throughput stands without it
         log("counting pass failed: %s" % e)
 
+    batch = None
+    if args.batch:
+        try:
+            batch = batch_leg(fumeng_amd, ctx)
+        except Exception as e:
+            log("batch leg failed: %s" % e)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -184,7 +216,7 @@ def main():
            "config": {"workload": "displaced icosphere %d tris + ground, 64 Disney materials, 2048x1024 HDR env, "
                                   "thin lens, %dx%d, %d spp, depth 12, %s" % (20 * args.freq ** 2, W, H, spp, args.mode),
                       "mode": args.mode, "width": W, "height": H, "spp": spp, "depth": 12},
-           "mrays_per_s": mrays, "paths": paths, "rays": rays, "other_form": other,
+           "mrays_per_s": mrays, "paths": paths, "rays": rays, "other_form": other, "batch": batch,
            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out))
     scene.close()

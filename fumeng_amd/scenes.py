@@ -201,3 +201,23 @@ def icosphere_scene(freq=224, width=1920, height=1080, spp=256, depth=12, seed=2
     env = s.add_image(sky_env(env_size[0], env_size[1], seed))
     s.add_env(env)
     return s
+
+
+def batch_jobs(n=64, seed=4, radius=4.0):
+    """Uneven renders of one scene: seeded camera positions on a sphere of
+    the given radius looking at the origin (pin-hole, 40 degrees), resolution
+    r x r with r ~ U{128..1024}, spp ~ U{16..256}, depth ~ U{4..16}.  Job
+    dicts for Scene.render_batch."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n):
+        v = rng.standard_normal(3)
+        v /= np.linalg.norm(v)
+        up = [0.0, 0.0, 1.0] if abs(v[2]) < 0.99 else [0.0, 1.0, 0.0]
+        r = int(rng.integers(128, 1025))
+        spp = int(rng.integers(16, 257))
+        depth = int(rng.integers(4, 17))
+        cam = {"type": "pin_hole", "pos": (radius * v).tolist(), "look_at": [0.0, 0.0, 0.0], "up": up, "fov": 40.0,
+               "aspect": 1.0, "focal_distance": 1.0, "lens_radius": 0.0}
+        jobs.append({"camera": cam, "width": r, "height": r, "spp": spp, "depth": depth, "seed": 1000 + i})
+    return jobs
