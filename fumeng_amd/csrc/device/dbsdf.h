@@ -151,6 +151,7 @@ struct Bsdf {
     v3 refl, refr; float eta_i, eta_t;
     /* Disney */
     float metallic, specular, specularTint, roughness, sheen, sheenTint, clearcoat, specTrans, diffTrans, flatness, ior;
+    float anisotropic, cc_gloss, st_rough;   /* parameters only the derived terms use */
     bool thin;
     float ax, ay, tax, tay, rax, ray, cc_rough, ior_r, diffuse_weight;
     v3 ctint, cspec0;
@@ -361,38 +362,11 @@ struct Bsdf {
     FMD v3 albedo() const { return type == FM_MAT_SPECULAR_TRANSMISSION ? refl : base; }
 };
 
-/* Material::CreateBSDF for a hit: frame + texture lookups + derived terms
- * (disney.cpp:375-461, 627-643).  TYPE >= 0 states the material type at
- * compile time (a kernel that only shades one material class), so the other
- * classes' code drops out of it. */
-template <int TYPE = -1>
-FMD Bsdf make_bsdf(const fm_dscene& S, int mat, v3 ns, v3 ss, v3 wo_w, float u, float v) {
-    Bsdf b;
-    const fm_dmaterial& M = S.materials[mat];
-    const int type = TYPE >= 0 ? TYPE : M.type;
-    b.type = type;
-    b.zs = ns;
-    b.xs = normalize(ss);
-    b.ys = cross(b.zs, b.xs);
-    if (type == FM_MAT_LAMBERT) { b.base = tex_sample(S, M.tex[0], u, v); return b; }
-    if (type == FM_MAT_SPECULAR_TRANSMISSION) {
-        b.refl = ld3(M.refl); b.refr = ld3(M.refr); b.eta_i = M.eta_i; b.eta_t = M.eta_t; b.base = b.refl;
-        return b;
-    }
-    b.base = tex_sample(S, M.tex[FM_D_BASECOLOR], u, v);
-    float p[15];
-#pragma unroll
-    for (int k = 1; k < 15; k++) {
-        const v3 t = tex_sample(S, M.tex[k], u, v);
-        p[k] = (k == FM_D_METALLIC) ? t.z : (k == FM_D_ROUGHNESS ? t.y : t.x);
-    }
-    b.metallic = p[1]; b.specular = p[2]; b.specularTint = p[3]; b.roughness = p[4];
-    const float anisotropic = p[5];
-    b.sheen = p[6]; b.sheenTint = p[7]; b.clearcoat = p[8];
-    const float ccg = p[9];
-    b.specTrans = p[10];
-    const float str = p[11];
-    b.diffTrans = p[12]; b.flatness = p[13]; b.ior = p[14]; b.thin = M.thin != 0;
+/* DisneyBSDF constructor terms from the parameters held in b and its frame
+ * (disney.cpp:375-461); wo_w is the outgoing direction (the lobe weights
+ * use the Fresnel term at wo). */
+FMD void disney_derive(Bsdf& b, v3 wo_w) {
+    const float anisotropic = b.anisotropic, ccg = b.cc_gloss, str = b.st_rough;
     const float bl = lum(b.base);
     b.ctint = bl > 0.0f ? b.base * (1.0f / bl) : splat(1.0f);
     b.cc_rough = lerp1(0.1f, 0.0f, ccg);
@@ -428,6 +402,41 @@ FMD Bsdf make_bsdf(const fm_dscene& S, int mat, v3 ns, v3 ss, v3 wo_w, float u, 
     b.c_dr = b.w_dr;
     b.c_cc = b.w_cc + b.c_dr;
     b.c_dt = b.w_dt + b.c_cc;
+}
+
+/* Material::CreateBSDF for a hit: frame + texture lookups + derived terms
+ * (disney.cpp:375-461, 627-643).  TYPE >= 0 states the material type at
+ * compile time (a kernel that only shades one material class), so the other
+ * classes' code drops out of it. */
+template <int TYPE = -1>
+FMD Bsdf make_bsdf(const fm_dscene& S, int mat, v3 ns, v3 ss, v3 wo_w, float u, float v) {
+    Bsdf b;
+    const fm_dmaterial& M = S.materials[mat];
+    const int type = TYPE >= 0 ? TYPE : M.type;
+    b.type = type;
+    b.zs = ns;
+    b.xs = normalize(ss);
+    b.ys = cross(b.zs, b.xs);
+    if (type == FM_MAT_LAMBERT) { b.base = tex_sample(S, M.tex[0], u, v); return b; }
+    if (type == FM_MAT_SPECULAR_TRANSMISSION) {
+        b.refl = ld3(M.refl); b.refr = ld3(M.refr); b.eta_i = M.eta_i; b.eta_t = M.eta_t; b.base = b.refl;
+        return b;
+    }
+    b.base = tex_sample(S, M.tex[FM_D_BASECOLOR], u, v);
+    float p[15];
+#pragma unroll
+    for (int k = 1; k < 15; k++) {
+        const v3 t = tex_sample(S, M.tex[k], u, v);
+        p[k] = (k == FM_D_METALLIC) ? t.z : (k == FM_D_ROUGHNESS ? t.y : t.x);
+    }
+    b.metallic = p[1]; b.specular = p[2]; b.specularTint = p[3]; b.roughness = p[4];
+    b.anisotropic = p[5];
+    b.sheen = p[6]; b.sheenTint = p[7]; b.clearcoat = p[8];
+    b.cc_gloss = p[9];
+    b.specTrans = p[10];
+    b.st_rough = p[11];
+    b.diffTrans = p[12]; b.flatness = p[13]; b.ior = p[14]; b.thin = M.thin != 0;
+    disney_derive(b, wo_w);
     return b;
 }
 

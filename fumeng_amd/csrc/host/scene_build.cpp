@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <limits>
 
 namespace fmh {
@@ -334,6 +335,91 @@ struct SahBuilder {
 };
 
 }  // namespace
+
+/* Env sampling tables of one env light (EnvImportanceSampler,
+ * env_importance_sampler.h:28-95, with Distribution1D / 2D of
+ * distribution_{1d,2d}.cpp), built in FP64 from the texels and stored FP32:
+ * luminance x sin(theta) at texel centres through the bilinear sampler,
+ * normalised to mean 1; conditional and marginal CDFs; guide tables.
+ * fresh = false rewrites the tables at the offsets already in e (same size:
+ * new texel values of the same image). */
+void build_env_tables(HostScene& hs, fm_denv& e, const std::function<void(int, int, double*)>& texel, bool fresh) {
+    const int W = e.W, H = e.H;
+    std::vector<double> lum((size_t)W * H);
+    double total = 0.0;
+    for (int j = 0; j < H; j++) {
+        const double v = ((double)j + 0.5) / H;
+        const double st = std::sin(kPId * v);
+        for (int i = 0; i < W; i++) {
+            const double u = ((double)i + 0.5) / W;
+            const double ru = u * W, rv = v * H;
+            const int u0 = (int)std::min(std::max(ru - 0.5, 0.0), (double)(W - 1));
+            const int u1 = std::min(u0 + 1, W - 1);
+            const double pu = std::min(std::max(ru - 0.5 - u0, 0.0), 1.0);
+            const int v0 = (int)std::min(std::max(rv - 0.5, 0.0), (double)(H - 1));
+            const int v1 = std::min(v0 + 1, H - 1);
+            const double pv = std::min(std::max(rv - 0.5 - v0, 0.0), 1.0);
+            double a[3], b[3], c3[3], d3[3], rgb[3];
+            texel(u0, v0, a); texel(u1, v0, b); texel(u0, v1, c3); texel(u1, v1, d3);
+            for (int c = 0; c < 3; c++) rgb[c] = (a[c] * (1 - pu) + b[c] * pu) * (1 - pv) + (c3[c] * (1 - pu) + d3[c] * pu) * pv;
+            const double L = (0.2126 * rgb[0] + 0.7152 * rgb[1] + 0.0722 * rgb[2]) * st;
+            lum[(size_t)j * W + i] = L;
+            total += L;
+        }
+    }
+    if (!(total > 0.0)) throw BuildError(FM_ERR_INVALID, "env map has zero luminance (invalid 1d distribution)");
+    const double scale = (double)W * (double)H / total;
+    std::vector<float>& T = hs.env_tables;
+    auto alloc = [&](size_t n, int64_t old) {
+        if (!fresh) return old;
+        const size_t o = T.size(); T.resize(o + n); return (int64_t)o;
+    };
+    e.off_marg_cdf = alloc((size_t)H + 1, e.off_marg_cdf);
+    e.off_marg_val = alloc((size_t)H, e.off_marg_val);
+    e.off_cond_cdf = alloc((size_t)H * (W + 1), e.off_cond_cdf);
+    e.off_cond_val = alloc((size_t)H * W, e.off_cond_val);
+    e.off_cond_sum = alloc((size_t)H, e.off_cond_sum);
+    std::vector<double> rowsum((size_t)H), cdf((size_t)W + 1);
+    for (int j = 0; j < H; j++) {
+        cdf[0] = 0.0;
+        for (int i = 0; i < W; i++) cdf[(size_t)i + 1] = cdf[(size_t)i] + lum[(size_t)j * W + i] * scale / W;
+        const double s = cdf[(size_t)W];
+        rowsum[(size_t)j] = s;
+        for (int i = 0; i <= W; i++) T[(size_t)e.off_cond_cdf + (size_t)j * (W + 1) + i] = s > 0 ? (float)(cdf[(size_t)i] / s) : 0.0f;
+        for (int i = 0; i < W; i++) T[(size_t)e.off_cond_val + (size_t)j * W + i] = (float)(lum[(size_t)j * W + i] * scale);
+        T[(size_t)e.off_cond_sum + j] = (float)s;
+    }
+    std::vector<double> mc((size_t)H + 1);
+    mc[0] = 0.0;
+    for (int j = 0; j < H; j++) mc[(size_t)j + 1] = mc[(size_t)j] + rowsum[(size_t)j] / H;
+    const double ms = mc[(size_t)H];
+    for (int j = 0; j <= H; j++) T[(size_t)e.off_marg_cdf + j] = (float)(mc[(size_t)j] / ms);
+    for (int j = 0; j < H; j++) T[(size_t)e.off_marg_val + j] = (float)rowsum[(size_t)j];
+    e.marg_sum = (float)ms;
+    /* guide tables for the device search: entry g = last index with
+     * cdf <= g / G over the stored FP32 cdf (G a power of two, so g / G
+     * and u * G are exact); a query u then only searches between the
+     * entries of floor(u G) and floor(u G) + 1 and finds the same index */
+    auto last_le = [](const float* cdf, int n, float u) {
+        int lo = 0, hi = n;
+        while (hi - lo > 1) { const int mid = (lo + hi) >> 1; if (cdf[mid] <= u) lo = mid; else hi = mid; }
+        return lo;
+    };
+    auto pow2_at_least = [](int n, int cap) { int g = 1; while (g < n && g < cap) g <<= 1; return g; };
+    e.guide_m = pow2_at_least(H, 4096);
+    e.guide_c = pow2_at_least(W / 8, 512);
+    e.off_marg_guide = alloc((size_t)e.guide_m + 1, e.off_marg_guide);
+    e.off_cond_guide = alloc((size_t)H * (e.guide_c + 1), e.off_cond_guide);
+    for (int g = 0; g <= e.guide_m; g++) {
+        const int32_t ix = last_le(&T[(size_t)e.off_marg_cdf], H, (float)g / (float)e.guide_m);
+        std::memcpy(&T[(size_t)e.off_marg_guide + g], &ix, 4);
+    }
+    for (int j = 0; j < H; j++)
+        for (int g = 0; g <= e.guide_c; g++) {
+            const int32_t ix = last_le(&T[(size_t)e.off_cond_cdf + (size_t)j * (W + 1)], W, (float)g / (float)e.guide_c);
+            std::memcpy(&T[(size_t)e.off_cond_guide + (size_t)j * (e.guide_c + 1) + g], &ix, 4);
+        }
+}
 
 /* PinHoleCamera / ThinLensCamera constructors (pin_hole_camera.cpp:5-16,
  * thin_lens_camera.cpp:14-21): camera_to_world = inverse(lookAtLH), film
@@ -689,77 +775,7 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
                 for (int c = 0; c < 3; c++) rgb[c] = s[c];
             }
         };
-        std::vector<double> lum((size_t)W * H);
-        double total = 0.0;
-        for (int j = 0; j < H; j++) {
-            const double v = ((double)j + 0.5) / H;
-            const double st = std::sin(kPId * v);
-            for (int i = 0; i < W; i++) {
-                const double u = ((double)i + 0.5) / W;
-                const double ru = u * W, rv = v * H;
-                const int u0 = (int)std::min(std::max(ru - 0.5, 0.0), (double)(W - 1));
-                const int u1 = std::min(u0 + 1, W - 1);
-                const double pu = std::min(std::max(ru - 0.5 - u0, 0.0), 1.0);
-                const int v0 = (int)std::min(std::max(rv - 0.5, 0.0), (double)(H - 1));
-                const int v1 = std::min(v0 + 1, H - 1);
-                const double pv = std::min(std::max(rv - 0.5 - v0, 0.0), 1.0);
-                double a[3], b[3], c3[3], d3[3], rgb[3];
-                texel(u0, v0, a); texel(u1, v0, b); texel(u0, v1, c3); texel(u1, v1, d3);
-                for (int c = 0; c < 3; c++) rgb[c] = (a[c] * (1 - pu) + b[c] * pu) * (1 - pv) + (c3[c] * (1 - pu) + d3[c] * pu) * pv;
-                const double L = (0.2126 * rgb[0] + 0.7152 * rgb[1] + 0.0722 * rgb[2]) * st;
-                lum[(size_t)j * W + i] = L;
-                total += L;
-            }
-        }
-        if (!(total > 0.0)) throw BuildError(FM_ERR_INVALID, "env map has zero luminance (invalid 1d distribution)");
-        const double scale = (double)W * (double)H / total;
-        std::vector<float>& T = hs.env_tables;
-        auto alloc = [&](size_t n) { const size_t o = T.size(); T.resize(o + n); return (int64_t)o; };
-        e.off_marg_cdf = alloc((size_t)H + 1);
-        e.off_marg_val = alloc((size_t)H);
-        e.off_cond_cdf = alloc((size_t)H * (W + 1));
-        e.off_cond_val = alloc((size_t)H * W);
-        e.off_cond_sum = alloc((size_t)H);
-        std::vector<double> rowsum((size_t)H), cdf((size_t)W + 1);
-        for (int j = 0; j < H; j++) {
-            cdf[0] = 0.0;
-            for (int i = 0; i < W; i++) cdf[(size_t)i + 1] = cdf[(size_t)i] + lum[(size_t)j * W + i] * scale / W;
-            const double s = cdf[(size_t)W];
-            rowsum[(size_t)j] = s;
-            for (int i = 0; i <= W; i++) T[(size_t)e.off_cond_cdf + (size_t)j * (W + 1) + i] = s > 0 ? (float)(cdf[(size_t)i] / s) : 0.0f;
-            for (int i = 0; i < W; i++) T[(size_t)e.off_cond_val + (size_t)j * W + i] = (float)(lum[(size_t)j * W + i] * scale);
-            T[(size_t)e.off_cond_sum + j] = (float)s;
-        }
-        std::vector<double> mc((size_t)H + 1);
-        mc[0] = 0.0;
-        for (int j = 0; j < H; j++) mc[(size_t)j + 1] = mc[(size_t)j] + rowsum[(size_t)j] / H;
-        const double ms = mc[(size_t)H];
-        for (int j = 0; j <= H; j++) T[(size_t)e.off_marg_cdf + j] = (float)(mc[(size_t)j] / ms);
-        for (int j = 0; j < H; j++) T[(size_t)e.off_marg_val + j] = (float)rowsum[(size_t)j];
-        e.marg_sum = (float)ms;
-        /* guide tables for the device search: entry g = last index with
-         * cdf <= g / G over the stored FP32 cdf (G a power of two, so g / G
-         * and u * G are exact); a query u then only searches between the
-         * entries of floor(u G) and floor(u G) + 1 and finds the same index */
-        auto last_le = [](const float* cdf, int n, float u) {
-            int lo = 0, hi = n;
-            while (hi - lo > 1) { const int mid = (lo + hi) >> 1; if (cdf[mid] <= u) lo = mid; else hi = mid; }
-            return lo;
-        };
-        auto pow2_at_least = [](int n, int cap) { int g = 1; while (g < n && g < cap) g <<= 1; return g; };
-        e.guide_m = pow2_at_least(H, 4096);
-        e.guide_c = pow2_at_least(W / 8, 512);
-        e.off_marg_guide = alloc((size_t)e.guide_m + 1);
-        e.off_cond_guide = alloc((size_t)H * (e.guide_c + 1));
-        for (int g = 0; g <= e.guide_m; g++) {
-            const int32_t ix = last_le(&T[(size_t)e.off_marg_cdf], H, (float)g / (float)e.guide_m);
-            std::memcpy(&T[(size_t)e.off_marg_guide + g], &ix, 4);
-        }
-        for (int j = 0; j < H; j++)
-            for (int g = 0; g <= e.guide_c; g++) {
-                const int32_t ix = last_le(&T[(size_t)e.off_cond_cdf + (size_t)j * (W + 1)], W, (float)g / (float)e.guide_c);
-                std::memcpy(&T[(size_t)e.off_cond_guide + (size_t)j * (e.guide_c + 1) + g], &ix, 4);
-            }
+        build_env_tables(hs, e, texel, true);
         L.kind = FM_DLIGHT_ENV;
         L.aux = (int32_t)hs.envs.size();
         if (hs.env_light < 0) { hs.env_light = (int32_t)hs.lights.size(); hs.env = L.aux; }

@@ -5,6 +5,7 @@
 #include "../../../include/fm_gpu.h"
 #include "../fm_layout.h"
 
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -52,6 +53,10 @@ struct HostScene {
     fm_renderer_desc renderer{};
     fm_camera_desc camera_desc{};
 };
+
+/* Env sampling tables of env light e from its texels (RGB per texel, linear);
+ * fresh = false rewrites them in place (same image size). */
+void build_env_tables(HostScene& hs, fm_denv& e, const std::function<void(int, int, double*)>& texel, bool fresh);
 
 /* Device camera record for a camera description (per-job cameras). */
 void make_camera(const fm_camera_desc& c, fm_dcamera& dc);
