@@ -170,6 +170,50 @@ class Scene:
         _check(lib().fm_render_batch(self.ctx.h, self.h, cj, n, C.byref(p), done, C.byref(st)), self.ctx.h)
         return films, np.array(done[:n]), st
 
+    # ---- differentiable parameters -------------------------------------------
+    def env_shape(self):
+        """(H, W) of the first env light's image, or None."""
+        i = self.info()
+        return (i.env_height, i.env_width) if i.has_env and i.env_width > 0 else None
+
+    def params(self):
+        """(material_params (num_materials, 6) float32 [base r g b, roughness,
+        metallic, specular], NaN where not a constant Disney value;
+        env_texels (H, W, 3) float32 or None)."""
+        nm = self.info().num_materials
+        mp = np.zeros((nm, A.FM_DPARAM_PER_MATERIAL), np.float32)
+        env = None
+        if self.env_shape() is not None:
+            env = np.zeros(self.env_shape() + (3,), np.float32)
+        _check(lib().fm_params_get(self.ctx.h, self.h, _ptr(mp), _ptr(env) if env is not None else None), self.ctx.h)
+        return mp, env
+
+    def set_params(self, material_params=None, env_texels=None):
+        mp = None if material_params is None else np.ascontiguousarray(material_params, np.float32)
+        env = None if env_texels is None else np.ascontiguousarray(env_texels, np.float32)
+        _check(lib().fm_params_set(self.ctx.h, self.h, _ptr(mp) if mp is not None else None,
+                                   _ptr(env) if env is not None else None), self.ctx.h)
+
+    def adjoint(self, dl_dimg, spp=0, seed=1, sample_offset=0, depth=0, width=0, height=0, rr_depth=-1, rr_coef=0.0,
+                direct_loop=0, grid_blocks=0):
+        """dL/d(parameters) for dl_dimg = dL/d(image) (H, W, 3); returns
+        (g_material (num_materials, 6) float64, g_env (H, W, 3) float64 or
+        None, stats)."""
+        info = self.info()
+        W = width or info.renderer.width
+        H = height or info.renderer.height
+        g = np.ascontiguousarray(dl_dimg, np.float32).reshape(H, W, 3)
+        gm = np.zeros((info.num_materials, A.FM_DPARAM_PER_MATERIAL), np.float64)
+        ge = None
+        if self.env_shape() is not None:
+            ge = np.zeros(self.env_shape() + (3,), np.float64)
+        p = A.fm_render_params(width=width, height=height, spp=spp, depth=depth, seed=seed, sample_offset=sample_offset,
+                               rr_depth=rr_depth, rr_coef=rr_coef, direct_loop=direct_loop, grid_blocks=grid_blocks)
+        st = A.fm_render_stats()
+        _check(lib().fm_render_adjoint(self.ctx.h, self.h, C.byref(p), _ptr(g), _ptr(gm),
+                                       _ptr(ge) if ge is not None else None, C.byref(st)), self.ctx.h)
+        return gm, ge, st
+
     def trace(self, rays, any_hit=False):
         """rays: (N, 8) float32 [ox oy oz tmin dx dy dz tmax]; returns fm_hit records."""
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)

@@ -29,6 +29,8 @@ FM_ACCEL_BVH, FM_ACCEL_NAIVE, FM_ACCEL_BVH_REFERENCE_TREE = 0, 1, 2
 FM_MODE_MEGAKERNEL, FM_MODE_WAVEFRONT = 0, 1
 FM_RENDER_TIME_TRACE = 1
 FM_RENDER_GRAPH = 2
+FM_DPARAM_PER_MATERIAL = 6
+DPARAM_NAMES = ["base_r", "base_g", "base_b", "roughness", "metallic", "specular"]
 
 F3 = C.c_float * 3
 
@@ -90,7 +92,8 @@ class fm_scene_desc(C.Structure):
 class fm_scene_info(C.Structure):
     _fields_ = [("num_geometries", C.c_int64), ("num_bvh_nodes", C.c_int64), ("bvh_depth", C.c_int32),
                 ("num_lights", C.c_int32), ("num_materials", C.c_int32), ("has_env", C.c_int32),
-                ("device_bytes", C.c_int64), ("renderer", fm_renderer_desc), ("camera", fm_camera_desc)]
+                ("device_bytes", C.c_int64), ("renderer", fm_renderer_desc), ("camera", fm_camera_desc),
+                ("env_width", C.c_int32), ("env_height", C.c_int32)]
 
 
 class fm_render_params(C.Structure):
@@ -169,6 +172,10 @@ EXPORTS = {
     "fm_render_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(fm_job), C.c_int, C.POINTER(fm_render_params),
                                   C.POINTER(C.c_double), C.POINTER(fm_batch_stats)]),
     "fm_set_counting": (C.c_int, [C.c_void_p, C.c_int]),
+    "fm_params_get": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fm_params_set": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fm_render_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(fm_render_params), C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.POINTER(fm_render_stats)]),
     "fm_trace": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p]),
     "fm_camera_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "fm_bsdf_eval": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),

@@ -25,7 +25,7 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
              "-I" + os.path.join(ROCM, "include"), "-D__HIP_PLATFORM_AMD__"]
 
 HOST_SOURCES = ["api.cpp", "host/loader.cpp", "host/scene_build.cpp"]
-HIP_SOURCES = ["kernels.hip", "wavefront.hip"]
+HIP_SOURCES = ["kernels.hip", "wavefront.hip", "adjoint.hip"]
 
 
 def _run(cmd):

@@ -1,0 +1,75 @@
+/*
+ * Path-replay adjoint kernel (device/dadjoint.h): one thread per (pixel,
+ * sample) of a persistent grid; the first replay is the forward path
+ * (render_path), the second accumulates parameter gradients.  Material
+ * gradients collect in LDS per block (FP32 atomics, num_materials x 6) and
+ * reach the FP64 global sums once per block; env texel gradients go to FP64
+ * global sums directly (they are spread over many texels).
+ */
+#include "launch.h"
+#include "device/dadjoint.h"
+
+using namespace fmd;
+
+namespace {
+
+__global__ void __launch_bounds__(128) k_adjoint(fm_dscene S, fm::AdjointArgs A, int stack_levels) {
+    extern __shared__ uint32_t lds[];
+    uint32_t* stk = lds + threadIdx.x;
+    const int stride = blockDim.x;
+    float* gl = A.lds_grad ? reinterpret_cast<float*>(lds + (size_t)stack_levels * blockDim.x) : nullptr;
+    const int ng = S.num_materials * AD_NPARAM;
+    if (gl)
+        for (int k = threadIdx.x; k < ng; k += blockDim.x) gl[k] = 0.0f;
+    __syncthreads();
+    AdSink K;
+    K.lds = gl; K.g_mat = A.g_mat; K.g_env = A.g_env; K.env_w = A.env_w; K.env_h = A.env_h;
+    PathCfg cfg;
+    cfg.depth = A.cfg_raw.depth; cfg.direct = A.cfg_raw.direct; cfg.rr_depth = A.cfg_raw.rr_depth; cfg.rr_coef = A.cfg_raw.rr_coef;
+    const int64_t npix = (int64_t)A.W * A.H;
+    unsigned long long paths = 0;
+    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < A.total_items; w += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = w / npix;
+        const int64_t pix = w - s * npix;
+        const v3 g = mk(A.dl_dimg[3 * pix], A.dl_dimg[3 * pix + 1], A.dl_dimg[3 * pix + 2]) * (1.0f / (float)A.spp);
+        paths++;
+        if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f) continue;
+        const int row = (int)(pix / A.W), i = (int)(pix - (int64_t)row * A.W);
+        const int j = A.H - 1 - row;
+        Rng rng;
+        rng.init(A.seed, (uint32_t)pix, (uint32_t)(A.sample_offset + s), 0u);
+        const float fx = rng.next(), fy = rng.next();
+        const float lx = rng.next(), ly = rng.next();
+        const float px = ((float)i + fx) / (float)A.W, py = ((float)j + fy) / (float)A.H;
+        const Ray cr = camera_ray(S.camera, px, py, lx, ly);
+        Rng rng2 = rng;
+        const PathOut po = render_path<false>(S, cfg, cr, rng, stk, stride);
+        if (!(po.color.x == po.color.x && po.color.y == po.color.y && po.color.z == po.color.z)) continue;
+        adjoint_walk(S, cfg, cr, rng2, stk, stride, g, po.color, K);
+    }
+    __syncthreads();
+    if (gl)
+        for (int k = threadIdx.x; k < ng; k += blockDim.x)
+            if (gl[k] != 0.0f) atomicAdd(A.g_mat + k, (double)gl[k]);
+    for (int off = 32; off > 0; off >>= 1) paths += __shfl_xor(paths, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(A.counters, paths);
+}
+
+}  // namespace
+
+namespace fm {
+
+hipError_t launch_adjoint(const fm_dscene& S, const AdjointArgs& A, int block, int grid, int stack_levels, hipStream_t st) {
+    size_t lds = (size_t)stack_levels * block * sizeof(uint32_t);
+    if (A.lds_grad) lds += (size_t)S.num_materials * AD_NPARAM * sizeof(float);
+    hipLaunchKernelGGL(k_adjoint, dim3(grid), dim3(block), lds, st, S, A, stack_levels);
+    return hipGetLastError();
+}
+
+int adjoint_blocks_per_cu(int block, size_t lds) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_adjoint, block, lds) != hipSuccess || n <= 0) return 1;
+    return n;
+}
+
+}  // namespace fm

@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -119,6 +120,7 @@ struct fm_ctx {
     WfBuffers wf;
     DevBuf wf_passes, wf_units, wf_cams, wf_pass_idx;   /* wavefront tables of the last run */
     DevBuf batch_film;                                  /* FP64 sums of fm_render_batch jobs */
+    DevBuf adj_in, adj_gmat, adj_genv;                  /* adjoint: image gradient, FP64 parameter gradients */
     std::vector<hipEvent_t> trace_ev;  /* pairs around wavefront trace launches */
     std::vector<hipEvent_t> pass_ev;   /* end of each wavefront pass */
 };
@@ -483,6 +485,10 @@ int fm_scene_get_info(const fm_scene* s, fm_scene_info* o) {
     o->device_bytes = s->device_bytes;
     o->renderer = s->hs.renderer;
     o->camera = s->hs.camera_desc;
+    if (s->hs.env >= 0) {
+        o->env_width = s->hs.envs[(size_t)s->hs.env].W;
+        o->env_height = s->hs.envs[(size_t)s->hs.env].H;
+    }
     return FM_OK;
 }
 
@@ -666,6 +672,164 @@ int fm_render_batch(fm_ctx* ctx, fm_scene* s, const fm_job* jobs, int n, const f
                 if (J.normal) J.normal[k] = (float)(h[(size_t)(2 * m + k)] * inv);
                 if (J.sumsq) J.sumsq[k] = h[(size_t)(3 * m + k)];
             }
+        }
+        return FM_OK;
+    });
+}
+
+/* ---- differentiable parameters ------------------------------------------ */
+
+namespace {
+
+/* Parameter p (FM_DPARAM_*) of a Disney material: texture slot and the
+ * channel the BSDF reads (metallic .z, roughness .y, others .x; disney.cpp:629-643). */
+void dparam_slot(int p, int& slot, int& ch) {
+    switch (p) {
+        case 0: slot = FM_D_BASECOLOR; ch = 0; break;
+        case 1: slot = FM_D_BASECOLOR; ch = 1; break;
+        case 2: slot = FM_D_BASECOLOR; ch = 2; break;
+        case 3: slot = FM_D_ROUGHNESS; ch = 1; break;
+        case 4: slot = FM_D_METALLIC; ch = 2; break;
+        default: slot = FM_D_SPECULAR; ch = 0; break;
+    }
+}
+
+const fm_dimage* env_image(const fm_scene* s) {
+    if (s->hs.env < 0) return nullptr;
+    const fm_dimage& im = s->hs.images[(size_t)s->hs.envs[(size_t)s->hs.env].image];
+    return im.format == 0 ? &im : nullptr;
+}
+
+}  // namespace
+
+int fm_params_get(fm_ctx* ctx, fm_scene* s, float* material_params, float* env_texels) {
+    if (!ctx || !s) return fail(ctx, FM_ERR_INVALID, "NULL argument");
+    return guarded(ctx, [&]() -> int {
+        const auto& M = s->hs.materials;
+        if (material_params)
+            for (size_t m = 0; m < M.size(); m++)
+                for (int p = 0; p < FM_DPARAM_PER_MATERIAL; p++) {
+                    int slot, ch;
+                    dparam_slot(p, slot, ch);
+                    const bool ok = M[m].type == FM_MAT_DISNEY && M[m].tex[slot].type == FM_TEX_CONSTANT;
+                    material_params[m * FM_DPARAM_PER_MATERIAL + p] = ok ? M[m].tex[slot].value[ch] : NAN;
+                }
+        if (env_texels) {
+            const fm_dimage* im = env_image(s);
+            if (!im) return fail(ctx, FM_ERR_UNSUPPORTED, "no env light with a linear float image");
+            const float* t = reinterpret_cast<const float*>(s->hs.texels.data() + im->offset);
+            for (int64_t k = 0; k < (int64_t)im->w * im->h; k++)
+                for (int c = 0; c < 3; c++) env_texels[3 * k + c] = t[4 * k + c];
+        }
+        return FM_OK;
+    });
+}
+
+int fm_params_set(fm_ctx* ctx, fm_scene* s, const float* material_params, const float* env_texels) {
+    if (!ctx || !s) return fail(ctx, FM_ERR_INVALID, "NULL argument");
+    return guarded(ctx, [&]() -> int {
+        HIPCHECK(hipSetDevice(ctx->device));
+        fmh::HostScene& hs = s->hs;
+        hipStream_t st = ctx->stream;
+        if (material_params) {
+            for (size_t m = 0; m < hs.materials.size(); m++) {
+                fm_dmaterial& M = hs.materials[m];
+                if (M.type != FM_MAT_DISNEY) continue;
+                for (int p = 0; p < FM_DPARAM_PER_MATERIAL; p++) {
+                    int slot, ch;
+                    dparam_slot(p, slot, ch);
+                    const float v = material_params[m * FM_DPARAM_PER_MATERIAL + p];
+                    if (M.tex[slot].type != FM_TEX_CONSTANT || v != v) continue;
+                    if (slot == FM_D_BASECOLOR) M.tex[slot].value[ch] = v;
+                    else for (int c = 0; c < 3; c++) M.tex[slot].value[c] = v;
+                }
+            }
+            HIPCHECK(hipMemcpyAsync(s->materials.p, hs.materials.data(), hs.materials.size() * sizeof(fm_dmaterial),
+                                    hipMemcpyHostToDevice, st));
+        }
+        if (env_texels) {
+            const fm_dimage* im = env_image(s);
+            if (!im) return fail(ctx, FM_ERR_UNSUPPORTED, "no env light with a linear float image");
+            float* t = reinterpret_cast<float*>(hs.texels.data() + im->offset);
+            const int64_t n = (int64_t)im->w * im->h;
+            for (int64_t k = 0; k < n; k++)
+                for (int c = 0; c < 3; c++) t[4 * k + c] = env_texels[3 * k + c];
+            fm_denv& e = hs.envs[(size_t)hs.env];
+            const int W = e.W;
+            fmh::build_env_tables(hs, e, [&](int u, int v, double rgb[3]) {
+                const float* q = t + 4 * ((size_t)v * W + (size_t)u);
+                for (int c = 0; c < 3; c++) rgb[c] = q[c];
+            }, false);
+            HIPCHECK(hipMemcpyAsync((char*)s->texels.p + im->offset, t, (size_t)n * 16, hipMemcpyHostToDevice, st));
+            HIPCHECK(hipMemcpyAsync(s->env_tables.p, hs.env_tables.data(), hs.env_tables.size() * sizeof(float),
+                                    hipMemcpyHostToDevice, st));
+            HIPCHECK(hipMemcpyAsync(s->envs.p, hs.envs.data(), hs.envs.size() * sizeof(fm_denv), hipMemcpyHostToDevice, st));
+        }
+        HIPCHECK(hipStreamSynchronize(st));
+        return FM_OK;
+    });
+}
+
+int fm_render_adjoint(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, const float* dl_dimg, double* g_material,
+                      double* g_env, fm_render_stats* stats) {
+    if (!ctx || !s || !p || !dl_dimg) return fail(ctx, FM_ERR_INVALID, "NULL argument");
+    return guarded(ctx, [&]() -> int {
+        const fm_renderer_desc& rd = s->hs.renderer;
+        fm::AdjointArgs A{};
+        A.W = p->width > 0 ? p->width : rd.width;
+        A.H = p->height > 0 ? p->height : rd.height;
+        A.spp = p->spp > 0 ? p->spp : rd.spp;
+        if (A.W <= 0 || A.H <= 0 || A.spp <= 0) return fail(ctx, FM_ERR_INVALID, "width, height and spp must be positive");
+        A.cfg_raw.depth = p->depth > 0 ? p->depth : rd.depth;
+        A.cfg_raw.direct = p->direct_loop > 0 ? p->direct_loop : rd.direct_loop;
+        A.cfg_raw.rr_depth = p->rr_depth >= 0 ? p->rr_depth : rd.rr_depth;
+        A.cfg_raw.rr_coef = p->rr_coef > 0 ? p->rr_coef : rd.rr_coef;
+        if (A.cfg_raw.direct <= 0) return fail(ctx, FM_ERR_INVALID, "direct_loop must be positive");
+        A.seed = p->seed;
+        A.sample_offset = p->sample_offset;
+        const int64_t npix = (int64_t)A.W * A.H;
+        A.total_items = npix * A.spp;
+        HIPCHECK(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        const int nm = s->d.num_materials;
+        const fm_dimage* im = env_image(s);
+        const int64_t nenv = im ? (int64_t)im->w * im->h * 3 : 0;
+        ctx->adj_in.alloc((size_t)npix * 3 * sizeof(float));
+        HIPCHECK(hipMemcpyAsync(ctx->adj_in.p, dl_dimg, (size_t)npix * 3 * sizeof(float), hipMemcpyHostToDevice, st));
+        if (ctx->adj_gmat.n < (size_t)nm * FM_DPARAM_PER_MATERIAL * sizeof(double))
+            ctx->adj_gmat.alloc((size_t)nm * FM_DPARAM_PER_MATERIAL * sizeof(double));
+        HIPCHECK(fm::launch_zero_f64(ctx->adj_gmat.as<double>(), (int64_t)nm * FM_DPARAM_PER_MATERIAL, st));
+        if (nenv > 0) {
+            if (ctx->adj_genv.n < (size_t)nenv * sizeof(double)) ctx->adj_genv.alloc((size_t)nenv * sizeof(double));
+            HIPCHECK(fm::launch_zero_f64(ctx->adj_genv.as<double>(), nenv, st));
+        }
+        HIPCHECK(fm::launch_zero_f64((double*)ctx->counters.p, 4, st));
+        A.dl_dimg = ctx->adj_in.as<float>();
+        A.g_mat = ctx->adj_gmat.as<double>();
+        A.g_env = nenv > 0 ? ctx->adj_genv.as<double>() : nullptr;
+        A.env_w = im ? im->w : 0; A.env_h = im ? im->h : 0;
+        A.lds_grad = (int64_t)nm * FM_DPARAM_PER_MATERIAL <= 4096 ? 1 : 0;
+        A.counters = ctx->counters.as<unsigned long long>();
+        const int block = 128, levels = FM_STACK_DEPTH;
+        const size_t lds = (size_t)levels * block * 4 + (A.lds_grad ? (size_t)nm * FM_DPARAM_PER_MATERIAL * 4 : 0);
+        int grid = p->grid_blocks > 0 ? p->grid_blocks : ctx->num_cus * fm::adjoint_blocks_per_cu(block, lds) * 2;
+        const int64_t need = (A.total_items + block - 1) / block;
+        if (grid > need) grid = (int)need;
+        HIPCHECK(hipEventRecord(ctx->ev0, st));
+        HIPCHECK(fm::launch_adjoint(s->d, A, block, std::max(grid, 1), levels, st));
+        HIPCHECK(hipEventRecord(ctx->ev1, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        if (g_material)
+            HIPCHECK(hipMemcpy(g_material, A.g_mat, (size_t)nm * FM_DPARAM_PER_MATERIAL * sizeof(double), hipMemcpyDeviceToHost));
+        if (g_env && nenv > 0) HIPCHECK(hipMemcpy(g_env, A.g_env, (size_t)nenv * sizeof(double), hipMemcpyDeviceToHost));
+        if (stats) {
+            memset(stats, 0, sizeof(*stats));
+            float ms = 0.0f;
+            HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            unsigned long long c[4];
+            HIPCHECK(hipMemcpy(c, A.counters, sizeof c, hipMemcpyDeviceToHost));
+            stats->kernel_ms = ms;
+            stats->paths = (int64_t)c[0];
         }
         return FM_OK;
     });

@@ -102,6 +102,22 @@ hipError_t launch_set_pass(int* pass_idx, int v, hipStream_t st);
 /* Persistent grid sizes for the wavefront kernels on this device. */
 void wavefront_grids(int num_cus, int stack_levels, WfLaunch& L);
 
+/* Path-replay adjoint over (pixel, sample) work items, sample-major. */
+struct AdjointArgs {
+    int W, H, spp, sample_offset;
+    uint64_t seed;
+    struct { int depth, direct, rr_depth; float rr_coef; } cfg_raw;
+    int64_t total_items;
+    const float* dl_dimg;     /* H x W x 3, row 0 = top */
+    double* g_mat;            /* num_materials x 6 */
+    double* g_env;            /* env texels x 3, or null */
+    int env_w, env_h;
+    int lds_grad;             /* accumulate material gradients in LDS first */
+    unsigned long long* counters;
+};
+hipError_t launch_adjoint(const fm_dscene& S, const AdjointArgs& A, int block, int grid, int stack_levels, hipStream_t st);
+int adjoint_blocks_per_cu(int block, size_t lds);
+
 hipError_t launch_trace(const fm_dscene& S, const fm_ray* rays, int64_t n, int any, fm_hit* out, hipStream_t st);
 hipError_t launch_camera(const fm_dscene& S, const float* nl, int64_t n, fm_ray* out, hipStream_t st);
 hipError_t launch_bsdf(const fm_dscene& S, const fm_bsdf_query* q, int64_t n, fm_bsdf_result* out, hipStream_t st);

@@ -221,3 +221,31 @@ def batch_jobs(n=64, seed=4, radius=4.0):
                "aspect": 1.0, "focal_distance": 1.0, "lens_radius": 0.0}
         jobs.append({"camera": cam, "width": r, "height": r, "spp": spp, "depth": depth, "seed": 1000 + i})
     return jobs
+
+
+def inverse_scene(freq=71, width=256, height=256, spp=32, depth=6, seed=5, env_size=(512, 256), n_materials=64):
+    """Inverse-rendering workload: the displaced-icosphere geometry at the
+    given frequency (71: 100,820 triangles + ground), n_materials Disney
+    materials, an env map of env_size, a square image, depth 6."""
+    s = icosphere_scene(freq=freq, width=width, height=height, spp=spp, depth=depth, seed=seed, env_size=env_size,
+                        n_materials=n_materials)
+    s.camera = dict(s.camera, aspect=float(width) / float(height))
+    return s
+
+
+def perturb_params(material_params, env_texels, seed=5, amount=0.25, env_sigma=0.5):
+    """A seeded start point away from the given parameters: Disney values
+    moved by U(-amount, amount) and clipped to [0.02, 0.98] (NaN entries
+    kept), env texels scaled by exp(env_sigma N(0, 1)) on an 8 x 4 grid of
+    blocks (smooth, so the image changes visibly)."""
+    rng = np.random.default_rng(seed)
+    mp = np.array(material_params, np.float32, copy=True)
+    ok = np.isfinite(mp)
+    mp[ok] = np.clip(mp[ok] + rng.uniform(-amount, amount, mp.shape)[ok], 0.02, 0.98).astype(np.float32)
+    env = None
+    if env_texels is not None:
+        H, W = env_texels.shape[:2]
+        blocks = np.exp(env_sigma * rng.standard_normal((4, 8))).astype(np.float32)
+        scale = np.kron(blocks, np.ones((H // 4 + 1, W // 8 + 1), np.float32))[:H, :W]
+        env = (env_texels * scale[:, :, None]).astype(np.float32)
+    return mp, env

@@ -203,6 +203,7 @@ typedef struct fm_scene_info {
     int64_t device_bytes;     /* scene bytes resident in HBM */
     fm_renderer_desc renderer;
     fm_camera_desc camera;
+    int32_t env_width, env_height;   /* first env light's image (0 if none) */
 } fm_scene_info;
 int fm_scene_get_info(const fm_scene* scene, fm_scene_info* out);
 
@@ -295,6 +296,30 @@ typedef struct fm_batch_stats {
  * for many renders of one scene; it has no single counterpart there. */
 int fm_render_batch(fm_ctx* ctx, fm_scene* scene, const fm_job* jobs, int n, const fm_render_params* common,
                     double* job_done_ms, fm_batch_stats* stats);
+
+/* ---- differentiable rendering --------------------------------------------- */
+
+/* Per material, FM_DPARAM_PER_MATERIAL values: Disney baseColor r, g, b,
+ * roughness, metallic, specular (constant textures; metallic and roughness
+ * as the BSDF reads them, disney.cpp:629-643).  Other materials and
+ * image-textured slots read as NaN and ignore writes.  Env texels are the
+ * first env light's linear float image, height x width x 3 (row 0 first). */
+enum { FM_DPARAM_PER_MATERIAL = 6 };
+int fm_params_get(fm_ctx* ctx, fm_scene* scene, float* material_params, float* env_texels);
+/* Either pointer may be NULL.  New env texels rebuild the env sampling
+ * tables (FP64, on the host) and upload them. */
+int fm_params_set(fm_ctx* ctx, fm_scene* scene, const float* material_params, const float* env_texels);
+
+/* Path-replay adjoint (no counterpart in the reference, which only renders
+ * forward).  For the paths of `params` (seed, spp, sample_offset, sizes,
+ * depth as in fm_render) and dl_dimg = dL/d(mean image), height x width x 3
+ * (row 0 = top), writes dL/d(material parameters) (num_materials x 6) and
+ * dL/d(env texels) (env height x width x 3) as FP64 sums; either output may
+ * be NULL.  Each path is recomputed from its random numbers, pdfs and MIS
+ * weights are held fixed; the BSDF's parameter derivatives are central
+ * differences of its FP32 value at the path's directions. */
+int fm_render_adjoint(fm_ctx* ctx, fm_scene* scene, const fm_render_params* params, const float* dl_dimg,
+                      double* g_material, double* g_env, fm_render_stats* stats);
 /* Enables the node / primitive counters in later renders (0 = off). */
 int fm_set_counting(fm_ctx* ctx, int enabled);
 

@@ -129,3 +129,21 @@ def convergence_stats(mean, sumsq, n, ref):
     rel = float(np.mean(diff * diff / (r * r + 1e-2)))
     return {"relmse": rel, "mean_z": float(np.mean(z[np.isfinite(z)])), "frac_z3": float(np.mean(np.abs(z) > 3.0)),
             "max_abs_z": float(np.max(np.abs(z))), "nonfinite_pixels": int(np.sum(~finite)), "z": z}
+
+
+def two_planes_scene(depth=3, rough=0.4, metal=0.3, width=16, height=16):
+    """A Disney floor and a Disney wall under a sky env with a soft sun: a
+    smooth test bed for parameter derivatives (no small geometry whose
+    visibility jumps under finite differences)."""
+    s = SceneDesc()
+    s.camera = {"type": "pin_hole", "pos": [0.0, -3.0, 1.2], "look_at": [0.0, 0.0, 0.0], "up": [0.0, 0.0, 1.0],
+                "fov": 40.0, "aspect": float(width) / height, "focal_distance": 1.0, "lens_radius": 0.0}
+    s.renderer.update(width=width, height=height, spp=8, depth=depth, rr_depth=10)
+    m0 = s.add_disney(basecolor=[0.6, 0.5, 0.3], roughness=rough, metallic=metal, specular=0.5, clearcoat=0.3, sheen=0.2)
+    m1 = s.add_disney(basecolor=[0.3, 0.6, 0.4], roughness=0.5, metallic=0.1, specular=0.5)
+    floor = s.add_mesh(np.array([[-2, -2, 0], [2, -2, 0], [2, 2, 0], [-2, 2, 0]], np.float32), np.array([[0, 1, 2], [0, 2, 3]]))
+    wall = s.add_mesh(np.array([[-2, 1, 0], [2, 1, 0], [2, 1, 3], [-2, 1, 3]], np.float32), np.array([[0, 2, 1], [0, 3, 2]]))
+    s.add_mesh_object(floor, material=m0)
+    s.add_mesh_object(wall, material=m1)
+    s.add_env(s.add_image(scenes.sky_env(64, 32, 3, sun_peak=50.0, sun_sigma_deg=8.0)))
+    return s
