@@ -69,6 +69,65 @@ def batch_leg(fumeng_amd, ctx):
     return res
 
 
+def adjoint_leg(fumeng_amd, ctx, iters=6):
+    """Inverse rendering: the displaced icosphere at frequency 71 (100,822
+    geometries), 64 Disney materials, 512x256 env, 256x256, depth 6.  Adam
+    on all 384 material parameters and 393,216 env texel values from a
+    perturbed start towards a target rendered from the hidden values; one
+    iteration = set parameters (env tables rebuilt in FP64) + 32-spp forward
+    render + 32-spp path-replay adjoint + optimiser step."""
+    import numpy as np
+    import torch
+    from fumeng_amd import scenes, _abi
+    from fumeng_amd.torch_ext import render
+    sc = ctx.scene(scenes.inverse_scene())
+    mp, env = sc.params()
+    target = torch.from_numpy(sc.render(spp=256, seed=5, mode=_abi.FM_MODE_WAVEFRONT)[0])
+    start_mp, start_env = scenes.perturb_params(mp, env, seed=5)
+    mask = torch.from_numpy(np.isfinite(mp).astype(np.float32))
+    theta = torch.tensor(np.nan_to_num(start_mp, nan=0.0), requires_grad=True)
+    env_t = torch.tensor(start_env, requires_grad=True)
+    opt = torch.optim.Adam([{"params": [theta], "lr": 0.03}, {"params": [env_t], "lr": 0.03 * float(env.mean())}])
+    nan_rows = torch.full_like(theta, float("nan"))
+    times, losses, fwd_ms, adj_ms = [], [], [], []
+    orig_render, orig_adjoint = sc.render, sc.adjoint
+
+    def timed_render(**kw):
+        out = orig_render(**kw)
+        fwd_ms.append(out[4].kernel_ms)
+        return out
+
+    def timed_adjoint(*a, **kw):
+        out = orig_adjoint(*a, **kw)
+        adj_ms.append(out[2].kernel_ms)
+        return out
+    sc.render, sc.adjoint = timed_render, timed_adjoint
+    for it in range(iters + 1):
+        t0 = time.perf_counter()
+        opt.zero_grad()
+        params = torch.where(mask > 0, theta, nan_rows)
+        img = render(sc, params, env_t, spp=32, seed=100 + it, spp_adjoint=32, seed_adjoint=900 + it)
+        loss = torch.mean((img - target) ** 2)
+        loss.backward()
+        theta.grad = torch.nan_to_num(theta.grad) * mask
+        opt.step()
+        with torch.no_grad():
+            theta.clamp_(0.0, 1.0)
+            env_t.clamp_(min=0.0)
+        if it > 0:                       # the first iteration is the warm-up
+            times.append(time.perf_counter() - t0)
+        losses.append(float(loss))
+    sc.close()
+    res = {"workload": "inverse rendering, 100,822 geometries, 64 Disney materials, 512x256 env, 256x256, "
+                       "32 + 32 spp, depth 6", "params": int(mask.sum().item()) + env.size,
+           "ms_per_iter": 1e3 * float(np.mean(times)), "forward_kernel_ms": float(np.mean(fwd_ms[1:])),
+           "adjoint_kernel_ms": float(np.mean(adj_ms[1:])), "iters": iters,
+           "loss_first": losses[0], "loss_last": losses[-1]}
+    log("adjoint: %.1f ms/iter (forward kernel %.1f ms, adjoint kernel %.1f ms), loss %.4g -> %.4g"
+        % (res["ms_per_iter"], res["forward_kernel_ms"], res["adjoint_kernel_ms"], losses[0], losses[-1]))
+    return res
+
+
 def profiled_traffic(kernel):
     """HBM bytes per launch of `kernel` on the default workload, from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by a
@@ -109,6 +168,8 @@ def main():
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--mode", choices=["megakernel", "wavefront"], default="wavefront")
+    ap.add_argument("--no-adjoint", dest="adjoint", action="store_false",
+                    help="skip the inverse-rendering (adjoint) leg")
     ap.add_argument("--no-batch", dest="batch", action="store_false",
                     help="skip the batched-jobs leg (64 uneven renders of the box scene)")
     ap.add_argument("--no-compare", dest="compare", action="store_false",
@@ -203,6 +264,13 @@ def main():
         except Exception as e:
             log("batch leg failed: %s" % e)
 
+    adjoint = None
+    if args.adjoint:
+        try:
+            adjoint = adjoint_leg(fumeng_amd, ctx)
+        except Exception as e:
+            log("adjoint leg failed: %s" % e)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -216,7 +284,7 @@ def main():
            "config": {"workload": "displaced icosphere %d tris + ground, 64 Disney materials, 2048x1024 HDR env, "
                                   "thin lens, %dx%d, %d spp, depth 12, %s" % (20 * args.freq ** 2, W, H, spp, args.mode),
                       "mode": args.mode, "width": W, "height": H, "spp": spp, "depth": 12},
-           "mrays_per_s": mrays, "paths": paths, "rays": rays, "other_form": other, "batch": batch,
+           "mrays_per_s": mrays, "paths": paths, "rays": rays, "other_form": other, "batch": batch, "adjoint": adjoint,
            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out))
     scene.close()
