@@ -128,6 +128,35 @@ def adjoint_leg(fumeng_amd, ctx, iters=6):
     return res
 
 
+def stress_leg(fumeng_amd, ctx):
+    """Large-geometry stress scene: 400 displaced spheres (9.8 M triangles),
+    200 sRGB 2048^2 textures, 10 rectangle lights, ground; 1920x1080, 128
+    spp, depth 10; one wavefront frame (scene generation and upload are
+    reported, not timed into the throughput)."""
+    from fumeng_amd import scenes, _abi
+    t0 = time.perf_counter()
+    desc = scenes.stress_scene()
+    gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sc = ctx.scene(desc)
+    build = time.perf_counter() - t0
+    info = sc.info()
+    W, H, spp = desc.renderer["width"], desc.renderer["height"], desc.renderer["spp"]
+    del desc
+    sc.render(seed=2, spp=1, read=False, mode=_abi.FM_MODE_WAVEFRONT)        # warm-up
+    t0 = time.perf_counter()
+    _, _, _, _, st = sc.render(seed=1, read=False, mode=_abi.FM_MODE_WAVEFRONT, time_trace=True)
+    dt = time.perf_counter() - t0
+    sc.close()
+    res = {"workload": "stress: 400 displaced spheres (9,800,012 geometries), 200 textures 2048^2, 10 rect lights, "
+                       "1920x1080, 128 spp, depth 10", "mpaths_per_s": W * H * spp / dt / 1e6,
+           "mrays_per_s": st.rays / dt / 1e6, "ms_per_frame": dt * 1e3, "trace_ms": st.trace_ms,
+           "generate_s": gen, "build_upload_s": build, "device_gb": info.device_bytes / 1e9}
+    log("stress: %.1f Mpaths/s, %.1f Mrays/s (%.0f ms per frame), built in %.1f s, %.2f GB in HBM"
+        % (res["mpaths_per_s"], res["mrays_per_s"], dt * 1e3, build, res["device_gb"]))
+    return res
+
+
 def profiled_traffic(kernel):
     """HBM bytes per launch of `kernel` on the default workload, from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by a
@@ -168,6 +197,8 @@ def main():
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--mode", choices=["megakernel", "wavefront"], default="wavefront")
+    ap.add_argument("--no-stress", dest="stress", action="store_false",
+                    help="skip the 9.8 M-triangle textured stress scene leg")
     ap.add_argument("--no-adjoint", dest="adjoint", action="store_false",
                     help="skip the inverse-rendering (adjoint) leg")
     ap.add_argument("--no-batch", dest="batch", action="store_false",
@@ -264,6 +295,13 @@ def main():
         except Exception as e:
             log("batch leg failed: %s" % e)
 
+    stress = None
+    if args.stress:
+        try:
+            stress = stress_leg(fumeng_amd, ctx)
+        except Exception as e:
+            log("stress leg failed: %s" % e)
+
     adjoint = None
     if args.adjoint:
         try:
@@ -284,7 +322,7 @@ def main():
            "config": {"workload": "displaced icosphere %d tris + ground, 64 Disney materials, 2048x1024 HDR env, "
                                   "thin lens, %dx%d, %d spp, depth 12, %s" % (20 * args.freq ** 2, W, H, spp, args.mode),
                       "mode": args.mode, "width": W, "height": H, "spp": spp, "depth": 12},
-           "mrays_per_s": mrays, "paths": paths, "rays": rays, "other_form": other, "batch": batch, "adjoint": adjoint,
+           "mrays_per_s": mrays, "paths": paths, "rays": rays, "other_form": other, "batch": batch, "adjoint": adjoint, "stress": stress,
            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out))
     scene.close()

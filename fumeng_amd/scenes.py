@@ -11,7 +11,7 @@ All values are float32 and fully determined by the seed.
 """
 import numpy as np
 
-from .scene import SceneDesc, rgbe_decode, rgbe_encode
+from .scene import SceneDesc, image_tex, rgbe_decode, rgbe_encode
 
 
 def box_scene(width=512, height=512, spp=64, depth=8, rr_depth=3, rr_coef=0.85, accelerator="bvh"):
@@ -249,3 +249,81 @@ def perturb_params(material_params, env_texels, seed=5, amount=0.25, env_sigma=0
         scale = np.kron(blocks, np.ones((H // 4 + 1, W // 8 + 1), np.float32))[:H, :W]
         env = (env_texels * scale[:, :, None]).astype(np.float32)
     return mp, env
+
+
+def noise_texture(size, seed, cells=6, octaves=2, block=4):
+    """Seeded smooth colour texture: fBm of value noise with smoothstep
+    interpolation, computed at size / block and replicated block x block,
+    as uint8 sRGB RGB (size x size x 3)."""
+    rng = np.random.default_rng(seed)
+    full = size
+    size = max(1, size // block)
+    img = np.zeros((size, size, 3), np.float32)
+    amp, total = 1.0, 0.0
+    for o in range(octaves):
+        n = cells * 2 ** o
+        g = rng.random((n + 1, n + 1, 3)).astype(np.float32)
+        x = np.arange(size, dtype=np.float32) * (n / size)
+        i0 = x.astype(np.int32)
+        f = x - i0
+        f = (f * f * (3.0 - 2.0 * f))[:, None]
+        rows = g[i0] * (1.0 - f)[:, :, None] + g[i0 + 1] * f[:, :, None]          # (size, n + 1, 3)
+        img += amp * (rows[:, i0] * (1.0 - f.T)[:, :, None] + rows[:, i0 + 1] * f.T[:, :, None])
+        total += amp
+        amp *= 0.5
+    img /= total
+    lo = rng.random(3).astype(np.float32) * 0.3
+    hi = 0.5 + rng.random(3).astype(np.float32) * 0.5
+    small = np.clip((lo + (hi - lo) * img) * 255.0 + 0.5, 0, 255).astype(np.uint8)
+    if size * block != full or block == 1:
+        return np.ascontiguousarray(np.resize(np.repeat(np.repeat(small, block, 0), block, 1), (full, full, 3)))
+    return np.ascontiguousarray(np.repeat(np.repeat(small, block, 0), block, 1))
+
+
+def stress_scene(n_spheres=400, freq=35, n_textures=200, tex_size=2048, width=1920, height=1080, spp=128, depth=10,
+                 seed=3, n_lights=10, ground=True):
+    """Large-geometry stress scene: n_spheres displaced geodesic spheres
+    (freq 35: 24,500 triangles each, 9.8 M in total) with centres uniform
+    in a 40 x 40 area at z = r, r ~ U[0.3, 1.2], displaced by 0.15 r;
+    n_lights 2 x 2 rectangle lights at z = 10 facing down with grey radiance
+    U[5, 20]; one Disney material per sphere drawn like the icosphere's,
+    the even spheres' baseColor an n_textures-image set of tex_size^2
+    seeded colour noise textures (spherical UVs); a 60 x 60 Lambert ground;
+    pin-hole camera at (0, -30, 15) looking at the origin, fov 50;
+    1920 x 1080, 128 spp, depth 10 (Russian roulette inactive)."""
+    rng = np.random.default_rng(seed)
+    s = SceneDesc()
+    s.camera = {"type": "pin_hole", "pos": [0.0, -30.0, 15.0], "look_at": [0.0, 0.0, 0.0], "up": [0.0, 0.0, 1.0],
+                "fov": 50.0, "aspect": float(width) / height, "focal_distance": 1.0, "lens_radius": 0.0}
+    s.renderer = {"width": width, "height": height, "spp": spp, "thread_count": 8, "depth": depth, "direct_loop": 1,
+                  "rr_depth": 10, "rr_coef": 0.85, "output": "stress"}
+    verts, tris = geodesic_sphere(freq)
+    uv = np.stack([0.5 + np.arctan2(verts[:, 1], verts[:, 0]) / (2.0 * np.pi),
+                   np.arccos(np.clip(verts[:, 2], -1.0, 1.0)) / np.pi], 1).astype(np.float32)
+    tex_ids = [s.add_image(noise_texture(tex_size, seed * 1000 + t)) for t in range(n_textures)]
+    for k in range(n_spheres):
+        r = float(rng.uniform(0.3, 1.2))
+        cx, cy = rng.uniform(-20.0, 20.0, 2)
+        d = fbm(verts * 1.0, seed * 7919 + k)
+        d = d / max(1e-9, np.abs(d).max())
+        pos = (verts * (r * (1.0 + 0.15 * d))[:, None] + np.array([cx, cy, r])).astype(np.float32)
+        m = s.add_mesh(pos, tris, texcoords=uv)
+        q = rng.random(8)
+        base = [float(q[0]), float(q[1]), float(q[2])]
+        if k % 2 == 0 and k // 2 < n_textures:
+            base = image_tex(tex_ids[k // 2])
+        mat = s.add_disney(name="sphere%d" % k, basecolor=base, roughness=float(q[3]), metallic=float(q[4]),
+                           specular=float(q[5]), clearcoat=float(q[6]), sheen=float(q[7]))
+        s.add_mesh_object(m, material=mat)
+    if ground:
+        gm = s.add_lambert(0.5, name="ground")
+        g = s.add_mesh(np.array([[-30, -30, 0], [30, -30, 0], [30, 30, 0], [-30, 30, 0]], np.float32),
+                       np.array([[0, 1, 2], [0, 2, 3]], np.int32))
+        s.add_mesh_object(g, material=gm)
+    black = s.add_lambert(0.0, name="light")
+    for li in range(n_lights):
+        x, y = rng.uniform(-18.0, 18.0, 2)
+        L = float(rng.uniform(5.0, 20.0))
+        s.add_rectangle((x - 1, y - 1, 10.0), (x - 1, y + 1, 10.0), (x + 1, y + 1, 10.0), (x + 1, y - 1, 10.0),
+                        material=black, emittance=L)
+    return s

@@ -123,3 +123,21 @@ def test_trace_timing_flag(gpu_ctx):
     assert 0.0 < b[4].trace_ms <= b[4].kernel_ms
     m = g.render(seed=9, mode=MK, time_trace=True)[4]
     assert m.trace_ms == m.kernel_ms
+
+
+@pytest.mark.parametrize("mode", [MK, WF])
+def test_textured_many_light_scene_matches_oracle(gpu_ctx, mode):
+    """A scaled-down stress scene: displaced spheres with spherical UVs,
+    sRGB image textures on half of the Disney baseColors, ten rectangle
+    lights (ten shadow records per vertex), a ground plane; common random
+    numbers against the FP32 oracle, both forms."""
+    s = scenes.stress_scene(n_spheres=8, freq=6, n_textures=4, tex_size=64, width=32, height=18, spp=8, depth=5)
+    g = gpu_ctx.scene(s)
+    assert g.info().num_lights == 10
+    cg, ag, ng, qg, st = g.render(seed=3, mode=mode)
+    co, ao, no, qo = oracle.OracleScene(s).render(0, 0, 32, 18, spp=8, seed=3, threads=8)
+    close = np.abs(cg - co) <= 1e-3 * np.maximum(1.0, np.abs(co))
+    print("pixels equal to 1e-3:", close.mean(), "mean", cg.mean(), co.mean())
+    assert close.mean() > 0.9
+    assert abs(cg.mean() - co.mean()) < 0.03 * co.mean()
+    assert np.mean(np.abs(ag - ao) < 1e-4) > 0.97
