@@ -119,10 +119,11 @@ FMD bool sphere_core(const Ray& r, const fm_dsphere& s, float tmax_incl, float& 
 /* BBox::IsIntersect with the current t_max.  `fast` uses fminf/fmaxf, equal
  * to std::min/max when no slab value is NaN. */
 template <bool FAST>
-FMD bool box_hit(const float* b, const Ray& r, const RayPre& rp, float tmax, float& tnear) {
+FMD bool box_hit6(float x0, float y0, float z0, float x1, float y1, float z1, const Ray& r, const RayPre& rp, float tmax,
+                  float& tnear) {
 #pragma clang fp contract(off)
-    const float lx = rp.inv.x * (b[0] - r.o.x), ly = rp.inv.y * (b[1] - r.o.y), lz = rp.inv.z * (b[2] - r.o.z);
-    const float hx = rp.inv.x * (b[3] - r.o.x), hy = rp.inv.y * (b[4] - r.o.y), hz = rp.inv.z * (b[5] - r.o.z);
+    const float lx = rp.inv.x * (x0 - r.o.x), ly = rp.inv.y * (y0 - r.o.y), lz = rp.inv.z * (z0 - r.o.z);
+    const float hx = rp.inv.x * (x1 - r.o.x), hy = rp.inv.y * (y1 - r.o.y), hz = rp.inv.z * (z1 - r.o.z);
     if (FAST) {
         const float n = fmaxf(fmaxf(r.tmin, fminf(lx, hx)), fmaxf(fminf(ly, hy), fminf(lz, hz)));
         const float f = fminf(fminf(tmax, fmaxf(lx, hx)), fminf(fmaxf(ly, hy), fmaxf(lz, hz)));
@@ -139,6 +140,16 @@ FMD bool box_hit(const float* b, const Ray& r, const RayPre& rp, float tmax, flo
         tnear = n;
         return n <= f;
     }
+}
+template <bool FAST>
+FMD bool box_hit(const float* b, const Ray& r, const RayPre& rp, float tmax, float& tnear) {
+    return box_hit6<FAST>(b[0], b[1], b[2], b[3], b[4], b[5], r, rp, tmax, tnear);
+}
+/* Either form, chosen per ray (lanes of a wave may differ). */
+FMD bool box_hit_any(float x0, float y0, float z0, float x1, float y1, float z1, const Ray& r, const RayPre& rp, float tmax,
+                     float& tnear) {
+    return rp.fast ? box_hit6<true>(x0, y0, z0, x1, y1, z1, r, rp, tmax, tnear)
+                   : box_hit6<false>(x0, y0, z0, x1, y1, z1, r, rp, tmax, tnear);
 }
 
 struct Hit {
@@ -224,42 +235,66 @@ FMD bool test_slot_any(const fm_dscene& S, const Ray& r, const RayPre& rp, int32
 /* Work counters of the counting variant (node visits, primitive tests). */
 struct Cnt { uint32_t nodes = 0, prims = 0; };
 
-/* Traversal with a per-thread stack in LDS: stack[level * stride + lane]. */
-template <bool FAST, bool COUNT>
-FMD void closest_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& h, uint32_t* stk, int stride, Cnt& cn) {
+/* Traversal with a per-thread stack in LDS, stack[level * stride + lane],
+ * in the speculative "while-while" form of Aila & Laine (HPG 2009): a lane
+ * that reaches a leaf postpones it and keeps descending until every active
+ * lane of the wave holds a leaf, then the leaves are tested together; the
+ * closest hit does not depend on the order (order-free tie rule). */
+#define FM_TRAV_NONE 0x7fffffff   /* no node / no pending leaf (leaf refs are negative) */
+
+template <bool FAST, bool ANY, bool COUNT>
+FMD bool traverse_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& h, uint32_t* stk, int stride, Cnt& cn) {
     float tn;
-    if (!box_hit<FAST>(S.root_box, r, rp, h.t, tn)) return;
-    int32_t ref = S.root_ref;
+    if (!box_hit<FAST>(S.root_box, r, rp, ANY ? r.tmax : h.t, tn)) return false;
+    int32_t node = S.root_ref, leaf = FM_TRAV_NONE;
+    if (node < 0) { leaf = node; node = FM_TRAV_NONE; }
     int sp = 0;
-    for (;;) {
-        if (ref >= 0) {
+    auto pop = [&]() -> int32_t {
+        if (sp == 0) return FM_TRAV_NONE;
+        sp--;
+        return (int32_t)stk[sp * stride];
+    };
+    while (node != FM_TRAV_NONE || leaf != FM_TRAV_NONE) {
+        while (node != FM_TRAV_NONE && node >= 0) {
             if (COUNT) cn.nodes++;
-            const float4* nd = reinterpret_cast<const float4*>(S.nodes + ref);
+            const float4* nd = reinterpret_cast<const float4*>(S.nodes + node);
             const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2];
             const int4 cr = reinterpret_cast<const int4*>(nd)[3];
-            const float b0[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
-            const float b1[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+            const float tmax = ANY ? r.tmax : h.t;
             float t0, t1;
-            const bool h0 = box_hit<FAST>(b0, r, rp, h.t, t0);
-            const bool h1 = box_hit<FAST>(b1, r, rp, h.t, t1);
+            const bool h0 = box_hit6<FAST>(q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, r, rp, tmax, t0);
+            const bool h1 = box_hit6<FAST>(q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r, rp, tmax, t1);
             if (h0 && h1) {
-                const bool first0 = t0 <= t1;
+                const bool first0 = ANY || t0 <= t1;
                 stk[sp * stride] = (uint32_t)(first0 ? cr.y : cr.x);
                 sp++;
-                ref = first0 ? cr.x : cr.y;
-                continue;
+                node = first0 ? cr.x : cr.y;
+            } else if (h0) {
+                node = cr.x;
+            } else if (h1) {
+                node = cr.y;
+            } else {
+                node = pop();
             }
-            if (h0) { ref = cr.x; continue; }
-            if (h1) { ref = cr.y; continue; }
-        } else {
-            const int32_t start = FM_LEAF_START(ref), cnt = FM_LEAF_COUNT(ref);
-            if (COUNT) cn.prims += (uint32_t)cnt;
-            for (int k = 0; k < cnt; k++) test_slot_closest(S, r, rp, start + k, h);
+            if (node < 0 && leaf == FM_TRAV_NONE) { leaf = node; node = pop(); }
+            if (__all(leaf != FM_TRAV_NONE)) break;
         }
-        if (sp == 0) break;
-        sp--;
-        ref = (int32_t)stk[sp * stride];
+        while (leaf != FM_TRAV_NONE) {
+            const int32_t start = FM_LEAF_START(leaf), count = FM_LEAF_COUNT(leaf);
+            if (ANY) {
+                for (int k = 0; k < count; k++) {
+                    if (COUNT) cn.prims++;
+                    if (test_slot_any(S, r, rp, start + k)) return true;
+                }
+            } else {
+                if (COUNT) cn.prims += (uint32_t)count;
+                for (int k = 0; k < count; k++) test_slot_closest(S, r, rp, start + k, h);
+            }
+            if (node < 0) { leaf = node; node = pop(); }
+            else leaf = FM_TRAV_NONE;
+        }
     }
+    return false;
 }
 
 template <bool COUNT = false>
@@ -270,43 +305,9 @@ FMD Hit trace_closest(const fm_dscene& S, const Ray& r, uint32_t* stk, int strid
     const RayPre rp = precompute(r);
     Cnt dummy;
     Cnt& c = COUNT ? *cn : dummy;
-    if (rp.fast) closest_impl<true, COUNT>(S, r, rp, h, stk, stride, c);
-    else closest_impl<false, COUNT>(S, r, rp, h, stk, stride, c);
+    if (rp.fast) traverse_impl<true, false, COUNT>(S, r, rp, h, stk, stride, c);
+    else traverse_impl<false, false, COUNT>(S, r, rp, h, stk, stride, c);
     return h;
-}
-
-template <bool FAST, bool COUNT>
-FMD bool any_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, uint32_t* stk, int stride, Cnt& cn) {
-    float tn;
-    if (!box_hit<FAST>(S.root_box, r, rp, r.tmax, tn)) return false;
-    int32_t ref = S.root_ref;
-    int sp = 0;
-    for (;;) {
-        if (ref >= 0) {
-            if (COUNT) cn.nodes++;
-            const float4* nd = reinterpret_cast<const float4*>(S.nodes + ref);
-            const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2];
-            const int4 cr = reinterpret_cast<const int4*>(nd)[3];
-            const float b0[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
-            const float b1[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
-            float t0, t1;
-            const bool h0 = box_hit<FAST>(b0, r, rp, r.tmax, t0);
-            const bool h1 = box_hit<FAST>(b1, r, rp, r.tmax, t1);
-            if (h0 && h1) { stk[sp * stride] = (uint32_t)cr.y; sp++; ref = cr.x; continue; }
-            if (h0) { ref = cr.x; continue; }
-            if (h1) { ref = cr.y; continue; }
-        } else {
-            const int32_t start = FM_LEAF_START(ref), cnt = FM_LEAF_COUNT(ref);
-            for (int k = 0; k < cnt; k++) {
-                if (COUNT) cn.prims++;
-                if (test_slot_any(S, r, rp, start + k)) return true;
-            }
-        }
-        if (sp == 0) break;
-        sp--;
-        ref = (int32_t)stk[sp * stride];
-    }
-    return false;
 }
 
 template <bool COUNT = false>
@@ -315,7 +316,10 @@ FMD bool trace_any(const fm_dscene& S, const Ray& r, uint32_t* stk, int stride, 
     const RayPre rp = precompute(r);
     Cnt dummy;
     Cnt& c = COUNT ? *cn : dummy;
-    return rp.fast ? any_impl<true, COUNT>(S, r, rp, stk, stride, c) : any_impl<false, COUNT>(S, r, rp, stk, stride, c);
+    Hit h;
+    h.t = r.tmax; h.slot = -1; h.kind = -1; h.tie = 0; h.half = 0; h.b0 = h.b1 = h.b2 = 0.0f;
+    return rp.fast ? traverse_impl<true, true, COUNT>(S, r, rp, h, stk, stride, c)
+                   : traverse_impl<false, true, COUNT>(S, r, rp, h, stk, stride, c);
 }
 
 /* Hit point as the geometry's GetIntersect fills it (triangle.cpp:211-248,

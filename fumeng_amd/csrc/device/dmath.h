@@ -28,7 +28,12 @@ FMD float length(v3 a) { return sqrtf(dot(a, a)); }
 /* glm::normalize = v * (1 / sqrt(dot(v, v))) */
 FMD v3 normalize(v3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
 FMD float absdot(v3 a, v3 b) { return fabsf(dot(a, b)); }
-FMD float comp(v3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+FMD float comp(v3 a, int i) {
+    float r = a.z;
+    r = i == 1 ? a.y : r;
+    r = i == 0 ? a.x : r;
+    return r;
+}
 FMD v3 ld3(const float* p) { return mk(p[0], p[1], p[2]); }
 FMD v3 f4to3(float4 a) { return mk(a.x, a.y, a.z); }
 FMD float lum(v3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
