@@ -197,6 +197,8 @@ def main():
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--mode", choices=["megakernel", "wavefront"], default="wavefront")
+    ap.add_argument("--bvh-width", type=int, choices=[2, 4], default=2,
+                    help="traversal tree width (the closest hit does not depend on it)")
     ap.add_argument("--no-stress", dest="stress", action="store_false",
                     help="skip the 9.8 M-triangle textured stress scene leg")
     ap.add_argument("--no-adjoint", dest="adjoint", action="store_false",
@@ -224,11 +226,14 @@ def main():
     W, H, spp = desc.renderer["width"], desc.renderer["height"], desc.renderer["spp"]
     log("scene generated: %d geometries in %.1f s" % (desc.num_geometries(), time.perf_counter() - t0))
     ctx = fumeng_amd.Context(0)
+    ctx.set_bvh_width(args.bvh_width)
     t0 = time.perf_counter()
     scene = ctx.scene(desc)
     info = scene.info()
-    log("scene built and uploaded in %.1f s: %d BVH nodes, depth %d, %.1f MB in HBM"
-        % (time.perf_counter() - t0, info.num_bvh_nodes, info.bvh_depth, info.device_bytes / 1e6))
+    log("scene built and uploaded in %.1f s: %d BVH nodes, depth %d; traversal tree %d wide, %d nodes, "
+        "%d stack entries; %.1f MB in HBM"
+        % (time.perf_counter() - t0, info.num_bvh_nodes, info.bvh_depth, info.trav_width, info.num_trav_nodes,
+           info.trav_stack, info.device_bytes / 1e6))
 
     from fumeng_amd import _abi
     mode = _abi.FM_MODE_WAVEFRONT if args.mode == "wavefront" else _abi.FM_MODE_MEGAKERNEL
@@ -267,8 +272,9 @@ def main():
 
     # Roofline of the traversal kernel (the wavefront's trace launches, or the
     # whole megakernel): algorithmic bytes per ray = the node and primitive
-    # records a BVH2 traversal must fetch (64 B per visited node, 48 B per
-    # tested primitive), counted by the counting variant of the same kernels
+    # records the traversal must fetch (64 B per visited two-wide node or
+    # 128 B per four-wide node, 48 B per tested primitive), counted by the
+    # counting variant of the same kernels
     # on a 1-spp pass, times the rays of the timed steps, over the summed
     # duration of those launches (HIP events around each launch).
     roofline = None
@@ -277,14 +283,16 @@ def main():
         _, _, _, _, cst = scene.render(seed=2, spp=1, read=False, mode=mode)
         fumeng_amd.lib().fm_set_counting(ctx.h, 0)
         if cst.node_visits > 0 and cst.rays > 0 and trace_ms > 0:
-            bytes_per_ray = (cst.node_visits * 64.0 + cst.prim_tests * 48.0) / cst.rays
+            bytes_per_ray = (cst.node_visits * float(info.trav_node_bytes) + cst.prim_tests * 48.0) / cst.rays
             achieved = bytes_per_ray * rays / (trace_ms * 1e-3) / 1e9
             default_run = (args.freq, args.spp, args.block, args.grid, args.chunk) == (224, 256, 0, 0, 0)
-            kname = "k_wf_trace<false>" if mode == _abi.FM_MODE_WAVEFRONT else "k_render_mega<false>"
+            kname = ("k_wf_trace<false, %d>" % info.trav_width) if mode == _abi.FM_MODE_WAVEFRONT else "k_render_mega<false>"
             traffic = profiled_traffic(kname) if default_run else None
             roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kname,
-                        "bytes_per_ray": bytes_per_ray, "kernel_ms_per_step": trace_ms / args.steps}
+                        "bytes_per_ray": bytes_per_ray, "kernel_ms_per_step": trace_ms / args.steps,
+                        "nodes_per_ray": cst.node_visits / cst.rays, "prims_per_ray": cst.prim_tests / cst.rays,
+                        "node_bytes": info.trav_node_bytes}
     except Exception as e:  # counting is diagnostic; the throughput stands without it
         log("counting pass failed: %s" % e)
 
@@ -321,7 +329,8 @@ def main():
            "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scene)",
            "config": {"workload": "displaced icosphere %d tris + ground, 64 Disney materials, 2048x1024 HDR env, "
                                   "thin lens, %dx%d, %d spp, depth 12, %s" % (20 * args.freq ** 2, W, H, spp, args.mode),
-                      "mode": args.mode, "width": W, "height": H, "spp": spp, "depth": 12},
+                      "mode": args.mode, "width": W, "height": H, "spp": spp, "depth": 12,
+                      "bvh_width": args.bvh_width},
            "mrays_per_s": mrays, "paths": paths, "rays": rays, "other_form": other, "batch": batch, "adjoint": adjoint, "stress": stress,
            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out))

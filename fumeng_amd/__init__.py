@@ -73,6 +73,10 @@ class Context:
         except Exception:
             pass
 
+    def set_bvh_width(self, width):
+        """Traversal tree width (2, the default, or 4) of the scenes created afterwards."""
+        _check(lib().fm_set_bvh_width(self.h, int(width)), self.h)
+
     def scene(self, desc):
         """Upload a SceneDesc (fm_scene_create)."""
         d, keep = desc.to_c()

@@ -93,7 +93,9 @@ class fm_scene_info(C.Structure):
     _fields_ = [("num_geometries", C.c_int64), ("num_bvh_nodes", C.c_int64), ("bvh_depth", C.c_int32),
                 ("num_lights", C.c_int32), ("num_materials", C.c_int32), ("has_env", C.c_int32),
                 ("device_bytes", C.c_int64), ("renderer", fm_renderer_desc), ("camera", fm_camera_desc),
-                ("env_width", C.c_int32), ("env_height", C.c_int32)]
+                ("env_width", C.c_int32), ("env_height", C.c_int32), ("trav_width", C.c_int32),
+                ("trav_stack", C.c_int32), ("num_trav_nodes", C.c_int64), ("trav_node_bytes", C.c_int32),
+                ("pad_", C.c_int32)]
 
 
 class fm_render_params(C.Structure):
@@ -172,6 +174,7 @@ EXPORTS = {
     "fm_render_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(fm_job), C.c_int, C.POINTER(fm_render_params),
                                   C.POINTER(C.c_double), C.POINTER(fm_batch_stats)]),
     "fm_set_counting": (C.c_int, [C.c_void_p, C.c_int]),
+    "fm_set_bvh_width": (C.c_int, [C.c_void_p, C.c_int32]),
     "fm_params_get": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fm_params_set": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fm_render_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(fm_render_params), C.c_void_p, C.c_void_p,

@@ -116,6 +116,7 @@ struct fm_ctx {
     int film_w = 0, film_h = 0;
     int64_t film_spp = 0;     /* samples accumulated into the device film */
     int counting = 0;
+    int bvh_width = 2;        /* traversal tree width of scenes created next */
     std::set<fm_scene*> scenes;  /* live scenes, released by fm_shutdown */
     WfBuffers wf;
     DevBuf wf_passes, wf_units, wf_cams, wf_pass_idx;   /* wavefront tables of the last run */
@@ -128,7 +129,7 @@ struct fm_ctx {
 struct fm_scene {
     fm_ctx* ctx = nullptr;
     fmh::HostScene hs;        /* host copy (kept for info and tests) */
-    DevBuf nodes, prims, meta, tri_sh, rects, spheres, materials, lights, envs, images, texels, env_tables, lut;
+    DevBuf nodes, nodes4, prims, meta, tri_sh, rects, spheres, materials, lights, envs, images, texels, env_tables, lut;
     fm_dscene d{};
     int64_t device_bytes = 0;
 };
@@ -167,6 +168,7 @@ void upload_scene(fm_scene* s) {
     hipStream_t st = s->ctx->stream;
     fmh::HostScene& hs = s->hs;
     s->nodes.upload(hs.nodes, st);
+    s->nodes4.upload(hs.nodes4, st);
     s->prims.upload(hs.prims, st);
     s->meta.upload(hs.meta, st);
     s->tri_sh.upload(hs.tri_sh, st);
@@ -182,6 +184,10 @@ void upload_scene(fm_scene* s) {
     HIPCHECK(hipStreamSynchronize(st));
     fm_dscene& d = s->d;
     d.nodes = s->nodes.as<fm_dnode>();
+    d.nodes4 = s->nodes4.as<fm_dnode4>();
+    d.width = hs.trav_width;
+    d.stack_levels = hs.trav_stack;
+    d.node_bytes = (uint32_t)(hs.nodes.size() * sizeof(fm_dnode));
     d.prims = s->prims.as<float>();
     d.meta = s->meta.as<int32_t>();
     d.tri_sh = s->tri_sh.as<float>();
@@ -204,7 +210,7 @@ void upload_scene(fm_scene* s) {
     memcpy(d.root_box, hs.root_box, sizeof d.root_box);
     d.camera = hs.camera;
     s->device_bytes = 0;
-    for (DevBuf* b : {&s->nodes, &s->prims, &s->meta, &s->tri_sh, &s->rects, &s->spheres, &s->materials, &s->lights,
+    for (DevBuf* b : {&s->nodes, &s->nodes4, &s->prims, &s->meta, &s->tri_sh, &s->rects, &s->spheres, &s->materials, &s->lights,
                       &s->envs, &s->images, &s->texels, &s->env_tables, &s->lut})
         s->device_bytes += (int64_t)b->n;
 }
@@ -337,7 +343,17 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
     L.grid_trace = grid; L.grid_shade = grid;
     for (const fm_dmaterial& m : s->hs.materials)
         if (m.type >= 0 && m.type + 1 < fm::WQ_NCLS) L.has_class[m.type + 1] = true;
-    fm::wavefront_grids(ctx->num_cus, std::max(1, s->hs.trav_depth), L);
+    {
+        bool env = false, area = false, other = false;
+        for (const fm_dlight& l : s->hs.lights) {
+            if (l.kind == FM_DLIGHT_ENV) env = true;
+            else if (l.kind == FM_DLIGHT_TRI || l.kind == FM_DLIGHT_RECT || l.kind == FM_DLIGHT_SPHERE) area = true;
+            else other = true;
+        }
+        L.lights = (env && !area && !other) ? fm::LK_ENV : ((!env && !other) ? fm::LK_AREA : fm::LK_ANY);
+    }
+    L.width = s->hs.trav_width;
+    fm::wavefront_grids(ctx->num_cus, std::max(1, s->hs.trav_stack), L);
 
     const bool count = ctx->counting != 0;
     const bool timed = (flags & FM_RENDER_TIME_TRACE) != 0;
@@ -448,6 +464,7 @@ int fm_scene_create(fm_ctx* ctx, const fm_scene_desc* desc, fm_scene** out) {
         HIPCHECK(hipSetDevice(ctx->device));
         std::unique_ptr<fm_scene> s(new fm_scene());
         s->ctx = ctx;
+        s->hs.trav_width = ctx->bvh_width;
         fmh::build_host_scene(*desc, s->hs);
         upload_scene(s.get());
         *out = s.release();
@@ -489,12 +506,23 @@ int fm_scene_get_info(const fm_scene* s, fm_scene_info* o) {
         o->env_width = s->hs.envs[(size_t)s->hs.env].W;
         o->env_height = s->hs.envs[(size_t)s->hs.env].H;
     }
+    o->trav_width = s->hs.trav_width;
+    o->trav_stack = s->hs.trav_stack;
+    o->num_trav_nodes = (int64_t)(s->hs.trav_width == 4 ? s->hs.nodes4.size() : s->hs.nodes.size());
+    o->trav_node_bytes = s->hs.trav_width == 4 ? (int32_t)sizeof(fm_dnode4) : (int32_t)sizeof(fm_dnode);
     return FM_OK;
 }
 
 int fm_set_counting(fm_ctx* ctx, int enabled) {
     if (!ctx) return fail(nullptr, FM_ERR_INVALID, "NULL ctx");
     ctx->counting = enabled;
+    return FM_OK;
+}
+
+int fm_set_bvh_width(fm_ctx* ctx, int32_t width) {
+    if (!ctx) return fail(nullptr, FM_ERR_INVALID, "NULL ctx");
+    if (width != 2 && width != 4) return fail(ctx, FM_ERR_INVALID, "BVH width must be 2 or 4");
+    ctx->bvh_width = width;
     return FM_OK;
 }
 
@@ -547,7 +575,7 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
         if (!wave) {
             int grid = p->grid_blocks;
             if (grid <= 0) {
-                int per_cu = fm::mega_blocks_per_cu(block);
+                int per_cu = fm::mega_blocks_per_cu(s->d, block);
                 if (per_cu <= 0) per_cu = 1;
                 grid = ctx->num_cus * per_cu * 4;
             }
@@ -810,7 +838,7 @@ int fm_render_adjoint(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, const
         A.env_w = im ? im->w : 0; A.env_h = im ? im->h : 0;
         A.lds_grad = (int64_t)nm * FM_DPARAM_PER_MATERIAL <= 4096 ? 1 : 0;
         A.counters = ctx->counters.as<unsigned long long>();
-        const int block = 128, levels = FM_STACK_DEPTH;
+        const int block = 128, levels = std::max(1, s->hs.trav_stack);
         const size_t lds = (size_t)levels * block * 4 + (A.lds_grad ? (size_t)nm * FM_DPARAM_PER_MATERIAL * 4 : 0);
         int grid = p->grid_blocks > 0 ? p->grid_blocks : ctx->num_cus * fm::adjoint_blocks_per_cu(block, lds) * 2;
         const int64_t need = (A.total_items + block - 1) / block;

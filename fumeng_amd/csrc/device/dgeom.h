@@ -242,32 +242,48 @@ struct Cnt { uint32_t nodes = 0, prims = 0; };
  * closest hit does not depend on the order (order-free tie rule). */
 #define FM_TRAV_NONE 0x7fffffff   /* no node / no pending leaf (leaf refs are negative) */
 
+/* Node records through a buffer resource: 32-bit byte offsets (one shift
+ * per node instead of 64-bit address arithmetic) and hardware range checks
+ * (a read past the array returns zeros instead of faulting). */
+FMD __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+FMD float4 buf_load4(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
+}
+FMD int4 buf_load4i(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    return make_int4((int)v[0], (int)v[1], (int)v[2], (int)v[3]);
+}
+
 template <bool FAST, bool ANY, bool COUNT>
 FMD bool traverse_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& h, uint32_t* stk, int stride, Cnt& cn) {
+    const __amdgpu_buffer_rsrc_t nrs = buf_rsrc(S.nodes, S.node_bytes);
     float tn;
     if (!box_hit<FAST>(S.root_box, r, rp, ANY ? r.tmax : h.t, tn)) return false;
     int32_t node = S.root_ref, leaf = FM_TRAV_NONE;
     if (node < 0) { leaf = node; node = FM_TRAV_NONE; }
-    int sp = 0;
+    int sp = 0;   /* stack depth times stride: no multiply per push / pop */
     auto pop = [&]() -> int32_t {
         if (sp == 0) return FM_TRAV_NONE;
-        sp--;
-        return (int32_t)stk[sp * stride];
+        sp -= stride;
+        return (int32_t)stk[sp];
     };
     while (node != FM_TRAV_NONE || leaf != FM_TRAV_NONE) {
         while (node != FM_TRAV_NONE && node >= 0) {
             if (COUNT) cn.nodes++;
-            const float4* nd = reinterpret_cast<const float4*>(S.nodes + node);
-            const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2];
-            const int4 cr = reinterpret_cast<const int4*>(nd)[3];
+            const uint32_t off = (uint32_t)node << 6;
+            const float4 q0 = buf_load4(nrs, off), q1 = buf_load4(nrs, off + 16u), q2 = buf_load4(nrs, off + 32u);
+            const int4 cr = buf_load4i(nrs, off + 48u);
             const float tmax = ANY ? r.tmax : h.t;
             float t0, t1;
             const bool h0 = box_hit6<FAST>(q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, r, rp, tmax, t0);
             const bool h1 = box_hit6<FAST>(q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r, rp, tmax, t1);
             if (h0 && h1) {
                 const bool first0 = ANY || t0 <= t1;
-                stk[sp * stride] = (uint32_t)(first0 ? cr.y : cr.x);
-                sp++;
+                stk[sp] = (uint32_t)(first0 ? cr.y : cr.x);
+                sp += stride;
                 node = first0 ? cr.x : cr.y;
             } else if (h0) {
                 node = cr.x;
@@ -297,7 +313,85 @@ FMD bool traverse_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& 
     return false;
 }
 
-template <bool COUNT = false>
+/* The same traversal over four-wide nodes.  The children a ray enters are
+ * ordered by entry distance with one 4-key sorting network (key = distance
+ * bits with the child index in the two low bits; entry distances are >= 0,
+ * so the unsigned order of the bits is their order); the nearest is entered
+ * and the others are pushed far to near. */
+FMD int32_t child_of4(int4 c, uint32_t key) {
+    const uint32_t i = key & 3u;
+    return i == 0u ? c.x : (i == 1u ? c.y : (i == 2u ? c.z : c.w));
+}
+FMD void sort2u(uint32_t& a, uint32_t& b) {
+    const uint32_t lo = min(a, b), hi = max(a, b);
+    a = lo; b = hi;
+}
+
+template <bool FAST, bool ANY, bool COUNT>
+FMD bool traverse4_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& h, uint32_t* stk, int stride, Cnt& cn) {
+    float tn;
+    if (!box_hit<FAST>(S.root_box, r, rp, ANY ? r.tmax : h.t, tn)) return false;
+    int32_t node = S.root_ref, leaf = FM_TRAV_NONE;
+    if (node < 0) { leaf = node; node = FM_TRAV_NONE; }
+    int sp = 0;   /* stack depth times stride: no multiply per push / pop */
+    auto pop = [&]() -> int32_t {
+        if (sp == 0) return FM_TRAV_NONE;
+        sp -= stride;
+        return (int32_t)stk[sp];
+    };
+    constexpr uint32_t MISS = 0xffffffffu;
+    while (node != FM_TRAV_NONE || leaf != FM_TRAV_NONE) {
+        while (node != FM_TRAV_NONE && node >= 0) {
+            if (COUNT) cn.nodes++;
+            const float4* nd = reinterpret_cast<const float4*>(S.nodes4 + node);
+            const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
+            const int4 cr = reinterpret_cast<const int4*>(nd)[6];
+            const float tmax = ANY ? r.tmax : h.t;
+            /* all four tests unconditionally (children 0 and 1 always exist;
+             * an empty slot's box is masked after the test), so that the
+             * whole node is fetched at once */
+            float t0, t1, t2, t3;
+            const bool b0 = box_hit6<FAST>(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, r, rp, tmax, t0);
+            const bool b1 = box_hit6<FAST>(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, r, rp, tmax, t1);
+            const bool b2 = box_hit6<FAST>(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, r, rp, tmax, t2) && cr.z != FM_NODE4_EMPTY;
+            const bool b3 = box_hit6<FAST>(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, r, rp, tmax, t3) && cr.w != FM_NODE4_EMPTY;
+            uint32_t k0 = b0 ? (__float_as_uint(t0) & ~3u) | 0u : MISS;
+            uint32_t k1 = b1 ? (__float_as_uint(t1) & ~3u) | 1u : MISS;
+            uint32_t k2 = b2 ? (__float_as_uint(t2) & ~3u) | 2u : MISS;
+            uint32_t k3 = b3 ? (__float_as_uint(t3) & ~3u) | 3u : MISS;
+            sort2u(k0, k1); sort2u(k2, k3); sort2u(k0, k2); sort2u(k1, k3); sort2u(k1, k2);
+            if (k0 == MISS) {
+                node = pop();
+            } else {
+                if (k3 != MISS) { stk[sp] = (uint32_t)child_of4(cr, k3); sp += stride; }
+                if (k2 != MISS) { stk[sp] = (uint32_t)child_of4(cr, k2); sp += stride; }
+                if (k1 != MISS) { stk[sp] = (uint32_t)child_of4(cr, k1); sp += stride; }
+                node = child_of4(cr, k0);
+            }
+            if (node < 0 && leaf == FM_TRAV_NONE) { leaf = node; node = pop(); }
+            if (__all(leaf != FM_TRAV_NONE)) break;
+        }
+        while (leaf != FM_TRAV_NONE) {
+            const int32_t start = FM_LEAF_START(leaf), count = FM_LEAF_COUNT(leaf);
+            if (ANY) {
+                for (int k = 0; k < count; k++) {
+                    if (COUNT) cn.prims++;
+                    if (test_slot_any(S, r, rp, start + k)) return true;
+                }
+            } else {
+                if (COUNT) cn.prims += (uint32_t)count;
+                for (int k = 0; k < count; k++) test_slot_closest(S, r, rp, start + k, h);
+            }
+            if (node < 0) { leaf = node; node = pop(); }
+            else leaf = FM_TRAV_NONE;
+        }
+    }
+    return false;
+}
+
+/* WIDTH = 2 or 4 compiles one traversal only (the caller knows the scene's
+ * width); 0 branches on S.width. */
+template <bool COUNT = false, int WIDTH = 0>
 FMD Hit trace_closest(const fm_dscene& S, const Ray& r, uint32_t* stk, int stride, Cnt* cn = nullptr) {
     Hit h;
     h.t = r.tmax; h.slot = -1; h.kind = -1; h.tie = 0; h.half = 0; h.b0 = h.b1 = h.b2 = 0.0f;
@@ -305,12 +399,17 @@ FMD Hit trace_closest(const fm_dscene& S, const Ray& r, uint32_t* stk, int strid
     const RayPre rp = precompute(r);
     Cnt dummy;
     Cnt& c = COUNT ? *cn : dummy;
-    if (rp.fast) traverse_impl<true, false, COUNT>(S, r, rp, h, stk, stride, c);
-    else traverse_impl<false, false, COUNT>(S, r, rp, h, stk, stride, c);
+    if (WIDTH == 4 || (WIDTH == 0 && S.width == 4)) {
+        if (rp.fast) traverse4_impl<true, false, COUNT>(S, r, rp, h, stk, stride, c);
+        else traverse4_impl<false, false, COUNT>(S, r, rp, h, stk, stride, c);
+    } else {
+        if (rp.fast) traverse_impl<true, false, COUNT>(S, r, rp, h, stk, stride, c);
+        else traverse_impl<false, false, COUNT>(S, r, rp, h, stk, stride, c);
+    }
     return h;
 }
 
-template <bool COUNT = false>
+template <bool COUNT = false, int WIDTH = 0>
 FMD bool trace_any(const fm_dscene& S, const Ray& r, uint32_t* stk, int stride, Cnt* cn = nullptr) {
     if (S.num_slots == 0) return false;
     const RayPre rp = precompute(r);
@@ -318,6 +417,9 @@ FMD bool trace_any(const fm_dscene& S, const Ray& r, uint32_t* stk, int stride, 
     Cnt& c = COUNT ? *cn : dummy;
     Hit h;
     h.t = r.tmax; h.slot = -1; h.kind = -1; h.tie = 0; h.half = 0; h.b0 = h.b1 = h.b2 = 0.0f;
+    if (WIDTH == 4 || (WIDTH == 0 && S.width == 4))
+        return rp.fast ? traverse4_impl<true, true, COUNT>(S, r, rp, h, stk, stride, c)
+                       : traverse4_impl<false, true, COUNT>(S, r, rp, h, stk, stride, c);
     return rp.fast ? traverse_impl<true, true, COUNT>(S, r, rp, h, stk, stride, c)
                    : traverse_impl<false, true, COUNT>(S, r, rp, h, stk, stride, c);
 }

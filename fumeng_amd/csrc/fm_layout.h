@@ -28,13 +28,25 @@
 #define FM_LEAF_START(r) ((int32_t)(((uint32_t)(~(r))) >> 3))
 #define FM_LEAF_COUNT(r) ((int32_t)(((uint32_t)(~(r))) & 7u))
 
-#define FM_STACK_DEPTH 32
+#define FM_STACK_DEPTH 32   /* depth limit of the binary traversal trees */
+#define FM_STACK_MAX 64     /* LDS traversal stack entries per lane, at most */
+
+/* Traversal trees are two-wide (fm_dnode) or four-wide (fm_dnode4). */
+#define FM_NODE4_EMPTY 0x7fffffff   /* unused child of a four-wide node */
 
 typedef struct fm_dnode {
     float b[12];     /* c0 min xyz, c0 max xyz, c1 min xyz, c1 max xyz */
     int32_t child[2];
     int32_t pad[2];
 } fm_dnode;          /* 64 bytes */
+
+/* Four child boxes as a structure of arrays (one float4 per bound plane,
+ * lane k = child k), then the four child refs. */
+typedef struct fm_dnode4 {
+    float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
+    int32_t child[4];
+    int32_t pad[4];
+} fm_dnode4;         /* 128 bytes */
 
 typedef struct fm_drect {
     float pa[4], pb[4], pc[4], pd[4]; /* xyz + pad */
@@ -108,7 +120,8 @@ typedef struct fm_dcamera {
 
 /* Everything a kernel needs, passed by value. */
 typedef struct fm_dscene {
-    const fm_dnode* nodes;
+    const fm_dnode* nodes;       /* width 2 */
+    const fm_dnode4* nodes4;     /* width 4 */
     const float* prims;          /* float4 x 3 per slot */
     const int32_t* meta;         /* int4 per slot */
     const float* tri_sh;         /* float4 x 4 per slot */
@@ -128,8 +141,10 @@ typedef struct fm_dscene {
     int32_t env;                 /* env table index or -1 */
     int32_t naive;               /* brute force in object order */
     int32_t num_materials;
-    int32_t pad;
+    int32_t width;               /* traversal tree: 2 or 4 */
     float root_box[6];
+    int32_t stack_levels;        /* LDS stack entries a traversal needs */
+    uint32_t node_bytes;         /* size of nodes (two-wide), < 4 GiB */
     fm_dcamera camera;
 } fm_dscene;
 

@@ -630,6 +630,7 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
     for (const TNode& t : tn)
         if (t.leaf && t.b - t.a > 7) throw BuildError(FM_ERR_UNSUPPORTED, "leaf larger than 7 primitives");
     hs.trav_depth = tree_depth;
+    if (hs.trav_width != 2 && hs.trav_width != 4) throw BuildError(FM_ERR_INVALID, "traversal tree width must be 2 or 4");
 
     /* ---- flatten: interior nodes in index order become device nodes ---- */
     std::vector<int32_t> dev_id(tn.size(), -1);
@@ -640,19 +641,78 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
         if (n.leaf) return FM_LEAF_REF(n.a, n.b - n.a);
         return dev_id[(size_t)i];
     };
-    hs.nodes.assign((size_t)nint, fm_dnode{});
-    for (size_t i = 0; i < tn.size(); i++) {
-        const TNode& n = tn[i];
-        if (n.leaf) continue;
-        fm_dnode& o = hs.nodes[(size_t)dev_id[i]];
-        const float* a = &node_boxes[(size_t)n.a * 6];
-        const float* b = &node_boxes[(size_t)n.b * 6];
-        for (int k = 0; k < 6; k++) { o.b[k] = a[k]; o.b[6 + k] = b[k]; }
-        o.child[0] = ref_of(n.a);
-        o.child[1] = ref_of(n.b);
+    hs.nodes.clear();
+    hs.nodes4.clear();
+    if (hs.trav_width == 2) {
+        hs.nodes.assign((size_t)nint, fm_dnode{});
+        for (size_t i = 0; i < tn.size(); i++) {
+            const TNode& n = tn[i];
+            if (n.leaf) continue;
+            fm_dnode& o = hs.nodes[(size_t)dev_id[i]];
+            const float* a = &node_boxes[(size_t)n.a * 6];
+            const float* b = &node_boxes[(size_t)n.b * 6];
+            for (int k = 0; k < 6; k++) { o.b[k] = a[k]; o.b[6 + k] = b[k]; }
+            o.child[0] = ref_of(n.a);
+            o.child[1] = ref_of(n.b);
+        }
+        if (hs.nodes.size() * sizeof(fm_dnode) >= ((size_t)1 << 32))
+            throw BuildError(FM_ERR_UNSUPPORTED, "two-wide traversal tree of 4 GiB or more");
+        /* one entry pushed per inner node on the way down */
+        hs.trav_stack = std::max(1, tree_depth - 1);
+    } else if (G > 0 && !tn[0].leaf) {
+        /* Four-wide tree collapsed from the binary one: a node starts from
+         * the two children of a binary node and opens its inner child of
+         * largest surface area until it holds four children or only leaves.
+         * The traversal pushes at most (children - 1) refs per node on the
+         * way down, so the stack never holds more than the largest sum of
+         * (children - 1) over a root-to-leaf path. */
+        auto area = [&](int64_t i) {
+            const float* b = &node_boxes[(size_t)i * 6];
+            const double dx = (double)b[3] - b[0], dy = (double)b[4] - b[1], dz = (double)b[5] - b[2];
+            return dx * dy + dy * dz + dz * dx;
+        };
+        std::function<int32_t(int64_t, int&)> make4 = [&](int64_t bin, int& need) -> int32_t {
+            int64_t c[4] = {tn[(size_t)bin].a, tn[(size_t)bin].b, -1, -1};
+            int n = 2;
+            while (n < 4) {
+                int best = -1;
+                double ba = -1.0;
+                for (int k = 0; k < n; k++)
+                    if (!tn[(size_t)c[k]].leaf && area(c[k]) > ba) { ba = area(c[k]); best = k; }
+                if (best < 0) break;
+                const int64_t x = c[best];
+                c[best] = tn[(size_t)x].a;
+                c[n++] = tn[(size_t)x].b;
+            }
+            const int32_t id = (int32_t)hs.nodes4.size();
+            hs.nodes4.emplace_back();
+            int32_t refs[4] = {FM_NODE4_EMPTY, FM_NODE4_EMPTY, FM_NODE4_EMPTY, FM_NODE4_EMPTY};
+            int below = 0;
+            for (int k = 0; k < n; k++) {
+                if (tn[(size_t)c[k]].leaf) { refs[k] = ref_of(c[k]); continue; }
+                int nk = 0;
+                refs[k] = make4(c[k], nk);
+                below = std::max(below, nk);
+            }
+            fm_dnode4& o = hs.nodes4[(size_t)id];
+            for (int k = 0; k < 4; k++) {
+                o.child[k] = refs[k];
+                const float* b = k < n ? &node_boxes[(size_t)c[k] * 6] : nullptr;
+                o.lo_x[k] = b ? b[0] : 0.0f; o.lo_y[k] = b ? b[1] : 0.0f; o.lo_z[k] = b ? b[2] : 0.0f;
+                o.hi_x[k] = b ? b[3] : 0.0f; o.hi_y[k] = b ? b[4] : 0.0f; o.hi_z[k] = b ? b[5] : 0.0f;
+            }
+            need = (n - 1) + below;
+            return id;
+        };
+        int need = 0;
+        make4(0, need);
+        hs.trav_stack = std::max(1, need);
+    } else {
+        hs.trav_stack = 1;
     }
+    if (hs.trav_stack > FM_STACK_MAX) throw BuildError(FM_ERR_UNSUPPORTED, "traversal needs more stack than FM_STACK_MAX");
     if (G > 0) {
-        hs.root_ref = ref_of(0);
+        hs.root_ref = hs.trav_width == 4 && !tn[0].leaf ? 0 : ref_of(0);
         for (int k = 0; k < 6; k++) hs.root_box[k] = node_boxes[(size_t)k];
     }
 

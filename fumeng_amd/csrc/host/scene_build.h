@@ -29,10 +29,13 @@ struct HostScene {
     std::vector<RefNode> ref_nodes;
     std::vector<int32_t> order;         /* geometry index per leaf slot */
     int32_t bvh_depth = 0;          /* reference tree */
-    int32_t trav_depth = 0;         /* traversal tree */
+    int32_t trav_depth = 0;         /* traversal tree (binary form) */
+    int32_t trav_width = 2;         /* set before the build: 2 or 4 */
+    int32_t trav_stack = 1;         /* LDS stack entries the traversal needs */
     int64_t num_geoms = 0;
 
-    std::vector<fm_dnode> nodes;
+    std::vector<fm_dnode> nodes;        /* width 2 */
+    std::vector<fm_dnode4> nodes4;      /* width 4 */
     std::vector<float> prims;            /* 12 floats per slot */
     std::vector<int32_t> meta;           /* 4 ints per slot */
     std::vector<float> tri_sh;           /* 16 floats per slot */

@@ -8,7 +8,7 @@
  *   k_trace         closest / any hit for a batch of rays (parity hook)
  *   k_camera, k_bsdf, k_light, k_env   parity hooks for the other rows
  *
- * Every traversal keeps its stack in LDS: FM_STACK_DEPTH entries per lane,
+ * Every traversal keeps its stack in LDS: the scene's stack_levels entries per lane,
  * stored level-major ([level][lane]) so that concurrent pushes of a wave hit
  * distinct banks.
  */
@@ -182,11 +182,11 @@ inline int blocks_for(int64_t n, int bs) { return (int)std::min<int64_t>((n + bs
 
 namespace fm {
 
-static size_t stack_bytes(int bs) { return (size_t)FM_STACK_DEPTH * (size_t)bs * sizeof(uint32_t); }
+static size_t stack_bytes(const fm_dscene& S, int bs) { return (size_t)S.stack_levels * (size_t)bs * sizeof(uint32_t); }
 
 hipError_t launch_trace(const fm_dscene& S, const fm_ray* rays, int64_t n, int any, fm_hit* out, hipStream_t st) {
     const int bs = 128;
-    hipLaunchKernelGGL(k_trace, dim3(blocks_for(n, bs)), dim3(bs), stack_bytes(bs), st, S, rays, n, any, out);
+    hipLaunchKernelGGL(k_trace, dim3(blocks_for(n, bs)), dim3(bs), stack_bytes(S, bs), st, S, rays, n, any, out);
     return hipGetLastError();
 }
 hipError_t launch_camera(const fm_dscene& S, const float* nl, int64_t n, fm_ray* out, hipStream_t st) {
@@ -210,13 +210,13 @@ hipError_t launch_zero_f64(double* p, int64_t n, hipStream_t st) {
     return hipGetLastError();
 }
 hipError_t launch_render_mega(const fm_dscene& S, const RenderArgs& A, int block, int grid, bool count, hipStream_t st) {
-    if (count) hipLaunchKernelGGL(k_render_mega<true>, dim3(grid), dim3(block), stack_bytes(block), st, S, A);
-    else hipLaunchKernelGGL(k_render_mega<false>, dim3(grid), dim3(block), stack_bytes(block), st, S, A);
+    if (count) hipLaunchKernelGGL(k_render_mega<true>, dim3(grid), dim3(block), stack_bytes(S, block), st, S, A);
+    else hipLaunchKernelGGL(k_render_mega<false>, dim3(grid), dim3(block), stack_bytes(S, block), st, S, A);
     return hipGetLastError();
 }
-int mega_blocks_per_cu(int block) {
+int mega_blocks_per_cu(const fm_dscene& S, int block) {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_render_mega<false>, block, stack_bytes(block)) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_render_mega<false>, block, stack_bytes(S, block)) != hipSuccess) return 0;
     return n;
 }
 

@@ -28,6 +28,10 @@ enum { WQ_EXT = 0, WQ_SH = 1, WQ_CLS = 2, WQ_NCLS = 4, WQ_N = WQ_CLS + WQ_NCLS }
 /* shading classes: WQ_CLS + c, c = 0 path end (miss, emitter seen by the
  * camera, or only deferred direct light left), 1 + material type */
 enum { WC_END = 0, WC_LAMBERT = 1, WC_DISNEY = 2, WC_SPEC = 3 };
+/* light kinds of a scene, for shading kernels compiled without the others:
+ * env only (plus nothing else), area lights only (triangle / rectangle /
+ * sphere, no env), anything */
+enum { LK_ANY = 0, LK_ENV = 1, LK_AREA = 2 };
 
 /* A unit of a pass: k samples of every pixel of one job (paths
  * path_base + j * npix + pixel, j < k; samples sample_base + j).  `film`
@@ -89,6 +93,8 @@ struct WfLaunch {
     int grid_class[WQ_NCLS];       /* persistent grid per shading class (its own occupancy) */
     int stack_levels;
     bool has_class[WQ_NCLS];
+    int lights;                    /* LK_* */
+    int width;                     /* traversal tree width of the scene */
 };
 
 /* Kernel sequence of one pass (camera, n_iter bounce iterations, film, pass
@@ -126,7 +132,7 @@ hipError_t launch_env(const fm_dscene& S, const float* dirs, int64_t n, float* p
 hipError_t launch_zero_f64(double* p, int64_t n, hipStream_t st);
 hipError_t launch_render_mega(const fm_dscene& S, const RenderArgs& A, int block, int grid, bool count, hipStream_t st);
 /* Blocks per CU the megakernel reaches at this block size (occupancy query). */
-int mega_blocks_per_cu(int block);
+int mega_blocks_per_cu(const fm_dscene& S, int block);
 
 }  // namespace fm
 #endif

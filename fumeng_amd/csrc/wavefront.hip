@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(256) k_wf_camera(fm_dscene S, WfArgs A) {
  * slots for the closest hit, [n_ext, n_ext + n_sh) shadow records for the
  * any hit, so the two share the machine (and the latency of a short tail
  * iteration is paid once). */
-template <bool COUNT>
+template <bool COUNT, int WIDTH>
 __global__ void __launch_bounds__(256) k_wf_trace(fm_dscene S, WfArgs A) {
     uint32_t* stk = wf_stack();
     const int stride = blockDim.x;
@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(256) k_wf_trace(fm_dscene S, WfArgs A) {
         if (i < (int64_t)n_ext) {
             const int r = q[i];
             const Ray tr = load_ray(A, r);
-            const Hit h = trace_closest<COUNT>(S, tr, stk, stride, &cn);
+            const Hit h = trace_closest<COUNT, WIDTH>(S, tr, stk, stride, &cn);
             reinterpret_cast<float4*>(A.hit_a)[r] =
                 make_float4(h.t, __int_as_float(h.slot), __int_as_float((h.kind & 0xff) | (h.half << 8)), h.b0);
             reinterpret_cast<float2*>(A.hit_b)[r] = make_float2(h.b1, h.b2);
@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(256) k_wf_trace(fm_dscene S, WfArgs A) {
             const float4 d = reinterpret_cast<const float4*>(A.sh_d)[k];
             Ray tr;
             tr.o = xyz(o); tr.tmin = o.w; tr.d = xyz(d); tr.tmax = d.w;
-            const bool occluded = trace_any<COUNT>(S, tr, stk, stride, &cn);
+            const bool occluded = trace_any<COUNT, WIDTH>(S, tr, stk, stride, &cn);
             A.sh_vis[k] = occluded ? 1 : 2;
         }
     }
@@ -212,6 +212,7 @@ __global__ void __launch_bounds__(256) k_wf_trace(fm_dscene S, WfArgs A) {
 /* Direct light of the previous vertex, in the megakernel's order: per pass
  * the light samples whose shadow rays were unoccluded, then the MisBSDF
  * contribution (path_tracing.cpp:84-92, 122-229). */
+template <int LK>
 FMD v3 resolve_direct(const fm_dscene& S, const WfArgs& A, int64_t p) {
     v3 direct = splat(0.0f);
     for (int pass = 0; pass < A.D; pass++) {
@@ -230,7 +231,7 @@ FMD v3 resolve_direct(const fm_dscene& S, const WfArgs& A, int64_t p) {
         const Ray tr = load_ray(A, A.P + m);
         const Hit h = load_hit(A, A.P + m);
         if (h.slot < 0) {
-            if (S.env >= 0) {
+            if (LK != fm::LK_AREA && S.env >= 0) {
                 const fm_denv& E = S.envs[S.env];
                 const v3 f = msf * env_radiance(S, E, -tr.d) * absdot(tr.d, ng);
                 if (delta) direct += beta * (f / mpdf);
@@ -239,7 +240,7 @@ FMD v3 resolve_direct(const fm_dscene& S, const WfArgs& A, int64_t p) {
                     direct += beta * (f / mpdf * ((mpdf * mpdf) / (mpdf * mpdf + lp * lp)));
                 }
             }
-        } else {
+        } else if (LK != fm::LK_ENV) {
             const Surf ls = fill_hit(S, tr, h);
             if (ls.light >= 0) {
                 const fm_dlight& L = S.lights[ls.light];
@@ -259,7 +260,14 @@ FMD v3 resolve_direct(const fm_dscene& S, const WfArgs& A, int64_t p) {
     return direct;
 }
 
-template <int C>
+/* Light sampling for a scene whose lights are all of the kinds LK allows. */
+template <int LK>
+FMD LightSample light_sample_k(const fm_dscene& S, const fm_dlight& L, v3 pos, v3 s) {
+    if (LK == fm::LK_ENV) return env_sample(S, S.envs[L.aux], pos, s);
+    return light_sample(S, L, pos, s);
+}
+
+template <int C, int LK>
 __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
     const int it = A.iter, nx = it + 1;
     const unsigned n = *cnt(A, it, fm::WQ_CLS + C);
@@ -272,7 +280,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
         v3 beta = xyz(b4), color = xyz(c4);
         int depth = __float_as_int(b4.w);
         const uint32_t dim = __float_as_uint(c4.w);
-        if (fl & F_PENDING) color += resolve_direct(S, A, p) / (float)A.D;
+        if (fl & F_PENDING) color += resolve_direct<LK>(S, A, p) / (float)A.D;
         bool any = false, term = true;
         uint32_t dim_out = dim;
         if constexpr (C == fm::WC_END) {
@@ -280,8 +288,8 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
                 const Ray tr = load_ray(A, p);
                 const Hit h = load_hit(A, p);
                 if (h.slot < 0) {
-                    if (depth == 1 && S.env >= 0) color = env_radiance(S, S.envs[S.env], -tr.d);
-                } else {
+                    if (LK != fm::LK_AREA && depth == 1 && S.env >= 0) color = env_radiance(S, S.envs[S.env], -tr.d);
+                } else if (LK != fm::LK_ENV) {
                     const Surf sf = fill_hit(S, tr, h);
                     if (dot(sf.ng, sf.wo) > 0) color += beta * ld3(S.lights[sf.light].radiance);
                 }
@@ -305,7 +313,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
                 for (int li = 0; li < A.NL; li++) {
                     const int64_t k = ((int64_t)p * A.D + pass) * A.NL + li;
                     const fm_dlight& L = S.lights[li];
-                    const LightSample ls = light_sample(S, L, sf.pos, rng.next3());
+                    const LightSample ls = light_sample_k<LK>(S, L, sf.pos, rng.next3());
                     bool ok = false;
                     if (!(ls.pdf < FM_EPS_PDF || length(ls.radiance) < FM_EPS_PDF)) {
                         const v3 f = b.eval(sf.wo, ls.wi);
@@ -414,18 +422,44 @@ int occupancy(K kern, int block, size_t lds) {
 
 namespace fm {
 
+void launch_shades(const fm_dscene& S, const WfArgs& A, const WfLaunch& L, hipStream_t st);
+
 void wavefront_grids(int num_cus, int stack_levels, WfLaunch& L) {
     L.stack_levels = stack_levels;
     if (L.block_trace <= 0) L.block_trace = 256;
     if (L.block_shade <= 0) L.block_shade = 256;
     const size_t lds = (size_t)stack_levels * L.block_trace * sizeof(uint32_t);
-    if (L.grid_trace <= 0) L.grid_trace = num_cus * occupancy(k_wf_trace<false>, L.block_trace, lds);
+    if (L.grid_trace <= 0)
+        L.grid_trace = num_cus * (L.width == 4 ? occupancy(k_wf_trace<false, 4>, L.block_trace, lds)
+                                               : occupancy(k_wf_trace<false, 2>, L.block_trace, lds));
     const int g = L.grid_shade;
-    L.grid_class[WC_END] = g > 0 ? g : num_cus * occupancy(k_wf_shade<WC_END>, L.block_shade, 0);
-    L.grid_class[WC_LAMBERT] = g > 0 ? g : num_cus * occupancy(k_wf_shade<WC_LAMBERT>, L.block_shade, 0);
-    L.grid_class[WC_DISNEY] = g > 0 ? g : num_cus * occupancy(k_wf_shade<WC_DISNEY>, L.block_shade, 0);
-    L.grid_class[WC_SPEC] = g > 0 ? g : num_cus * occupancy(k_wf_shade<WC_SPEC>, L.block_shade, 0);
+    auto occ4 = [&](auto k0, auto k1, auto k2, auto k3) {
+        L.grid_class[WC_END] = g > 0 ? g : num_cus * occupancy(k0, L.block_shade, 0);
+        L.grid_class[WC_LAMBERT] = g > 0 ? g : num_cus * occupancy(k1, L.block_shade, 0);
+        L.grid_class[WC_DISNEY] = g > 0 ? g : num_cus * occupancy(k2, L.block_shade, 0);
+        L.grid_class[WC_SPEC] = g > 0 ? g : num_cus * occupancy(k3, L.block_shade, 0);
+    };
+    if (L.lights == LK_ENV) occ4(k_wf_shade<WC_END, LK_ENV>, k_wf_shade<WC_LAMBERT, LK_ENV>, k_wf_shade<WC_DISNEY, LK_ENV>, k_wf_shade<WC_SPEC, LK_ENV>);
+    else if (L.lights == LK_AREA) occ4(k_wf_shade<WC_END, LK_AREA>, k_wf_shade<WC_LAMBERT, LK_AREA>, k_wf_shade<WC_DISNEY, LK_AREA>, k_wf_shade<WC_SPEC, LK_AREA>);
+    else occ4(k_wf_shade<WC_END, LK_ANY>, k_wf_shade<WC_LAMBERT, LK_ANY>, k_wf_shade<WC_DISNEY, LK_ANY>, k_wf_shade<WC_SPEC, LK_ANY>);
     if (L.grid_shade <= 0) L.grid_shade = L.grid_class[WC_END];
+}
+
+template <int LK>
+void launch_shades_k(const fm_dscene& S, const WfArgs& A, const WfLaunch& L, hipStream_t st) {
+    hipLaunchKernelGGL((k_wf_shade<WC_END, LK>), dim3(L.grid_class[WC_END]), dim3(L.block_shade), 0, st, S, A);
+    if (L.has_class[WC_LAMBERT])
+        hipLaunchKernelGGL((k_wf_shade<WC_LAMBERT, LK>), dim3(L.grid_class[WC_LAMBERT]), dim3(L.block_shade), 0, st, S, A);
+    if (L.has_class[WC_DISNEY])
+        hipLaunchKernelGGL((k_wf_shade<WC_DISNEY, LK>), dim3(L.grid_class[WC_DISNEY]), dim3(L.block_shade), 0, st, S, A);
+    if (L.has_class[WC_SPEC])
+        hipLaunchKernelGGL((k_wf_shade<WC_SPEC, LK>), dim3(L.grid_class[WC_SPEC]), dim3(L.block_shade), 0, st, S, A);
+}
+
+void launch_shades(const fm_dscene& S, const WfArgs& A, const WfLaunch& L, hipStream_t st) {
+    if (L.lights == LK_ENV) launch_shades_k<LK_ENV>(S, A, L, st);
+    else if (L.lights == LK_AREA) launch_shades_k<LK_AREA>(S, A, L, st);
+    else launch_shades_k<LK_ANY>(S, A, L, st);
 }
 
 hipError_t launch_wavefront_pass(const fm_dscene& S, WfArgs A, const WfLaunch& L, bool count, hipStream_t st,
@@ -436,13 +470,15 @@ hipError_t launch_wavefront_pass(const fm_dscene& S, WfArgs A, const WfLaunch& L
     for (int it = 0; it < A.n_iter; it++) {
         A.iter = it;
         if (ev) hipEventRecord(ev[2 * it], st);
-        if (count) hipLaunchKernelGGL(k_wf_trace<true>, dim3(L.grid_trace), dim3(L.block_trace), lds, st, S, A);
-        else hipLaunchKernelGGL(k_wf_trace<false>, dim3(L.grid_trace), dim3(L.block_trace), lds, st, S, A);
+        if (L.width == 4) {
+            if (count) hipLaunchKernelGGL((k_wf_trace<true, 4>), dim3(L.grid_trace), dim3(L.block_trace), lds, st, S, A);
+            else hipLaunchKernelGGL((k_wf_trace<false, 4>), dim3(L.grid_trace), dim3(L.block_trace), lds, st, S, A);
+        } else {
+            if (count) hipLaunchKernelGGL((k_wf_trace<true, 2>), dim3(L.grid_trace), dim3(L.block_trace), lds, st, S, A);
+            else hipLaunchKernelGGL((k_wf_trace<false, 2>), dim3(L.grid_trace), dim3(L.block_trace), lds, st, S, A);
+        }
         if (ev) hipEventRecord(ev[2 * it + 1], st);
-        hipLaunchKernelGGL(k_wf_shade<WC_END>, dim3(L.grid_class[WC_END]), dim3(L.block_shade), 0, st, S, A);
-        if (L.has_class[WC_LAMBERT]) hipLaunchKernelGGL(k_wf_shade<WC_LAMBERT>, dim3(L.grid_class[WC_LAMBERT]), dim3(L.block_shade), 0, st, S, A);
-        if (L.has_class[WC_DISNEY]) hipLaunchKernelGGL(k_wf_shade<WC_DISNEY>, dim3(L.grid_class[WC_DISNEY]), dim3(L.block_shade), 0, st, S, A);
-        if (L.has_class[WC_SPEC]) hipLaunchKernelGGL(k_wf_shade<WC_SPEC>, dim3(L.grid_class[WC_SPEC]), dim3(L.block_shade), 0, st, S, A);
+        launch_shades(S, A, L, st);
     }
     const int film_grid = (int)std::min<int64_t>((L.max_pix + 255) / 256, (int64_t)L.grid_shade * 4);
     hipLaunchKernelGGL(k_wf_film, dim3(std::max(film_grid, 1)), dim3(256), 0, st, A);

@@ -204,6 +204,11 @@ typedef struct fm_scene_info {
     fm_renderer_desc renderer;
     fm_camera_desc camera;
     int32_t env_width, env_height;   /* first env light's image (0 if none) */
+    int32_t trav_width;       /* traversal tree the device walks: 2 or 4 wide */
+    int32_t trav_stack;       /* stack entries per traversal (LDS) */
+    int64_t num_trav_nodes;   /* its inner nodes */
+    int32_t trav_node_bytes;  /* bytes per node: 64 (2 wide) or 128 (4 wide) */
+    int32_t pad_;
 } fm_scene_info;
 int fm_scene_get_info(const fm_scene* scene, fm_scene_info* out);
 
@@ -322,6 +327,12 @@ int fm_render_adjoint(fm_ctx* ctx, fm_scene* scene, const fm_render_params* para
                       double* g_material, double* g_env, fm_render_stats* stats);
 /* Enables the node / primitive counters in later renders (0 = off). */
 int fm_set_counting(fm_ctx* ctx, int enabled);
+/* Width (2 or 4, default 2) of the traversal tree of the scenes created on
+ * this context afterwards.  The four-wide tree is the binary one with nodes
+ * merged (half the node visits, twice the box tests per visit); the closest
+ * hit does not depend on the width (ties are resolved by reference slot).  No counterpart in the reference (whose BVHAggregate
+ * is binary, bvh_aggregate.cpp:38-79): a device traversal parameter. */
+int fm_set_bvh_width(fm_ctx* ctx, int32_t width);
 
 /* ---- parity hooks (batched queries against the uploaded scene) ----------- */
 
