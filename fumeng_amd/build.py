@@ -26,6 +26,10 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
 
 HOST_SOURCES = ["api.cpp", "host/loader.cpp", "host/scene_build.cpp"]
 HIP_SOURCES = ["kernels.hip", "wavefront.hip", "adjoint.hip"]
+# wavefront.hip aggregates its queue atomics per wave itself and issues
+# those of several queues back to back; the compiler's own atomic
+# aggregation would put a wait after each of them.
+HIP_FILE_FLAGS = {"wavefront.hip": ["-mllvm", "-amdgpu-atomic-optimizer-strategy=None"]}
 
 
 def _run(cmd):
@@ -40,7 +44,7 @@ def _compile(src):
     path = os.path.join(CSRC, src)
     obj = os.path.join(OBJ, src.replace("/", "_") + ".o")
     if src.endswith(".hip"):
-        cmd = [os.path.join(ROCM, "bin", "hipcc")] + HIP_FLAGS + ["-c", path, "-o", obj]
+        cmd = [os.path.join(ROCM, "bin", "hipcc")] + HIP_FLAGS + HIP_FILE_FLAGS.get(src, []) + ["-c", path, "-o", obj]
     else:
         cmd = ["g++"] + CXX_FLAGS + ["-c", path, "-o", obj]
     _run(cmd)

@@ -93,7 +93,7 @@ struct WfBuffers {
             for (DevBuf& b : q_sh) b.alloc((size_t)PDL * 4);
             cap_pdl = PDL;
         }
-        if (IT > cap_it) { cap_it = 0; counts.alloc((size_t)IT * fm::WQ_N * 4); cap_it = IT; }
+        if (IT > cap_it) { cap_it = 0; counts.alloc((size_t)IT * fm::WQ_N * fm::WQ_CNT_STRIDE * 4); cap_it = IT; }
     }
     int64_t bytes() const {
         int64_t n = 0;

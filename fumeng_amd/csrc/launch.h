@@ -25,6 +25,7 @@ struct RenderArgs {
 
 /* Wavefront form.  Per-iteration queue counters, [iteration][WQ_*]. */
 enum { WQ_EXT = 0, WQ_SH = 1, WQ_CLS = 2, WQ_NCLS = 4, WQ_N = WQ_CLS + WQ_NCLS };
+enum { WQ_CNT_STRIDE = 64 };   /* unsigned entries between queue counters */
 /* shading classes: WQ_CLS + c, c = 0 path end (miss, emitter seen by the
  * camera, or only deferred direct light left), 1 + material type */
 enum { WC_END = 0, WC_LAMBERT = 1, WC_DISNEY = 2, WC_SPEC = 3 };
@@ -82,7 +83,7 @@ struct WfArgs {
     int* q_ext[2];     /* ray queues, ping-pong by iteration */
     int* q_sh[2];      /* shadow queues */
     int* q_cls[2];     /* WQ_NCLS segments of P path ids */
-    unsigned* counts;  /* [(n_iter + 1) * WQ_N] */
+    unsigned* counts;  /* [(n_iter + 1) * WQ_N * WQ_CNT_STRIDE] */
     unsigned long long* counters;
 };
 

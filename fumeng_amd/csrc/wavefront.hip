@@ -65,6 +65,42 @@ __device__ __forceinline__ void wave_push(unsigned* count, int* q, bool pred, in
     }
 }
 
+/* N pushes of one wave at once: push k appends v[k] for the lanes with
+ * pred[k] to queue q[k], whose counter is count[grp[k]] (grp ascending;
+ * pushes of one group fill consecutive ranges in k order).  One lane issues
+ * the atomics of all groups back to back, so the wave waits for one round
+ * trip instead of one per queue (device-scope atomics are the slowest
+ * accesses of a shade step). */
+template <int N, int G>
+__device__ __forceinline__ void wave_push_n(unsigned* const (&count)[G], int* const (&q)[N], const int (&grp)[N],
+                                            const bool (&pred)[N], const int (&v)[N]) {
+    unsigned long long m[N];
+    unsigned off[N], tot[G], base[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) { tot[g] = 0; base[g] = 0; }
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        m[k] = __ballot(pred[k]);
+        off[k] = tot[grp[k]];
+        tot[grp[k]] += (unsigned)__popcll(m[k]);
+    }
+    const unsigned long long act = __ballot(true);
+    const int leader = __ffsll((long long)act) - 1;
+    if ((int)lane_id() == leader) {
+#pragma unroll
+        for (int g = 0; g < G; g++)
+            if (tot[g]) base[g] = atomicAdd(count[g], tot[g]);
+    }
+#pragma unroll
+    for (int g = 0; g < G; g++) base[g] = __builtin_amdgcn_readlane(base[g], leader);
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        if (!pred[k]) continue;
+        const unsigned below = __builtin_amdgcn_mbcnt_hi((unsigned)(m[k] >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m[k], 0u));
+        q[k][base[grp[k]] + off[k] + below] = v[k];
+    }
+}
+
 __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, unsigned long long v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     if ((threadIdx.x & 63) == 0) atomicAdd(dst, v);
@@ -98,7 +134,9 @@ FMD Hit load_hit(const WfArgs& A, int64_t r) {
 
 FMD int64_t grid_id() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 FMD int64_t grid_stride() { return (int64_t)gridDim.x * blockDim.x; }
-FMD unsigned* cnt(const WfArgs& A, int iter, int q) { return A.counts + (size_t)iter * fm::WQ_N + q; }
+/* one queue counter per 256 bytes, so that the atomics of different
+ * queues go to different cache lines */
+FMD unsigned* cnt(const WfArgs& A, int iter, int q) { return A.counts + ((size_t)iter * fm::WQ_N + q) * fm::WQ_CNT_STRIDE; }
 
 /* ---- camera ------------------------------------------------------------- */
 
@@ -124,8 +162,8 @@ FMD int find_unit_pix(const WfArgs& A, const fm::WfPass& ps, int64_t q) {
 __global__ void __launch_bounds__(256) k_wf_camera(fm_dscene S, WfArgs A) {
     const fm::WfPass ps = A.passes[*A.pass_idx];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        for (int k = 0; k < (A.n_iter + 1) * fm::WQ_N; k++) A.counts[k] = 0u;
-        A.counts[fm::WQ_EXT] = (unsigned)ps.P;
+        for (int k = 0; k < (A.n_iter + 1) * fm::WQ_N; k++) A.counts[(size_t)k * fm::WQ_CNT_STRIDE] = 0u;
+        *cnt(A, 0, fm::WQ_EXT) = (unsigned)ps.P;
         atomicAdd(A.counters + 0, (unsigned long long)ps.P);
     }
     for (int64_t p = grid_id(); p < ps.P; p += grid_stride()) {
@@ -187,9 +225,13 @@ __global__ void __launch_bounds__(256) k_wf_trace(fm_dscene S, WfArgs A) {
                     if (!(depth == 1 && me.z >= 0)) cls = 1 + S.materials[me.y].type;
                 }
             }
-#pragma unroll
-            for (int c = 0; c < fm::WQ_NCLS; c++)
-                wave_push(cnt(A, it, fm::WQ_CLS + c), qc + (size_t)c * A.P, cls == c, r);
+            unsigned* const cc[fm::WQ_NCLS] = {cnt(A, it, fm::WQ_CLS), cnt(A, it, fm::WQ_CLS + 1), cnt(A, it, fm::WQ_CLS + 2),
+                                               cnt(A, it, fm::WQ_CLS + 3)};
+            int* const qq[fm::WQ_NCLS] = {qc, qc + (size_t)A.P, qc + 2 * (size_t)A.P, qc + 3 * (size_t)A.P};
+            const int gg[fm::WQ_NCLS] = {0, 1, 2, 3};
+            const bool pc[fm::WQ_NCLS] = {cls == 0, cls == 1, cls == 2, cls == 3};
+            const int vv[fm::WQ_NCLS] = {r, r, r, r};
+            wave_push_n(cc, qq, gg, pc, vv);
         } else {
             const int k = qs[i - (int64_t)n_ext];
             const float4 o = reinterpret_cast<const float4*>(A.sh_o)[k];
@@ -283,6 +325,12 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
         if (fl & F_PENDING) color += resolve_direct<LK>(S, A, p) / (float)A.D;
         bool any = false, term = true;
         uint32_t dim_out = dim;
+        /* with one light sample and one MisBSDF ray per vertex (uniform) the
+         * shadow and MisBSDF pushes are deferred and issued with the
+         * continuation and path-end pushes at once */
+        const bool single = A.D == 1 && A.NL == 1;
+        bool sh_ok = false, mis_ok = false;
+        int sh_v = 0, mis_v = 0;
         if constexpr (C == fm::WC_END) {
             if (fl & F_CONT) {
                 const Ray tr = load_ray(A, p);
@@ -331,7 +379,8 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
                         }
                     }
                     A.sh_vis[k] = ok ? 1 : 0;
-                    wave_push(cnt(A, nx, fm::WQ_SH), A.q_sh[nx & 1], ok, (int)k);
+                    if (single) { sh_ok = ok; sh_v = (int)k; }
+                    else wave_push(cnt(A, nx, fm::WQ_SH), A.q_sh[nx & 1], ok, (int)k);
                     any = any || ok;
                 }
                 const int64_t m = (int64_t)p * A.D + pass;
@@ -346,7 +395,8 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
                 } else {
                     reinterpret_cast<float4*>(A.mis_ng)[m] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 }
-                wave_push(cnt(A, nx, fm::WQ_EXT), A.q_ext[nx & 1], ok, (int)(A.P + m));
+                if (single) { mis_ok = ok; mis_v = (int)(A.P + m); }
+                else wave_push(cnt(A, nx, fm::WQ_EXT), A.q_ext[nx & 1], ok, (int)(A.P + m));
                 any = any || ok;
             }
             /* continuation (path_tracing.cpp:93-113) */
@@ -367,8 +417,15 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
             }
             dim_out = rng.dim;
         }
-        wave_push(cnt(A, nx, fm::WQ_EXT), A.q_ext[nx & 1], !term, p);
-        wave_push(cnt(A, nx, fm::WQ_CLS + fm::WC_END), A.q_cls[nx & 1] + (size_t)fm::WC_END * A.P, term && any, p);
+        {
+            unsigned* const cc[3] = {cnt(A, nx, fm::WQ_SH), cnt(A, nx, fm::WQ_EXT), cnt(A, nx, fm::WQ_CLS + fm::WC_END)};
+            int* const qq[4] = {A.q_sh[nx & 1], A.q_ext[nx & 1], A.q_ext[nx & 1],
+                                A.q_cls[nx & 1] + (size_t)fm::WC_END * A.P};
+            const int gg[4] = {0, 1, 1, 2};
+            const bool pp[4] = {sh_ok, mis_ok, !term, term && any};
+            const int vv[4] = {sh_v, mis_v, p, p};
+            wave_push_n(cc, qq, gg, pp, vv);
+        }
         reinterpret_cast<float4*>(A.st_beta)[p] = f4(beta, __int_as_float(depth));
         reinterpret_cast<float4*>(A.st_color)[p] = f4(color, __uint_as_float(dim_out));
         A.st_flags[p] = (fl & ~0xff) | (any ? F_PENDING : 0) | (term ? 0 : F_CONT);
