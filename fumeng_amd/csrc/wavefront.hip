@@ -302,6 +302,56 @@ FMD v3 resolve_direct(const fm_dscene& S, const WfArgs& A, int64_t p) {
     return direct;
 }
 
+/* Every queue entry of one shade step of a wave with one round trip of
+ * atomics: shadow records (lane bit pass * NL + light of sb), MisBSDF rays
+ * (bit pass of mb), the continuation and the path end.  Entries of one
+ * queue fill consecutive ranges in bit order. */
+FMD void push_step(const WfArgs& A, int nx, int64_t p, uint64_t sb, uint32_t mb, bool cont, bool end) {
+    const int nsb = A.D * A.NL, nmb = A.D;
+    const bool any_sh = __ballot(sb != 0ull) != 0ull, any_mis = __ballot(mb != 0u) != 0ull;
+    unsigned t_sh = 0, t_ext = 0;
+    if (any_sh)
+        for (int b = 0; b < nsb; b++) t_sh += (unsigned)__popcll(__ballot((sb >> b) & 1ull));
+    if (any_mis)
+        for (int b = 0; b < nmb; b++) t_ext += (unsigned)__popcll(__ballot((mb >> b) & 1u));
+    const unsigned long long mc = __ballot(cont), me = __ballot(end);
+    t_ext += (unsigned)__popcll(mc);
+    const unsigned t_end = (unsigned)__popcll(me);
+    unsigned b_sh = 0, b_ext = 0, b_end = 0;
+    const int leader = __ffsll((long long)__ballot(true)) - 1;
+    if ((int)lane_id() == leader) {
+        if (t_sh) b_sh = atomicAdd(cnt(A, nx, fm::WQ_SH), t_sh);
+        if (t_ext) b_ext = atomicAdd(cnt(A, nx, fm::WQ_EXT), t_ext);
+        if (t_end) b_end = atomicAdd(cnt(A, nx, fm::WQ_CLS + fm::WC_END), t_end);
+    }
+    b_sh = __builtin_amdgcn_readlane(b_sh, leader);
+    b_ext = __builtin_amdgcn_readlane(b_ext, leader);
+    b_end = __builtin_amdgcn_readlane(b_end, leader);
+    auto rank = [](unsigned long long m) {
+        return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+    };
+    if (any_sh) {
+        int* q = A.q_sh[nx & 1];
+        for (int b = 0; b < nsb; b++) {
+            const bool on = (sb >> b) & 1ull;
+            const unsigned long long m = __ballot(on);
+            if (on) q[b_sh + rank(m)] = (int)(p * nsb + b);
+            b_sh += (unsigned)__popcll(m);
+        }
+    }
+    int* qe = A.q_ext[nx & 1];
+    if (any_mis) {
+        for (int b = 0; b < nmb; b++) {
+            const bool on = (mb >> b) & 1u;
+            const unsigned long long m = __ballot(on);
+            if (on) qe[b_ext + rank(m)] = (int)(A.P + p * nmb + b);
+            b_ext += (unsigned)__popcll(m);
+        }
+    }
+    if (cont) qe[b_ext + rank(mc)] = (int)p;
+    if (end) (A.q_cls[nx & 1] + (size_t)fm::WC_END * A.P)[b_end + rank(me)] = (int)p;
+}
+
 /* Light sampling for a scene whose lights are all of the kinds LK allows. */
 template <int LK>
 FMD LightSample light_sample_k(const fm_dscene& S, const fm_dlight& L, v3 pos, v3 s) {
@@ -325,12 +375,12 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
         if (fl & F_PENDING) color += resolve_direct<LK>(S, A, p) / (float)A.D;
         bool any = false, term = true;
         uint32_t dim_out = dim;
-        /* with one light sample and one MisBSDF ray per vertex (uniform) the
-         * shadow and MisBSDF pushes are deferred and issued with the
-         * continuation and path-end pushes at once */
-        const bool single = A.D == 1 && A.NL == 1;
-        bool sh_ok = false, mis_ok = false;
-        int sh_v = 0, mis_v = 0;
+        /* the shadow and MisBSDF pushes are kept as lane bits (uniformly,
+         * when they fit) and issued with the continuation and path-end
+         * pushes in one round trip of atomics */
+        const bool bits = A.D * A.NL <= 64 && A.D <= 32;
+        uint64_t sb = 0;
+        uint32_t mb = 0;
         if constexpr (C == fm::WC_END) {
             if (fl & F_CONT) {
                 const Ray tr = load_ray(A, p);
@@ -379,7 +429,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
                         }
                     }
                     A.sh_vis[k] = ok ? 1 : 0;
-                    if (single) { sh_ok = ok; sh_v = (int)k; }
+                    if (bits) sb |= (uint64_t)(ok ? 1 : 0) << (pass * A.NL + li);
                     else wave_push(cnt(A, nx, fm::WQ_SH), A.q_sh[nx & 1], ok, (int)k);
                     any = any || ok;
                 }
@@ -395,7 +445,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
                 } else {
                     reinterpret_cast<float4*>(A.mis_ng)[m] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 }
-                if (single) { mis_ok = ok; mis_v = (int)(A.P + m); }
+                if (bits) mb |= (uint32_t)(ok ? 1 : 0) << pass;
                 else wave_push(cnt(A, nx, fm::WQ_EXT), A.q_ext[nx & 1], ok, (int)(A.P + m));
                 any = any || ok;
             }
@@ -417,15 +467,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
             }
             dim_out = rng.dim;
         }
-        {
-            unsigned* const cc[3] = {cnt(A, nx, fm::WQ_SH), cnt(A, nx, fm::WQ_EXT), cnt(A, nx, fm::WQ_CLS + fm::WC_END)};
-            int* const qq[4] = {A.q_sh[nx & 1], A.q_ext[nx & 1], A.q_ext[nx & 1],
-                                A.q_cls[nx & 1] + (size_t)fm::WC_END * A.P};
-            const int gg[4] = {0, 1, 1, 2};
-            const bool pp[4] = {sh_ok, mis_ok, !term, term && any};
-            const int vv[4] = {sh_v, mis_v, p, p};
-            wave_push_n(cc, qq, gg, pp, vv);
-        }
+        push_step(A, nx, p, sb, mb, !term, term && any);
         reinterpret_cast<float4*>(A.st_beta)[p] = f4(beta, __int_as_float(depth));
         reinterpret_cast<float4*>(A.st_color)[p] = f4(color, __uint_as_float(dim_out));
         A.st_flags[p] = (fl & ~0xff) | (any ? F_PENDING : 0) | (term ? 0 : F_CONT);
