@@ -208,39 +208,54 @@ __global__ void __launch_bounds__(256) k_wf_trace(fm_dscene S, WfArgs A) {
     const int* qs = A.q_sh[it & 1];
     int* qc = A.q_cls[it & 1];
     Cnt cn;
-    for (int64_t i = grid_id(); i < total; i += grid_stride()) {
-        if (i < (int64_t)n_ext) {
-            const int r = q[i];
-            const Ray tr = load_ray(A, r);
-            const Hit h = trace_closest<COUNT, WIDTH>(S, tr, stk, stride, &cn);
-            reinterpret_cast<float4*>(A.hit_a)[r] =
-                make_float4(h.t, __int_as_float(h.slot), __int_as_float((h.kind & 0xff) | (h.half << 8)), h.b0);
-            reinterpret_cast<float2*>(A.hit_b)[r] = make_float2(h.b1, h.b2);
-            int cls = -1;
-            if (r < A.P) {
-                cls = fm::WC_END;
-                if (h.slot >= 0) {
-                    const int4 me = reinterpret_cast<const int4*>(S.meta)[h.slot];
-                    const int depth = __float_as_int(reinterpret_cast<const float4*>(A.st_beta)[r].w);
-                    if (!(depth == 1 && me.z >= 0)) cls = 1 + S.materials[me.y].type;
+    /* Each lane takes two items per round and the classification pushes of
+     * both go out in one round of atomics: the wave waits for the atomics'
+     * round trip once per two traversals. */
+    const int64_t gs = grid_stride();
+    for (int64_t i0 = grid_id(); i0 < total; i0 += 2 * gs) {
+        int cls0 = -1, cls1 = -1, r0 = 0, r1 = 0;
+#pragma nounroll
+        for (int u = 0; u < 2; u++) {
+            const int64_t i = i0 + u * gs;
+            if (i >= total) break;
+            if (i < (int64_t)n_ext) {
+                const int r = q[i];
+                const Ray tr = load_ray(A, r);
+                const Hit h = trace_closest<COUNT, WIDTH>(S, tr, stk, stride, &cn);
+                reinterpret_cast<float4*>(A.hit_a)[r] =
+                    make_float4(h.t, __int_as_float(h.slot), __int_as_float((h.kind & 0xff) | (h.half << 8)), h.b0);
+                reinterpret_cast<float2*>(A.hit_b)[r] = make_float2(h.b1, h.b2);
+                int cls = -1;
+                if (r < A.P) {
+                    cls = fm::WC_END;
+                    if (h.slot >= 0) {
+                        const int4 me = reinterpret_cast<const int4*>(S.meta)[h.slot];
+                        const int depth = __float_as_int(reinterpret_cast<const float4*>(A.st_beta)[r].w);
+                        if (!(depth == 1 && me.z >= 0)) cls = 1 + S.materials[me.y].type;
+                    }
                 }
+                if (u == 0) { cls0 = cls; r0 = r; } else { cls1 = cls; r1 = r; }
+            } else {
+                const int k = qs[i - (int64_t)n_ext];
+                const float4 o = reinterpret_cast<const float4*>(A.sh_o)[k];
+                const float4 d = reinterpret_cast<const float4*>(A.sh_d)[k];
+                Ray tr;
+                tr.o = xyz(o); tr.tmin = o.w; tr.d = xyz(d); tr.tmax = d.w;
+                const bool occluded = trace_any<COUNT, WIDTH>(S, tr, stk, stride, &cn);
+                A.sh_vis[k] = occluded ? 1 : 2;
             }
-            unsigned* const cc[fm::WQ_NCLS] = {cnt(A, it, fm::WQ_CLS), cnt(A, it, fm::WQ_CLS + 1), cnt(A, it, fm::WQ_CLS + 2),
-                                               cnt(A, it, fm::WQ_CLS + 3)};
-            int* const qq[fm::WQ_NCLS] = {qc, qc + (size_t)A.P, qc + 2 * (size_t)A.P, qc + 3 * (size_t)A.P};
-            const int gg[fm::WQ_NCLS] = {0, 1, 2, 3};
-            const bool pc[fm::WQ_NCLS] = {cls == 0, cls == 1, cls == 2, cls == 3};
-            const int vv[fm::WQ_NCLS] = {r, r, r, r};
-            wave_push_n(cc, qq, gg, pc, vv);
-        } else {
-            const int k = qs[i - (int64_t)n_ext];
-            const float4 o = reinterpret_cast<const float4*>(A.sh_o)[k];
-            const float4 d = reinterpret_cast<const float4*>(A.sh_d)[k];
-            Ray tr;
-            tr.o = xyz(o); tr.tmin = o.w; tr.d = xyz(d); tr.tmax = d.w;
-            const bool occluded = trace_any<COUNT, WIDTH>(S, tr, stk, stride, &cn);
-            A.sh_vis[k] = occluded ? 1 : 2;
         }
+        unsigned* const cc[fm::WQ_NCLS] = {cnt(A, it, fm::WQ_CLS), cnt(A, it, fm::WQ_CLS + 1), cnt(A, it, fm::WQ_CLS + 2),
+                                           cnt(A, it, fm::WQ_CLS + 3)};
+        int* const q0 = qc;
+        int* const q1 = qc + (size_t)A.P;
+        int* const q2 = qc + 2 * (size_t)A.P;
+        int* const q3 = qc + 3 * (size_t)A.P;
+        int* const qq[2 * fm::WQ_NCLS] = {q0, q0, q1, q1, q2, q2, q3, q3};
+        const int gg[2 * fm::WQ_NCLS] = {0, 0, 1, 1, 2, 2, 3, 3};
+        const bool pc[2 * fm::WQ_NCLS] = {cls0 == 0, cls1 == 0, cls0 == 1, cls1 == 1, cls0 == 2, cls1 == 2, cls0 == 3, cls1 == 3};
+        const int vv[2 * fm::WQ_NCLS] = {r0, r1, r0, r1, r0, r1, r0, r1};
+        wave_push_n(cc, qq, gg, pc, vv);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(A.counters + 1, (unsigned long long)total);
     if (COUNT) {
