@@ -267,6 +267,14 @@ def host_bvh(desc):
     return boxes, leaf, a, b, order
 
 
+def host_scene_info(desc, bvh_width=2):
+    """fm_scene_info of the whole host build of a SceneDesc (no GPU needed)."""
+    d, keep = desc.to_c()
+    i = A.fm_scene_info()
+    _check(lib().fm_host_scene_info(C.byref(d), int(bvh_width), C.byref(i)))
+    return i
+
+
 class SceneLoader:
     """Mirror of the reference's SceneLoader (create/scene_loader.h:278-307):
     load(path) then draw_frame() renders with the scene's renderer block and

@@ -175,6 +175,7 @@ EXPORTS = {
                                   C.POINTER(C.c_double), C.POINTER(fm_batch_stats)]),
     "fm_set_counting": (C.c_int, [C.c_void_p, C.c_int]),
     "fm_set_bvh_width": (C.c_int, [C.c_void_p, C.c_int32]),
+    "fm_host_scene_info": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(fm_scene_info)]),
     "fm_params_get": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fm_params_set": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fm_render_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(fm_render_params), C.c_void_p, C.c_void_p,

@@ -490,26 +490,30 @@ void fm_scene_destroy(fm_scene* s) {
     delete s;
 }
 
+static void fill_info(const fmh::HostScene& hs, int64_t device_bytes, fm_scene_info* o) {
+    memset(o, 0, sizeof(*o));
+    o->num_geometries = hs.num_geoms;
+    o->num_bvh_nodes = (int64_t)hs.ref_nodes.size();
+    o->bvh_depth = hs.bvh_depth;
+    o->num_lights = (int32_t)hs.lights.size();
+    o->num_materials = (int32_t)hs.materials.size();
+    o->has_env = hs.env >= 0;
+    o->device_bytes = device_bytes;
+    o->renderer = hs.renderer;
+    o->camera = hs.camera_desc;
+    if (hs.env >= 0) {
+        o->env_width = hs.envs[(size_t)hs.env].W;
+        o->env_height = hs.envs[(size_t)hs.env].H;
+    }
+    o->trav_width = hs.trav_width;
+    o->trav_stack = hs.trav_stack;
+    o->num_trav_nodes = (int64_t)(hs.trav_width == 4 ? hs.nodes4.size() : hs.nodes.size());
+    o->trav_node_bytes = hs.trav_width == 4 ? (int32_t)sizeof(fm_dnode4) : (int32_t)sizeof(fm_dnode);
+}
+
 int fm_scene_get_info(const fm_scene* s, fm_scene_info* o) {
     if (!s || !o) return fail(nullptr, FM_ERR_INVALID, "NULL argument");
-    memset(o, 0, sizeof(*o));
-    o->num_geometries = s->hs.num_geoms;
-    o->num_bvh_nodes = (int64_t)s->hs.ref_nodes.size();
-    o->bvh_depth = s->hs.bvh_depth;
-    o->num_lights = (int32_t)s->hs.lights.size();
-    o->num_materials = (int32_t)s->hs.materials.size();
-    o->has_env = s->hs.env >= 0;
-    o->device_bytes = s->device_bytes;
-    o->renderer = s->hs.renderer;
-    o->camera = s->hs.camera_desc;
-    if (s->hs.env >= 0) {
-        o->env_width = s->hs.envs[(size_t)s->hs.env].W;
-        o->env_height = s->hs.envs[(size_t)s->hs.env].H;
-    }
-    o->trav_width = s->hs.trav_width;
-    o->trav_stack = s->hs.trav_stack;
-    o->num_trav_nodes = (int64_t)(s->hs.trav_width == 4 ? s->hs.nodes4.size() : s->hs.nodes.size());
-    o->trav_node_bytes = s->hs.trav_width == 4 ? (int32_t)sizeof(fm_dnode4) : (int32_t)sizeof(fm_dnode);
+    fill_info(s->hs, s->device_bytes, o);
     return FM_OK;
 }
 
@@ -942,6 +946,18 @@ int fm_host_build_bvh(const fm_scene_desc* desc, int64_t* num_nodes, float* boxe
             }
         }
         if (order) for (size_t i = 0; i < hs.order.size(); i++) order[i] = hs.order[i];
+        return FM_OK;
+    });
+}
+
+int fm_host_scene_info(const fm_scene_desc* desc, int32_t bvh_width, fm_scene_info* out) {
+    if (!desc || !out) return fail(nullptr, FM_ERR_INVALID, "NULL argument");
+    if (bvh_width != 2 && bvh_width != 4) return fail(nullptr, FM_ERR_INVALID, "BVH width must be 2 or 4");
+    return guarded(nullptr, [&]() -> int {
+        fmh::HostScene hs;
+        hs.trav_width = bvh_width;
+        fmh::build_host_scene(*desc, hs);
+        fill_info(hs, 0, out);
         return FM_OK;
     });
 }

@@ -159,13 +159,15 @@ struct Hit {
     int32_t tie;     /* tie key of the winner (slot, or geometry id for "naive") */
     int32_t half;    /* rectangle: 1 = (pa,pb,pc), 0 = (pc,pd,pa) */
     float b0, b1, b2;
+    int32_t info;    /* FM_PRIM_INFO of the winner's slot */
 };
 
 /* Leaf slot test for the closest hit with the order-free tie rule. */
 FMD void test_slot_closest(const fm_dscene& S, const Ray& r, const RayPre& rp, int32_t p, Hit& h) {
     const float4* pr = reinterpret_cast<const float4*>(S.prims) + 3 * (size_t)p;
     const float4 a = pr[0];
-    const int kind = __float_as_int(a.w);
+    const int kind = FM_PRIM_KIND(__float_as_int(a.w));
+    const int info = FM_PRIM_INFO(__float_as_int(a.w));
     const bool have = h.slot >= 0;
     if (kind == FM_PRIM_SPHERE) {
         const int32_t key = __float_as_int(pr[2].w);
@@ -174,7 +176,7 @@ FMD void test_slot_closest(const fm_dscene& S, const Ray& r, const RayPre& rp, i
         float t;
         if (!sphere_core(r, s, h.t, t)) return;  /* t_min <= t <= t_max */
         if (t < h.t || !have || h.kind != FM_PRIM_SPHERE || key > h.tie) {
-            h.t = t; h.slot = p; h.kind = FM_PRIM_SPHERE; h.tie = key; h.half = 0;
+            h.t = t; h.slot = p; h.kind = FM_PRIM_SPHERE; h.tie = key; h.half = 0; h.info = info;
         }
         return;
     }
@@ -198,17 +200,20 @@ FMD void test_slot_closest(const fm_dscene& S, const Ray& r, const RayPre& rp, i
     if (tri) {
         if (tri_core(r, rp, f4to3(a), f4to3(b), f4to3(c), t, b0, b1, b2) && accept(t)) {
             h.t = t; h.slot = p; h.kind = FM_PRIM_TRI; h.tie = key; h.half = 0; h.b0 = b0; h.b1 = b1; h.b2 = b2;
+            h.info = info;
         }
         return;
     }
     /* rectangle: (pa,pb,pc) first, (pc,pd,pa) only if the first is rejected */
     if (tri_core(r, rp, f4to3(a), f4to3(b), f4to3(c), t, b0, b1, b2) && accept(t)) {
         h.t = t; h.slot = p; h.kind = FM_PRIM_RECT; h.tie = key; h.half = 1; h.b0 = b0; h.b1 = b1; h.b2 = b2;
+        h.info = info;
         return;
     }
     const v3 pd = ld3(S.rects[__float_as_int(b.w)].pd);
     if (tri_core(r, rp, f4to3(c), pd, f4to3(a), t, b0, b1, b2) && accept(t)) {
         h.t = t; h.slot = p; h.kind = FM_PRIM_RECT; h.tie = key; h.half = 0; h.b0 = b0; h.b1 = b1; h.b2 = b2;
+        h.info = info;
     }
 }
 
@@ -216,7 +221,7 @@ FMD void test_slot_closest(const fm_dscene& S, const Ray& r, const RayPre& rp, i
 FMD bool test_slot_any(const fm_dscene& S, const Ray& r, const RayPre& rp, int32_t p) {
     const float4* pr = reinterpret_cast<const float4*>(S.prims) + 3 * (size_t)p;
     const float4 a = pr[0];
-    const int kind = __float_as_int(a.w);
+    const int kind = FM_PRIM_KIND(__float_as_int(a.w));
     const float4 b = pr[1];
     float t, b0, b1, b2;
     if (kind == FM_PRIM_SPHERE) {
@@ -394,7 +399,7 @@ FMD bool traverse4_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit&
 template <bool COUNT = false, int WIDTH = 0>
 FMD Hit trace_closest(const fm_dscene& S, const Ray& r, uint32_t* stk, int stride, Cnt* cn = nullptr) {
     Hit h;
-    h.t = r.tmax; h.slot = -1; h.kind = -1; h.tie = 0; h.half = 0; h.b0 = h.b1 = h.b2 = 0.0f;
+    h.t = r.tmax; h.slot = -1; h.kind = -1; h.tie = 0; h.half = 0; h.b0 = h.b1 = h.b2 = 0.0f; h.info = 0;
     if (S.num_slots == 0) return h;
     const RayPre rp = precompute(r);
     Cnt dummy;
@@ -416,7 +421,7 @@ FMD bool trace_any(const fm_dscene& S, const Ray& r, uint32_t* stk, int stride, 
     Cnt dummy;
     Cnt& c = COUNT ? *cn : dummy;
     Hit h;
-    h.t = r.tmax; h.slot = -1; h.kind = -1; h.tie = 0; h.half = 0; h.b0 = h.b1 = h.b2 = 0.0f;
+    h.t = r.tmax; h.slot = -1; h.kind = -1; h.tie = 0; h.half = 0; h.b0 = h.b1 = h.b2 = 0.0f; h.info = 0;
     if (WIDTH == 4 || (WIDTH == 0 && S.width == 4))
         return rp.fast ? traverse4_impl<true, true, COUNT>(S, r, rp, h, stk, stride, c)
                        : traverse4_impl<false, true, COUNT>(S, r, rp, h, stk, stride, c);

@@ -22,6 +22,12 @@
 
 #define FM_META_HAS_NORMAL 1
 
+/* w of a slot's first float4: FM_PRIM_* | shading class << 8 (1 + material
+ * type) | emissive << 16, so that a closest hit is classified without
+ * reading the slot's meta and material */
+#define FM_PRIM_KIND(w) ((w) & 0xff)
+#define FM_PRIM_INFO(w) ((w) >> 8)
+
 /* child ref: >= 0 inner node index; < 0 leaf: ~((start << 3) | count) */
 #define FM_LEAF_REF(start, count) (~(int32_t)(((uint32_t)(start) << 3) | (uint32_t)(count)))
 #define FM_REF_IS_LEAF(r) ((r) < 0)

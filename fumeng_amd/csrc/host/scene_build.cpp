@@ -743,6 +743,12 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
             pr[4] = s.radius_world; pr[7] = ibits(ge.aux);
         }
         pr[11] = ibits(hs.naive ? g : ref_pos[(size_t)g]);
+        {
+            /* the material table is built below; its types are the descriptor's */
+            const int32_t mt = ge.material < d.num_materials ? d.materials[ge.material].type : FM_MAT_LAMBERT;
+            const int32_t cls = 1 + mt;
+            pr[3] = ibits(ge.type | (cls << 8) | ((ge.light >= 0 ? 1 : 0) << 16));
+        }
         int32_t* me = &hs.meta[p * 4];
         me[0] = g; me[1] = ge.material; me[2] = ge.light; me[3] = flags;
     }

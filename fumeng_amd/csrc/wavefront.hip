@@ -127,7 +127,7 @@ FMD Hit load_hit(const WfArgs& A, int64_t r) {
     Hit h;
     h.t = a.x; h.slot = __float_as_int(a.y);
     const int kh = __float_as_int(a.z);
-    h.kind = (kh << 24) >> 24; h.half = kh >> 8; h.tie = 0;
+    h.kind = (kh << 24) >> 24; h.half = kh >> 8; h.tie = 0; h.info = 0;
     h.b0 = a.w; h.b1 = b.x; h.b2 = b.y;
     return h;
 }
@@ -227,11 +227,12 @@ __global__ void __launch_bounds__(256) k_wf_trace(fm_dscene S, WfArgs A) {
                 reinterpret_cast<float2*>(A.hit_b)[r] = make_float2(h.b1, h.b2);
                 int cls = -1;
                 if (r < A.P) {
+                    /* emitters seen directly from the camera end the path */
                     cls = fm::WC_END;
                     if (h.slot >= 0) {
-                        const int4 me = reinterpret_cast<const int4*>(S.meta)[h.slot];
-                        const int depth = __float_as_int(reinterpret_cast<const float4*>(A.st_beta)[r].w);
-                        if (!(depth == 1 && me.z >= 0)) cls = 1 + S.materials[me.y].type;
+                        const bool emissive = (h.info >> 8) & 1;
+                        if (!emissive || __float_as_int(reinterpret_cast<const float4*>(A.st_beta)[r].w) != 1)
+                            cls = h.info & 0xff;
                     }
                 }
                 if (u == 0) { cls0 = cls; r0 = r; } else { cls1 = cls; r1 = r; }

@@ -415,6 +415,12 @@ int fm_env_eval(fm_ctx* ctx, fm_scene* scene, const float* dirs, int64_t n,
 int fm_host_build_bvh(const fm_scene_desc* desc, int64_t* num_nodes,
                       float* boxes, int32_t* is_leaf, int64_t* idx_a,
                       int64_t* idx_b, int64_t* num_geoms, int64_t* order);
+/* The whole host build of fm_scene_create (tables, traversal tree of the
+ * given width, primitive records) without a device; fills the facts
+ * fm_scene_get_info reports, device_bytes = 0.  For tools and CPU tests of
+ * the builder; no counterpart in the reference (SceneLoader::ParseScene,
+ * scene_loader.h:293-302, builds and renders in one object). */
+int fm_host_scene_info(const fm_scene_desc* desc, int32_t bvh_width, fm_scene_info* out);
 /* Parses a scene JSON into a descriptor held by an opaque handle. */
 typedef struct fm_loaded_desc fm_loaded_desc;
 int fm_desc_load(const char* json_path, fm_loaded_desc** out,

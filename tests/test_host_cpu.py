@@ -80,6 +80,27 @@ def test_host_builder_equals_reference_topology(freq):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("name", ["mesh", "box", "stress_small"])
+def test_host_build_of_both_traversal_widths(name):
+    """The whole host build (no device) for both tree widths: a two-wide
+    traversal pushes at most one entry per level (stack = depth - 1), the
+    four-wide tree merges nodes (fewer of them, each of 128 bytes) and its
+    exact stack bound stays within three entries per four-wide level."""
+    s = {"mesh": lambda: mesh_scene(freq=20), "box": lambda: scenes.box_scene(width=16, height=16),
+         "stress_small": lambda: scenes.stress_scene(n_spheres=6, freq=6, n_textures=2, tex_size=32, width=16,
+                                                     height=16)}[name]()
+    i2 = fumeng_amd.host_scene_info(s, 2)
+    i4 = fumeng_amd.host_scene_info(s, 4)
+    assert i2.device_bytes == 0 and i2.num_geometries == i4.num_geometries > 0
+    assert (i2.trav_width, i2.trav_node_bytes, i4.trav_width, i4.trav_node_bytes) == (2, 64, 4, 128)
+    assert i2.num_trav_nodes == i2.num_geometries - 1 or i2.num_trav_nodes < i2.num_geometries
+    assert 0 < i4.num_trav_nodes <= i2.num_trav_nodes
+    depth = i2.trav_stack + 1
+    assert 1 <= i4.trav_stack <= 3 * ((depth + 1) // 2)
+    with pytest.raises(fumeng_amd.FumengError):
+        fumeng_amd.host_scene_info(s, 3)
+
+
 def _load_desc(path):
     h = C.c_void_p()
     dp = C.POINTER(A.fm_scene_desc)()
