@@ -77,6 +77,20 @@ class Context:
         """Traversal tree width (2, the default, or 4) of the scenes created afterwards."""
         _check(lib().fm_set_bvh_width(self.h, int(width)), self.h)
 
+    def post_process(self, ops, rgb):
+        """The reference's output step on the device (fm_post_process):
+        returns (post-processed float image, uint8 image).  ops: list of
+        {"type": "tonemapping" | "linear_to_srgb" | "srgb_to_linear", "lum": ...}."""
+        rgb = np.ascontiguousarray(rgb, np.float32)
+        arr = (A.fm_post_op * max(1, len(ops)))()
+        for i, o in enumerate(ops):
+            arr[i].type = A.POST_TYPES[o["type"]] if isinstance(o, dict) else int(o)
+            arr[i].lum = float(o.get("lum", 1.0)) if isinstance(o, dict) else 1.0
+        out = np.empty_like(rgb)
+        out8 = np.empty(rgb.shape, np.uint8)
+        _check(lib().fm_post_process(self.h, arr, len(ops), _ptr(rgb), rgb.size // 3, _ptr(out), _ptr(out8)), self.h)
+        return out, out8
+
     def scene(self, desc):
         """Upload a SceneDesc (fm_scene_create)."""
         d, keep = desc.to_c()
@@ -114,6 +128,15 @@ class Scene:
         i = A.fm_scene_info()
         _check(lib().fm_scene_get_info(self.h, C.byref(i)))
         return i
+
+    def post_ops(self):
+        """The scene's post-process list as dicts ({"type": ..., "lum": ...})."""
+        arr = (A.fm_post_op * A.FM_MAX_POST_OPS)()
+        n = C.c_int32()
+        _check(lib().fm_scene_post_ops(self.h, arr, A.FM_MAX_POST_OPS, C.byref(n)))
+        names = {v: k for k, v in A.POST_TYPES.items()}
+        return [dict({"type": names[arr[i].type]}, **({"lum": arr[i].lum} if arr[i].type == A.FM_POST_TONEMAPPING else {}))
+                for i in range(n.value)]
 
     def render(self, width=0, height=0, spp=0, depth=0, seed=1, rr_depth=-1, rr_coef=0.0, direct_loop=0,
                mode=A.FM_MODE_MEGAKERNEL, sample_offset=0, block_size=0, grid_blocks=0, chunk=0, accumulate=False,

@@ -1,7 +1,7 @@
 """ctypes mirror of include/fm_gpu.h (structure layouts and constants)."""
 import ctypes as C
 
-FM_ABI_VERSION = 1
+FM_ABI_VERSION = 2
 
 FM_OK = 0
 FM_ERR_INVALID = -1
@@ -79,6 +79,16 @@ class fm_renderer_desc(C.Structure):
                 ("output", C.c_char * 256)]
 
 
+FM_POST_TONEMAPPING, FM_POST_LINEAR_TO_SRGB, FM_POST_SRGB_TO_LINEAR, FM_POST_OIDN_DENOISE = 1, 2, 3, 4
+FM_MAX_POST_OPS = 16
+POST_TYPES = {"tonemapping": FM_POST_TONEMAPPING, "linear_to_srgb": FM_POST_LINEAR_TO_SRGB,
+              "srgb_to_linear": FM_POST_SRGB_TO_LINEAR, "oidn_denoise": FM_POST_OIDN_DENOISE}
+
+
+class fm_post_op(C.Structure):
+    _fields_ = [("type", C.c_int32), ("lum", C.c_float)]
+
+
 class fm_scene_desc(C.Structure):
     _fields_ = [("abi_version", C.c_uint32), ("accelerator", C.c_int32),
                 ("camera", fm_camera_desc), ("renderer", fm_renderer_desc),
@@ -86,7 +96,8 @@ class fm_scene_desc(C.Structure):
                 ("num_materials", C.c_int32), ("materials", C.POINTER(fm_material_desc)),
                 ("num_meshes", C.c_int32), ("meshes", C.POINTER(fm_mesh_desc)),
                 ("num_objects", C.c_int32), ("objects", C.POINTER(fm_object_desc)),
-                ("num_lights", C.c_int32), ("lights", C.POINTER(fm_light_desc))]
+                ("num_lights", C.c_int32), ("lights", C.POINTER(fm_light_desc)),
+                ("num_post_ops", C.c_int32), ("post_ops", C.POINTER(fm_post_op))]
 
 
 class fm_scene_info(C.Structure):
@@ -175,6 +186,9 @@ EXPORTS = {
                                   C.POINTER(C.c_double), C.POINTER(fm_batch_stats)]),
     "fm_set_counting": (C.c_int, [C.c_void_p, C.c_int]),
     "fm_set_bvh_width": (C.c_int, [C.c_void_p, C.c_int32]),
+    "fm_scene_post_ops": (C.c_int, [C.c_void_p, C.POINTER(fm_post_op), C.c_int32, C.POINTER(C.c_int32)]),
+    "fm_post_process": (C.c_int, [C.c_void_p, C.POINTER(fm_post_op), C.c_int32, C.c_void_p, C.c_int64, C.c_void_p,
+                                  C.c_void_p]),
     "fm_host_scene_info": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(fm_scene_info)]),
     "fm_params_get": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fm_params_set": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -930,6 +930,45 @@ int fm_env_eval(fm_ctx* ctx, fm_scene* s, const float* dirs, int64_t n, float* p
     });
 }
 
+int fm_scene_post_ops(const fm_scene* s, fm_post_op* ops, int32_t max_ops, int32_t* n_ops) {
+    if (!s || !n_ops || (max_ops > 0 && !ops)) return fail(nullptr, FM_ERR_INVALID, "bad argument");
+    *n_ops = (int32_t)s->hs.post_ops.size();
+    for (int32_t i = 0; i < *n_ops && i < max_ops; i++) ops[i] = s->hs.post_ops[(size_t)i];
+    return FM_OK;
+}
+
+int fm_post_process(fm_ctx* ctx, const fm_post_op* ops, int32_t n_ops, const float* rgb_in, int64_t n_pixels,
+                    float* rgb_out, uint8_t* rgb8) {
+    if (!ctx || n_pixels < 0 || n_ops < 0 || (n_ops > 0 && !ops) || (n_pixels > 0 && !rgb_in))
+        return fail(ctx, FM_ERR_INVALID, "bad argument");
+    if (n_ops > FM_MAX_POST_OPS) return fail(ctx, FM_ERR_INVALID, "more than FM_MAX_POST_OPS post-process ops");
+    fm::PostOps po{};
+    po.n = n_ops;
+    for (int32_t i = 0; i < n_ops; i++) {
+        if (ops[i].type == FM_POST_OIDN_DENOISE)
+            return fail(ctx, FM_ERR_UNSUPPORTED, "oidn_denoise: the OIDN denoiser is not available");
+        if (ops[i].type < FM_POST_TONEMAPPING || ops[i].type > FM_POST_SRGB_TO_LINEAR)
+            return fail(ctx, FM_ERR_INVALID, "unknown post-process type");
+        po.op[i] = ops[i];
+    }
+    if (n_pixels == 0 || (!rgb_out && !rgb8)) return FM_OK;
+    return guarded(ctx, [&]() -> int {
+        HIPCHECK(hipSetDevice(ctx->device));
+        const int64_t n = n_pixels * 3;
+        DevBuf din, dout, d8;
+        din.alloc((size_t)n * sizeof(float));
+        if (rgb_out) dout.alloc((size_t)n * sizeof(float));
+        if (rgb8) d8.alloc((size_t)n);
+        HIPCHECK(hipMemcpyAsync(din.p, rgb_in, (size_t)n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHECK(fm::launch_post(po, din.as<float>(), n, rgb_out ? dout.as<float>() : nullptr,
+                                 rgb8 ? d8.as<unsigned char>() : nullptr, ctx->stream));
+        if (rgb_out) HIPCHECK(hipMemcpyAsync(rgb_out, dout.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        if (rgb8) HIPCHECK(hipMemcpyAsync(rgb8, d8.p, (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+        return FM_OK;
+    });
+}
+
 int fm_host_build_bvh(const fm_scene_desc* desc, int64_t* num_nodes, float* boxes, int32_t* is_leaf, int64_t* idx_a,
                       int64_t* idx_b, int64_t* num_geoms, int64_t* order) {
     if (!desc || !num_nodes || !num_geoms) return fail(nullptr, FM_ERR_INVALID, "NULL argument");

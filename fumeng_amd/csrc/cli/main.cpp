@@ -1,9 +1,19 @@
 /*
  * fumeng_gpu <scene.json>: the CMD_EXE front end (src/cmd/src/main.cpp:13-35)
  * over the C ABI.  Loads the scene with the reference loader semantics,
- * renders it on GPU 0 and writes <output>.png, <output>_albedo.png and
- * <output>_normal.png (render_target.h:26-31, image.cpp:97-110: clamp to
- * [0,1], round(255 x)) plus <output>.pfm with the float colour film.
+ * renders it on GPU 0 (wavefront form) and writes <output>.png,
+ * <output>_albedo.png and <output>_normal.png (render_target.h:26-31).
+ *
+ * Output step (renderer.h:124-142): the frame is rendered in ten steps of a
+ * tenth of the samples; after each of the first nine the running means are
+ * written as progressive snapshots, after the last the scene's post-process
+ * list (tone mapping, sRGB conversions) is applied to the colour and the
+ * final images are written.  The bytes come from fm_post_process
+ * (RealToUInt8, image.cpp:97-110).  The reference's snapshots hold finished
+ * pixels only (its thread pool fills the image pixel by pixel); these hold
+ * every pixel at a tenth, two tenths, ... of the samples.  <output>.pfm keeps
+ * the float colour film before post-processing.  An "oidn_denoise" entry is
+ * skipped with a warning (no OIDN here).
  * Returns 0 on success and -1 on any error, like the reference.
  */
 #include "../../../include/fm_gpu.h"
@@ -40,7 +50,7 @@ static void chunk(FILE* f, const char* type, const std::vector<unsigned char>& d
     fwrite(b.data(), 1, b.size(), f);
 }
 /* 8-bit RGB PNG with stored (uncompressed) deflate blocks */
-static bool write_png(const std::string& path, int w, int h, const float* rgb) {
+static bool write_png(const std::string& path, int w, int h, const unsigned char* rgb8) {
     FILE* f = fopen(path.c_str(), "wb");
     if (!f) return false;
     static const unsigned char sig[8] = {137, 80, 78, 71, 13, 10, 26, 10};
@@ -53,11 +63,7 @@ static bool write_png(const std::string& path, int w, int h, const float* rgb) {
     raw.reserve((size_t)h * (1 + 3 * (size_t)w));
     for (int y = 0; y < h; y++) {
         raw.push_back(0);
-        for (int x = 0; x < 3 * w; x++) {
-            float c = rgb[(size_t)y * 3 * w + x];
-            c = c > 1.0f ? 1.0f : (c < 0.0f ? 0.0f : c);
-            raw.push_back((unsigned char)std::round(255.0f * c));
-        }
+        raw.insert(raw.end(), rgb8 + (size_t)y * 3 * w, rgb8 + (size_t)(y + 1) * 3 * w);
     }
     std::vector<unsigned char> z;
     z.push_back(0x78); z.push_back(0x01);
@@ -99,22 +105,63 @@ int main(int argc, char** argv) {
     fm_scene_info info;
     fm_scene_get_info(s, &info);
     const int w = info.renderer.width, h = info.renderer.height;
-    std::vector<float> color((size_t)w * h * 3), albedo(color.size()), normal(color.size());
+    std::vector<float> color((size_t)w * h * 3), albedo(color.size()), normal(color.size()), post(color.size());
+    std::vector<unsigned char> c8(color.size()), a8(color.size()), n8(color.size());
+    std::vector<fm_post_op> ops(FM_MAX_POST_OPS);
+    int32_t n_ops = 0;
+    fm_scene_post_ops(s, ops.data(), FM_MAX_POST_OPS, &n_ops);
+    ops.resize((size_t)n_ops);
+    for (size_t k = 0; k < ops.size();)
+        if (ops[k].type == FM_POST_OIDN_DENOISE) {
+            fprintf(stderr, "warning: oidn_denoise is not available, skipped\n");
+            ops.erase(ops.begin() + (long)k);
+        } else {
+            k++;
+        }
+    const std::string out = info.renderer.output;
+    auto write_all = [&](const std::vector<fm_post_op>& post_ops) {
+        const int64_t n = (int64_t)w * h;
+        return fm_post_process(ctx, post_ops.data(), (int32_t)post_ops.size(), color.data(), n, post.data(), c8.data()) == FM_OK &&
+               fm_post_process(ctx, nullptr, 0, albedo.data(), n, nullptr, a8.data()) == FM_OK &&
+               fm_post_process(ctx, nullptr, 0, normal.data(), n, nullptr, n8.data()) == FM_OK &&
+               write_png(out + ".png", w, h, c8.data()) && write_png(out + "_albedo.png", w, h, a8.data()) &&
+               write_png(out + "_normal.png", w, h, n8.data());
+    };
     fm_render_params p;
     memset(&p, 0, sizeof p);
     p.rr_depth = -1;
-    p.mode = FM_MODE_MEGAKERNEL;
+    p.mode = FM_MODE_WAVEFRONT;
     p.seed = 1;
     fm_film film;
     memset(&film, 0, sizeof film);
     film.color = color.data(); film.albedo = albedo.data(); film.normal = normal.data();
-    fm_render_stats st;
-    if (fm_render(ctx, s, &p, &film, &st) != FM_OK) { fprintf(stderr, "render: %s\n", fm_last_error(ctx)); fm_scene_destroy(s); fm_shutdown(ctx); return -1; }
-    const std::string out = info.renderer.output;
-    bool ok = write_png(out + ".png", w, h, color.data()) && write_png(out + "_albedo.png", w, h, albedo.data()) &&
-              write_png(out + "_normal.png", w, h, normal.data()) && write_pfm(out + ".pfm", w, h, color.data());
-    printf("rendered %dx%d at %d spp in %.1f ms (%.2f Mpaths/s, %.2f Mrays/s)\n", w, h, info.renderer.spp, st.kernel_ms,
-           st.paths / (st.kernel_ms * 1e3), st.rays / (st.kernel_ms * 1e3));
+    const int spp = info.renderer.spp > 0 ? info.renderer.spp : 1;
+    double ms = 0.0;
+    int64_t paths = 0, rays = 0;
+    int done = 0;
+    bool ok = true;
+    for (int step = 1; step <= 10 && ok; step++) {
+        const int target = (int)((int64_t)spp * step / 10);
+        if (target <= done) continue;
+        p.spp = target - done;
+        p.sample_offset = done;
+        p.accumulate = done > 0 ? 1 : 0;
+        fm_render_stats st;
+        if (fm_render(ctx, s, &p, &film, &st) != FM_OK) {
+            fprintf(stderr, "render: %s\n", fm_last_error(ctx));
+            fm_scene_destroy(s);
+            fm_shutdown(ctx);
+            return -1;
+        }
+        ms += st.kernel_ms; paths += st.paths; rays += st.rays;
+        done = target;
+        printf("Current progress is : %.2f\n", (double)done / spp);
+        ok = write_all(done < spp ? std::vector<fm_post_op>() : ops);
+    }
+    ok = ok && write_pfm(out + ".pfm", w, h, color.data());
+    if (!ok) fprintf(stderr, "output: %s\n", fm_last_error(ctx));
+    printf("rendered %dx%d at %d spp in %.1f ms (%.2f Mpaths/s, %.2f Mrays/s)\n", w, h, spp, ms,
+           paths / (ms * 1e3), rays / (ms * 1e3));
     fm_scene_destroy(s);
     fm_shutdown(ctx);
     return ok ? 0 : -1;

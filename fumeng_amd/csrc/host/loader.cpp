@@ -518,6 +518,23 @@ struct Loader {
             if (os->type == Json::ARRAY) for (const Json& o : os->arr) render_object(o);
         if (const Json* ls = s->find("lights"))
             if (ls->type == Json::ARRAY) for (const Json& l : ls->arr) lights(l);
+        /* ParsePostProcess (scene_loader.h:254-275); an unknown type is
+         * skipped with a warning, like the reference */
+        if (const Json* pp = cfg.find("post_process"))
+            if (pp->type == Json::ARRAY)
+                for (const Json& e : pp->arr) {
+                    std::string t;
+                    load_str(e, "type", t);
+                    fm_post_op op{};
+                    op.lum = 1.0f;
+                    if (t == "tonemapping") { op.type = FM_POST_TONEMAPPING; load_real(e, "lum", op.lum); }
+                    else if (t == "linear_to_srgb") op.type = FM_POST_LINEAR_TO_SRGB;
+                    else if (t == "srgb_to_linear") op.type = FM_POST_SRGB_TO_LINEAR;
+                    else if (t == "oidn_denoise") op.type = FM_POST_OIDN_DENOISE;
+                    else { fprintf(stderr, "[fumeng] Post process type not supported: %s.\n", t.c_str()); continue; }
+                    if (out.post_ops.size() >= FM_MAX_POST_OPS) throw LoadError(FM_ERR_PARSE, "too many post_process entries");
+                    out.post_ops.push_back(op);
+                }
         out.finish();
     }
 };
@@ -544,6 +561,7 @@ void LoadedScene::finish() {
     desc.num_meshes = (int32_t)meshes.size(); desc.meshes = meshes.data();
     desc.num_objects = (int32_t)objects.size(); desc.objects = objects.data();
     desc.num_lights = (int32_t)lights.size(); desc.lights = lights.data();
+    desc.num_post_ops = (int32_t)post_ops.size(); desc.post_ops = post_ops.data();
 }
 
 void load_scene_json(const std::string& path, LoadedScene& out) {

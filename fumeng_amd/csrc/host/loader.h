@@ -38,6 +38,7 @@ struct LoadedScene {
     std::vector<fm_material_desc> materials;
     std::vector<fm_object_desc> objects;
     std::vector<fm_light_desc> lights;
+    std::vector<fm_post_op> post_ops;
     void finish();
 };
 

@@ -454,6 +454,11 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
     hs.renderer = d.renderer;
     hs.camera_desc = d.camera;
     hs.naive = d.accelerator == FM_ACCEL_NAIVE;
+    if (d.num_post_ops < 0 || d.num_post_ops > FM_MAX_POST_OPS || (d.num_post_ops > 0 && !d.post_ops))
+        throw BuildError(FM_ERR_INVALID, "post_ops: at most FM_MAX_POST_OPS entries");
+    hs.post_ops.assign(d.post_ops, d.post_ops + d.num_post_ops);
+    for (const fm_post_op& op : hs.post_ops)
+        if (op.type < FM_POST_TONEMAPPING || op.type > FM_POST_OIDN_DENOISE) throw BuildError(FM_ERR_INVALID, "post_ops: unknown type");
     const int default_mat = d.num_materials;
 
     /* ---- geometries in object order ---- */

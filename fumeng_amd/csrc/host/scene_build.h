@@ -55,6 +55,7 @@ struct HostScene {
     fm_dcamera camera{};
     fm_renderer_desc renderer{};
     fm_camera_desc camera_desc{};
+    std::vector<fm_post_op> post_ops;   /* colour post-process list */
 };
 
 /* Env sampling tables of env light e from its texels (RGB per texel, linear);

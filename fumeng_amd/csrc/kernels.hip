@@ -172,6 +172,39 @@ __global__ void __launch_bounds__(256) k_render_mega(fm_dscene S, fm::RenderArgs
     }
 }
 
+/* ---- output step (post/tonemapping.cpp:10-31, utils.h:120-137) ---- */
+FMD float filmic_core(float x) {
+#pragma clang fp contract(off)
+    const float A = 0.22f, B = 0.30f, C = 0.10f, D = 0.20f, E = 0.01f, F = 0.30f;
+    return ((x * (A * x + C * B) + D * E) / (x * (A * x + B) + D * F)) - E / F;
+}
+/* powf through double: the correctly rounded float power (the host libm's
+ * powf is within 1 ulp of it) */
+FMD float pow_f(float x, float y) { return (float)pow((double)x, (double)y); }
+FMD float post_op(const fm_post_op& op, float c) {
+#pragma clang fp contract(off)
+    if (op.type == FM_POST_TONEMAPPING) return filmic_core(1.6f * c * op.lum) / filmic_core(11.2f);
+    if (op.type == FM_POST_LINEAR_TO_SRGB) {
+        if (c <= 0.0031308f) return 12.92f * c;
+        return 1.055f * pow_f(c, 1.0f / 2.4f) - 0.055f;
+    }
+    if (c <= 0.04045f) return c * 1.0f / 12.92f;     /* FM_POST_SRGB_TO_LINEAR */
+    return pow_f((c + 0.055f) * 1.0f / 1.055f, 2.4f);
+}
+__global__ void k_post(fm::PostOps ops, const float* __restrict__ in, int64_t n, float* __restrict__ out,
+                       unsigned char* __restrict__ out8) {
+#pragma clang fp contract(off)
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float c = in[i];
+        for (int k = 0; k < ops.n; k++) c = post_op(ops.op[k], c);
+        if (out) out[i] = c;
+        if (out8) {
+            const float cl = c > 1.0f ? 1.0f : (c < 0.0f ? 0.0f : c);
+            out8[i] = cl != cl ? (unsigned char)0 : (unsigned char)roundf(255.0f * cl);
+        }
+    }
+}
+
 __global__ void k_zero_f64(double* p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0.0;
 }
@@ -203,6 +236,10 @@ hipError_t launch_light(const fm_dscene& S, const fm_light_query* q, int64_t n, 
 }
 hipError_t launch_env(const fm_dscene& S, const float* dirs, int64_t n, float* pdf, float* rgb, hipStream_t st) {
     hipLaunchKernelGGL(k_env, dim3(blocks_for(n, 256)), dim3(256), 0, st, S, dirs, n, pdf, rgb);
+    return hipGetLastError();
+}
+hipError_t launch_post(const PostOps& ops, const float* in, int64_t n, float* out, unsigned char* out8, hipStream_t st) {
+    hipLaunchKernelGGL(k_post, dim3(blocks_for(n, 256)), dim3(256), 0, st, ops, in, n, out, out8);
     return hipGetLastError();
 }
 hipError_t launch_zero_f64(double* p, int64_t n, hipStream_t st) {

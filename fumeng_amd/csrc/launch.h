@@ -131,6 +131,12 @@ hipError_t launch_bsdf(const fm_dscene& S, const fm_bsdf_query* q, int64_t n, fm
 hipError_t launch_light(const fm_dscene& S, const fm_light_query* q, int64_t n, fm_light_result* out, hipStream_t st);
 hipError_t launch_env(const fm_dscene& S, const float* dirs, int64_t n, float* pdf, float* rgb, hipStream_t st);
 hipError_t launch_zero_f64(double* p, int64_t n, hipStream_t st);
+struct PostOps {
+    int n;
+    fm_post_op op[FM_MAX_POST_OPS];
+};
+/* n floats (3 per pixel) through the post-process ops, then RealToUInt8 */
+hipError_t launch_post(const PostOps& ops, const float* in, int64_t n, float* out, unsigned char* out8, hipStream_t st);
 hipError_t launch_render_mega(const fm_dscene& S, const RenderArgs& A, int block, int grid, bool count, hipStream_t st);
 /* Blocks per CU the megakernel reaches at this block size (occupancy query). */
 int mega_blocks_per_cu(const fm_dscene& S, int block);

@@ -44,6 +44,7 @@ class SceneDesc:
         self.renderer = {"width": 1, "height": 1, "spp": 10, "thread_count": 8, "depth": 10, "direct_loop": 1,
                          "rr_depth": 10, "rr_coef": 0.85, "output": "result"}
         self.accelerator = "bvh"
+        self.post_process = []  # [{"type": "tonemapping", "lum": 1.0}, {"type": "linear_to_srgb"}, ...]
         self.images = []      # float32 HxWx3 linear (hdr) or uint8 HxWx3 sRGB
         self.materials = []   # dicts
         self.material_names = []
@@ -194,12 +195,17 @@ class SceneDesc:
                 lights[i].type = A.FM_LIGHT_DIRECTIONAL
                 lights[i].radiance[:] = l["radiance"]
                 lights[i].direction[:] = l["direction"]
-        keep += [imgs, mats, meshes, objs, lights]
+        posts = (A.fm_post_op * max(1, len(self.post_process)))()
+        for i, pp in enumerate(self.post_process):
+            posts[i].type = A.POST_TYPES[pp["type"]]
+            posts[i].lum = float(pp.get("lum", 1.0))
+        keep += [imgs, mats, meshes, objs, lights, posts]
         d.num_images, d.images = len(self.images), C.cast(imgs, C.POINTER(A.fm_image_desc))
         d.num_materials, d.materials = len(self.materials), C.cast(mats, C.POINTER(A.fm_material_desc))
         d.num_meshes, d.meshes = len(self.meshes), C.cast(meshes, C.POINTER(A.fm_mesh_desc))
         d.num_objects, d.objects = len(self.objects), C.cast(objs, C.POINTER(A.fm_object_desc))
         d.num_lights, d.lights = len(self.lights), C.cast(lights, C.POINTER(A.fm_light_desc))
+        d.num_post_ops, d.post_ops = len(self.post_process), C.cast(posts, C.POINTER(A.fm_post_op))
         return d, keep
 
     # ---- files -------------------------------------------------------------
@@ -268,6 +274,8 @@ class SceneDesc:
         doc = {"camera": cam, "renderer": dict(self.renderer, type="path_tracer"),
                "scene": {"accelerator": {"type": self.accelerator}, "materials": mats, "render_objects": objs,
                          "lights": lights}}
+        if self.post_process:
+            doc["post_process"] = [dict(pp, **({"lum": _f(pp["lum"])} if "lum" in pp else {})) for pp in self.post_process]
         path = os.path.join(directory, name)
         with open(path, "w") as f:
             json.dump(doc, f, indent=1)

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define FM_ABI_VERSION 1u
+#define FM_ABI_VERSION 2u
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -161,6 +161,20 @@ typedef struct fm_renderer_desc {
  * FM_ACCEL_BVH traverses a binned-SAH tree; FM_ACCEL_BVH_REFERENCE_TREE
  * traverses the reference builder's own topology (bvh_aggregate.cpp:38-79). */
 enum { FM_ACCEL_BVH = 0, FM_ACCEL_NAIVE = 1, FM_ACCEL_BVH_REFERENCE_TREE = 2 };
+
+/* Post-process operations of the JSON "post_process" list
+ * (scene_loader.h:254-275), applied in order to the colour image once the
+ * frame is complete (renderer.h:131-142): ToneMapping (tonemapping.cpp:5-31,
+ * key "lum", default 1), LinearToSrgb / SrgbToLinear (post/linear_to_srgb.cpp,
+ * post/srgb_to_linear.cpp over utils.h:120-132), and the OIDN denoiser, which
+ * is not available here (fm_post_process rejects it). */
+enum { FM_POST_TONEMAPPING = 1, FM_POST_LINEAR_TO_SRGB = 2, FM_POST_SRGB_TO_LINEAR = 3, FM_POST_OIDN_DENOISE = 4 };
+#define FM_MAX_POST_OPS 16
+typedef struct fm_post_op {
+    int32_t type;          /* FM_POST_* */
+    float lum;             /* tonemapping: ToneMappingConfig::lum */
+} fm_post_op;
+
 typedef struct fm_scene_desc {
     uint32_t abi_version;  /* FM_ABI_VERSION */
     int32_t accelerator;   /* FM_ACCEL_* */
@@ -171,6 +185,7 @@ typedef struct fm_scene_desc {
     int32_t num_meshes;     const fm_mesh_desc* meshes;
     int32_t num_objects;    const fm_object_desc* objects;
     int32_t num_lights;     const fm_light_desc* lights;
+    int32_t num_post_ops;   const fm_post_op* post_ops;   /* at most FM_MAX_POST_OPS */
 } fm_scene_desc;
 
 /* ---- context and scene -------------------------------------------------- */
@@ -266,6 +281,21 @@ int fm_render(fm_ctx* ctx, fm_scene* scene, const fm_render_params* params,
               fm_film* film, fm_render_stats* stats);
 /* Copies the current device film (means over the samples rendered so far). */
 int fm_film_read(fm_ctx* ctx, fm_film* film);
+
+/* ---- output step -------------------------------------------------------- */
+
+/* The scene's post-process list; *n_ops receives its length (at most
+ * max_ops entries are copied). */
+int fm_scene_post_ops(const fm_scene* scene, fm_post_op* ops, int32_t max_ops, int32_t* n_ops);
+/* The reference's output step on the device for n_pixels RGB float pixels
+ * (host arrays, 3 floats per pixel): the ops in order (PostProcess::Process),
+ * then Image::save_to_file's RealToUInt8 per channel (image.cpp:97-110,
+ * utils.h:134-137: clamp to [0, 1], round(255 x); a NaN gives 0, the
+ * reference's float-to-uint8 conversion on x86-64).  rgb_out receives the
+ * post-processed floats, rgb8 the bytes; either may be NULL.  The albedo and
+ * normal images take n_ops = 0. */
+int fm_post_process(fm_ctx* ctx, const fm_post_op* ops, int32_t n_ops, const float* rgb_in, int64_t n_pixels,
+                    float* rgb_out, uint8_t* rgb8);
 
 /* One render of a batch: the scene seen through its own camera at its own
  * resolution, samples, depth and seed.  Fields <= 0 take the scene's

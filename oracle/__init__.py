@@ -40,6 +40,7 @@ def lib():
                                     P, P, P, P]),
             "or_philox": (None, [P, P, P]),
             "or_abi_version": (C.c_int, []),
+            "or_post_process": (None, [C.POINTER(A.fm_post_op), C.c_int, P, I64, P, P]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -135,3 +136,17 @@ def philox(ctr, key):
     o = np.zeros(4, np.uint32)
     lib().or_philox(_p(c), _p(k), _p(o))
     return o
+
+
+def post_process(ops, rgb):
+    """Reference output step on a float image: (post-processed floats, bytes).
+    ops: list of {"type": ..., "lum": ...} as in the scene JSON."""
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    arr = (A.fm_post_op * max(1, len(ops)))()
+    for i, o in enumerate(ops):
+        arr[i].type = A.POST_TYPES[o["type"]]
+        arr[i].lum = float(o.get("lum", 1.0))
+    out = np.empty_like(rgb)
+    out8 = np.empty(rgb.shape, np.uint8)
+    lib().or_post_process(arr, len(ops), rgb.ctypes.data, rgb.size, out.ctypes.data, out8.ctypes.data)
+    return out, out8

@@ -481,6 +481,37 @@ template <typename R> static int render_t(or_handle* hd, const fm_render_params*
 
 #define DISPATCH(fn, ...) (h->fp64 ? fn<double>(h, __VA_ARGS__) : fn<float>(h, __VA_ARGS__))
 
+/* ---- output step (FP32 build: real = float) ----
+ * ToneMapping::FilmicCore / Filmic (post/tonemapping.cpp:10-31),
+ * SrgbToLinear / LinearToSrgb (core/utils.h:120-132), RealToUInt8
+ * (core/utils.h:134-137) with Clamp (utils.h:68-71); std::pow on floats is
+ * the host libm's powf. */
+namespace post {
+static float filmic_core(float x) {
+    const float A = 0.22f, B = 0.30f, C = 0.10f, D = 0.20f, E = 0.01f, F = 0.30f;
+    return ((x * (A * x + C * B) + D * E) / (x * (A * x + B) + D * F)) - E / F;
+}
+static float filmic(float value, float lum) {
+    static const float white = filmic_core(11.2f);
+    return filmic_core(1.6f * value * lum) / white;
+}
+static float srgb_to_linear(float c) {
+    if (c <= 0.04045f) return c * 1.0f / 12.92f;
+    return std::pow((c + 0.055f) * 1.0f / 1.055f, 2.4f);
+}
+static float linear_to_srgb(float c) {
+    if (c <= 0.0031308f) return 12.92f * c;
+    return 1.055f * std::pow(c, (float)(1.0f / 2.4f)) - 0.055f;
+}
+static uint8_t real_to_uint8(float c) {
+    c = c > 1.0f ? 1.0f : (c < 0.0f ? 0.0f : c);
+    const double r = round(255.0f * c);
+    /* static_cast<uint8_t> of a NaN: x86-64 converts through a 32-bit
+     * integer (0x80000000), whose low byte is 0 */
+    return r != r ? (uint8_t)0 : (uint8_t)r;
+}
+}  // namespace post
+
 extern "C" {
 
 void* or_scene_create(const fm_scene_desc* d, int fp64, char* err, int64_t errlen) {
@@ -540,5 +571,19 @@ int or_render(void* p, const fm_render_params* prm, int x0, int y0, int w, int h
 }
 void or_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) { philox4x32_10(ctr, key, out); }
 int or_abi_version(void) { return (int)FM_ABI_VERSION; }
+/* PostProcess::Process over the colour image in list order, then
+ * Image::save_to_file's byte conversion (image.cpp:97-110). */
+void or_post_process(const fm_post_op* ops, int n_ops, const float* in, int64_t n, float* out, uint8_t* out8) {
+    for (int64_t i = 0; i < n; i++) {
+        float c = in[i];
+        for (int k = 0; k < n_ops; k++) {
+            if (ops[k].type == FM_POST_TONEMAPPING) c = post::filmic(c, ops[k].lum);
+            else if (ops[k].type == FM_POST_LINEAR_TO_SRGB) c = post::linear_to_srgb(c);
+            else if (ops[k].type == FM_POST_SRGB_TO_LINEAR) c = post::srgb_to_linear(c);
+        }
+        if (out) out[i] = c;
+        if (out8) out8[i] = post::real_to_uint8(c);
+    }
+}
 
 }  /* extern "C" */

@@ -214,3 +214,24 @@ def test_geodesic_sphere_counts_and_closure():
     e = np.sort(np.concatenate([t[:, [0, 1]], t[:, [1, 2]], t[:, [2, 0]]]), axis=1)
     _, counts = np.unique(e, axis=0, return_counts=True)
     assert np.all(counts == 2)
+
+
+def test_loader_reads_the_post_process_list(tmp_path, capfd):
+    s = mesh_scene(freq=4)
+    s.post_process = [{"type": "tonemapping", "lum": 1.75}, {"type": "linear_to_srgb"}, {"type": "oidn_denoise"},
+                      {"type": "srgb_to_linear"}]
+    path = s.write(str(tmp_path))
+    import json
+    doc = json.load(open(path))
+    doc["post_process"].insert(1, {"type": "sharpen"})        # unknown: skipped with a warning
+    json.dump(doc, open(path, "w"))
+    rc, h, dp, err = _load_desc(path)
+    assert rc == 0, err
+    d = dp.contents
+    got = [(d.post_ops[i].type, d.post_ops[i].lum) for i in range(d.num_post_ops)]
+    assert got == [(A.FM_POST_TONEMAPPING, np.float32(1.75)), (A.FM_POST_LINEAR_TO_SRGB, 1.0),
+                   (A.FM_POST_OIDN_DENOISE, 1.0), (A.FM_POST_SRGB_TO_LINEAR, 1.0)]
+    fumeng_amd.lib().fm_desc_free(h)
+    assert "sharpen" in capfd.readouterr().err
+    info = fumeng_amd.host_scene_info(s)       # the builder accepts the list
+    assert info.num_geometries > 0

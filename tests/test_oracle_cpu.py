@@ -316,3 +316,55 @@ def test_box_render_float_vs_double_consistent():
     b = oracle.OracleScene(s, fp64=True).render(0, 0, 24, 24, spp=8, seed=3, threads=4)[0]
     close = np.abs(a - b) <= 1e-3 * np.maximum(1, np.abs(b))
     assert close.mean() > 0.9   # 8 paths per pixel, up to 8 bounces: a pixel differs once one path diverges
+
+
+# ---- output step ----------------------------------------------------------
+
+def _hable(x):
+    """Hable's filmic curve (Uncharted 2, published constants: shoulder 0.22,
+    linear strength 0.30, linear angle 0.10, toe 0.20, toe numerator 0.01,
+    toe denominator 0.30), in float64."""
+    A_, B_, C_, D_, E_, F_ = 0.22, 0.30, 0.10, 0.20, 0.01, 0.30
+    x = np.asarray(x, np.float64)
+    return (x * (A_ * x + C_ * B_) + D_ * E_) / (x * (A_ * x + B_) + D_ * F_) - E_ / F_
+
+
+def test_post_tonemapping_is_the_published_filmic_curve():
+    x = np.linspace(0.0, 40.0, 4001).astype(np.float32)
+    for lum in (1.0, 0.35, 2.5):
+        out, _ = oracle.post_process([{"type": "tonemapping", "lum": lum}], x)
+        ref = _hable(1.6 * x.astype(np.float64) * np.float32(lum)) / _hable(11.2)
+        assert np.max(np.abs(out - ref)) < 2e-6
+        assert np.all(np.diff(out) >= 0)                 # monotone
+    # the white point maps to 1
+    out, _ = oracle.post_process([{"type": "tonemapping", "lum": 1.0}], np.array([11.2 / 1.6], np.float32))
+    assert abs(out[0] - 1.0) < 1e-6
+
+
+def test_post_srgb_pair_is_the_iec_61966_transfer():
+    c = np.linspace(0.0, 1.0, 100001).astype(np.float32)
+    enc, _ = oracle.post_process([{"type": "linear_to_srgb"}], c)
+    # IEC 61966-2-1 in float64; its two pieces meet at the breakpoint
+    ref = np.where(c <= 0.0031308, 12.92 * c.astype(np.float64), 1.055 * c.astype(np.float64) ** (1 / 2.4) - 0.055)
+    assert np.max(np.abs(enc - ref)) < 3e-7
+    assert abs(12.92 * 0.0031308 - (1.055 * 0.0031308 ** (1 / 2.4) - 0.055)) < 1e-6
+    assert abs(float(oracle.post_process([{"type": "linear_to_srgb"}], np.float32([0.18]))[0][0]) - 0.46135613) < 1e-6
+    back, _ = oracle.post_process([{"type": "srgb_to_linear"}], enc)
+    assert np.max(np.abs(back - c)) < 2e-6                # inverse pair
+    both, _ = oracle.post_process([{"type": "linear_to_srgb"}, {"type": "srgb_to_linear"}], c)
+    assert np.array_equal(both, back)                     # ops apply in list order
+
+
+def test_post_bytes_round_half_away_from_zero_and_clamp():
+    def tie(k):
+        """a float c with fl(255 c) == k + 0.5 exactly"""
+        c = np.float32((k + 0.5) / 255.0)
+        while np.float32(255.0) * c < np.float32(k + 0.5):
+            c = np.nextafter(c, np.float32(2))
+        while np.float32(255.0) * c > np.float32(k + 0.5):
+            c = np.nextafter(c, np.float32(-2))
+        assert np.float32(255.0) * c == np.float32(k + 0.5)
+        return c
+    vals = np.array([0.0, 1.0, 2.0, -1.0, np.nan, np.inf, -np.inf, tie(126), tie(127), tie(0), 0.5], np.float32)
+    _, b = oracle.post_process([], vals)
+    assert list(b) == [0, 255, 255, 0, 0, 255, 0, 127, 128, 1, 128]   # 126.5 -> 127: away from zero, not to even
