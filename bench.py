@@ -165,9 +165,9 @@ def profiled_traffic(kernel):
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         for line in open(path):
             if line.startswith("bench-traffic:"):
-                parts = line.split()
-                if len(parts) == 3 and parts[1] == kernel:
-                    return int(parts[2])
+                name, _, count = line[len("bench-traffic:"):].strip().rpartition(" ")
+                if name == kernel and count.isdigit():
+                    return int(count)
     return None
 
 
