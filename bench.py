@@ -151,9 +151,17 @@ def stress_leg(fumeng_amd, ctx):
     res = {"workload": "stress: 400 displaced spheres (9,800,012 geometries), 200 textures 2048^2, 10 rect lights, "
                        "1920x1080, 128 spp, depth 10", "mpaths_per_s": W * H * spp / dt / 1e6,
            "mrays_per_s": st.rays / dt / 1e6, "ms_per_frame": dt * 1e3, "trace_ms": st.trace_ms,
-           "generate_s": gen, "build_upload_s": build, "device_gb": info.device_bytes / 1e9}
+           "generate_s": gen, "build_upload_s": build, "device_gb": info.device_bytes / 1e9,
+           "build_report": {"threads": info.build_threads, "geometry_ms": info.build_ms_geometry,
+                            "reference_bvh_ms": info.build_ms_ref_bvh, "traversal_tree_ms": info.build_ms_trav_tree,
+                            "tables_ms": info.build_ms_tables, "host_total_ms": info.build_ms_total,
+                            "upload_ms": info.upload_ms, "host_gb": info.host_bytes / 1e9}}
     log("stress: %.1f Mpaths/s, %.1f Mrays/s (%.0f ms per frame), built in %.1f s, %.2f GB in HBM"
         % (res["mpaths_per_s"], res["mrays_per_s"], dt * 1e3, build, res["device_gb"]))
+    log("stress build on %d threads: geometry %.0f ms, reference BVH %.0f ms, traversal tree %.0f ms, tables %.0f ms, "
+        "upload %.0f ms; host copy %.2f GB" % (info.build_threads, info.build_ms_geometry, info.build_ms_ref_bvh,
+                                               info.build_ms_trav_tree, info.build_ms_tables, info.upload_ms,
+                                               info.host_bytes / 1e9))
     return res
 
 

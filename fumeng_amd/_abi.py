@@ -106,7 +106,9 @@ class fm_scene_info(C.Structure):
                 ("device_bytes", C.c_int64), ("renderer", fm_renderer_desc), ("camera", fm_camera_desc),
                 ("env_width", C.c_int32), ("env_height", C.c_int32), ("trav_width", C.c_int32),
                 ("trav_stack", C.c_int32), ("num_trav_nodes", C.c_int64), ("trav_node_bytes", C.c_int32),
-                ("pad_", C.c_int32)]
+                ("build_threads", C.c_int32), ("build_ms_geometry", C.c_double), ("build_ms_ref_bvh", C.c_double),
+                ("build_ms_trav_tree", C.c_double), ("build_ms_tables", C.c_double), ("build_ms_total", C.c_double),
+                ("upload_ms", C.c_double), ("host_bytes", C.c_int64), ("trav_hash", C.c_uint64)]
 
 
 class fm_render_params(C.Structure):

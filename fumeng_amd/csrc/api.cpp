@@ -9,6 +9,7 @@
 #include "host/loader.h"
 #include "host/scene_build.h"
 
+#include <chrono>
 #include <hip/hip_runtime_api.h>
 
 #include <cmath>
@@ -132,6 +133,7 @@ struct fm_scene {
     DevBuf nodes, nodes4, prims, meta, tri_sh, rects, spheres, materials, lights, envs, images, texels, env_tables, lut;
     fm_dscene d{};
     int64_t device_bytes = 0;
+    double upload_ms = 0.0;
 };
 
 struct fm_loaded_desc {
@@ -139,6 +141,13 @@ struct fm_loaded_desc {
 };
 
 namespace {
+
+template <typename V> int64_t vbytes(const V& v) { return (int64_t)(v.size() * sizeof(v[0])); }
+template <typename V> uint64_t fnv1a(const V& v, uint64_t h = 1469598103934665603ull) {
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(v.data());
+    for (size_t i = 0; i < v.size() * sizeof(v[0]); i++) { h ^= b[i]; h *= 1099511628211ull; }
+    return h;
+}
 
 int fail(fm_ctx* ctx, int code, const std::string& msg) {
     if (ctx) ctx->err = msg;
@@ -165,6 +174,7 @@ template <typename F> int guarded(fm_ctx* ctx, F&& f) {
 }
 
 void upload_scene(fm_scene* s) {
+    const auto t0 = std::chrono::steady_clock::now();
     hipStream_t st = s->ctx->stream;
     fmh::HostScene& hs = s->hs;
     s->nodes.upload(hs.nodes, st);
@@ -213,6 +223,7 @@ void upload_scene(fm_scene* s) {
     for (DevBuf* b : {&s->nodes, &s->nodes4, &s->prims, &s->meta, &s->tri_sh, &s->rects, &s->spheres, &s->materials, &s->lights,
                       &s->envs, &s->images, &s->texels, &s->env_tables, &s->lut})
         s->device_bytes += (int64_t)b->n;
+    s->upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 /* Copies n items in, runs a launcher, copies m items out. */
@@ -490,7 +501,7 @@ void fm_scene_destroy(fm_scene* s) {
     delete s;
 }
 
-static void fill_info(const fmh::HostScene& hs, int64_t device_bytes, fm_scene_info* o) {
+static void fill_info(const fmh::HostScene& hs, int64_t device_bytes, double upload_ms, fm_scene_info* o) {
     memset(o, 0, sizeof(*o));
     o->num_geometries = hs.num_geoms;
     o->num_bvh_nodes = (int64_t)hs.ref_nodes.size();
@@ -509,11 +520,23 @@ static void fill_info(const fmh::HostScene& hs, int64_t device_bytes, fm_scene_i
     o->trav_stack = hs.trav_stack;
     o->num_trav_nodes = (int64_t)(hs.trav_width == 4 ? hs.nodes4.size() : hs.nodes.size());
     o->trav_node_bytes = hs.trav_width == 4 ? (int32_t)sizeof(fm_dnode4) : (int32_t)sizeof(fm_dnode);
+    o->build_threads = hs.report.threads;
+    o->build_ms_geometry = hs.report.geometry_ms;
+    o->build_ms_ref_bvh = hs.report.ref_bvh_ms;
+    o->build_ms_trav_tree = hs.report.trav_tree_ms;
+    o->build_ms_tables = hs.report.tables_ms;
+    o->build_ms_total = hs.report.total_ms;
+    o->upload_ms = upload_ms;
+    o->host_bytes = vbytes(hs.ref_nodes) + vbytes(hs.order) + vbytes(hs.nodes) + vbytes(hs.nodes4) + vbytes(hs.prims) +
+                    vbytes(hs.meta) + vbytes(hs.tri_sh) + vbytes(hs.rects) + vbytes(hs.spheres) + vbytes(hs.materials) +
+                    vbytes(hs.lights) + vbytes(hs.envs) + vbytes(hs.images) + vbytes(hs.texels) + vbytes(hs.env_tables) +
+                    vbytes(hs.srgb_lut);
+    o->trav_hash = fnv1a(hs.prims, fnv1a(hs.nodes4, fnv1a(hs.nodes)));
 }
 
 int fm_scene_get_info(const fm_scene* s, fm_scene_info* o) {
     if (!s || !o) return fail(nullptr, FM_ERR_INVALID, "NULL argument");
-    fill_info(s->hs, s->device_bytes, o);
+    fill_info(s->hs, s->device_bytes, s->upload_ms, o);
     return FM_OK;
 }
 
@@ -996,7 +1019,7 @@ int fm_host_scene_info(const fm_scene_desc* desc, int32_t bvh_width, fm_scene_in
         fmh::HostScene hs;
         hs.trav_width = bvh_width;
         fmh::build_host_scene(*desc, hs);
-        fill_info(hs, 0, out);
+        fill_info(hs, 0, 0.0, out);
         return FM_OK;
     });
 }

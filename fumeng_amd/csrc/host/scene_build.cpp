@@ -19,6 +19,11 @@
 #include "scene_build.h"
 
 #include <algorithm>
+#include <sched.h>
+#include <atomic>
+#include <cstdlib>
+#include <thread>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -165,24 +170,85 @@ void push_tight(std::vector<float>& out, const F3* p, int n) {
     for (int k = 0; k < 3; k++) out.push_back(mx[k]);
 }
 
+/* Host threads for the builds: the two halves of a node are independent
+ * (disjoint ranges, the parent's sort or partition fixes their input), so
+ * the top levels of both builders run their subtrees on separate threads
+ * and splice the results back in pre-order; the trees are identical to the
+ * sequential builds. */
+int build_threads() {
+    /* the CPUs this process may run on, at most 64; OMP_NUM_THREADS and
+     * FUMENG_BUILD_THREADS cap them (FUMENG_BUILD_THREADS=1: sequential) */
+    cpu_set_t set;
+    int n = sched_getaffinity(0, sizeof set, &set) == 0 ? CPU_COUNT(&set) : (int)std::thread::hardware_concurrency();
+    n = std::max(1, std::min(n, 64));
+    for (const char* var : {"OMP_NUM_THREADS", "FUMENG_BUILD_THREADS"})
+        if (const char* e = std::getenv(var)) {
+            const int v = std::atoi(e);
+            if (v >= 1) n = std::min(n, v);
+        }
+    return n;
+}
+int par_levels(int threads) {
+    int l = 0;
+    while ((1 << l) < 2 * threads) l++;
+    return l;
+}
+constexpr size_t kParMin = 1 << 15;   /* smallest subtree handed to its own thread */
+
+/* f(begin, end) over contiguous chunks of [0, n) on up to `threads` threads */
+template <typename F>
+void parallel_for(size_t n, int threads, F&& f) {
+    const size_t t = std::max<size_t>(1, std::min<size_t>((size_t)threads, n / 4096 + 1));
+    if (t == 1) { f((size_t)0, n); return; }
+    std::vector<std::thread> pool;
+    const size_t step = (n + t - 1) / t;
+    for (size_t k = 1; k < t; k++) {
+        const size_t b = k * step, e = std::min(n, b + step);
+        if (b < e) pool.emplace_back([&f, b, e]() { f(b, e); });
+    }
+    f((size_t)0, std::min(n, step));
+    for (std::thread& th : pool) th.join();
+}
+
 struct Builder {
     std::vector<BInfo> infos;
+    struct Out {
+        std::vector<RefNode> nodes;
+        std::vector<int32_t> order;
+        int depth = 0;
+    };
     std::vector<RefNode> nodes;
     std::vector<int32_t> order;
     int depth = 0;
+    int par = 0;
+    void build(size_t start, size_t end, int level) {
+        Out o;
+        rec(o, start, end, level, par);
+        nodes.swap(o.nodes); order.swap(o.order); depth = o.depth;
+    }
+    /* the second half's nodes and leaf ranges follow the first half's */
+    static void splice(Out& o, Out& part) {
+        const int64_t dn = (int64_t)o.nodes.size(), dord = (int64_t)o.order.size();
+        for (RefNode n : part.nodes) {
+            if (n.leaf) { n.a += dord; n.b += dord; } else { n.a += dn; n.b += dn; }
+            o.nodes.push_back(n);
+        }
+        o.order.insert(o.order.end(), part.order.begin(), part.order.end());
+        o.depth = std::max(o.depth, part.depth);
+    }
     /* bvh_aggregate.cpp:38-79 */
-    int64_t build(size_t start, size_t end, int level) {
-        depth = std::max(depth, level);
+    int64_t rec(Out& o, size_t start, size_t end, int level, int par_left) {
+        o.depth = std::max(o.depth, level);
         const size_t size = end - start;
         if (size <= 5) {
             Box box;
-            const int64_t s0 = (int64_t)order.size();
-            for (size_t i = start; i < end; i++) { order.push_back(infos[i].geom); box.grow(infos[i].box); }
+            const int64_t s0 = (int64_t)o.order.size();
+            for (size_t i = start; i < end; i++) { o.order.push_back(infos[i].geom); box.grow(infos[i].box); }
             RefNode n;
             memcpy(n.mn, box.mn, sizeof n.mn); memcpy(n.mx, box.mx, sizeof n.mx);
-            n.leaf = true; n.a = s0; n.b = (int64_t)order.size();
-            nodes.push_back(n);
-            return (int64_t)nodes.size() - 1;
+            n.leaf = true; n.a = s0; n.b = (int64_t)o.order.size();
+            o.nodes.push_back(n);
+            return (int64_t)o.nodes.size() - 1;
         }
         Box root;
         for (size_t i = start; i < end; i++) root.grow(infos[i].box);
@@ -190,16 +256,28 @@ struct Builder {
         std::sort(infos.begin() + (long)start, infos.begin() + (long)end, [ax](const BInfo& a, const BInfo& b) {
             return a.box.mn[ax] + a.box.mx[ax] < b.box.mn[ax] + b.box.mx[ax];
         });
-        const int64_t id = (int64_t)nodes.size();
+        const int64_t id = (int64_t)o.nodes.size();
         RefNode n;
         memcpy(n.mn, root.mn, sizeof n.mn); memcpy(n.mx, root.mx, sizeof n.mx);
         n.leaf = false; n.a = 0; n.b = 0;
-        nodes.push_back(n);
+        o.nodes.push_back(n);
         const size_t mid = (start + end) / 2;
-        const int64_t a = build(start, mid, level + 1);
-        const int64_t b = build(mid, end, level + 1);
-        nodes[(size_t)id].a = a;
-        nodes[(size_t)id].b = b;
+        int64_t a, b;
+        if (par_left > 0 && size >= 2 * kParMin) {
+            Out lo, hi;
+            std::thread t([&]() { rec(lo, start, mid, level + 1, par_left - 1); });
+            rec(hi, mid, end, level + 1, par_left - 1);
+            t.join();
+            a = (int64_t)o.nodes.size();
+            splice(o, lo);
+            b = (int64_t)o.nodes.size();
+            splice(o, hi);
+        } else {
+            a = rec(o, start, mid, level + 1, 0);
+            b = rec(o, mid, end, level + 1, 0);
+        }
+        o.nodes[(size_t)id].a = a;
+        o.nodes[(size_t)id].b = b;
         return id;
     }
 };
@@ -225,42 +303,106 @@ float srgb_to_linear(float c) {
 
 /* Binned SAH builder over true bounds (16 bins, three axes).  Leaves hold at
  * most 7 primitives; when the remaining depth budget gets tight it switches
- * to median splits so that the tree always fits FM_STACK_DEPTH levels. */
+ * to median splits so that the tree always fits FM_STACK_DEPTH levels.
+ * Each primitive's centroid and bounds travel with it through the
+ * partitions (40-byte records), so every pass over a node's range streams
+ * through memory; the three axes are binned in one pass, and the passes of
+ * large nodes run on the build threads. */
 struct SahBuilder {
     struct N { bool leaf; int64_t a, b; };
-    const std::vector<float>& bb;
+    struct Ref { float c[3], lo[3], hi[3]; int32_t id; };
+    static constexpr int B = 16;
+    struct Bins {
+        int cnt[3][B];
+        float mn[3][B][3], mx[3][B][3];
+        void clear() {
+            for (int ax = 0; ax < 3; ax++)
+                for (int b = 0; b < B; b++) {
+                    cnt[ax][b] = 0;
+                    for (int k = 0; k < 3; k++) { mn[ax][b][k] = INFINITY; mx[ax][b][k] = -INFINITY; }
+                }
+        }
+        void merge(const Bins& o) {
+            for (int ax = 0; ax < 3; ax++)
+                for (int b = 0; b < B; b++) {
+                    cnt[ax][b] += o.cnt[ax][b];
+                    for (int k = 0; k < 3; k++) { mn[ax][b][k] = std::min(mn[ax][b][k], o.mn[ax][b][k]); mx[ax][b][k] = std::max(mx[ax][b][k], o.mx[ax][b][k]); }
+                }
+        }
+    };
+    struct Bounds {
+        float cmn[3] = {INFINITY, INFINITY, INFINITY}, cmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        float pmn[3] = {INFINITY, INFINITY, INFINITY}, pmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        void add(const Ref& r) {
+            for (int k = 0; k < 3; k++) {
+                cmn[k] = std::min(cmn[k], r.c[k]); cmx[k] = std::max(cmx[k], r.c[k]);
+                pmn[k] = std::min(pmn[k], r.lo[k]); pmx[k] = std::max(pmx[k], r.hi[k]);
+            }
+        }
+        void merge(const Bounds& o) {
+            for (int k = 0; k < 3; k++) {
+                cmn[k] = std::min(cmn[k], o.cmn[k]); cmx[k] = std::max(cmx[k], o.cmx[k]);
+                pmn[k] = std::min(pmn[k], o.pmn[k]); pmx[k] = std::max(pmx[k], o.pmx[k]);
+            }
+        }
+    };
     int64_t n;
+    std::vector<Ref> refs;
     std::vector<int32_t> order;
-    std::vector<float> cen;
     std::vector<N> nodes;
-    SahBuilder(const std::vector<float>& b, int64_t count) : bb(b), n(count) {
-        order.resize((size_t)n);
-        cen.resize((size_t)n * 3);
+    int par = 0, threads = 1;
+    SahBuilder(const std::vector<float>& bb, int64_t count) : n(count) {
+        refs.resize((size_t)n);
         for (int64_t i = 0; i < n; i++) {
-            order[(size_t)i] = (int32_t)i;
-            for (int k = 0; k < 3; k++) cen[(size_t)i * 3 + k] = 0.5f * (bb[(size_t)i * 6 + k] + bb[(size_t)i * 6 + 3 + k]);
+            Ref& r = refs[(size_t)i];
+            r.id = (int32_t)i;
+            for (int k = 0; k < 3; k++) {
+                r.lo[k] = bb[(size_t)i * 6 + k];
+                r.hi[k] = bb[(size_t)i * 6 + 3 + k];
+                r.c[k] = 0.5f * (r.lo[k] + r.hi[k]);
+            }
         }
     }
     static float area(const float* mn, const float* mx) {
         const float dx = std::max(0.0f, mx[0] - mn[0]), dy = std::max(0.0f, mx[1] - mn[1]), dz = std::max(0.0f, mx[2] - mn[2]);
         return 2.0f * (dx * dy + dy * dz + dz * dx);
     }
-    void build() { nodes.reserve((size_t)(2 * n / 3 + 8)); rec(0, n, 1); }
-    int64_t rec(int64_t s0, int64_t s1, int depth) {
+    void build() {
+        nodes.reserve((size_t)(2 * n / 3 + 8));
+        rec(nodes, 0, n, 1, par);
+        order.resize((size_t)n);
+        for (int64_t i = 0; i < n; i++) order[(size_t)i] = refs[(size_t)i].id;
+        std::vector<Ref>().swap(refs);
+    }
+    /* a pass over [s0, s1): on the build threads when the range is large */
+    template <typename T, typename F>
+    T pass(int64_t s0, int64_t s1, F&& body) {
+        const int64_t cnt = s1 - s0;
+        const int t = cnt >= (int64_t)(1 << 18) ? threads : 1;
+        std::vector<T> part((size_t)t);
+        for (T& x : part) x = T{};
+        if (t == 1) {
+            body(part[0], s0, s1);
+        } else {
+            std::vector<std::thread> pool;
+            const int64_t step = (cnt + t - 1) / t;
+            for (int k = 0; k < t; k++) {
+                const int64_t b = s0 + k * step, e = std::min(s1, b + step);
+                if (b < e) pool.emplace_back([&, k, b, e]() { body(part[(size_t)k], b, e); });
+            }
+            for (std::thread& th : pool) th.join();
+        }
+        for (int k = 1; k < t; k++) part[0].merge(part[(size_t)k]);
+        return part[0];
+    }
+    int64_t rec(std::vector<N>& nodes, int64_t s0, int64_t s1, int depth, int par_left) {
         const int64_t id = (int64_t)nodes.size();
         nodes.push_back(N{true, s0, s1});
         const int64_t cnt = s1 - s0;
         if (cnt <= 2) return id;
-        float cmn[3] = {INFINITY, INFINITY, INFINITY}, cmx[3] = {-INFINITY, -INFINITY, -INFINITY};
-        float pmn[3] = {INFINITY, INFINITY, INFINITY}, pmx[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (int64_t i = s0; i < s1; i++) {
-            const int32_t g = order[(size_t)i];
-            for (int k = 0; k < 3; k++) {
-                const float c = cen[(size_t)g * 3 + k];
-                cmn[k] = std::min(cmn[k], c); cmx[k] = std::max(cmx[k], c);
-                pmn[k] = std::min(pmn[k], bb[(size_t)g * 6 + k]); pmx[k] = std::max(pmx[k], bb[(size_t)g * 6 + 3 + k]);
-            }
-        }
+        const Bounds bd = pass<Bounds>(s0, s1, [&](Bounds& o, int64_t b, int64_t e) {
+            for (int64_t i = b; i < e; i++) o.add(refs[(size_t)i]);
+        });
         /* levels a median split would still need below this node */
         int need = 0;
         for (int64_t m = (cnt + 6) / 7; m > 1; m = (m + 1) / 2) need++;
@@ -268,65 +410,88 @@ struct SahBuilder {
         int best_axis = -1;
         int best_bin = -1;
         float best_cost = INFINITY;
-        constexpr int B = 16;
+        float sc[3];
+        for (int ax = 0; ax < 3; ax++) {
+            const float ext = bd.cmx[ax] - bd.cmn[ax];
+            sc[ax] = ext > 0.0f ? B / ext : 0.0f;
+        }
         if (!force_median) {
-            for (int ax = 0; ax < 3; ax++) {
-                const float ext = cmx[ax] - cmn[ax];
-                if (!(ext > 0.0f)) continue;
-                int bc[B] = {0};
-                float bmn[B][3], bmx[B][3];
-                for (int b = 0; b < B; b++) for (int k = 0; k < 3; k++) { bmn[b][k] = INFINITY; bmx[b][k] = -INFINITY; }
-                const float sc = B / ext;
-                for (int64_t i = s0; i < s1; i++) {
-                    const int32_t g = order[(size_t)i];
-                    int b = (int)((cen[(size_t)g * 3 + ax] - cmn[ax]) * sc);
-                    b = std::min(std::max(b, 0), B - 1);
-                    bc[b]++;
-                    for (int k = 0; k < 3; k++) { bmn[b][k] = std::min(bmn[b][k], bb[(size_t)g * 6 + k]); bmx[b][k] = std::max(bmx[b][k], bb[(size_t)g * 6 + 3 + k]); }
+            Bins bins;
+            bins = pass<Bins>(s0, s1, [&](Bins& o, int64_t b, int64_t e) {
+                o.clear();
+                for (int64_t i = b; i < e; i++) {
+                    const Ref& r = refs[(size_t)i];
+                    for (int ax = 0; ax < 3; ax++) {
+                        int bi = (int)((r.c[ax] - bd.cmn[ax]) * sc[ax]);
+                        bi = std::min(std::max(bi, 0), B - 1);
+                        o.cnt[ax][bi]++;
+                        for (int k = 0; k < 3; k++) {
+                            o.mn[ax][bi][k] = std::min(o.mn[ax][bi][k], r.lo[k]);
+                            o.mx[ax][bi][k] = std::max(o.mx[ax][bi][k], r.hi[k]);
+                        }
+                    }
                 }
+            });
+            for (int ax = 0; ax < 3; ax++) {
+                if (!(bd.cmx[ax] - bd.cmn[ax] > 0.0f)) continue;
                 float ra[B], rmn[3] = {INFINITY, INFINITY, INFINITY}, rmx[3] = {-INFINITY, -INFINITY, -INFINITY};
                 int rc[B], racc = 0;
                 for (int b = B - 1; b > 0; b--) {
-                    racc += bc[b];
-                    for (int k = 0; k < 3; k++) { rmn[k] = std::min(rmn[k], bmn[b][k]); rmx[k] = std::max(rmx[k], bmx[b][k]); }
+                    racc += bins.cnt[ax][b];
+                    for (int k = 0; k < 3; k++) { rmn[k] = std::min(rmn[k], bins.mn[ax][b][k]); rmx[k] = std::max(rmx[k], bins.mx[ax][b][k]); }
                     rc[b] = racc; ra[b] = area(rmn, rmx);
                 }
                 float lmn[3] = {INFINITY, INFINITY, INFINITY}, lmx[3] = {-INFINITY, -INFINITY, -INFINITY};
                 int lacc = 0;
                 for (int b = 0; b < B - 1; b++) {
-                    lacc += bc[b];
-                    for (int k = 0; k < 3; k++) { lmn[k] = std::min(lmn[k], bmn[b][k]); lmx[k] = std::max(lmx[k], bmx[b][k]); }
+                    lacc += bins.cnt[ax][b];
+                    for (int k = 0; k < 3; k++) { lmn[k] = std::min(lmn[k], bins.mn[ax][b][k]); lmx[k] = std::max(lmx[k], bins.mx[ax][b][k]); }
                     if (lacc == 0 || rc[b + 1] == 0) continue;
                     const float cost = area(lmn, lmx) * (float)lacc + ra[b + 1] * (float)rc[b + 1];
                     if (cost < best_cost) { best_cost = cost; best_axis = ax; best_bin = b; }
                 }
             }
         }
-        const float parent_area = area(pmn, pmx);
+        const float parent_area = area(bd.pmn, bd.pmx);
         /* SAH with traversal cost 1 and intersection cost 1 per primitive */
         if (!force_median && cnt <= 7 && (best_axis < 0 || 1.0f + best_cost / std::max(parent_area, 1e-30f) >= (float)cnt)) return id;
         int64_t mid;
         if (force_median || best_axis < 0) {
             int ax = 0;
-            for (int k = 1; k < 3; k++) if (cmx[k] - cmn[k] > cmx[ax] - cmn[ax]) ax = k;
+            for (int k = 1; k < 3; k++) if (bd.cmx[k] - bd.cmn[k] > bd.cmx[ax] - bd.cmn[ax]) ax = k;
             mid = (s0 + s1) / 2;
-            std::nth_element(order.begin() + s0, order.begin() + mid, order.begin() + s1, [&](int32_t x, int32_t y) {
-                const float cx = cen[(size_t)x * 3 + ax], cy = cen[(size_t)y * 3 + ax];
-                return cx < cy || (cx == cy && x < y);
+            std::nth_element(refs.begin() + s0, refs.begin() + mid, refs.begin() + s1, [ax](const Ref& x, const Ref& y) {
+                return x.c[ax] < y.c[ax] || (x.c[ax] == y.c[ax] && x.id < y.id);
             });
         } else {
-            const float ext = cmx[best_axis] - cmn[best_axis];
-            const float sc = B / ext;
             const int ax = best_axis;
-            auto it = std::partition(order.begin() + s0, order.begin() + s1, [&](int32_t g) {
-                int b = (int)((cen[(size_t)g * 3 + ax] - cmn[ax]) * sc);
+            const float cmn = bd.cmn[ax], s = sc[ax];
+            auto it = std::partition(refs.begin() + s0, refs.begin() + s1, [&](const Ref& r) {
+                int b = (int)((r.c[ax] - cmn) * s);
                 b = std::min(std::max(b, 0), B - 1);
                 return b <= best_bin;
             });
-            mid = it - order.begin();
+            mid = it - refs.begin();
         }
-        const int64_t l = rec(s0, mid, depth + 1);
-        const int64_t r = rec(mid, s1, depth + 1);
+        int64_t l, r;
+        if (par_left > 0 && cnt >= (int64_t)(2 * kParMin) && mid > s0 && mid < s1) {
+            /* leaf ranges are positions in refs: global, no shift needed */
+            std::vector<N> lo, hi;
+            std::thread t([&]() { rec(lo, s0, mid, depth + 1, par_left - 1); });
+            rec(hi, mid, s1, depth + 1, par_left - 1);
+            t.join();
+            for (std::vector<N>* part : {&lo, &hi}) {
+                const int64_t dn = (int64_t)nodes.size();
+                (part == &lo ? l : r) = dn;
+                for (N x : *part) {
+                    if (!x.leaf) { x.a += dn; x.b += dn; }
+                    nodes.push_back(x);
+                }
+            }
+        } else {
+            l = rec(nodes, s0, mid, depth + 1, 0);
+            r = rec(nodes, mid, s1, depth + 1, 0);
+        }
         nodes[(size_t)id].leaf = false;
         nodes[(size_t)id].a = l;
         nodes[(size_t)id].b = r;
@@ -448,6 +613,17 @@ void make_camera(const fm_camera_desc& c, fm_dcamera& dc) {
 }
 
 void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
+    using clk = std::chrono::steady_clock;
+    const clk::time_point t_start = clk::now();
+    clk::time_point t_mark = t_start;
+    auto lap = [&](double& acc) {
+        const clk::time_point t = clk::now();
+        acc += std::chrono::duration<double, std::milli>(t - t_mark).count();
+        t_mark = t;
+    };
+    hs.report = BuildReport{};
+    const int nthreads = build_threads();
+    hs.report.threads = nthreads;
     if (d.abi_version != FM_ABI_VERSION) throw BuildError(FM_ERR_VERSION, "fm_scene_desc.abi_version mismatch");
     if (d.num_images < 0 || d.num_materials < 0 || d.num_meshes < 0 || d.num_objects < 0 || d.num_lights < 0)
         throw BuildError(FM_ERR_INVALID, "negative count in descriptor");
@@ -469,6 +645,16 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
     std::vector<fm_drect> rects;
     std::vector<fm_dsphere> spheres;
     std::vector<fm_dlight> area_lights;
+    {
+        size_t nt = 0;
+        for (int oi = 0; oi < d.num_objects; oi++)
+            if (d.objects[oi].shape == FM_SHAPE_TRIANGLE_MESH && d.objects[oi].mesh >= 0 && d.objects[oi].mesh < d.num_meshes)
+                nt += (size_t)d.meshes[d.objects[oi].mesh].num_triangles;
+        tris.reserve(nt);
+        geos.reserve(nt + (size_t)d.num_objects);
+        bounds.reserve(nt + (size_t)d.num_objects);
+        tight.reserve(6 * (nt + (size_t)d.num_objects));
+    }
     for (int oi = 0; oi < d.num_objects; oi++) {
         const fm_object_desc& o = d.objects[oi];
         const int mat = (o.material >= 0 && o.material < d.num_materials) ? o.material : default_mat;
@@ -486,38 +672,56 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
             if (o.mesh < 0 || o.mesh >= d.num_meshes) throw BuildError(FM_ERR_INVALID, "object mesh index out of range");
             const fm_mesh_desc& m = d.meshes[o.mesh];
             const Xf xf = Xf::TRS(o.translation, o.rotation, o.scale);
-            for (int64_t f = 0; f < m.num_triangles; f++) {
-                const int32_t* ix = m.indices + 9 * f;
-                Tri t{};
-                for (int c = 0; c < 3; c++) {
-                    const int32_t vi = ix[3 * c], ni = ix[3 * c + 1], ti = ix[3 * c + 2];
-                    if (vi < 0 || vi >= m.num_positions || ni >= m.num_normals || ti >= m.num_texcoords)
-                        throw BuildError(FM_ERR_INVALID, "mesh index out of range");
-                    const float* p = m.positions + 3 * (int64_t)vi;
-                    t.v[c] = xf.point(f3(p[0], p[1], p[2]));
-                    if (ti >= 0) { t.uv[c][0] = m.texcoords[2 * (int64_t)ti]; t.uv[c][1] = 1.0f - m.texcoords[2 * (int64_t)ti + 1]; }
-                }
-                t.has_normal = ix[1] >= 0 && ix[4] >= 0 && ix[7] >= 0;
-                if (t.has_normal)
+            /* faces are baked on the build threads into slots reserved in
+             * object order; emitting faces then add their lights in order */
+            const size_t nf = (size_t)m.num_triangles, t0 = tris.size(), g0 = geos.size();
+            tris.resize(t0 + nf);
+            geos.resize(g0 + nf);
+            bounds.resize(g0 + nf);
+            tight.resize(6 * (g0 + nf));
+            std::atomic<bool> bad_index(false);
+            parallel_for(nf, nthreads, [&](size_t f0, size_t f1) {
+                for (size_t f = f0; f < f1; f++) {
+                    const int32_t* ix = m.indices + 9 * (int64_t)f;
+                    Tri t{};
+                    bool ok = true;
                     for (int c = 0; c < 3; c++) {
-                        const float* n = m.normals + 3 * (int64_t)ix[3 * c + 1];
-                        t.n[c] = xf.normal(f3(n[0], n[1], n[2]));
+                        const int32_t vi = ix[3 * c], ni = ix[3 * c + 1], ti = ix[3 * c + 2];
+                        if (vi < 0 || vi >= m.num_positions || ni >= m.num_normals || ti >= m.num_texcoords) { ok = false; break; }
+                        const float* p = m.positions + 3 * (int64_t)vi;
+                        t.v[c] = xf.point(f3(p[0], p[1], p[2]));
+                        if (ti >= 0) { t.uv[c][0] = m.texcoords[2 * (int64_t)ti]; t.uv[c][1] = 1.0f - m.texcoords[2 * (int64_t)ti + 1]; }
                     }
-                t.area = len(cross(sub(t.v[1], t.v[0]), sub(t.v[2], t.v[0]))) * 0.5f;
-                Box b; b.grow(t.v[0]); b.grow(t.v[1]); b.grow(t.v[2]);
-                push_tight(tight, t.v, 3);
-                int32_t light = -1;
-                if (emissive) {
+                    if (!ok) { bad_index = true; continue; }
+                    t.has_normal = ix[1] >= 0 && ix[4] >= 0 && ix[7] >= 0;
+                    if (t.has_normal)
+                        for (int c = 0; c < 3; c++) {
+                            const float* n = m.normals + 3 * (int64_t)ix[3 * c + 1];
+                            t.n[c] = xf.normal(f3(n[0], n[1], n[2]));
+                        }
+                    t.area = len(cross(sub(t.v[1], t.v[0]), sub(t.v[2], t.v[0]))) * 0.5f;
+                    Box& b = bounds[g0 + f];
+                    b = Box{};
+                    b.grow(t.v[0]); b.grow(t.v[1]); b.grow(t.v[2]);
+                    float* tb = &tight[6 * (g0 + f)];
+                    for (int k = 0; k < 3; k++) {
+                        tb[k] = std::min(std::min(comp(t.v[0], k), comp(t.v[1], k)), comp(t.v[2], k));
+                        tb[3 + k] = std::max(std::max(comp(t.v[0], k), comp(t.v[1], k)), comp(t.v[2], k));
+                    }
+                    tris[t0 + f] = t;
+                    geos[g0 + f] = Geo{FM_PRIM_TRI, (int32_t)(t0 + f), mat, -1};
+                }
+            });
+            if (bad_index) throw BuildError(FM_ERR_INVALID, "mesh index out of range");
+            if (emissive)
+                for (size_t f = 0; f < nf; f++) {
+                    const Tri& t = tris[t0 + f];
                     fm_dlight L{};
                     L.kind = FM_DLIGHT_TRI; L.has_normal = t.has_normal;
                     for (int c = 0; c < 3; c++) { put4(L.p[c], t.v[c], 0.0f); put4(L.n[c], t.n[c], 0.0f); }
                     L.p[0][3] = (float)(1.0 / (double)t.area);  /* triangle.cpp:293 */
-                    light = add_light(L);
+                    geos[g0 + f].light = add_light(L);
                 }
-                geos.push_back(Geo{FM_PRIM_TRI, (int32_t)tris.size(), mat, light});
-                tris.push_back(t);
-                bounds.push_back(b);
-            }
         } else if (o.shape == FM_SHAPE_RECTANGLE) {
             fm_drect r{};
             const F3 pa = f3(o.pa[0], o.pa[1], o.pa[2]), pb = f3(o.pb[0], o.pb[1], o.pb[2]);
@@ -566,15 +770,18 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
     const size_t G = geos.size();
     if (G > (size_t)1 << 28) throw BuildError(FM_ERR_UNSUPPORTED, "more than 2^28 geometries");
     hs.num_geoms = (int64_t)G;
+    lap(hs.report.geometry_ms);
 
     /* ---- reference BVH ---- */
     Builder B;
+    B.par = par_levels(nthreads);
     B.infos.resize(G);
     for (size_t i = 0; i < G; i++) { B.infos[i].geom = (int32_t)i; B.infos[i].box = bounds[i]; }
     if (G > 0) B.build(0, G, 1);
     hs.ref_nodes.swap(B.nodes);
     hs.order.swap(B.order);
     hs.bvh_depth = B.depth;
+    lap(hs.report.ref_bvh_ms);
     if (only_bvh) return;
     if (hs.bvh_depth > FM_STACK_DEPTH) throw BuildError(FM_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
 
@@ -600,6 +807,8 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
         for (size_t i = 0; i < hs.ref_nodes.size(); i++) tn[i] = TNode{hs.ref_nodes[i].leaf, hs.ref_nodes[i].a, hs.ref_nodes[i].b};
     } else {
         SahBuilder sb(tight, (int64_t)G);
+        sb.par = par_levels(nthreads);
+        sb.threads = nthreads;
         if (G > 0) sb.build();
         slot_geom.swap(sb.order);
         tn.resize(sb.nodes.size());
@@ -721,12 +930,15 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
         for (int k = 0; k < 6; k++) hs.root_box[k] = node_boxes[(size_t)k];
     }
 
+    lap(hs.report.trav_tree_ms);
+
     /* ---- slots (w of the third float4 = tie key: reference slot, or the
      * geometry index for the brute-force "naive" accelerator) ---- */
     hs.prims.assign(G * 12, 0.0f);
     hs.meta.assign(G * 4, 0);
     hs.tri_sh.assign(G * 16, 0.0f);
-    for (size_t p = 0; p < G; p++) {
+    parallel_for(G, nthreads, [&](size_t p0, size_t p1) {
+    for (size_t p = p0; p < p1; p++) {
         const int32_t g = slot_geom[p];
         const Geo& ge = geos[(size_t)g];
         float* pr = &hs.prims[p * 12];
@@ -757,10 +969,13 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
         int32_t* me = &hs.meta[p * 4];
         me[0] = g; me[1] = ge.material; me[2] = ge.light; me[3] = flags;
     }
+    });
     hs.rects.swap(rects);
     hs.spheres.swap(spheres);
 
-    /* ---- images: float4 texels for .hdr, RGBA8 for 8-bit sRGB ---- */
+    /* ---- images: float4 texels for .hdr, RGBA8 for 8-bit sRGB ----
+     * offsets first (256-byte aligned), one allocation, then the images are
+     * converted on the build threads */
     size_t off = 0;
     for (int i = 0; i < d.num_images; i++) {
         const fm_image_desc& im = d.images[i];
@@ -770,21 +985,33 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
         di.format = im.format == FM_IMAGE_U8_SRGB ? 1 : 0;
         off = (off + 255) & ~(size_t)255;
         di.offset = (int64_t)off;
-        const size_t n = (size_t)im.width * im.height;
-        const size_t bytes = n * (di.format ? 4 : 16);
-        hs.texels.resize(off + bytes);
-        unsigned char* dst = hs.texels.data() + off;
-        if (di.format) {
-            const unsigned char* s = (const unsigned char*)im.data;
-            for (size_t k = 0; k < n; k++) { dst[4 * k] = s[3 * k]; dst[4 * k + 1] = s[3 * k + 1]; dst[4 * k + 2] = s[3 * k + 2]; dst[4 * k + 3] = 255; }
-        } else {
-            const float* s = (const float*)im.data;
-            float* o = (float*)dst;
-            for (size_t k = 0; k < n; k++) { o[4 * k] = s[3 * k]; o[4 * k + 1] = s[3 * k + 1]; o[4 * k + 2] = s[3 * k + 2]; o[4 * k + 3] = 0.0f; }
-        }
-        off += bytes;
+        off += (size_t)im.width * im.height * (di.format ? 4 : 16);
         hs.images.push_back(di);
     }
+    hs.texels.resize(off);
+    std::vector<std::pair<int, size_t>> rows;   /* (image, row) work items */
+    for (int i = 0; i < d.num_images; i++)
+        for (int r = 0; r < d.images[i].height; r++) rows.emplace_back(i, (size_t)r);
+    parallel_for(rows.size(), nthreads, [&](size_t r0, size_t r1) {
+        for (size_t q = r0; q < r1; q++) {
+            const fm_image_desc& im = d.images[rows[q].first];
+            const fm_dimage& di = hs.images[(size_t)rows[q].first];
+            const size_t w = (size_t)im.width, k0 = rows[q].second * w;
+            unsigned char* dst = hs.texels.data() + di.offset;
+            if (di.format) {
+                const unsigned char* s = (const unsigned char*)im.data;
+                for (size_t k = k0; k < k0 + w; k++) {
+                    dst[4 * k] = s[3 * k]; dst[4 * k + 1] = s[3 * k + 1]; dst[4 * k + 2] = s[3 * k + 2]; dst[4 * k + 3] = 255;
+                }
+            } else {
+                const float* s = (const float*)im.data;
+                float* o = (float*)dst;
+                for (size_t k = k0; k < k0 + w; k++) {
+                    o[4 * k] = s[3 * k]; o[4 * k + 1] = s[3 * k + 1]; o[4 * k + 2] = s[3 * k + 2]; o[4 * k + 3] = 0.0f;
+                }
+            }
+        }
+    });
     if (hs.texels.empty()) hs.texels.resize(256);
     hs.srgb_lut.resize(256);
     for (int k = 0; k < 256; k++) hs.srgb_lut[(size_t)k] = srgb_to_linear((float)k / 255.0f);
@@ -857,6 +1084,8 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
 
     /* ---- camera (pin_hole_camera.cpp:5-16) ---- */
     make_camera(d.camera, hs.camera);
+    lap(hs.report.tables_ms);
+    hs.report.total_ms = std::chrono::duration<double, std::milli>(clk::now() - t_start).count();
 }
 
 }  // namespace fmh

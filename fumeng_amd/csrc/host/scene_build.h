@@ -24,6 +24,16 @@ struct RefNode {
     int64_t a, b;
 };
 
+/* Wall time of the host build's phases. */
+struct BuildReport {
+    double geometry_ms = 0;    /* shapes, meshes, area lights in object order */
+    double ref_bvh_ms = 0;     /* reference builder (tie order) */
+    double trav_tree_ms = 0;   /* traversal tree, boxes, flattening */
+    double tables_ms = 0;      /* slots, images, materials, lights, env tables */
+    double total_ms = 0;
+    int threads = 1;           /* host threads the build used */
+};
+
 /* Host copy of everything that goes to HBM, plus facts for fm_scene_info. */
 struct HostScene {
     std::vector<RefNode> ref_nodes;
@@ -56,6 +66,7 @@ struct HostScene {
     fm_renderer_desc renderer{};
     fm_camera_desc camera_desc{};
     std::vector<fm_post_op> post_ops;   /* colour post-process list */
+    BuildReport report;
 };
 
 /* Env sampling tables of env light e from its texels (RGB per texel, linear);

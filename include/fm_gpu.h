@@ -223,7 +223,11 @@ typedef struct fm_scene_info {
     int32_t trav_stack;       /* stack entries per traversal (LDS) */
     int64_t num_trav_nodes;   /* its inner nodes */
     int32_t trav_node_bytes;  /* bytes per node: 64 (2 wide) or 128 (4 wide) */
-    int32_t pad_;
+    int32_t build_threads;    /* host threads the build used */
+    /* build report: wall time of the host build's phases and the upload */
+    double build_ms_geometry, build_ms_ref_bvh, build_ms_trav_tree, build_ms_tables, build_ms_total, upload_ms;
+    int64_t host_bytes;       /* host copy kept for the scene's lifetime */
+    uint64_t trav_hash;       /* FNV-1a of the traversal nodes and slot records */
 } fm_scene_info;
 int fm_scene_get_info(const fm_scene* scene, fm_scene_info* out);
 

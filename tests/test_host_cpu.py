@@ -235,3 +235,22 @@ def test_loader_reads_the_post_process_list(tmp_path, capfd):
     assert "sharpen" in capfd.readouterr().err
     info = fumeng_amd.host_scene_info(s)       # the builder accepts the list
     assert info.num_geometries > 0
+
+
+def test_threaded_builds_equal_sequential_builds(monkeypatch):
+    """The builders split the top levels over threads: the reference
+    topology (node array and leaf order) and the traversal tree with its
+    slot records are the same as a one-thread build, on a scene large enough
+    for the threaded levels (81,920 triangles)."""
+    s = mesh_scene(freq=64, with_box=False, with_spheres=False)
+    par = [fumeng_amd.host_scene_info(s, w) for w in (2, 4)]
+    ref_par = fumeng_amd.host_bvh(s)
+    monkeypatch.setenv("FUMENG_BUILD_THREADS", "1")
+    seq = [fumeng_amd.host_scene_info(s, w) for w in (2, 4)]
+    ref_seq = fumeng_amd.host_bvh(s)
+    assert seq[0].build_threads == 1 and par[0].build_threads >= 1
+    for a, b in zip(par, seq):
+        assert a.trav_hash == b.trav_hash and a.num_trav_nodes == b.num_trav_nodes and a.trav_stack == b.trav_stack
+    for a, b in zip(ref_par, ref_seq):
+        assert np.array_equal(a, b)
+    assert np.array_equal(ref_seq[4], oracle.OracleScene(s).bvh()[4])   # leaf order = the oracle's
