@@ -129,6 +129,19 @@ class Scene:
         _check(lib().fm_scene_get_info(self.h, C.byref(i)))
         return i
 
+    def autotune(self, trial_spp=16, mode=A.FM_MODE_WAVEFRONT, width=0, height=0, depth=0, seed=1):
+        """fm_autotune: returns {"best": {...}, "baseline": {...}, "trials": [...]}
+        with block_size / grid_blocks / chunk / kernel_ms per configuration;
+        pass best's block_size, grid_blocks and chunk to render()."""
+        p = A.fm_render_params(width=width, height=height, depth=depth, rr_depth=-1, mode=mode, seed=seed)
+        res = A.fm_tune_result()
+        arr = (A.fm_tune_trial * 64)()
+        _check(lib().fm_autotune(self.ctx.h, self.h, C.byref(p), int(trial_spp), C.byref(res), arr, 64), self.ctx.h)
+        conv = lambda t: {"block_size": t.block_size, "grid_blocks": t.grid_blocks, "chunk": t.chunk,
+                          "kernel_ms": t.kernel_ms}
+        return {"best": conv(res.best), "baseline": conv(res.baseline), "trial_spp": res.trial_spp,
+                "trials": [conv(arr[i]) for i in range(min(res.num_trials, 64))]}
+
     def post_ops(self):
         """The scene's post-process list as dicts ({"type": ..., "lum": ...})."""
         arr = (A.fm_post_op * A.FM_MAX_POST_OPS)()

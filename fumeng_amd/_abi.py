@@ -89,6 +89,16 @@ class fm_post_op(C.Structure):
     _fields_ = [("type", C.c_int32), ("lum", C.c_float)]
 
 
+class fm_tune_trial(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("block_size", C.c_int32), ("grid_blocks", C.c_int32), ("chunk", C.c_int32),
+                ("kernel_ms", C.c_double)]
+
+
+class fm_tune_result(C.Structure):
+    _fields_ = [("best", fm_tune_trial), ("baseline", fm_tune_trial), ("num_trials", C.c_int32),
+                ("trial_spp", C.c_int32)]
+
+
 class fm_scene_desc(C.Structure):
     _fields_ = [("abi_version", C.c_uint32), ("accelerator", C.c_int32),
                 ("camera", fm_camera_desc), ("renderer", fm_renderer_desc),
@@ -188,6 +198,8 @@ EXPORTS = {
                                   C.POINTER(C.c_double), C.POINTER(fm_batch_stats)]),
     "fm_set_counting": (C.c_int, [C.c_void_p, C.c_int]),
     "fm_set_bvh_width": (C.c_int, [C.c_void_p, C.c_int32]),
+    "fm_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(fm_render_params), C.c_int32,
+                              C.POINTER(fm_tune_result), C.c_void_p, C.c_int32]),
     "fm_scene_post_ops": (C.c_int, [C.c_void_p, C.POINTER(fm_post_op), C.c_int32, C.POINTER(C.c_int32)]),
     "fm_post_process": (C.c_int, [C.c_void_p, C.POINTER(fm_post_op), C.c_int32, C.c_void_p, C.c_int64, C.c_void_p,
                                   C.c_void_p]),

@@ -546,6 +546,59 @@ int fm_set_counting(fm_ctx* ctx, int enabled) {
     return FM_OK;
 }
 
+int fm_autotune(fm_ctx* ctx, fm_scene* s, const fm_render_params* base, int32_t trial_spp, fm_tune_result* out,
+                fm_tune_trial* trials, int32_t max_trials) {
+    if (!ctx || !s || !base || !out || (max_trials > 0 && !trials)) return fail(ctx, FM_ERR_INVALID, "bad argument");
+    memset(out, 0, sizeof *out);
+    fm_render_params p = *base;
+    p.spp = trial_spp > 0 ? trial_spp : 16;
+    p.accumulate = 0;
+    p.flags = 0;
+    p.sample_offset = 0;
+    const bool wave = p.mode == FM_MODE_WAVEFRONT;
+    /* the defaults first, then every other combination */
+    std::vector<fm_tune_trial> cand;
+    {
+        fm_tune_trial t{};
+        t.mode = p.mode; t.chunk = wave ? 0 : 16;
+        cand.push_back(t);
+    }
+    const std::vector<int> chunks = wave ? std::vector<int>{0, std::max(1, p.spp / 4)} : std::vector<int>{4, 16, 64};
+    for (int block : {64, 128, 256})
+        for (int fixed_grid : {0, 1})
+            for (int chunk : chunks) {
+                fm_tune_trial t{};
+                t.mode = p.mode; t.block_size = block; t.chunk = chunk;
+                if (fixed_grid)
+                    t.grid_blocks = wave ? ctx->num_cus * (2048 / block)
+                                         : ctx->num_cus * std::max(1, fm::mega_blocks_per_cu(s->d, block)) * 8;
+                cand.push_back(t);
+            }
+    fm_render_stats st;
+    {   /* a warm-up with the defaults sizes the buffers */
+        fm_render_params q = p;
+        q.block_size = 0; q.grid_blocks = 0; q.chunk = cand[0].chunk;
+        const int rc = fm_render(ctx, s, &q, nullptr, &st);
+        if (rc != FM_OK) return rc;
+    }
+    size_t best = 0;
+    for (size_t i = 0; i < cand.size(); i++) {
+        fm_tune_trial& t = cand[i];
+        fm_render_params q = p;
+        q.block_size = t.block_size; q.grid_blocks = t.grid_blocks; q.chunk = t.chunk;
+        const int rc = fm_render(ctx, s, &q, nullptr, &st);
+        if (rc != FM_OK) return rc;
+        t.kernel_ms = st.kernel_ms;
+        if (t.kernel_ms < cand[best].kernel_ms) best = i;
+    }
+    out->baseline = cand[0];
+    out->best = cand[best];
+    out->num_trials = (int32_t)cand.size();
+    out->trial_spp = p.spp;
+    for (int32_t i = 0; i < max_trials && i < (int32_t)cand.size(); i++) trials[i] = cand[(size_t)i];
+    return FM_OK;
+}
+
 int fm_set_bvh_width(fm_ctx* ctx, int32_t width) {
     if (!ctx) return fail(nullptr, FM_ERR_INVALID, "NULL ctx");
     if (width != 2 && width != 4) return fail(ctx, FM_ERR_INVALID, "BVH width must be 2 or 4");

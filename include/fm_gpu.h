@@ -359,6 +359,31 @@ int fm_params_set(fm_ctx* ctx, fm_scene* scene, const float* material_params, co
  * differences of its FP32 value at the path's directions. */
 int fm_render_adjoint(fm_ctx* ctx, fm_scene* scene, const fm_render_params* params, const float* dl_dimg,
                       double* g_material, double* g_env, fm_render_stats* stats);
+/* ---- autotuner ------------------------------------------------------------ */
+
+/* One timed configuration of fm_autotune. */
+typedef struct fm_tune_trial {
+    int32_t mode, block_size, grid_blocks, chunk;
+    double kernel_ms;
+} fm_tune_trial;
+typedef struct fm_tune_result {
+    fm_tune_trial best;        /* copy into fm_render_params */
+    fm_tune_trial baseline;    /* the defaults (block 0, grid 0, chunk 0) */
+    int32_t num_trials;
+    int32_t trial_spp;
+} fm_tune_result;
+/* Times short renders of `scene` (the frame of `base`, trial_spp samples,
+ * <= 0: 16) for each launch configuration of the chosen form: block sizes
+ * 64 / 128 / 256; the grid each kernel's occupancy gives, or a fixed one
+ * (wavefront: 2048 threads per CU; megakernel: twice the occupancy grid);
+ * for the wavefront form passes of the whole trial or of a quarter of it
+ * (megakernel: chunks of 4, 16 or 64 samples).  Returns the fastest by device time; `trials` (may be
+ * NULL) receives up to max_trials of them.  Overwrites the device film.  No
+ * counterpart in the reference (its thread count is a JSON setting,
+ * renderer.h:70-77). */
+int fm_autotune(fm_ctx* ctx, fm_scene* scene, const fm_render_params* base, int32_t trial_spp,
+                fm_tune_result* out, fm_tune_trial* trials, int32_t max_trials);
+
 /* Enables the node / primitive counters in later renders (0 = off). */
 int fm_set_counting(fm_ctx* ctx, int enabled);
 /* Width (2 or 4, default 2) of the traversal tree of the scenes created on

@@ -91,3 +91,20 @@ def test_batch_rejects_bad_jobs(gpu_ctx):
     assert films[0][0].shape == (8, 8, 3)
     films, done, st = g.render_batch([])
     assert films == [] and st.paths == 0
+
+
+def test_autotune_picks_a_timed_configuration_that_renders_the_same_image(gpu_ctx):
+    """fm_autotune times every launch configuration once; its best is no
+    slower than the defaults and renders the same film (launch shape and
+    pass size do not change paths; passes only regroup the FP64 sums)."""
+    s = scenes.box_scene(width=64, height=48, spp=8)
+    g = gpu_ctx.scene(s)
+    for mode in (A.FM_MODE_WAVEFRONT, A.FM_MODE_MEGAKERNEL):
+        r = g.autotune(trial_spp=8, mode=mode)
+        assert r["best"]["kernel_ms"] <= r["baseline"]["kernel_ms"] and len(r["trials"]) >= 7
+        assert all(t["kernel_ms"] > 0 for t in r["trials"])
+        b = r["best"]
+        ref = g.render(seed=3, mode=mode)
+        got = g.render(seed=3, mode=mode, block_size=b["block_size"], grid_blocks=b["grid_blocks"], chunk=b["chunk"])
+        for a, b_ in zip(got[:3], ref[:3]):
+            assert np.allclose(a, b_, rtol=1e-6, atol=1e-7, equal_nan=True)

@@ -42,7 +42,8 @@ def test_struct_layouts_match_c(tmp_path):
     structs = ["fm_texture_desc", "fm_image_desc", "fm_material_desc", "fm_mesh_desc", "fm_object_desc",
                "fm_light_desc", "fm_camera_desc", "fm_renderer_desc", "fm_scene_desc", "fm_scene_info",
                "fm_render_params", "fm_film", "fm_render_stats", "fm_ray", "fm_hit", "fm_bsdf_query",
-               "fm_bsdf_result", "fm_light_query", "fm_light_result", "fm_job", "fm_batch_stats"]
+               "fm_bsdf_result", "fm_light_query", "fm_light_result", "fm_job", "fm_batch_stats", "fm_post_op",
+               "fm_tune_trial", "fm_tune_result"]
     src = ['#include "%s"' % HEADER, "#include <stdio.h>", "#include <stddef.h>", "int main(void){"]
     for s in structs:
         src.append('printf("%s %%zu\\n", sizeof(%s));' % (s, s))
