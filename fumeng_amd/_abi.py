@@ -2,6 +2,7 @@
 import ctypes as C
 
 FM_ABI_VERSION = 2
+FM_ACCEL_EMBREE = 3
 
 FM_OK = 0
 FM_ERR_INVALID = -1

@@ -130,7 +130,8 @@ struct fm_ctx {
 struct fm_scene {
     fm_ctx* ctx = nullptr;
     fmh::HostScene hs;        /* host copy (kept for info and tests) */
-    DevBuf nodes, nodes4, prims, meta, tri_sh, rects, spheres, materials, lights, envs, images, texels, env_tables, lut;
+    DevBuf nodes, nodes4, prims, meta, tri_sh, rects, spheres, materials, lights, mesh_tris, envs, images, texels, env_tables,
+        lut;
     fm_dscene d{};
     int64_t device_bytes = 0;
     double upload_ms = 0.0;
@@ -186,6 +187,7 @@ void upload_scene(fm_scene* s) {
     s->spheres.upload(hs.spheres, st);
     s->materials.upload(hs.materials, st);
     s->lights.upload(hs.lights, st);
+    s->mesh_tris.upload(hs.mesh_tris, st);
     s->envs.upload(hs.envs, st);
     s->images.upload(hs.images, st);
     s->texels.upload(hs.texels, st);
@@ -205,6 +207,7 @@ void upload_scene(fm_scene* s) {
     d.spheres = s->spheres.as<fm_dsphere>();
     d.materials = s->materials.as<fm_dmaterial>();
     d.lights = s->lights.as<fm_dlight>();
+    d.mesh_tris = s->mesh_tris.as<fm_dlight>();
     d.envs = s->envs.as<fm_denv>();
     d.images = s->images.as<fm_dimage>();
     d.texels = s->texels.as<unsigned char>();
@@ -220,7 +223,7 @@ void upload_scene(fm_scene* s) {
     memcpy(d.root_box, hs.root_box, sizeof d.root_box);
     d.camera = hs.camera;
     s->device_bytes = 0;
-    for (DevBuf* b : {&s->nodes, &s->nodes4, &s->prims, &s->meta, &s->tri_sh, &s->rects, &s->spheres, &s->materials, &s->lights,
+    for (DevBuf* b : {&s->nodes, &s->nodes4, &s->prims, &s->meta, &s->tri_sh, &s->rects, &s->spheres, &s->materials, &s->lights, &s->mesh_tris,
                       &s->envs, &s->images, &s->texels, &s->env_tables, &s->lut})
         s->device_bytes += (int64_t)b->n;
     s->upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -358,7 +361,9 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
         bool env = false, area = false, other = false;
         for (const fm_dlight& l : s->hs.lights) {
             if (l.kind == FM_DLIGHT_ENV) env = true;
-            else if (l.kind == FM_DLIGHT_TRI || l.kind == FM_DLIGHT_RECT || l.kind == FM_DLIGHT_SPHERE) area = true;
+            else if (l.kind == FM_DLIGHT_TRI || l.kind == FM_DLIGHT_RECT || l.kind == FM_DLIGHT_SPHERE ||
+                     l.kind == FM_DLIGHT_MESH)
+                area = true;
             else other = true;
         }
         L.lights = (env && !area && !other) ? fm::LK_ENV : ((!env && !other) ? fm::LK_AREA : fm::LK_ANY);
@@ -529,7 +534,7 @@ static void fill_info(const fmh::HostScene& hs, int64_t device_bytes, double upl
     o->upload_ms = upload_ms;
     o->host_bytes = vbytes(hs.ref_nodes) + vbytes(hs.order) + vbytes(hs.nodes) + vbytes(hs.nodes4) + vbytes(hs.prims) +
                     vbytes(hs.meta) + vbytes(hs.tri_sh) + vbytes(hs.rects) + vbytes(hs.spheres) + vbytes(hs.materials) +
-                    vbytes(hs.lights) + vbytes(hs.envs) + vbytes(hs.images) + vbytes(hs.texels) + vbytes(hs.env_tables) +
+                    vbytes(hs.lights) + vbytes(hs.mesh_tris) + vbytes(hs.envs) + vbytes(hs.images) + vbytes(hs.texels) + vbytes(hs.env_tables) +
                     vbytes(hs.srgb_lut);
     o->trav_hash = fnv1a(hs.prims, fnv1a(hs.nodes4, fnv1a(hs.nodes)));
 }

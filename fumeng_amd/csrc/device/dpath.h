@@ -104,10 +104,17 @@ FMD void sphere_point(const fm_dsphere& s, v3 pl, v3& pos, v3& ng) {
     pos = xf_point(s.o2w, pl);
     ng = xf_normal_inv(s.w2o, normalize(pl));
 }
-FMD LightSample light_sample(const fm_dscene& S, const fm_dlight& L, v3 pos, v3 s) {
+FMD LightSample light_sample(const fm_dscene& S, const fm_dlight& L_in, v3 pos, v3 s) {
     LightSample o;
     o.pdf = 0.0f; o.dist = 0.0f; o.radiance = splat(0.0f); o.wi = splat(0.0f); o.ref_pos = splat(0.0f); o.light_pos = splat(0.0f);
-    if (L.kind == FM_DLIGHT_ENV) return env_sample(S, S.envs[L.aux], pos, s);
+    if (L_in.kind == FM_DLIGHT_ENV) return env_sample(S, S.envs[L_in.aux], pos, s);
+    /* an "embree" mesh light picks a face by its area CDF with s.z
+     * (TriangleEmbree::Sample, triangle_embree.cpp:253-260), then samples
+     * that face with (s.x, s.y) under the mesh's area pdf */
+    const bool mesh = L_in.kind == FM_DLIGHT_MESH;
+    const fm_dlight& L = mesh ? S.mesh_tris[L_in.aux + cdf_search(S.env_tables + __float_as_int(L_in.p[1][0]),
+                                                                    __float_as_int(L_in.p[1][1]), s.z)]
+                              : L_in;
     if (L.kind == FM_DLIGHT_DIR) {
         const v3 d = ld3(L.dir);
         o.ref_pos = pos; o.light_pos = pos - d * 1.0e6f; o.wi = -d; o.radiance = ld3(L.radiance);
@@ -122,7 +129,7 @@ FMD LightSample light_sample(const fm_dscene& S, const fm_dlight& L, v3 pos, v3 
         const v3 v0 = ld3(L.p[0]), v1 = ld3(L.p[1]), v2 = ld3(L.p[2]);
         lp = b0 * v0 + b1 * v1 + b2 * v2;
         ln = L.has_normal ? b0 * ld3(L.n[0]) + b1 * ld3(L.n[1]) + b2 * ld3(L.n[2]) : normalize(cross(v0 - v2, v1 - v2));
-        pa = L.p[0][3];
+        pa = mesh ? L_in.p[0][3] : L.p[0][3];
     } else if (L.kind == FM_DLIGHT_RECT) {
         const fm_drect& R = S.rects[L.aux];
         const float su = sqrtf(s.x);
@@ -165,7 +172,7 @@ FMD LightSample light_sample(const fm_dscene& S, const fm_dlight& L, v3 pos, v3 
     const float dist = length(d);
     if (dist < FM_EPS) return o;
     const v3 wi = normalize(d);
-    o.ref_pos = pos; o.light_pos = lp; o.wi = wi; o.radiance = ld3(L.radiance);
+    o.ref_pos = pos; o.light_pos = lp; o.wi = wi; o.radiance = ld3(L_in.radiance);
     o.pdf = pa * dist * dist / absdot(ln, -wi);
     o.dist = dist;
     return o;
@@ -174,7 +181,7 @@ FMD LightSample light_sample(const fm_dscene& S, const fm_dlight& L, v3 pos, v3 
 FMD float area_pdf(const fm_dscene& S, const fm_dlight& L, v3 shd, v3 lp, v3 ln, v3 l2s) {
     if (dot(ln, l2s) < 0) return 0.0f;
     float pa;
-    if (L.kind == FM_DLIGHT_TRI) pa = L.p[0][3];
+    if (L.kind == FM_DLIGHT_TRI || L.kind == FM_DLIGHT_MESH) pa = L.p[0][3];   /* mesh: Pdf = 1 / Area() */
     else if (L.kind == FM_DLIGHT_RECT) pa = S.rects[L.aux].tangent[3];
     else {
         const fm_dsphere& sp = S.spheres[L.aux];

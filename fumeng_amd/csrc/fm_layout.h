@@ -85,6 +85,10 @@ typedef struct fm_dmaterial {
 #define FM_DLIGHT_SPHERE 2
 #define FM_DLIGHT_ENV 3
 #define FM_DLIGHT_DIR 4
+#define FM_DLIGHT_MESH 5    /* emissive mesh, "embree": aux = first triangle in mesh_tris,
+                               p[0][3] = 1 / area sum, int bits of p[1][0] = float
+                               offset of its area CDF in env_tables, of p[1][1] =
+                               triangle count */
 
 typedef struct fm_dlight {
     int32_t kind;       /* FM_DLIGHT_* */
@@ -135,6 +139,7 @@ typedef struct fm_dscene {
     const fm_dsphere* spheres;
     const fm_dmaterial* materials;
     const fm_dlight* lights;
+    const fm_dlight* mesh_tris;  /* triangles of mesh lights (FM_DLIGHT_TRI records) */
     const fm_denv* envs;
     const fm_dimage* images;
     const unsigned char* texels;

@@ -510,7 +510,7 @@ struct Loader {
         if (!s) throw LoadError(FM_ERR_PARSE, "Scene is not specified.");
         std::string acc = "bvh";
         if (const Json* a = s->find("accelerator")) load_str(*a, "type", acc);
-        d.accelerator = acc == "naive" ? FM_ACCEL_NAIVE : FM_ACCEL_BVH;
+        d.accelerator = acc == "naive" ? FM_ACCEL_NAIVE : (acc == "embree" ? FM_ACCEL_EMBREE : FM_ACCEL_BVH);
         if (const Json* ms = s->find("materials"))
             if (ms->type == Json::ARRAY) for (const Json& m : ms->arr) material(m);
         mat_ids.emplace("default_material", -1);

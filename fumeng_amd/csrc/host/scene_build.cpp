@@ -662,9 +662,9 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
         const F3 em = f3(o.emittance[0], o.emittance[1], o.emittance[2]);
         /* IsBlack (utils.h:86-97): all zero or any negative channel */
         const bool emissive = !((em.x == 0 && em.y == 0 && em.z == 0) || em.x < 0 || em.y < 0 || em.z < 0);
-        auto add_light = [&](fm_dlight L) -> int32_t {
+        auto add_light = [&](fm_dlight L, size_t geom) -> int32_t {
             L.radiance[0] = em.x; L.radiance[1] = em.y; L.radiance[2] = em.z;
-            L.geom = (int32_t)geos.size();
+            L.geom = (int32_t)geom;
             area_lights.push_back(L);
             return (int32_t)area_lights.size() - 1;
         };
@@ -713,15 +713,53 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
                 }
             });
             if (bad_index) throw BuildError(FM_ERR_INVALID, "mesh index out of range");
-            if (emissive)
+            if (emissive && d.accelerator == FM_ACCEL_EMBREE && nf > 0) {
+                /* one light for the mesh (triangle_embree.cpp:121-135, 253-270):
+                 * a face is chosen by Distribution1D over the face areas
+                 * (sample.z), a point on it from (sample.x, sample.y), and the
+                 * area pdf is 1 / (float sum of the face areas) */
+                fm_dlight M{};
+                M.kind = FM_DLIGHT_MESH;
+                M.aux = (int32_t)hs.mesh_tris.size();
+                M.has_normal = 0;
+                float area_sum = 0.0f;
+                std::vector<float> areas(nf);
+                for (size_t f = 0; f < nf; f++) {
+                    const Tri& t = tris[t0 + f];
+                    fm_dlight L{};
+                    L.kind = FM_DLIGHT_TRI; L.has_normal = t.has_normal;
+                    for (int c = 0; c < 3; c++) { put4(L.p[c], t.v[c], 0.0f); put4(L.n[c], t.n[c], 0.0f); }
+                    L.p[0][3] = (float)(1.0 / (double)t.area);
+                    L.geom = (int32_t)(g0 + f);
+                    hs.mesh_tris.push_back(L);
+                    areas[f] = t.area;
+                    area_sum += t.area;
+                }
+                M.p[0][3] = 1.0f / area_sum;
+                /* Distribution1D (distribution_1d.cpp:5-33) in float, as the
+                 * FP32 reference builds it */
+                const size_t off = hs.env_tables.size();
+                hs.env_tables.resize(off + nf + 1);
+                float* cdf = &hs.env_tables[off];
+                cdf[0] = 0.0f;
+                for (size_t i = 1; i <= nf; i++) cdf[i] = cdf[i - 1] + areas[i - 1] / (float)nf;
+                const float sum = cdf[nf];
+                if (!(sum < 1e-20f)) for (size_t i = 0; i <= nf; i++) cdf[i] /= sum;
+                const int32_t offi = (int32_t)off, count = (int32_t)nf;
+                memcpy(&M.p[1][0], &offi, 4);
+                memcpy(&M.p[1][1], &count, 4);
+                const int32_t li = add_light(M, g0);
+                for (size_t f = 0; f < nf; f++) geos[g0 + f].light = li;
+            } else if (emissive) {
                 for (size_t f = 0; f < nf; f++) {
                     const Tri& t = tris[t0 + f];
                     fm_dlight L{};
                     L.kind = FM_DLIGHT_TRI; L.has_normal = t.has_normal;
                     for (int c = 0; c < 3; c++) { put4(L.p[c], t.v[c], 0.0f); put4(L.n[c], t.n[c], 0.0f); }
                     L.p[0][3] = (float)(1.0 / (double)t.area);  /* triangle.cpp:293 */
-                    geos[g0 + f].light = add_light(L);
+                    geos[g0 + f].light = add_light(L, g0 + f);
                 }
+            }
         } else if (o.shape == FM_SHAPE_RECTANGLE) {
             fm_drect r{};
             const F3 pa = f3(o.pa[0], o.pa[1], o.pa[2]), pb = f3(o.pb[0], o.pb[1], o.pb[2]);
@@ -737,7 +775,7 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
             Box b; b.grow(pa); b.grow(pb); b.grow(pc); b.grow(pd);
             { const F3 ps[4] = {pa, pb, pc, pd}; push_tight(tight, ps, 4); }
             int32_t light = -1;
-            if (emissive) { fm_dlight L{}; L.kind = FM_DLIGHT_RECT; L.aux = (int32_t)rects.size(); light = add_light(L); }
+            if (emissive) { fm_dlight L{}; L.kind = FM_DLIGHT_RECT; L.aux = (int32_t)rects.size(); light = add_light(L, geos.size()); }
             geos.push_back(Geo{FM_PRIM_RECT, (int32_t)rects.size(), mat, light});
             rects.push_back(r);
             bounds.push_back(b);
@@ -759,7 +797,7 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
             for (int k = 0; k < 3; k++) tight.push_back(b.mn[k]);
             for (int k = 0; k < 3; k++) tight.push_back(b.mx[k]);
             int32_t light = -1;
-            if (emissive) { fm_dlight L{}; L.kind = FM_DLIGHT_SPHERE; L.aux = (int32_t)spheres.size(); light = add_light(L); }
+            if (emissive) { fm_dlight L{}; L.kind = FM_DLIGHT_SPHERE; L.aux = (int32_t)spheres.size(); light = add_light(L, geos.size()); }
             geos.push_back(Geo{FM_PRIM_SPHERE, (int32_t)spheres.size(), mat, light});
             spheres.push_back(s);
             bounds.push_back(b);

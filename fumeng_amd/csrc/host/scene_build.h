@@ -53,6 +53,7 @@ struct HostScene {
     std::vector<fm_dsphere> spheres;
     std::vector<fm_dmaterial> materials;
     std::vector<fm_dlight> lights;
+    std::vector<fm_dlight> mesh_tris;    /* triangles of "embree" mesh lights */
     std::vector<fm_denv> envs;
     std::vector<fm_dimage> images;
     std::vector<unsigned char> texels;

@@ -144,8 +144,8 @@ class SceneDesc:
         keep = []
         d = A.fm_scene_desc()
         d.abi_version = A.FM_ABI_VERSION
-        d.accelerator = {"naive": A.FM_ACCEL_NAIVE, "reference_tree": A.FM_ACCEL_BVH_REFERENCE_TREE}.get(
-            self.accelerator, A.FM_ACCEL_BVH)
+        d.accelerator = {"naive": A.FM_ACCEL_NAIVE, "reference_tree": A.FM_ACCEL_BVH_REFERENCE_TREE,
+                         "embree": A.FM_ACCEL_EMBREE}.get(self.accelerator, A.FM_ACCEL_BVH)
         camera_to_c(self.camera, d.camera)
         r = self.renderer
         for k in ("width", "height", "spp", "thread_count", "depth", "direct_loop", "rr_depth"):

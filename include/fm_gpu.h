@@ -160,7 +160,12 @@ typedef struct fm_renderer_desc {
  * resolved by the reference BVH's leaf order (or object order for "naive").
  * FM_ACCEL_BVH traverses a binned-SAH tree; FM_ACCEL_BVH_REFERENCE_TREE
  * traverses the reference builder's own topology (bvh_aggregate.cpp:38-79). */
-enum { FM_ACCEL_BVH = 0, FM_ACCEL_NAIVE = 1, FM_ACCEL_BVH_REFERENCE_TREE = 2 };
+enum { FM_ACCEL_BVH = 0, FM_ACCEL_NAIVE = 1, FM_ACCEL_BVH_REFERENCE_TREE = 2, FM_ACCEL_EMBREE = 3 };
+/* FM_ACCEL_EMBREE follows the reference built with USE_EMBREE for lights:
+ * an emissive triangle mesh is one area light sampled by area
+ * (TriangleEmbree::Sample / Pdf, triangle_embree.cpp:253-270) instead of
+ * one light per triangle; intersection and ties are those of FM_ACCEL_BVH
+ * (Embree leaves ties unspecified). */
 
 /* Post-process operations of the JSON "post_process" list
  * (scene_loader.h:254-275), applied in order to the colour image once the

@@ -28,11 +28,43 @@ template <typename R> struct RenderObject {
     int light = -1;  /* index into lights when emissive */
 };
 
+/* The light of an emissive mesh under "embree" (TriangleEmbree,
+ * triangle_embree.cpp:121-135, 253-270): m_area_sum and a Distribution1D
+ * over the face areas; Sample picks a face with sample.z and samples it with
+ * the whole sample, pdf = 1 / m_area_sum; Pdf = 1 / Area().  Intersection
+ * stays with the per-face objects. */
+template <typename R> struct MeshLightGeom : Geometry<R> {
+    std::vector<const Geometry<R>*> faces;
+    R area_sum = 0;
+    std::unique_ptr<Dist1D<R>> dist;
+    void Finish() {
+        std::vector<R> areas;
+        area_sum = R(0);
+        for (const Geometry<R>* f : faces) { area_sum += f->Area(); areas.push_back(f->Area()); }
+        dist.reset(new Dist1D<R>(areas));
+    }
+    bool IsIntersect(const Ray<R>&) const override { return false; }
+    bool GetIntersect(const Ray<R>&, HitPoint<R>*) const override { return false; }
+    R Area() const override { return area_sum; }
+    HitPoint<R> Sample(R* pdf, V3<R> s) const override {
+        R v, p;
+        size_t idx;
+        dist->Sample(s.z, &v, &p, &idx);
+        *pdf = R(1) / area_sum;
+        R tmp;
+        return faces[idx]->Sample(&tmp, s);
+    }
+    HitPoint<R> Sample(const HitPoint<R>&, R* pdf, V3<R> s) const override { return Sample(pdf, s); }
+    R Pdf(V3<R>, V3<R>) const override { return R(1) / Area(); }
+    BBox<R> WorldBound() const override { return BBox<R>(); }
+};
+
 template <typename R> struct Scene {
     std::vector<Image<R>> images;
     std::vector<Material<R>> materials;  /* desc materials + default_material */
     std::vector<RenderObject<R>> objects;
     std::vector<Light<R>> lights;
+    std::vector<std::unique_ptr<MeshLightGeom<R>>> mesh_lights;
     int env = -1;
     Camera<R> camera;
     fm_renderer_desc rd;
@@ -130,17 +162,28 @@ template <typename R> struct Scene {
                 gs.emplace_back(new Rectangle<R>(V3<R>(od.pa[0], od.pa[1], od.pa[2]), V3<R>(od.pb[0], od.pb[1], od.pb[2]),
                                                  V3<R>(od.pc[0], od.pc[1], od.pc[2]), V3<R>(od.pd[0], od.pd[1], od.pd[2])));
             }
+            const bool mesh_light = d.accelerator == FM_ACCEL_EMBREE && od.shape == FM_SHAPE_TRIANGLE_MESH &&
+                                    !IsBlack(em) && !gs.empty();
+            if (mesh_light) {
+                mesh_lights.emplace_back(new MeshLightGeom<R>());
+                Light<R> L; L.kind = 0; L.radiance = em; L.geom = mesh_lights.back().get();
+                lights.push_back(std::move(L));
+            }
             for (auto& g : gs) {
                 RenderObject<R> ro;
                 ro.geom = std::move(g);
                 ro.material = mat;
-                if (!IsBlack(em)) {
+                if (mesh_light) {
+                    mesh_lights.back()->faces.push_back(ro.geom.get());
+                    ro.light = (int)lights.size() - 1;
+                } else if (!IsBlack(em)) {
                     Light<R> L; L.kind = 0; L.radiance = em; L.geom = ro.geom.get();
                     ro.light = (int)lights.size();  /* area lights in object order */
                     lights.push_back(std::move(L));
                 }
                 objects.push_back(std::move(ro));
             }
+            if (mesh_light) mesh_lights.back()->Finish();
         }
         BuildBVH();
         for (int li = 0; li < d.num_lights; li++) {

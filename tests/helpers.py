@@ -147,3 +147,27 @@ def two_planes_scene(depth=3, rough=0.4, metal=0.3, width=16, height=16):
     s.add_mesh_object(wall, material=m1)
     s.add_env(s.add_image(scenes.sky_env(64, 32, 3, sun_peak=50.0, sun_sigma_deg=8.0)))
     return s
+
+
+def mesh_light_scene(accelerator="embree", width=16, height=16, spp=64, depth=3):
+    """A Lambert floor (z = 0) under an emissive triangle mesh facing down:
+    a 3 x 2 grid of quads of unequal sizes (12 faces), so that sampling the
+    mesh by area differs from sampling each face as its own light."""
+    s = SceneDesc()
+    s.accelerator = accelerator
+    s.camera = {"type": "pin_hole", "pos": [0.3, -2.5, 2.0], "look_at": [0, 0, 0.2], "up": [0, 0, 1], "fov": 45.0,
+                "aspect": width / height, "focal_distance": 1.0, "lens_radius": 0.0}
+    s.renderer.update(width=width, height=height, spp=spp, depth=depth, rr_depth=10)
+    floor = s.add_lambert(0.6)
+    s.add_rectangle((-3, -3, 0), (3, -3, 0), (3, 3, 0), (-3, 3, 0), floor)
+    xs = np.array([-0.8, -0.5, 0.1, 0.9], np.float32)
+    ys = np.array([-0.4, 0.0, 0.7], np.float32)
+    pos = np.array([[x, y, 1.2] for y in ys for x in xs], np.float32)
+    idx = []
+    for j in range(len(ys) - 1):
+        for i in range(len(xs) - 1):
+            a, b, c, d = j * 4 + i, j * 4 + i + 1, (j + 1) * 4 + i + 1, (j + 1) * 4 + i
+            idx += [[a, c, b], [a, d, c]]      # wound so that cross(v0 - v2, v1 - v2) points down
+    m = s.add_mesh(pos, np.array(idx))
+    s.add_mesh_object(m, emittance=[6.0, 5.0, 4.0])
+    return s

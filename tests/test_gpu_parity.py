@@ -309,3 +309,36 @@ def test_empty_scene_and_bad_arguments(gpu_ctx):
         g.render(mode=A.FM_MODE_WAVEFRONT + 5)
     with pytest.raises(FumengError):
         g.render(block_size=100)
+
+
+def test_embree_mesh_light_parity(gpu_ctx):
+    """With "embree" an emissive mesh is one light sampled by area.  Light
+    samples equal the oracle's; renders with common random numbers equal
+    the oracle's on most pixels (both forms); the image mean equals the
+    per-face-light ("bvh") render's within its noise."""
+    from helpers import mesh_light_scene
+    s = mesh_light_scene("embree", width=24, height=24, spp=64)
+    g, o = gpu_ctx.scene(s), oracle.OracleScene(s)
+    assert g.info().num_lights == o.info()["lights"] == 1
+    rng = np.random.default_rng(62)
+    n = 8192
+    q = np.zeros(n, dtype=np.dtype(A.fm_light_query))
+    q["light"] = 0
+    q["pos"] = np.column_stack([rng.uniform(-1, 1, n), rng.uniform(-1, 1, n), rng.uniform(0, 0.5, n)])
+    q["sample"] = rng.random((n, 3))
+    rg, ro = g.light_sample(q), o.light_sample(q)
+    for k in ("light_pos", "wi", "radiance"):
+        assert np.all(np.abs(rg[k] - ro[k]) < 1e-4), k
+    assert np.mean(_rel(rg["pdf"], ro["pdf"], 1e-6) < 1e-4) > 0.999
+    ref = o.render(0, 0, 24, 24, spp=64, seed=4, threads=16)[0]
+    for mode in (A.FM_MODE_WAVEFRONT, A.FM_MODE_MEGAKERNEL):
+        c = g.render(seed=4, mode=mode)[0]
+        close = np.abs(c - ref) <= 1e-3 * np.maximum(1.0, np.abs(ref))
+        assert close.mean() > 0.9, (mode, close.mean())
+    b = gpu_ctx.scene(mesh_light_scene("bvh", width=24, height=24, spp=64))
+    assert b.info().num_lights == 12
+    e_img, _, _, e_sq, _ = g.render(seed=9, spp=1024, mode=A.FM_MODE_WAVEFRONT)
+    b_img, _, _, b_sq, _ = b.render(seed=9, spp=1024, mode=A.FM_MODE_WAVEFRONT)
+    var = lambda c, sq: (np.maximum(sq / 1024 - c.astype(np.float64) ** 2, 0) / 1024).sum() / c.size ** 2
+    z = (e_img.mean() - b_img.mean()) / np.sqrt(var(e_img, e_sq) + var(b_img, b_sq))
+    assert abs(z) < 4, z

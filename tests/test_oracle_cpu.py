@@ -368,3 +368,69 @@ def test_post_bytes_round_half_away_from_zero_and_clamp():
     vals = np.array([0.0, 1.0, 2.0, -1.0, np.nan, np.inf, -np.inf, tie(126), tie(127), tie(0), 0.5], np.float32)
     _, b = oracle.post_process([], vals)
     assert list(b) == [0, 255, 255, 0, 0, 255, 0, 127, 128, 1, 128]   # 126.5 -> 127: away from zero, not to even
+
+
+# ---- "embree" mesh lights (Q26) -------------------------------------------
+
+def _mesh_light_faces():
+    from helpers import mesh_light_scene
+    s = mesh_light_scene("embree")
+    m = s.meshes[0]
+    tri = m["positions"][m["indices"][:, [0, 3, 6]]].astype(np.float64)     # (faces, 3 corners, xyz)
+    area = 0.5 * np.linalg.norm(np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0]), axis=1)
+    return s, tri, area
+
+
+def test_embree_mesh_light_samples_faces_by_area():
+    """One light for the whole mesh: faces are chosen in proportion to their
+    area, points are uniform on them, and the solid-angle pdf is
+    d^2 / (A_total |cos|) (TriangleEmbree::Sample / Pdf)."""
+    s, tri, area = _mesh_light_faces()
+    o = oracle.OracleScene(s)
+    assert o.info()["lights"] == 1
+    rng = np.random.default_rng(61)
+    n = 60000
+    q = np.zeros(n, dtype=np.dtype(A.fm_light_query))
+    q["light"] = 0
+    q["pos"] = np.column_stack([rng.uniform(-1, 1, n), rng.uniform(-1, 1, n), np.zeros(n)])
+    q["sample"] = rng.random((n, 3))
+    r = o.light_sample(q)
+    lp = r["light_pos"].astype(np.float64)
+    assert np.allclose(lp[:, 2], 1.2, atol=1e-5)
+    face = np.full(n, -1)
+    for k, t in enumerate(tri):                     # point-in-triangle in xy
+        v0, v1, v2 = t[:, :2]
+        def side(a, b):
+            return (b[0] - a[0]) * (lp[:, 1] - a[1]) - (b[1] - a[1]) * (lp[:, 0] - a[0])
+        s0, s1, s2 = side(v0, v1), side(v1, v2), side(v2, v0)
+        inside = ((s0 >= -1e-7) & (s1 >= -1e-7) & (s2 >= -1e-7)) | ((s0 <= 1e-7) & (s1 <= 1e-7) & (s2 <= 1e-7))
+        face[inside & (face < 0)] = k
+    assert np.all(face >= 0)
+    cnt = np.bincount(face, minlength=len(area))
+    p = area / area.sum()
+    z = (cnt - n * p) / np.sqrt(n * p * (1 - p))
+    assert np.max(np.abs(z)) < 5, z
+    d = lp - q["pos"].astype(np.float64)
+    dist = np.linalg.norm(d, axis=1)
+    cos = np.abs(d[:, 2]) / dist
+    expect = dist ** 2 / (area.sum() * cos)
+    assert np.max(np.abs(r["pdf"] - expect) / expect) < 1e-4
+
+
+def test_embree_and_bvh_mesh_lights_agree_in_expectation():
+    """Per-face lights ("bvh": one shadow ray per face) and the area-sampled
+    mesh light ("embree": one shadow ray per mesh) estimate the same image:
+    whole-image means within 4 standard errors, from the renders' own
+    per-pixel variances."""
+    from helpers import mesh_light_scene
+    means, vars_ = [], []
+    for acc in ("embree", "bvh"):
+        s = mesh_light_scene(acc, width=12, height=12)
+        spp = 256
+        c, _, _, sq = oracle.OracleScene(s).render(0, 0, 12, 12, spp=spp, seed=7, threads=8)
+        var_pix = np.maximum(sq / spp - c ** 2, 0.0) / spp
+        means.append(c.mean())
+        vars_.append(var_pix.sum() / c.size ** 2)
+    z = (means[0] - means[1]) / np.sqrt(vars_[0] + vars_[1])
+    print("embree %.5f bvh %.5f z %.2f" % (means[0], means[1], z))
+    assert abs(z) < 4
