@@ -49,6 +49,53 @@ FMD float env_pdf(const fm_dscene& S, const fm_denv& E, v3 dir) {
     const float cond = T[E.off_cond_val + (size_t)iv * E.W + iu] / T[E.off_cond_sum + iv];
     return cond / E.marg_sum / (2.0f * FM_PI * FM_PI * st);
 }
+/* env_radiance(E, -dir) and, if want_pdf, env_pdf(E, dir) of one escaped
+ * MisBSDF ray, with one direction transform and one texel fetch: the texel
+ * env_pdf reads ((int)(u W), (int)(v H)) is one of the four the bilinear
+ * lookup reads, and for float env images the host stores its conditional
+ * value (cond_val / cond_sum, the same float division) in the texel's w.
+ * Results equal the two separate calls bit for bit; 8-bit images and the
+ * wrap edge (u or v exactly 1) read the tables. */
+FMD v3 env_radiance_pdf(const fm_dscene& S, const fm_denv& E, v3 dir, bool want_pdf, float& pdf) {
+    const v3 d = m3_apply(E.w2l, dir);
+    const float z = clampf(d.z, -1.0f, 1.0f);
+    const float theta = acosf(z);
+    float phi = atan2f(d.y, d.x);
+    if (phi < 0) phi += 2.0f * FM_PI;
+    const float u = phi * FM_INV_PI * 0.5f, v = theta * FM_INV_PI;
+    const fm_dimage& im = S.images[E.image];
+    pdf = 0.0f;
+    if (im.format != 0) {
+        if (want_pdf) pdf = env_pdf(S, E, dir);
+        return E.factor * image_sample(S, im, FM_SAMPLER_LINEAR, wrapf(FM_WRAP_REPEAT, u), wrapf(FM_WRAP_REPEAT, v));
+    }
+    const float ru = wrapf(FM_WRAP_REPEAT, u) * (float)im.w, rv = wrapf(FM_WRAP_REPEAT, v) * (float)im.h;
+    const int u0 = (int)clampf(ru - 0.5f, 0.0f, (float)(im.w - 1));
+    const int u1 = min(u0 + 1, im.w - 1);
+    const float pu = clampf(ru - 0.5f - (float)u0, 0.0f, 1.0f);
+    const int v0 = (int)clampf(rv - 0.5f, 0.0f, (float)(im.h - 1));
+    const int v1 = min(v0 + 1, im.h - 1);
+    const float pv = clampf(rv - 0.5f - (float)v0, 0.0f, 1.0f);
+    const float4* T4 = reinterpret_cast<const float4*>(S.texels + im.offset);
+    const float4 ta = T4[(size_t)v0 * im.w + u0], tb = T4[(size_t)v0 * im.w + u1];
+    const float4 tc = T4[(size_t)v1 * im.w + u0], td = T4[(size_t)v1 * im.w + u1];
+    const v3 a = f4to3(ta), b = f4to3(tb), c = f4to3(tc), dd = f4to3(td);
+    const v3 rad = E.factor * ((a * (1.0f - pu) + b * pu) * (1.0f - pv) + (c * (1.0f - pu) + dd * pu) * pv);
+    if (want_pdf) {
+        const float st = sinf(theta);
+        if (st == 0.0f) return rad;
+        const int iu = min((int)(u * (float)E.W), E.W - 1);
+        const int iv = min((int)(v * (float)E.H), E.H - 1);
+        if (E.marg_sum < FM_EPS_PDF) return rad;
+        float cond;
+        if ((iu == u0 || iu == u1) && (iv == v0 || iv == v1))
+            cond = iv == v0 ? (iu == u0 ? ta.w : tb.w) : (iu == u0 ? tc.w : td.w);
+        else
+            cond = S.env_tables[E.off_cond_val + (size_t)iv * E.W + iu] / S.env_tables[E.off_cond_sum + iv];
+        pdf = cond / E.marg_sum / (2.0f * FM_PI * FM_PI * st);
+    }
+    return rad;
+}
 /* last index with cdf[idx] <= u among [0, n-1] */
 FMD int cdf_search(const float* cdf, int n, float u) {
     int lo = 0, hi = n;
@@ -273,12 +320,10 @@ FMD PathOut render_path(const fm_dscene& S, const PathCfg& cfg, Ray ray, Rng& rn
             if (h.slot < 0) {
                 if (S.env >= 0) {
                     const fm_denv& E = S.envs[S.env];
-                    const v3 f = ms.f * env_radiance(S, E, -tr.d) * absdot(tr.d, sf.ng);
+                    float lp;
+                    const v3 f = ms.f * env_radiance_pdf(S, E, tr.d, !ms.delta, lp) * absdot(tr.d, sf.ng);
                     if (ms.delta) direct += beta * (f / ms.pdf);
-                    else {
-                        const float lp = env_pdf(S, E, tr.d);
-                        direct += beta * (f / ms.pdf * ((ms.pdf * ms.pdf) / (ms.pdf * ms.pdf + lp * lp)));
-                    }
+                    else direct += beta * (f / ms.pdf * ((ms.pdf * ms.pdf) / (ms.pdf * ms.pdf + lp * lp)));
                 }
             } else {
                 const Surf ls = fill_hit(S, tr, h);

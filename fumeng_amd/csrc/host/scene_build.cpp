@@ -561,6 +561,17 @@ void build_env_tables(HostScene& hs, fm_denv& e, const std::function<void(int, i
     for (int j = 0; j <= H; j++) T[(size_t)e.off_marg_cdf + j] = (float)(mc[(size_t)j] / ms);
     for (int j = 0; j < H; j++) T[(size_t)e.off_marg_val + j] = (float)rowsum[(size_t)j];
     e.marg_sum = (float)ms;
+    /* a float env image keeps each texel's conditional pdf value in its w
+     * (the device's env_pdf division, in float), so that an escaped ray's
+     * radiance and pdf come from one fetch */
+    if (e.image >= 0 && (size_t)e.image < hs.images.size() && hs.images[(size_t)e.image].format == 0 &&
+        hs.images[(size_t)e.image].w == W && hs.images[(size_t)e.image].h == H) {
+        float* t4 = reinterpret_cast<float*>(hs.texels.data() + hs.images[(size_t)e.image].offset);
+        for (int j = 0; j < H; j++) {
+            const float sum = T[(size_t)e.off_cond_sum + j];
+            for (int i = 0; i < W; i++) t4[4 * ((size_t)j * W + i) + 3] = T[(size_t)e.off_cond_val + (size_t)j * W + i] / sum;
+        }
+    }
     /* guide tables for the device search: entry g = last index with
      * cdf <= g / G over the stored FP32 cdf (G a power of two, so g / G
      * and u * G are exact); a query u then only searches between the

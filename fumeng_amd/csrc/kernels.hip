@@ -112,8 +112,11 @@ __global__ void k_env(fm_dscene S, const float* __restrict__ dirs, int64_t n, fl
     if (i >= n) return;
     const fm_denv& E = S.envs[S.env];
     const v3 d = mk(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]);
-    pdf[i] = env_pdf(S, E, d);
-    const v3 c = env_radiance(S, E, -d);
+    /* the fused lookup the renders use for escaped rays; it falls back to
+     * env_pdf on 8-bit images and at the wrap edge */
+    float p;
+    const v3 c = env_radiance_pdf(S, E, d, true, p);
+    pdf[i] = p;
     rgb[3 * i] = c.x; rgb[3 * i + 1] = c.y; rgb[3 * i + 2] = c.z;
 }
 

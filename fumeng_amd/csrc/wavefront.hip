@@ -291,12 +291,10 @@ FMD v3 resolve_direct(const fm_dscene& S, const WfArgs& A, int64_t p) {
         if (h.slot < 0) {
             if (LK != fm::LK_AREA && S.env >= 0) {
                 const fm_denv& E = S.envs[S.env];
-                const v3 f = msf * env_radiance(S, E, -tr.d) * absdot(tr.d, ng);
+                float lp;
+                const v3 f = msf * env_radiance_pdf(S, E, tr.d, !delta, lp) * absdot(tr.d, ng);
                 if (delta) direct += beta * (f / mpdf);
-                else {
-                    const float lp = env_pdf(S, E, tr.d);
-                    direct += beta * (f / mpdf * ((mpdf * mpdf) / (mpdf * mpdf + lp * lp)));
-                }
+                else direct += beta * (f / mpdf * ((mpdf * mpdf) / (mpdf * mpdf + lp * lp)));
             }
         } else if (LK != fm::LK_ENV) {
             const Surf ls = fill_hit(S, tr, h);

@@ -249,6 +249,11 @@ def test_env_parity_against_fp64_tables(gpu_ctx):
     g, o = gpu_ctx.scene(s), oracle.OracleScene(s, fp64=True)
     rng = np.random.default_rng(28)
     d = random_sphere_dirs(rng, 20000).astype(np.float32)
+    # a few axis-aligned and near-seam directions besides the random ones
+    special = np.array([[0, 0, 1], [0, 0, -1], [1, -1e-8, 0], [1, 1e-8, 0], [1, 0, 1e-7], [0.5, -1e-9, -0.8]],
+                       np.float64)
+    special /= np.linalg.norm(special, axis=1, keepdims=True)
+    d = np.concatenate([d, special.astype(np.float32)])
     pg, cg = g.env_eval(d)
     po, co = o.env_eval(d)
     assert np.mean(_rel(pg, po, 1e-6) < 1e-4) > 0.999
