@@ -123,6 +123,7 @@ struct fm_ctx {
     DevBuf wf_passes, wf_units, wf_cams, wf_pass_idx;   /* wavefront tables of the last run */
     DevBuf batch_film;                                  /* FP64 sums of fm_render_batch jobs */
     DevBuf adj_in, adj_gmat, adj_genv;                  /* adjoint: image gradient, FP64 parameter gradients */
+    DevBuf adj_tot;                                     /* adjoint: replayed path radiance per work item */
     std::vector<hipEvent_t> trace_ev;  /* pairs around wavefront trace launches */
     std::vector<hipEvent_t> pass_ev;   /* end of each wavefront pass */
 };
@@ -923,6 +924,8 @@ int fm_render_adjoint(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, const
         A.env_w = im ? im->w : 0; A.env_h = im ? im->h : 0;
         A.lds_grad = (int64_t)nm * FM_DPARAM_PER_MATERIAL <= 4096 ? 1 : 0;
         A.counters = ctx->counters.as<unsigned long long>();
+        if (ctx->adj_tot.n < (size_t)A.total_items * 3 * sizeof(float)) ctx->adj_tot.alloc((size_t)A.total_items * 3 * sizeof(float));
+        A.totals = ctx->adj_tot.as<float>();
         const int block = 128, levels = std::max(1, s->hs.trav_stack);
         const size_t lds = (size_t)levels * block * 4 + (A.lds_grad ? (size_t)nm * FM_DPARAM_PER_MATERIAL * 4 : 0);
         int grid = p->grid_blocks > 0 ? p->grid_blocks : ctx->num_cus * fm::adjoint_blocks_per_cu(block, lds) * 2;

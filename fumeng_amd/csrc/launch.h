@@ -121,6 +121,7 @@ struct AdjointArgs {
     int env_w, env_h;
     int lds_grad;             /* accumulate material gradients in LDS first */
     unsigned long long* counters;
+    float* totals;            /* path radiance per work item (3 floats), from the forward replay */
 };
 hipError_t launch_adjoint(const fm_dscene& S, const AdjointArgs& A, int block, int grid, int stack_levels, hipStream_t st);
 int adjoint_blocks_per_cu(int block, size_t lds);
