@@ -132,22 +132,42 @@ def stress_leg(fumeng_amd, ctx):
     """Large-geometry stress scene: 400 displaced spheres (9.8 M triangles),
     200 sRGB 2048^2 textures, 10 rectangle lights, ground; 1920x1080, 128
     spp, depth 10; one wavefront frame (scene generation and upload are
-    reported, not timed into the throughput)."""
+    reported, not timed into the throughput).  The scene is built twice,
+    with the device's traversal-tree builder and with the host's, and a
+    frame is rendered from each."""
     from fumeng_amd import scenes, _abi
     t0 = time.perf_counter()
     desc = scenes.stress_scene()
     gen = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    sc = ctx.scene(desc)
-    build = time.perf_counter() - t0
-    info = sc.info()
     W, H, spp = desc.renderer["width"], desc.renderer["height"], desc.renderer["spp"]
+
+    def build_and_frame(builder):
+        ctx.set_bvh_builder(builder)
+        try:
+            t0 = time.perf_counter()
+            sc = ctx.scene(desc)
+            build = time.perf_counter() - t0
+        finally:
+            ctx.set_bvh_builder("host")
+        info = sc.info()
+        sc.render(seed=2, spp=1, read=False, mode=_abi.FM_MODE_WAVEFRONT)        # warm-up
+        t0 = time.perf_counter()
+        _, _, _, _, st = sc.render(seed=1, read=False, mode=_abi.FM_MODE_WAVEFRONT, time_trace=True)
+        dt = time.perf_counter() - t0
+        sc.close()
+        return info, build, st, dt
+
+    # the device-built tree of the same scene: build time against frame time
+    dinfo, dbuild, dst, ddt = build_and_frame("device")
+    device = {"build_upload_s": dbuild, "device_build_ms": dinfo.build_ms_device, "iterations": dinfo.build_iterations,
+              "host_total_ms": dinfo.build_ms_total, "trav_nodes": dinfo.num_trav_nodes, "trav_stack": dinfo.trav_stack,
+              "ms_per_frame": ddt * 1e3, "trace_ms": dst.trace_ms, "mpaths_per_s": W * H * spp / ddt / 1e6}
+    log("stress, device-built tree: built in %.1f s (host %.0f ms, device %.0f ms, %d iterations), %d nodes, "
+        "%d stack entries; %.0f ms per frame (traversal %.0f ms)"
+        % (dbuild, dinfo.build_ms_total, dinfo.build_ms_device, dinfo.build_iterations, dinfo.num_trav_nodes,
+           dinfo.trav_stack, ddt * 1e3, dst.trace_ms))
+    info, build, st, dt = build_and_frame("host")
     del desc
-    sc.render(seed=2, spp=1, read=False, mode=_abi.FM_MODE_WAVEFRONT)        # warm-up
-    t0 = time.perf_counter()
-    _, _, _, _, st = sc.render(seed=1, read=False, mode=_abi.FM_MODE_WAVEFRONT, time_trace=True)
-    dt = time.perf_counter() - t0
-    sc.close()
     res = {"workload": "stress: 400 displaced spheres (9,800,012 geometries), 200 textures 2048^2, 10 rect lights, "
                        "1920x1080, 128 spp, depth 10", "mpaths_per_s": W * H * spp / dt / 1e6,
            "mrays_per_s": st.rays / dt / 1e6, "ms_per_frame": dt * 1e3, "trace_ms": st.trace_ms,
@@ -155,7 +175,8 @@ def stress_leg(fumeng_amd, ctx):
            "build_report": {"threads": info.build_threads, "geometry_ms": info.build_ms_geometry,
                             "reference_bvh_ms": info.build_ms_ref_bvh, "traversal_tree_ms": info.build_ms_trav_tree,
                             "tables_ms": info.build_ms_tables, "host_total_ms": info.build_ms_total,
-                            "upload_ms": info.upload_ms, "host_gb": info.host_bytes / 1e9}}
+                            "upload_ms": info.upload_ms, "host_gb": info.host_bytes / 1e9},
+           "trav_nodes": info.num_trav_nodes, "trav_stack": info.trav_stack, "device_builder": device}
     log("stress: %.1f Mpaths/s, %.1f Mrays/s (%.0f ms per frame), built in %.1f s, %.2f GB in HBM"
         % (res["mpaths_per_s"], res["mrays_per_s"], dt * 1e3, build, res["device_gb"]))
     log("stress build on %d threads: geometry %.0f ms, reference BVH %.0f ms, traversal tree %.0f ms, tables %.0f ms, "
@@ -207,6 +228,8 @@ def main():
     ap.add_argument("--mode", choices=["megakernel", "wavefront"], default="wavefront")
     ap.add_argument("--bvh-width", type=int, choices=[2, 4], default=2,
                     help="traversal tree width (the closest hit does not depend on it)")
+    ap.add_argument("--bvh-builder", choices=["host", "device"], default="host",
+                    help="traversal tree builder: binned SAH on the host threads or PLOC on the GPU")
     ap.add_argument("--no-stress", dest="stress", action="store_false",
                     help="skip the 9.8 M-triangle textured stress scene leg")
     ap.add_argument("--no-adjoint", dest="adjoint", action="store_false",
@@ -228,6 +251,7 @@ def main():
 
     import fumeng_amd
     from fumeng_amd import scenes
+    from fumeng_amd import _abi as _abi_mod
 
     t0 = time.perf_counter()
     desc = scenes.icosphere_scene(freq=args.freq, spp=args.spp)
@@ -235,12 +259,15 @@ def main():
     log("scene generated: %d geometries in %.1f s" % (desc.num_geometries(), time.perf_counter() - t0))
     ctx = fumeng_amd.Context(0)
     ctx.set_bvh_width(args.bvh_width)
+    ctx.set_bvh_builder(args.bvh_builder)
     t0 = time.perf_counter()
     scene = ctx.scene(desc)
     info = scene.info()
-    log("scene built and uploaded in %.1f s: %d BVH nodes, depth %d; traversal tree %d wide, %d nodes, "
+    log("scene built and uploaded in %.1f s: %d BVH nodes, depth %d; traversal tree %d wide (%s builder%s), %d nodes, "
         "%d stack entries; %.1f MB in HBM"
-        % (time.perf_counter() - t0, info.num_bvh_nodes, info.bvh_depth, info.trav_width, info.num_trav_nodes,
+        % (time.perf_counter() - t0, info.num_bvh_nodes, info.bvh_depth, info.trav_width, args.bvh_builder,
+           ", %.1f ms on the device, %d iterations" % (info.build_ms_device, info.build_iterations)
+           if info.trav_builder == _abi_mod.FM_BUILDER_DEVICE else "", info.num_trav_nodes,
            info.trav_stack, info.device_bytes / 1e6))
 
     from fumeng_amd import _abi
@@ -338,7 +365,7 @@ def main():
            "config": {"workload": "displaced icosphere %d tris + ground, 64 Disney materials, 2048x1024 HDR env, "
                                   "thin lens, %dx%d, %d spp, depth 12, %s" % (20 * args.freq ** 2, W, H, spp, args.mode),
                       "mode": args.mode, "width": W, "height": H, "spp": spp, "depth": 12,
-                      "bvh_width": args.bvh_width},
+                      "bvh_width": args.bvh_width, "bvh_builder": args.bvh_builder},
            "mrays_per_s": mrays, "paths": paths, "rays": rays, "other_form": other, "batch": batch, "adjoint": adjoint, "stress": stress,
            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out))

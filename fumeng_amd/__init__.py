@@ -77,6 +77,13 @@ class Context:
         """Traversal tree width (2, the default, or 4) of the scenes created afterwards."""
         _check(lib().fm_set_bvh_width(self.h, int(width)), self.h)
 
+    def set_bvh_builder(self, builder):
+        """Traversal tree builder of the scenes created afterwards: "host"
+        (binned SAH on the host threads, the default) or "device" (PLOC on
+        the GPU, two-wide trees)."""
+        b = A.BUILDERS[builder] if isinstance(builder, str) else int(builder)
+        _check(lib().fm_set_bvh_builder(self.h, b), self.h)
+
     def post_process(self, ops, rgb):
         """The reference's output step on the device (fm_post_process):
         returns (post-processed float image, uint8 image).  ops: list of
@@ -141,6 +148,20 @@ class Scene:
                           "kernel_ms": t.kernel_ms}
         return {"best": conv(res.best), "baseline": conv(res.baseline), "trial_spp": res.trial_spp,
                 "trials": [conv(arr[i]) for i in range(min(res.num_trials, 64))]}
+
+    def traversal(self):
+        """(nodes, slot_geometry): the device's traversal tree (fm_dnode
+        records for two-wide trees, raw bytes for four-wide ones) and the
+        geometry index at each slot."""
+        i = self.info()
+        nb = int(i.num_trav_nodes) * int(i.trav_node_bytes)
+        raw = np.zeros(max(nb, 1), np.uint8)
+        slots = np.zeros(max(int(i.num_geometries), 1), np.int32)
+        _check(lib().fm_scene_traversal(self.h, raw.ctypes.data, nb, slots.ctypes.data, int(i.num_geometries)), self.ctx.h)
+        raw, slots = raw[:nb], slots[:int(i.num_geometries)]
+        if i.trav_width == 2:
+            return raw.view(np.dtype(A.fm_dnode)), slots
+        return raw, slots
 
     def post_ops(self):
         """The scene's post-process list as dicts ({"type": ..., "lum": ...})."""

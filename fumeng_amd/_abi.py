@@ -1,8 +1,11 @@
 """ctypes mirror of include/fm_gpu.h (structure layouts and constants)."""
 import ctypes as C
 
-FM_ABI_VERSION = 2
+FM_ABI_VERSION = 3
 FM_ACCEL_EMBREE = 3
+FM_BUILDER_HOST = 0
+FM_BUILDER_DEVICE = 1
+BUILDERS = {"host": FM_BUILDER_HOST, "device": FM_BUILDER_DEVICE}
 
 FM_OK = 0
 FM_ERR_INVALID = -1
@@ -119,7 +122,8 @@ class fm_scene_info(C.Structure):
                 ("trav_stack", C.c_int32), ("num_trav_nodes", C.c_int64), ("trav_node_bytes", C.c_int32),
                 ("build_threads", C.c_int32), ("build_ms_geometry", C.c_double), ("build_ms_ref_bvh", C.c_double),
                 ("build_ms_trav_tree", C.c_double), ("build_ms_tables", C.c_double), ("build_ms_total", C.c_double),
-                ("upload_ms", C.c_double), ("host_bytes", C.c_int64), ("trav_hash", C.c_uint64)]
+                ("upload_ms", C.c_double), ("host_bytes", C.c_int64), ("trav_hash", C.c_uint64),
+                ("trav_builder", C.c_int32), ("build_iterations", C.c_int32), ("build_ms_device", C.c_double)]
 
 
 class fm_render_params(C.Structure):
@@ -184,6 +188,12 @@ def _np_dtype(struct):
     return np.dtype(struct)
 
 
+class fm_dnode(C.Structure):
+    """Two-wide traversal node (fm_layout.h): child boxes, child refs (>= 0
+    node index, < 0 leaf ~((start << 3) | count))."""
+    _fields_ = [("b", C.c_float * 12), ("child", C.c_int32 * 2), ("pad", C.c_int32 * 2)]
+
+
 EXPORTS = {
     "fm_init": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
     "fm_shutdown": (None, [C.c_void_p]),
@@ -199,6 +209,8 @@ EXPORTS = {
                                   C.POINTER(C.c_double), C.POINTER(fm_batch_stats)]),
     "fm_set_counting": (C.c_int, [C.c_void_p, C.c_int]),
     "fm_set_bvh_width": (C.c_int, [C.c_void_p, C.c_int32]),
+    "fm_set_bvh_builder": (C.c_int, [C.c_void_p, C.c_int32]),
+    "fm_scene_traversal": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64]),
     "fm_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(fm_render_params), C.c_int32,
                               C.POINTER(fm_tune_result), C.c_void_p, C.c_int32]),
     "fm_scene_post_ops": (C.c_int, [C.c_void_p, C.POINTER(fm_post_op), C.c_int32, C.POINTER(C.c_int32)]),

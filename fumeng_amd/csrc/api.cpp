@@ -12,12 +12,14 @@
 #include <chrono>
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
 #include <set>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace {
@@ -118,6 +120,7 @@ struct fm_ctx {
     int64_t film_spp = 0;     /* samples accumulated into the device film */
     int counting = 0;
     int bvh_width = 2;        /* traversal tree width of scenes created next */
+    int bvh_builder = FM_BUILDER_HOST;   /* traversal tree builder of scenes created next */
     std::set<fm_scene*> scenes;  /* live scenes, released by fm_shutdown */
     WfBuffers wf;
     DevBuf wf_passes, wf_units, wf_cams, wf_pass_idx;   /* wavefront tables of the last run */
@@ -145,10 +148,24 @@ struct fm_loaded_desc {
 namespace {
 
 template <typename V> int64_t vbytes(const V& v) { return (int64_t)(v.size() * sizeof(v[0])); }
+/* FNV-1a over 8-byte words (then the tail bytes): a content hash of the
+ * traversal arrays, computed once per scene build */
 template <typename V> uint64_t fnv1a(const V& v, uint64_t h = 1469598103934665603ull) {
     const unsigned char* b = reinterpret_cast<const unsigned char*>(v.data());
-    for (size_t i = 0; i < v.size() * sizeof(v[0]); i++) { h ^= b[i]; h *= 1099511628211ull; }
+    const size_t n = v.size() * sizeof(v[0]);
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+        uint64_t w;
+        memcpy(&w, b + i, 8);
+        h ^= w;
+        h *= 1099511628211ull;
+    }
+    for (; i < n; i++) { h ^= b[i]; h *= 1099511628211ull; }
     return h;
+}
+
+void set_trav_hash(fmh::HostScene& hs) {
+    hs.trav_hash = hs.device_tree ? 0 : fnv1a(hs.prims, fnv1a(hs.nodes4, fnv1a(hs.nodes)));
 }
 
 int fail(fm_ctx* ctx, int code, const std::string& msg) {
@@ -228,6 +245,61 @@ void upload_scene(fm_scene* s) {
                       &s->envs, &s->images, &s->texels, &s->env_tables, &s->lut})
         s->device_bytes += (int64_t)b->n;
     s->upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+/* Device-built traversal tree (FM_BUILDER_DEVICE): the slot records were
+ * uploaded in geometry order; the builder makes the tree and the slot order,
+ * and the records are gathered into it. */
+void build_device_tree(fm_scene* s) {
+    fmh::HostScene& hs = s->hs;
+    const int64_t G = hs.num_geoms;
+    if (G == 0) { std::vector<float>().swap(hs.leaf_boxes); return; }
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st = s->ctx->stream;
+    DevBuf boxes;
+    boxes.upload(hs.leaf_boxes, st);
+    fm::BvhBuildOut bo;
+    const hipError_t e = fm::bvh_build_device(boxes.as<float>(), G, bo, st);
+    if (e != hipSuccess) throw HipError(std::string("device BVH build: ") + (bo.error ? bo.error : hipGetErrorString(e)));
+    s->nodes.release();
+    s->nodes.p = bo.nodes;
+    s->nodes.n = (size_t)std::max<int64_t>(bo.num_nodes, 1) * sizeof(fm_dnode);
+    DevBuf slot_src;   /* owns bo.slot_src */
+    slot_src.p = bo.slot_src;
+    slot_src.n = (size_t)G * sizeof(int32_t);
+    boxes.release();
+    if (bo.stack_levels > FM_STACK_MAX) throw fmh::BuildError(FM_ERR_UNSUPPORTED, "device-built tree needs more stack than FM_STACK_MAX");
+    if (bo.num_nodes * (int64_t)sizeof(fm_dnode) >= ((int64_t)1 << 32))
+        throw fmh::BuildError(FM_ERR_UNSUPPORTED, "two-wide traversal tree of 4 GiB or more");
+    const std::pair<DevBuf*, int> recs[3] = {{&s->prims, 12}, {&s->meta, 4}, {&s->tri_sh, 16}};
+    for (const auto& r : recs) {
+        DevBuf out;
+        out.alloc(r.first->n);
+        HIPCHECK(fm::launch_gather(r.first->as<float>(), out.as<float>(), slot_src.as<int32_t>(), G, r.second, st));
+        std::swap(r.first->p, out.p);   /* out now holds the old records and frees them */
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+    hs.device_nodes = bo.num_nodes;
+    hs.trav_stack = bo.stack_levels;
+    hs.trav_depth = bo.stack_levels + 1;
+    hs.root_ref = bo.root_ref;
+    memcpy(hs.root_box, bo.root_box, sizeof hs.root_box);
+    hs.build_iterations = bo.iterations;
+    std::vector<float>().swap(hs.leaf_boxes);
+    fm_dscene& d = s->d;
+    d.nodes = s->nodes.as<fm_dnode>();
+    d.node_bytes = (uint32_t)(bo.num_nodes * (int64_t)sizeof(fm_dnode));
+    d.stack_levels = hs.trav_stack;
+    d.root_ref = hs.root_ref;
+    memcpy(d.root_box, hs.root_box, sizeof d.root_box);
+    d.prims = s->prims.as<float>();
+    d.meta = s->meta.as<int32_t>();
+    d.tri_sh = s->tri_sh.as<float>();
+    s->device_bytes = 0;
+    for (DevBuf* b : {&s->nodes, &s->nodes4, &s->prims, &s->meta, &s->tri_sh, &s->rects, &s->spheres, &s->materials, &s->lights, &s->mesh_tris,
+                      &s->envs, &s->images, &s->texels, &s->env_tables, &s->lut})
+        s->device_bytes += (int64_t)b->n;
+    hs.device_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 /* Copies n items in, runs a launcher, copies m items out. */
@@ -482,8 +554,11 @@ int fm_scene_create(fm_ctx* ctx, const fm_scene_desc* desc, fm_scene** out) {
         std::unique_ptr<fm_scene> s(new fm_scene());
         s->ctx = ctx;
         s->hs.trav_width = ctx->bvh_width;
+        s->hs.builder = ctx->bvh_builder;
         fmh::build_host_scene(*desc, s->hs);
+        set_trav_hash(s->hs);
         upload_scene(s.get());
+        if (s->hs.device_tree) build_device_tree(s.get());
         *out = s.release();
         ctx->scenes.insert(*out);
         return FM_OK;
@@ -524,7 +599,7 @@ static void fill_info(const fmh::HostScene& hs, int64_t device_bytes, double upl
     }
     o->trav_width = hs.trav_width;
     o->trav_stack = hs.trav_stack;
-    o->num_trav_nodes = (int64_t)(hs.trav_width == 4 ? hs.nodes4.size() : hs.nodes.size());
+    o->num_trav_nodes = hs.device_tree ? hs.device_nodes : (int64_t)(hs.trav_width == 4 ? hs.nodes4.size() : hs.nodes.size());
     o->trav_node_bytes = hs.trav_width == 4 ? (int32_t)sizeof(fm_dnode4) : (int32_t)sizeof(fm_dnode);
     o->build_threads = hs.report.threads;
     o->build_ms_geometry = hs.report.geometry_ms;
@@ -537,7 +612,31 @@ static void fill_info(const fmh::HostScene& hs, int64_t device_bytes, double upl
                     vbytes(hs.meta) + vbytes(hs.tri_sh) + vbytes(hs.rects) + vbytes(hs.spheres) + vbytes(hs.materials) +
                     vbytes(hs.lights) + vbytes(hs.mesh_tris) + vbytes(hs.envs) + vbytes(hs.images) + vbytes(hs.texels) + vbytes(hs.env_tables) +
                     vbytes(hs.srgb_lut);
-    o->trav_hash = fnv1a(hs.prims, fnv1a(hs.nodes4, fnv1a(hs.nodes)));
+    o->trav_hash = hs.trav_hash;
+    o->trav_builder = hs.device_tree ? FM_BUILDER_DEVICE : FM_BUILDER_HOST;
+    o->build_iterations = hs.build_iterations;
+    o->build_ms_device = hs.device_build_ms;
+}
+
+int fm_scene_traversal(const fm_scene* s, void* nodes, int64_t max_bytes, int32_t* slot_geometry, int64_t max_slots) {
+    if (!s) return fail(nullptr, FM_ERR_INVALID, "NULL scene");
+    return guarded(s->ctx, [&]() -> int {
+        HIPCHECK(hipSetDevice(s->ctx->device));
+        const fmh::HostScene& hs = s->hs;
+        const bool w4 = s->d.width == 4;
+        const int64_t bytes = w4 ? vbytes(hs.nodes4) : (int64_t)s->d.node_bytes;
+        if (nodes && bytes > 0 && max_bytes > 0)
+            HIPCHECK(hipMemcpyAsync(nodes, w4 ? s->nodes4.p : s->nodes.p, (size_t)std::min(bytes, max_bytes), hipMemcpyDeviceToHost,
+                                    s->ctx->stream));
+        if (slot_geometry && hs.num_geoms > 0 && max_slots > 0) {
+            std::vector<int32_t> meta((size_t)hs.num_geoms * 4);
+            HIPCHECK(hipMemcpyAsync(meta.data(), s->meta.p, meta.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s->ctx->stream));
+            HIPCHECK(hipStreamSynchronize(s->ctx->stream));
+            for (int64_t i = 0; i < std::min<int64_t>(max_slots, hs.num_geoms); i++) slot_geometry[i] = meta[(size_t)i * 4];
+        }
+        HIPCHECK(hipStreamSynchronize(s->ctx->stream));
+        return FM_OK;
+    });
 }
 
 int fm_scene_get_info(const fm_scene* s, fm_scene_info* o) {
@@ -612,6 +711,13 @@ int fm_set_bvh_width(fm_ctx* ctx, int32_t width) {
     return FM_OK;
 }
 
+int fm_set_bvh_builder(fm_ctx* ctx, int32_t builder) {
+    if (!ctx) return fail(nullptr, FM_ERR_INVALID, "NULL ctx");
+    if (builder != FM_BUILDER_HOST && builder != FM_BUILDER_DEVICE) return fail(ctx, FM_ERR_INVALID, "unknown BVH builder");
+    ctx->bvh_builder = builder;
+    return FM_OK;
+}
+
 int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film, fm_render_stats* stats) {
     if (!ctx || !s || !p) return fail(ctx, FM_ERR_INVALID, "NULL argument");
     return guarded(ctx, [&]() -> int {
@@ -676,7 +782,7 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
             job.W = W; job.H = H; job.spp = spp; job.depth = std::max(A.cfg_raw.depth, 1);
             job.sample_offset = A.sample_offset; job.seed = A.seed; job.film = base;
             WfRunOut R;
-            wf_run(ctx, s, {job}, A.cfg_raw.direct, A.cfg_raw.rr_depth, A.cfg_raw.rr_coef, block, p->grid_blocks,
+            wf_run(ctx, s, {job}, A.cfg_raw.direct, A.cfg_raw.rr_depth, A.cfg_raw.rr_coef, p->block_size > 0 ? block : 0, p->grid_blocks,
                    p->chunk > 0 ? std::min(p->chunk, spp) : 0, p->flags, R);
             trace_ms = R.trace_ms;
         }
@@ -759,7 +865,7 @@ int fm_render_batch(fm_ctx* ctx, fm_scene* s, const fm_job* jobs, int n, const f
         HIPCHECK(fm::launch_zero_f64(film, off[(size_t)n], ctx->stream));
         HIPCHECK(fm::launch_zero_f64((double*)ctx->counters.p, 4, ctx->stream));
         WfRunOut R;
-        wf_run(ctx, s, wj, D, rr_depth, rr_coef, block, cp.grid_blocks, cp.chunk, cp.flags, R);
+        wf_run(ctx, s, wj, D, rr_depth, rr_coef, cp.block_size > 0 ? block : 0, cp.grid_blocks, cp.chunk, cp.flags, R);
         if (job_done_ms) for (int i = 0; i < n; i++) job_done_ms[i] = R.job_done_ms[(size_t)i];
         if (stats) {
             memset(stats, 0, sizeof(*stats));
@@ -1080,6 +1186,7 @@ int fm_host_scene_info(const fm_scene_desc* desc, int32_t bvh_width, fm_scene_in
         fmh::HostScene hs;
         hs.trav_width = bvh_width;
         fmh::build_host_scene(*desc, hs);
+        set_trav_hash(hs);
         fill_info(hs, 0, 0.0, out);
         return FM_OK;
     });

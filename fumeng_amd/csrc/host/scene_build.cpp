@@ -850,7 +850,15 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
     std::vector<float> node_boxes;    /* 6 per tree node */
     struct TNode { bool leaf; int64_t a, b; };
     std::vector<TNode> tn;
-    if (d.accelerator == FM_ACCEL_BVH_REFERENCE_TREE) {
+    hs.device_tree = hs.builder == FM_BUILDER_DEVICE && d.accelerator != FM_ACCEL_BVH_REFERENCE_TREE;
+    if (hs.builder != FM_BUILDER_HOST && hs.builder != FM_BUILDER_DEVICE) throw BuildError(FM_ERR_INVALID, "unknown BVH builder");
+    if (hs.trav_width != 2 && hs.trav_width != 4) throw BuildError(FM_ERR_INVALID, "traversal tree width must be 2 or 4");
+    if (hs.device_tree && hs.trav_width != 2) throw BuildError(FM_ERR_UNSUPPORTED, "the device builder makes two-wide trees only");
+    if (hs.device_tree) {
+        /* slots in geometry order; the device builds the tree and permutes them */
+        slot_geom.resize(G);
+        for (size_t p = 0; p < G; p++) slot_geom[p] = (int32_t)p;
+    } else if (d.accelerator == FM_ACCEL_BVH_REFERENCE_TREE) {
         slot_geom = hs.order;
         tn.resize(hs.ref_nodes.size());
         for (size_t i = 0; i < hs.ref_nodes.size(); i++) tn[i] = TNode{hs.ref_nodes[i].leaf, hs.ref_nodes[i].a, hs.ref_nodes[i].b};
@@ -863,120 +871,128 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
         tn.resize(sb.nodes.size());
         for (size_t i = 0; i < sb.nodes.size(); i++) tn[i] = TNode{sb.nodes[i].leaf, sb.nodes[i].a, sb.nodes[i].b};
     }
-    /* true bounds bottom-up (children have larger indices), padded by 2 ulps */
-    node_boxes.assign(tn.size() * 6, 0.0f);
-    for (size_t ii = tn.size(); ii-- > 0;) {
-        float* o = &node_boxes[ii * 6];
-        for (int k = 0; k < 3; k++) { o[k] = std::numeric_limits<float>::infinity(); o[3 + k] = -std::numeric_limits<float>::infinity(); }
-        auto merge = [&](const float* q) {
-            for (int k = 0; k < 3; k++) { o[k] = std::min(o[k], q[k]); o[3 + k] = std::max(o[3 + k], q[3 + k]); }
-        };
-        if (tn[ii].leaf) for (int64_t j = tn[ii].a; j < tn[ii].b; j++) merge(&tight[(size_t)slot_geom[(size_t)j] * 6]);
-        else { merge(&node_boxes[(size_t)tn[ii].a * 6]); merge(&node_boxes[(size_t)tn[ii].b * 6]); }
-    }
-    for (size_t ii = 0; ii < tn.size(); ii++)
-        for (int k = 0; k < 3; k++) {
-            float& lo = node_boxes[ii * 6 + k];
-            float& hi = node_boxes[ii * 6 + 3 + k];
-            lo = std::nextafter(std::nextafter(lo, -INFINITY), -INFINITY);
-            hi = std::nextafter(std::nextafter(hi, INFINITY), INFINITY);
-        }
-    int tree_depth = 0;
-    {
-        std::vector<int> dep(tn.size(), 1);
-        for (size_t ii = 0; ii < tn.size(); ii++) {
-            tree_depth = std::max(tree_depth, dep[ii]);
-            if (!tn[ii].leaf) { dep[(size_t)tn[ii].a] = dep[ii] + 1; dep[(size_t)tn[ii].b] = dep[ii] + 1; }
-        }
-    }
-    if (tree_depth > FM_STACK_DEPTH) throw BuildError(FM_ERR_UNSUPPORTED, "traversal tree deeper than the LDS stack");
-    for (const TNode& t : tn)
-        if (t.leaf && t.b - t.a > 7) throw BuildError(FM_ERR_UNSUPPORTED, "leaf larger than 7 primitives");
-    hs.trav_depth = tree_depth;
-    if (hs.trav_width != 2 && hs.trav_width != 4) throw BuildError(FM_ERR_INVALID, "traversal tree width must be 2 or 4");
-
-    /* ---- flatten: interior nodes in index order become device nodes ---- */
-    std::vector<int32_t> dev_id(tn.size(), -1);
-    int32_t nint = 0;
-    for (size_t i = 0; i < tn.size(); i++) if (!tn[i].leaf) dev_id[i] = nint++;
-    auto ref_of = [&](int64_t i) -> int32_t {
-        const TNode& n = tn[(size_t)i];
-        if (n.leaf) return FM_LEAF_REF(n.a, n.b - n.a);
-        return dev_id[(size_t)i];
-    };
-    hs.nodes.clear();
-    hs.nodes4.clear();
-    if (hs.trav_width == 2) {
-        hs.nodes.assign((size_t)nint, fm_dnode{});
-        for (size_t i = 0; i < tn.size(); i++) {
-            const TNode& n = tn[i];
-            if (n.leaf) continue;
-            fm_dnode& o = hs.nodes[(size_t)dev_id[i]];
-            const float* a = &node_boxes[(size_t)n.a * 6];
-            const float* b = &node_boxes[(size_t)n.b * 6];
-            for (int k = 0; k < 6; k++) { o.b[k] = a[k]; o.b[6 + k] = b[k]; }
-            o.child[0] = ref_of(n.a);
-            o.child[1] = ref_of(n.b);
-        }
-        if (hs.nodes.size() * sizeof(fm_dnode) >= ((size_t)1 << 32))
-            throw BuildError(FM_ERR_UNSUPPORTED, "two-wide traversal tree of 4 GiB or more");
-        /* one entry pushed per inner node on the way down */
-        hs.trav_stack = std::max(1, tree_depth - 1);
-    } else if (G > 0 && !tn[0].leaf) {
-        /* Four-wide tree collapsed from the binary one: a node starts from
-         * the two children of a binary node and opens its inner child of
-         * largest surface area until it holds four children or only leaves.
-         * The traversal pushes at most (children - 1) refs per node on the
-         * way down, so the stack never holds more than the largest sum of
-         * (children - 1) over a root-to-leaf path. */
-        auto area = [&](int64_t i) {
-            const float* b = &node_boxes[(size_t)i * 6];
-            const double dx = (double)b[3] - b[0], dy = (double)b[4] - b[1], dz = (double)b[5] - b[2];
-            return dx * dy + dy * dz + dz * dx;
-        };
-        std::function<int32_t(int64_t, int&)> make4 = [&](int64_t bin, int& need) -> int32_t {
-            int64_t c[4] = {tn[(size_t)bin].a, tn[(size_t)bin].b, -1, -1};
-            int n = 2;
-            while (n < 4) {
-                int best = -1;
-                double ba = -1.0;
-                for (int k = 0; k < n; k++)
-                    if (!tn[(size_t)c[k]].leaf && area(c[k]) > ba) { ba = area(c[k]); best = k; }
-                if (best < 0) break;
-                const int64_t x = c[best];
-                c[best] = tn[(size_t)x].a;
-                c[n++] = tn[(size_t)x].b;
-            }
-            const int32_t id = (int32_t)hs.nodes4.size();
-            hs.nodes4.emplace_back();
-            int32_t refs[4] = {FM_NODE4_EMPTY, FM_NODE4_EMPTY, FM_NODE4_EMPTY, FM_NODE4_EMPTY};
-            int below = 0;
-            for (int k = 0; k < n; k++) {
-                if (tn[(size_t)c[k]].leaf) { refs[k] = ref_of(c[k]); continue; }
-                int nk = 0;
-                refs[k] = make4(c[k], nk);
-                below = std::max(below, nk);
-            }
-            fm_dnode4& o = hs.nodes4[(size_t)id];
-            for (int k = 0; k < 4; k++) {
-                o.child[k] = refs[k];
-                const float* b = k < n ? &node_boxes[(size_t)c[k] * 6] : nullptr;
-                o.lo_x[k] = b ? b[0] : 0.0f; o.lo_y[k] = b ? b[1] : 0.0f; o.lo_z[k] = b ? b[2] : 0.0f;
-                o.hi_x[k] = b ? b[3] : 0.0f; o.hi_y[k] = b ? b[4] : 0.0f; o.hi_z[k] = b ? b[5] : 0.0f;
-            }
-            need = (n - 1) + below;
-            return id;
-        };
-        int need = 0;
-        make4(0, need);
-        hs.trav_stack = std::max(1, need);
-    } else {
+    if (hs.device_tree) {
+        hs.leaf_boxes.swap(tight);
+        hs.nodes.clear();
+        hs.nodes4.clear();
+        hs.trav_depth = 0;
         hs.trav_stack = 1;
-    }
-    if (hs.trav_stack > FM_STACK_MAX) throw BuildError(FM_ERR_UNSUPPORTED, "traversal needs more stack than FM_STACK_MAX");
-    if (G > 0) {
-        hs.root_ref = hs.trav_width == 4 && !tn[0].leaf ? 0 : ref_of(0);
-        for (int k = 0; k < 6; k++) hs.root_box[k] = node_boxes[(size_t)k];
+        hs.root_ref = 0;
+    } else {
+        /* true bounds bottom-up (children have larger indices), padded by 2 ulps */
+        node_boxes.assign(tn.size() * 6, 0.0f);
+        for (size_t ii = tn.size(); ii-- > 0;) {
+            float* o = &node_boxes[ii * 6];
+            for (int k = 0; k < 3; k++) { o[k] = std::numeric_limits<float>::infinity(); o[3 + k] = -std::numeric_limits<float>::infinity(); }
+            auto merge = [&](const float* q) {
+                for (int k = 0; k < 3; k++) { o[k] = std::min(o[k], q[k]); o[3 + k] = std::max(o[3 + k], q[3 + k]); }
+            };
+            if (tn[ii].leaf) for (int64_t j = tn[ii].a; j < tn[ii].b; j++) merge(&tight[(size_t)slot_geom[(size_t)j] * 6]);
+            else { merge(&node_boxes[(size_t)tn[ii].a * 6]); merge(&node_boxes[(size_t)tn[ii].b * 6]); }
+        }
+        for (size_t ii = 0; ii < tn.size(); ii++)
+            for (int k = 0; k < 3; k++) {
+                float& lo = node_boxes[ii * 6 + k];
+                float& hi = node_boxes[ii * 6 + 3 + k];
+                lo = std::nextafter(std::nextafter(lo, -INFINITY), -INFINITY);
+                hi = std::nextafter(std::nextafter(hi, INFINITY), INFINITY);
+            }
+        int tree_depth = 0;
+        {
+            std::vector<int> dep(tn.size(), 1);
+            for (size_t ii = 0; ii < tn.size(); ii++) {
+                tree_depth = std::max(tree_depth, dep[ii]);
+                if (!tn[ii].leaf) { dep[(size_t)tn[ii].a] = dep[ii] + 1; dep[(size_t)tn[ii].b] = dep[ii] + 1; }
+            }
+        }
+        if (tree_depth > FM_STACK_DEPTH) throw BuildError(FM_ERR_UNSUPPORTED, "traversal tree deeper than the LDS stack");
+        for (const TNode& t : tn)
+            if (t.leaf && t.b - t.a > 7) throw BuildError(FM_ERR_UNSUPPORTED, "leaf larger than 7 primitives");
+        hs.trav_depth = tree_depth;
+
+        /* ---- flatten: interior nodes in index order become device nodes ---- */
+        std::vector<int32_t> dev_id(tn.size(), -1);
+        int32_t nint = 0;
+        for (size_t i = 0; i < tn.size(); i++) if (!tn[i].leaf) dev_id[i] = nint++;
+        auto ref_of = [&](int64_t i) -> int32_t {
+            const TNode& n = tn[(size_t)i];
+            if (n.leaf) return FM_LEAF_REF(n.a, n.b - n.a);
+            return dev_id[(size_t)i];
+        };
+        hs.nodes.clear();
+        hs.nodes4.clear();
+        if (hs.trav_width == 2) {
+            hs.nodes.assign((size_t)nint, fm_dnode{});
+            for (size_t i = 0; i < tn.size(); i++) {
+                const TNode& n = tn[i];
+                if (n.leaf) continue;
+                fm_dnode& o = hs.nodes[(size_t)dev_id[i]];
+                const float* a = &node_boxes[(size_t)n.a * 6];
+                const float* b = &node_boxes[(size_t)n.b * 6];
+                for (int k = 0; k < 6; k++) { o.b[k] = a[k]; o.b[6 + k] = b[k]; }
+                o.child[0] = ref_of(n.a);
+                o.child[1] = ref_of(n.b);
+            }
+            if (hs.nodes.size() * sizeof(fm_dnode) >= ((size_t)1 << 32))
+                throw BuildError(FM_ERR_UNSUPPORTED, "two-wide traversal tree of 4 GiB or more");
+            /* one entry pushed per inner node on the way down */
+            hs.trav_stack = std::max(1, tree_depth - 1);
+        } else if (G > 0 && !tn[0].leaf) {
+            /* Four-wide tree collapsed from the binary one: a node starts from
+             * the two children of a binary node and opens its inner child of
+             * largest surface area until it holds four children or only leaves.
+             * The traversal pushes at most (children - 1) refs per node on the
+             * way down, so the stack never holds more than the largest sum of
+             * (children - 1) over a root-to-leaf path. */
+            auto area = [&](int64_t i) {
+                const float* b = &node_boxes[(size_t)i * 6];
+                const double dx = (double)b[3] - b[0], dy = (double)b[4] - b[1], dz = (double)b[5] - b[2];
+                return dx * dy + dy * dz + dz * dx;
+            };
+            std::function<int32_t(int64_t, int&)> make4 = [&](int64_t bin, int& need) -> int32_t {
+                int64_t c[4] = {tn[(size_t)bin].a, tn[(size_t)bin].b, -1, -1};
+                int n = 2;
+                while (n < 4) {
+                    int best = -1;
+                    double ba = -1.0;
+                    for (int k = 0; k < n; k++)
+                        if (!tn[(size_t)c[k]].leaf && area(c[k]) > ba) { ba = area(c[k]); best = k; }
+                    if (best < 0) break;
+                    const int64_t x = c[best];
+                    c[best] = tn[(size_t)x].a;
+                    c[n++] = tn[(size_t)x].b;
+                }
+                const int32_t id = (int32_t)hs.nodes4.size();
+                hs.nodes4.emplace_back();
+                int32_t refs[4] = {FM_NODE4_EMPTY, FM_NODE4_EMPTY, FM_NODE4_EMPTY, FM_NODE4_EMPTY};
+                int below = 0;
+                for (int k = 0; k < n; k++) {
+                    if (tn[(size_t)c[k]].leaf) { refs[k] = ref_of(c[k]); continue; }
+                    int nk = 0;
+                    refs[k] = make4(c[k], nk);
+                    below = std::max(below, nk);
+                }
+                fm_dnode4& o = hs.nodes4[(size_t)id];
+                for (int k = 0; k < 4; k++) {
+                    o.child[k] = refs[k];
+                    const float* b = k < n ? &node_boxes[(size_t)c[k] * 6] : nullptr;
+                    o.lo_x[k] = b ? b[0] : 0.0f; o.lo_y[k] = b ? b[1] : 0.0f; o.lo_z[k] = b ? b[2] : 0.0f;
+                    o.hi_x[k] = b ? b[3] : 0.0f; o.hi_y[k] = b ? b[4] : 0.0f; o.hi_z[k] = b ? b[5] : 0.0f;
+                }
+                need = (n - 1) + below;
+                return id;
+            };
+            int need = 0;
+            make4(0, need);
+            hs.trav_stack = std::max(1, need);
+        } else {
+            hs.trav_stack = 1;
+        }
+        if (hs.trav_stack > FM_STACK_MAX) throw BuildError(FM_ERR_UNSUPPORTED, "traversal needs more stack than FM_STACK_MAX");
+        if (G > 0) {
+            hs.root_ref = hs.trav_width == 4 && !tn[0].leaf ? 0 : ref_of(0);
+            for (int k = 0; k < 6; k++) hs.root_box[k] = node_boxes[(size_t)k];
+        }
     }
 
     lap(hs.report.trav_tree_ms);

@@ -41,6 +41,14 @@ struct HostScene {
     int32_t bvh_depth = 0;          /* reference tree */
     int32_t trav_depth = 0;         /* traversal tree (binary form) */
     int32_t trav_width = 2;         /* set before the build: 2 or 4 */
+    int32_t builder = FM_BUILDER_HOST;  /* set before the build: FM_BUILDER_* */
+    bool device_tree = false;       /* the traversal tree is left to the device builder: slots
+                                       are in geometry order, leaf_boxes holds their bounds */
+    std::vector<float> leaf_boxes;  /* 6 floats per slot (device builder input) */
+    int64_t device_nodes = 0;       /* device-built tree: inner nodes */
+    int32_t build_iterations = 0;   /* device-built tree: clustering iterations */
+    double device_build_ms = 0.0;
+    uint64_t trav_hash = 0;         /* of the host-built traversal nodes and slot records (0: device-built) */
     int32_t trav_stack = 1;         /* LDS stack entries the traversal needs */
     int64_t num_geoms = 0;
 

@@ -126,6 +126,26 @@ struct AdjointArgs {
 hipError_t launch_adjoint(const fm_dscene& S, const AdjointArgs& A, int block, int grid, int stack_levels, hipStream_t st);
 int adjoint_blocks_per_cu(int block, size_t lds);
 
+/* Device traversal-tree build (bvh_build.hip) over n primitive boxes (6
+ * floats each: min xyz, max xyz, in primitive order).  On success the
+ * caller owns nodes (num_nodes two-wide nodes in pre-order, at least one
+ * allocated) and slot_src (n ints: the primitive at each leaf slot), both
+ * hipMalloc'd. */
+struct BvhBuildOut {
+    fm_dnode* nodes = nullptr;
+    int32_t* slot_src = nullptr;
+    int64_t num_nodes = 0;
+    int32_t root_ref = 0;
+    float root_box[6] = {0, 0, 0, 0, 0, 0};   /* padded by 2 ulps like the node boxes */
+    int32_t stack_levels = 1;                  /* LDS stack entries a traversal needs */
+    int32_t iterations = 0;                    /* clustering iterations */
+    double ms = 0.0;
+    const char* error = nullptr;               /* set with a failure that is not a HIP error */
+};
+hipError_t bvh_build_device(const float* boxes, int64_t n, BvhBuildOut& out, hipStream_t st);
+/* dst[s] = src[idx[s]] for n records of `floats` floats (a multiple of 4) */
+hipError_t launch_gather(const float* src, float* dst, const int32_t* idx, int64_t n, int floats, hipStream_t st);
+
 hipError_t launch_trace(const fm_dscene& S, const fm_ray* rays, int64_t n, int any, fm_hit* out, hipStream_t st);
 hipError_t launch_camera(const fm_dscene& S, const float* nl, int64_t n, fm_ray* out, hipStream_t st);
 hipError_t launch_bsdf(const fm_dscene& S, const fm_bsdf_query* q, int64_t n, fm_bsdf_result* out, hipStream_t st);

@@ -539,12 +539,27 @@ void launch_shades(const fm_dscene& S, const WfArgs& A, const WfLaunch& L, hipSt
 
 void wavefront_grids(int num_cus, int stack_levels, WfLaunch& L) {
     L.stack_levels = stack_levels;
-    if (L.block_trace <= 0) L.block_trace = 256;
+    auto trace_blocks = [&](int block) {
+        const size_t lds = (size_t)stack_levels * block * sizeof(uint32_t);
+        return L.width == 4 ? occupancy(k_wf_trace<false, 4>, block, lds) : occupancy(k_wf_trace<false, 2>, block, lds);
+    };
+    if (L.block_trace <= 0) {
+        /* The block (256 or 128) that keeps the most trace waves resident
+         * per CU: its LDS stacks grow with the block (and with the tree's
+         * depth), registers cap the waves; equal counts keep the larger
+         * block.  One-wave blocks are left out: with deep trees the
+         * occupancy query counts more of them than run at once, and a
+         * persistent grid whose last blocks wait for a free slot runs their
+         * whole share after the rest (1.5x the trace time, measured). */
+        int best = 256, waves = 0;
+        for (int b : {256, 128}) {
+            const int w = trace_blocks(b) * (b / 64);
+            if (w > waves) { waves = w; best = b; }
+        }
+        L.block_trace = best;
+    }
     if (L.block_shade <= 0) L.block_shade = 256;
-    const size_t lds = (size_t)stack_levels * L.block_trace * sizeof(uint32_t);
-    if (L.grid_trace <= 0)
-        L.grid_trace = num_cus * (L.width == 4 ? occupancy(k_wf_trace<false, 4>, L.block_trace, lds)
-                                               : occupancy(k_wf_trace<false, 2>, L.block_trace, lds));
+    if (L.grid_trace <= 0) L.grid_trace = num_cus * trace_blocks(L.block_trace);
     const int g = L.grid_shade;
     auto occ4 = [&](auto k0, auto k1, auto k2, auto k3) {
         L.grid_class[WC_END] = g > 0 ? g : num_cus * occupancy(k0, L.block_shade, 0);

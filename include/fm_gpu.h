@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define FM_ABI_VERSION 2u
+#define FM_ABI_VERSION 3u
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -232,7 +232,11 @@ typedef struct fm_scene_info {
     /* build report: wall time of the host build's phases and the upload */
     double build_ms_geometry, build_ms_ref_bvh, build_ms_trav_tree, build_ms_tables, build_ms_total, upload_ms;
     int64_t host_bytes;       /* host copy kept for the scene's lifetime */
-    uint64_t trav_hash;       /* FNV-1a of the traversal nodes and slot records */
+    uint64_t trav_hash;       /* FNV-1a of the traversal nodes and slot records (host-built trees;
+                                 0 for device-built ones) */
+    int32_t trav_builder;     /* FM_BUILDER_* that made the traversal tree */
+    int32_t build_iterations; /* device builder: clustering iterations */
+    double build_ms_device;   /* device builder: wall time of the build and the slot permutation */
 } fm_scene_info;
 int fm_scene_get_info(const fm_scene* scene, fm_scene_info* out);
 
@@ -397,6 +401,25 @@ int fm_set_counting(fm_ctx* ctx, int enabled);
  * hit does not depend on the width (ties are resolved by reference slot).  No counterpart in the reference (whose BVHAggregate
  * is binary, bvh_aggregate.cpp:38-79): a device traversal parameter. */
 int fm_set_bvh_width(fm_ctx* ctx, int32_t width);
+/* Traversal tree builders of fm_set_bvh_builder. */
+enum {
+    FM_BUILDER_HOST = 0,    /* binned SAH on the host build threads (default) */
+    FM_BUILDER_DEVICE = 1   /* on the GPU: PLOC clustering in 63-bit Morton order, SAH leaf
+                               collapse; two-wide trees only */
+};
+/* Builder of the traversal trees of the scenes created on this context
+ * afterwards.  It applies where the host would build its SAH tree (every
+ * accelerator but FM_ACCEL_BVH_REFERENCE_TREE); the device builder needs
+ * width 2.  Any tree gives the same closest hits: ties go by reference slot.
+ * No counterpart in the reference (BVHAggregate::Build runs on the host,
+ * bvh_aggregate.cpp:38-79). */
+int fm_set_bvh_builder(fm_ctx* ctx, int32_t builder);
+/* Copies the device's traversal tree: num_trav_nodes nodes of
+ * trav_node_bytes each into `nodes` (max_bytes bytes at most), and the
+ * geometry index at each slot into slot_geometry (max_slots at most).
+ * Either pointer may be NULL.  For tests of the builders. */
+int fm_scene_traversal(const fm_scene* scene, void* nodes, int64_t max_bytes,
+                       int32_t* slot_geometry, int64_t max_slots);
 
 /* ---- parity hooks (batched queries against the uploaded scene) ----------- */
 
