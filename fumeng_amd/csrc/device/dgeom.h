@@ -38,12 +38,25 @@ struct RayPre {
     int kx, ky, kz;
     float sx, sy, sz;
     v3 inv;          /* 1 / d per component */
-    bool fast;       /* no zero direction component: slab values cannot be NaN */
+    v3 clo, chi;     /* fast path: t of a lo / hi plane x is fma(x, inv, -c) */
+    bool fast;       /* no zero direction component and finite constants */
 };
 
 FMD int trunc_iabs(float x) { const int i = (int)x; return i < 0 ? -i : i; }
 
-FMD RayPre precompute(const Ray& r) {
+/* The fast slab test computes a plane's t as one fma, fma(x, inv, -c).
+ * With c = o inv, each value would be off from the exact (x - o) / d by at
+ * most eps (|x| + 2 |o|) |inv| (the fma's and o inv's roundings) plus
+ * eps |t| (inv's own), eps = 2^-24: below
+ * err_k = 4 eps (X + |o_k|) |inv_k| per axis, X = the largest root-box
+ * coordinate (every node box lies inside it), with room for rounding c
+ * itself.  The constants move the near plane of each axis (lo if inv > 0,
+ * else hi) by -err_k and the far plane by +err_k, so every slab interval
+ * contains the exact one: the test passes every box the ray meets in exact
+ * arithmetic and culls nothing the hit search needs.  The widening is
+ * 4 eps (X + |o|) in scene units, far below any primitive; the 2-ulp
+ * padding of the node boxes covers the rest as before. */
+FMD RayPre precompute(const Ray& r, const fm_dscene& S) {
 #pragma clang fp contract(off)
     RayPre p;
     const int a0 = trunc_iabs(r.d.x), a1 = trunc_iabs(r.d.y), a2 = trunc_iabs(r.d.z);
@@ -58,7 +71,22 @@ FMD RayPre precompute(const Ray& r) {
     p.sy = -dy / dz;
     p.sz = 1.0f / dz;
     p.inv = mk(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
-    p.fast = r.d.x != 0.0f && r.d.y != 0.0f && r.d.z != 0.0f;
+    float X = 0.0f;
+    for (int k = 0; k < 6; k++) X = fmaxf(X, fabsf(S.root_box[k]));
+    auto consts = [&](float o, float inv, float& clo, float& chi) {
+        const float oi = o * inv;
+        const float err = 4.0f * 5.9604645e-8f * (X + fabsf(o)) * fabsf(inv);
+        const float e = inv > 0.0f ? err : -err;   /* lo is the near plane when inv > 0 */
+        clo = oi + e;
+        chi = oi - e;
+    };
+    consts(r.o.x, p.inv.x, p.clo.x, p.chi.x);
+    consts(r.o.y, p.inv.y, p.clo.y, p.chi.y);
+    consts(r.o.z, p.inv.z, p.clo.z, p.chi.z);
+    const float cmax = fmaxf(fmaxf(fmaxf(fabsf(p.clo.x), fabsf(p.clo.y)), fmaxf(fabsf(p.clo.z), fabsf(p.chi.x))),
+                             fmaxf(fabsf(p.chi.y), fabsf(p.chi.z)));
+    p.fast = r.d.x != 0.0f && r.d.y != 0.0f && r.d.z != 0.0f && cmax < INFINITY &&
+             fabsf(p.inv.x) < INFINITY && fabsf(p.inv.y) < INFINITY && fabsf(p.inv.z) < INFINITY;
     return p;
 }
 
@@ -122,14 +150,16 @@ template <bool FAST>
 FMD bool box_hit6(float x0, float y0, float z0, float x1, float y1, float z1, const Ray& r, const RayPre& rp, float tmax,
                   float& tnear) {
 #pragma clang fp contract(off)
-    const float lx = rp.inv.x * (x0 - r.o.x), ly = rp.inv.y * (y0 - r.o.y), lz = rp.inv.z * (z0 - r.o.z);
-    const float hx = rp.inv.x * (x1 - r.o.x), hy = rp.inv.y * (y1 - r.o.y), hz = rp.inv.z * (z1 - r.o.z);
     if (FAST) {
+        const float lx = fmaf(x0, rp.inv.x, -rp.clo.x), ly = fmaf(y0, rp.inv.y, -rp.clo.y), lz = fmaf(z0, rp.inv.z, -rp.clo.z);
+        const float hx = fmaf(x1, rp.inv.x, -rp.chi.x), hy = fmaf(y1, rp.inv.y, -rp.chi.y), hz = fmaf(z1, rp.inv.z, -rp.chi.z);
         const float n = fmaxf(fmaxf(r.tmin, fminf(lx, hx)), fmaxf(fminf(ly, hy), fminf(lz, hz)));
         const float f = fminf(fminf(tmax, fmaxf(lx, hx)), fminf(fmaxf(ly, hy), fmaxf(lz, hz)));
         tnear = n;
         return n <= f;
     } else {
+        const float lx = rp.inv.x * (x0 - r.o.x), ly = rp.inv.y * (y0 - r.o.y), lz = rp.inv.z * (z0 - r.o.z);
+        const float hx = rp.inv.x * (x1 - r.o.x), hy = rp.inv.y * (y1 - r.o.y), hz = rp.inv.z * (z1 - r.o.z);
         /* std::min(a, b) = (b < a) ? b : a ; std::max(a, b) = (a < b) ? b : a */
         const float ax = (hx < lx) ? hx : lx, ay = (hy < ly) ? hy : ly, az = (hz < lz) ? hz : lz;
         const float bx = (lx < hx) ? hx : lx, by = (ly < hy) ? hy : ly, bz = (lz < hz) ? hz : lz;
@@ -401,7 +431,7 @@ FMD Hit trace_closest(const fm_dscene& S, const Ray& r, uint32_t* stk, int strid
     Hit h;
     h.t = r.tmax; h.slot = -1; h.kind = -1; h.tie = 0; h.half = 0; h.b0 = h.b1 = h.b2 = 0.0f; h.info = 0;
     if (S.num_slots == 0) return h;
-    const RayPre rp = precompute(r);
+    const RayPre rp = precompute(r, S);
     Cnt dummy;
     Cnt& c = COUNT ? *cn : dummy;
     if (WIDTH == 4 || (WIDTH == 0 && S.width == 4)) {
@@ -417,7 +447,7 @@ FMD Hit trace_closest(const fm_dscene& S, const Ray& r, uint32_t* stk, int strid
 template <bool COUNT = false, int WIDTH = 0>
 FMD bool trace_any(const fm_dscene& S, const Ray& r, uint32_t* stk, int stride, Cnt* cn = nullptr) {
     if (S.num_slots == 0) return false;
-    const RayPre rp = precompute(r);
+    const RayPre rp = precompute(r, S);
     Cnt dummy;
     Cnt& c = COUNT ? *cn : dummy;
     Hit h;
