@@ -98,8 +98,22 @@ int main(int argc, char** argv) {
     if (argc < 2) { fprintf(stderr, "Please input the scene file.\n"); return -1; }
     const std::string scene = argv[1];
     if (scene.find(".json") == std::string::npos) { fprintf(stderr, "Please specify the correct scene file.\n"); return -1; }
+    /* optional, after the scene file: --bvh-builder host|device, --bvh-width 2|4
+     * (traversal tree of the scene; the image does not depend on them) */
+    int builder = FM_BUILDER_HOST, width = 2;
+    for (int i = 2; i + 1 < argc; i += 2) {
+        const std::string k = argv[i], v = argv[i + 1];
+        if (k == "--bvh-builder" && (v == "host" || v == "device")) builder = v == "device" ? FM_BUILDER_DEVICE : FM_BUILDER_HOST;
+        else if (k == "--bvh-width" && (v == "2" || v == "4")) width = v == "4" ? 4 : 2;
+        else { fprintf(stderr, "unknown option %s %s\n", k.c_str(), v.c_str()); return -1; }
+    }
     fm_ctx* ctx = nullptr;
     if (fm_init(0, &ctx) != FM_OK) { fprintf(stderr, "fm_init: %s\n", fm_last_error(nullptr)); return -1; }
+    if (fm_set_bvh_builder(ctx, builder) != FM_OK || fm_set_bvh_width(ctx, width) != FM_OK) {
+        fprintf(stderr, "options: %s\n", fm_last_error(ctx));
+        fm_shutdown(ctx);
+        return -1;
+    }
     fm_scene* s = nullptr;
     if (fm_scene_load(ctx, scene.c_str(), &s) != FM_OK) { fprintf(stderr, "load: %s\n", fm_last_error(ctx)); fm_shutdown(ctx); return -1; }
     fm_scene_info info;

@@ -142,3 +142,17 @@ def test_front_end_writes_the_reference_output_files(tmp_path):
         assert np.all(np.abs(sc - np.floor(sc) - 0.5) <= 8 * np.spacing(sc))
     for aov in ("shot_albedo.png", "shot_normal.png"):
         assert _read_png(str(tmp_path / aov)).shape == (24, 40, 3)
+    # the device-built traversal tree: the same film but for the pixels
+    # whose paths met a ray the reference's truncated shear makes sensitive
+    # (0 < |d.z| < 1e-2: ~400 rays per pixel here, and the box's walls send
+    # many such rays); the image means agree within 2 %
+    d2 = tmp_path / "device"
+    d2.mkdir()
+    path2 = s.write(str(d2))
+    r = subprocess.run([exe, path2, "--bvh-builder", "device"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    film2 = _read_pfm(str(d2 / "shot.pfm"))
+    assert np.mean(np.any(film2 != film, axis=-1)) <= 0.10
+    assert abs(film2.mean() - film.mean()) <= 0.02 * film.mean()
+    r = subprocess.run([exe, path2, "--bvh-builder", "gpu"], capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "unknown option" in r.stderr
