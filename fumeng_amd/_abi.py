@@ -21,7 +21,7 @@ FM_TEX_CONSTANT, FM_TEX_IMAGE = 0, 1
 FM_WRAP_REPEAT, FM_WRAP_CLAMP = 0, 1
 FM_SAMPLER_LINEAR, FM_SAMPLER_NEAREST = 0, 1
 FM_IMAGE_F32_LINEAR, FM_IMAGE_U8_SRGB = 0, 1
-FM_MAT_LAMBERT, FM_MAT_DISNEY, FM_MAT_SPECULAR_TRANSMISSION = 0, 1, 2
+FM_MAT_LAMBERT, FM_MAT_DISNEY, FM_MAT_SPECULAR_TRANSMISSION, FM_MAT_SPECULAR_REFLECTION = 0, 1, 2, 3
 DISNEY_PARAMS = ["basecolor", "metallic", "specular", "specularTint", "roughness", "anisotropic", "sheen",
                  "sheenTint", "clearcoat", "clearcoatGloss", "specTrans", "specTransRoughness", "diffTrans",
                  "flatness", "ior"]
@@ -51,7 +51,8 @@ class fm_image_desc(C.Structure):
 
 class fm_material_desc(C.Structure):
     _fields_ = [("type", C.c_int32), ("thin", C.c_int32), ("tex", fm_texture_desc * FM_D_NPARAM),
-                ("reflect_color", F3), ("refract_color", F3), ("eta_i", C.c_float), ("eta_t", C.c_float)]
+                ("reflect_color", F3), ("refract_color", F3), ("eta_i", C.c_float), ("eta_t", C.c_float),
+                ("cond_eta_i", F3), ("cond_eta_t", F3), ("cond_k", F3)]
 
 
 class fm_mesh_desc(C.Structure):

@@ -429,7 +429,7 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
     L.block_trace = block; L.block_shade = block;
     L.grid_trace = grid; L.grid_shade = grid;
     for (const fm_dmaterial& m : s->hs.materials)
-        if (m.type >= 0 && m.type + 1 < fm::WQ_NCLS) L.has_class[m.type + 1] = true;
+        if (m.type >= 0 && FM_SHADE_CLASS(m.type) < fm::WQ_NCLS) L.has_class[FM_SHADE_CLASS(m.type)] = true;
     {
         bool env = false, area = false, other = false;
         for (const fm_dlight& l : s->hs.lights) {

@@ -143,12 +143,37 @@ FMD float lerp1(float a, float b, float t) { return (1.0f - t) * a + t * b; }
 
 struct BsdfSample { v3 f, wi; float pdf; bool delta; };
 
+/* ConductorFresnel::CalFr (fresnel.h:90-108), one channel, operation for
+ * operation: eta2 = (eta_t / eta_i)^2, etak2 = (k / eta_i)^2. */
+FMD float conductor_fr(float eta2, float etak2, float cos_i) {
+#pragma clang fp contract(off)
+    if (cos_i <= 0.0f) return 0.0f;
+    const float cos2 = cos_i * cos_i;
+    const float sin2 = (0.0f < 1.0f - cos2) ? 1.0f - cos2 : 0.0f;
+    const float t0 = eta2 - etak2 - sin2;
+    const float a2b2 = sqrtf(t0 * t0 + 4.0f * eta2 * etak2);
+    const float t1 = a2b2 + cos2;
+    const float a = sqrtf(0.5f * (a2b2 + t0));
+    const float t2 = 2.0f * cos_i * a;
+    const float rs = (t1 - t2) / (t1 + t2);
+    const float t3 = cos2 * a2b2 + sin2 * sin2;
+    const float t4 = t2 * sin2;
+    const float rp = rs * (t3 - t4) / (t3 + t4);
+    return (rp + rs) * 0.5f;
+}
+
+/* make_bsdf's TYPE for a kernel that shades both delta materials
+ * (specular transmission and the conductor mirror), chosen per hit */
+#define FM_BSDF_DELTA 16
+
 struct Bsdf {
     int type;
     v3 xs, ys, zs;
     v3 base;
     /* specular transmission */
     v3 refl, refr; float eta_i, eta_t;
+    /* specular reflection (conductor) */
+    v3 c_eta2, c_etak2;
     /* Disney */
     float metallic, specular, specularTint, roughness, sheen, sheenTint, clearcoat, specTrans, diffTrans, flatness, ior;
     float anisotropic, cc_gloss, st_rough;   /* parameters only the derived terms use */
@@ -263,7 +288,7 @@ struct Bsdf {
     /* ---- BSDF interface ---- */
     FMD v3 eval_local(v3 wo, v3 wi) const {
         if (type == FM_MAT_LAMBERT) return base * FM_INV_PI;
-        if (type == FM_MAT_SPECULAR_TRANSMISSION) return splat(0.0f);
+        if (type == FM_MAT_SPECULAR_TRANSMISSION || type == FM_MAT_SPECULAR_REFLECTION) return splat(0.0f);
         v3 ret = splat(0.0f);
         if (wo.z * wi.z < 0) {
             if (thin) {
@@ -285,7 +310,7 @@ struct Bsdf {
     FMD float rt_fr(v3 wo) const { const float f = fr(wo.z); return 1.0f - ((1.0f - f) * specTrans * (1.0f - metallic)); }
     FMD float pdf_local(v3 wo, v3 wi) const {
         if (type == FM_MAT_LAMBERT) return wo.z * wi.z < 0.0f ? 0.0f : fabsf(wi.z) * FM_INV_PI;
-        if (type == FM_MAT_SPECULAR_TRANSMISSION) return 0.0f;
+        if (type == FM_MAT_SPECULAR_TRANSMISSION || type == FM_MAT_SPECULAR_REFLECTION) return 0.0f;
         float p = 0.0f;
         if (w_dr > 0.0f) p += w_dr * diff_pdf(wo, wi);
         if (w_cc > 0.0f) p += w_cc * cc_pdf(wo, wi);
@@ -307,6 +332,19 @@ struct Bsdf {
             v3 wi = cosine_hemisphere(s.x, s.y);
             if (wo.z * wi.z < 0.0f) wi.z *= -1.0f;
             o.pdf = pdf_local(wo, wi); o.f = eval_local(wo, wi); o.wi = world(wi); o.delta = false;
+            return o;
+        }
+        if (type == FM_MAT_SPECULAR_REFLECTION) {
+            /* SpecularReflectionBSDF::SampleBSDF (specular_reflection.cpp:27-36) */
+#pragma clang fp contract(off)
+            o.delta = true;
+            const v3 wi = mk(-wo.x, -wo.y, wo.z);
+            const float ci = normalize(wi).z;
+            const float ac = fabsf(ci);
+            o.f = mk(conductor_fr(c_eta2.x, c_etak2.x, ci) * base.x / ac, conductor_fr(c_eta2.y, c_etak2.y, ci) * base.y / ac,
+                     conductor_fr(c_eta2.z, c_etak2.z, ci) * base.z / ac);
+            o.wi = world(wi);
+            o.pdf = 1.0f;
             return o;
         }
         if (type == FM_MAT_SPECULAR_TRANSMISSION) {
@@ -412,7 +450,9 @@ template <int TYPE = -1>
 FMD Bsdf make_bsdf(const fm_dscene& S, int mat, v3 ns, v3 ss, v3 wo_w, float u, float v) {
     Bsdf b;
     const fm_dmaterial& M = S.materials[mat];
-    const int type = TYPE >= 0 ? TYPE : M.type;
+    const int type = TYPE == FM_BSDF_DELTA
+                         ? (M.type == FM_MAT_SPECULAR_REFLECTION ? FM_MAT_SPECULAR_REFLECTION : FM_MAT_SPECULAR_TRANSMISSION)
+                         : (TYPE >= 0 ? TYPE : M.type);
     b.type = type;
     b.zs = ns;
     b.xs = normalize(ss);
@@ -420,6 +460,10 @@ FMD Bsdf make_bsdf(const fm_dscene& S, int mat, v3 ns, v3 ss, v3 wo_w, float u, 
     if (type == FM_MAT_LAMBERT) { b.base = tex_sample(S, M.tex[0], u, v); return b; }
     if (type == FM_MAT_SPECULAR_TRANSMISSION) {
         b.refl = ld3(M.refl); b.refr = ld3(M.refr); b.eta_i = M.eta_i; b.eta_t = M.eta_t; b.base = b.refl;
+        return b;
+    }
+    if (type == FM_MAT_SPECULAR_REFLECTION) {   /* SpecularReflection::CreateBSDF (specular_reflection.cpp:54-58) */
+        b.base = ld3(M.refl); b.refl = b.base; b.c_eta2 = ld3(M.c_eta2); b.c_etak2 = ld3(M.c_etak2);
         return b;
     }
     b.base = tex_sample(S, M.tex[FM_D_BASECOLOR], u, v);

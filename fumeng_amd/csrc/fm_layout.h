@@ -23,8 +23,9 @@
 #define FM_META_HAS_NORMAL 1
 
 /* w of a slot's first float4: FM_PRIM_* | shading class << 8 (1 + material
- * type) | emissive << 16, so that a closest hit is classified without
- * reading the slot's meta and material */
+ * type; both delta materials share class 3) | emissive << 16, so that a
+ * closest hit is classified without reading the slot's meta and material */
+#define FM_SHADE_CLASS(mat_type) ((mat_type) == 3 ? 3 : 1 + (mat_type))
 #define FM_PRIM_KIND(w) ((w) & 0xff)
 #define FM_PRIM_INFO(w) ((w) >> 8)
 
@@ -78,6 +79,7 @@ typedef struct fm_dmaterial {
     fm_dtex tex[15];
     float refl[4], refr[4];
     float eta_i, eta_t, pad2[2];
+    float c_eta2[4], c_etak2[4];   /* conductor: (eta_t / eta_i)^2, (k / eta_i)^2 per channel */
 } fm_dmaterial;
 
 #define FM_DLIGHT_TRI 0

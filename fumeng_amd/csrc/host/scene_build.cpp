@@ -1028,7 +1028,7 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
         {
             /* the material table is built below; its types are the descriptor's */
             const int32_t mt = ge.material < d.num_materials ? d.materials[ge.material].type : FM_MAT_LAMBERT;
-            const int32_t cls = 1 + mt;
+            const int32_t cls = FM_SHADE_CLASS(mt);
             pr[3] = ibits(ge.type | (cls << 8) | ((ge.light >= 0 ? 1 : 0) << 16));
         }
         int32_t* me = &hs.meta[p * 4];
@@ -1090,7 +1090,8 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
             m.tex[0].value[0] = m.tex[0].value[1] = m.tex[0].value[2] = 1.0f;
         } else {
             const fm_material_desc& s = d.materials[i];
-            if (s.type != FM_MAT_LAMBERT && s.type != FM_MAT_DISNEY && s.type != FM_MAT_SPECULAR_TRANSMISSION)
+            if (s.type != FM_MAT_LAMBERT && s.type != FM_MAT_DISNEY && s.type != FM_MAT_SPECULAR_TRANSMISSION &&
+                s.type != FM_MAT_SPECULAR_REFLECTION)
                 throw BuildError(FM_ERR_INVALID, "material type");
             m.type = s.type; m.thin = s.thin ? 1 : 0;
             for (int k = 0; k < 15; k++) {
@@ -1099,6 +1100,16 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
             }
             for (int c = 0; c < 3; c++) { m.refl[c] = s.reflect_color[c]; m.refr[c] = s.refract_color[c]; }
             m.eta_i = s.eta_i; m.eta_t = s.eta_t;
+            if (s.type == FM_MAT_SPECULAR_REFLECTION)
+                for (int c = 0; c < 3; c++) {
+                    /* the ConductorFresnel constructor's terms, in FP32 (fresnel.h:77-84) */
+                    float e = s.cond_eta_t[c] / s.cond_eta_i[c];
+                    e *= e;
+                    float k = s.cond_k[c] / s.cond_eta_i[c];
+                    k *= k;
+                    m.c_eta2[c] = e;
+                    m.c_etak2[c] = k;
+                }
         }
         hs.materials.push_back(m);
     }

@@ -410,7 +410,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
             const Ray tr = load_ray(A, p);
             const Hit h = load_hit(A, p);
             const Surf sf = fill_hit(S, tr, h);
-            const Bsdf b = make_bsdf<C - 1>(S, sf.material, sf.ns, sf.ss, sf.wo, sf.u, sf.v);
+            const Bsdf b = make_bsdf<C == fm::WC_SPEC ? FM_BSDF_DELTA : C - 1>(S, sf.material, sf.ns, sf.ss, sf.wo, sf.u, sf.v);
             if (depth == 1) {
                 reinterpret_cast<float4*>(A.out_alb)[p] = f4(b.albedo(), 0.0f);
                 reinterpret_cast<float4*>(A.out_nrm)[p] = f4(sf.ng, 0.0f);

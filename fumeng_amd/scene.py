@@ -72,6 +72,17 @@ class SceneDesc:
         self.material_names.append(name or "mat%d" % len(self.materials))
         return len(self.materials) - 1
 
+    def add_conductor(self, color=1.0, eta_i=1.0, eta_t=(0.2, 0.92, 1.1), k=(3.9, 2.45, 2.14), name=None):
+        """A conductor mirror (the reference's CreateSpecularReflection:
+        colour, per-channel indices eta_i / eta_t and absorption k).  The
+        scene JSON has no such material type: the reference loader, and this
+        package's, warn and skip it; it exists for scenes built in code."""
+        f3 = lambda v: np.broadcast_to(np.float32(v), (3,)).tolist()
+        self.materials.append({"type": "specular_reflection", "reflect_color": f3(color), "eta_i": f3(eta_i),
+                               "eta_t": f3(eta_t), "k": f3(k)})
+        self.material_names.append(name or "mat%d" % len(self.materials))
+        return len(self.materials) - 1
+
     def add_disney(self, thin=False, name=None, **params):
         m = {"type": "disney", "thin": bool(thin)}
         for k in A.DISNEY_PARAMS:
@@ -239,6 +250,9 @@ class SceneDesc:
                 for k in A.DISNEY_PARAMS:
                     j[k] = tex_json(m[k])
                 j["thin"] = m["thin"]
+            elif m["type"] == "specular_reflection":
+                j.update({"reflect_color": [_f(x) for x in m["reflect_color"]], "eta_i": [_f(x) for x in m["eta_i"]],
+                          "eta_t": [_f(x) for x in m["eta_t"]], "k": [_f(x) for x in m["k"]]})
             else:
                 j.update({"eta_i": _f(m["eta_i"]), "eta_t": _f(m["eta_t"]),
                           "reflect_color": [_f(x) for x in m["reflect_color"]],
@@ -309,6 +323,12 @@ def _fill_material(x, m):
         x.thin = 1 if m["thin"] else 0
         for k, key in enumerate(A.DISNEY_PARAMS):
             tex(x.tex[k], m[key])
+    elif m["type"] == "specular_reflection":
+        x.type = A.FM_MAT_SPECULAR_REFLECTION
+        x.reflect_color[:] = m["reflect_color"]
+        x.cond_eta_i[:] = m["eta_i"]
+        x.cond_eta_t[:] = m["eta_t"]
+        x.cond_k[:] = m["k"]
     else:
         x.type = A.FM_MAT_SPECULAR_TRANSMISSION
         x.eta_i, x.eta_t = m["eta_i"], m["eta_t"]

@@ -81,7 +81,7 @@ typedef struct fm_image_desc {
 
 /* Materials: engine/include/engine/core/material.h:10-76 and
  * engine/include/engine/create/scene_loader.h:177-200. */
-enum { FM_MAT_LAMBERT = 0, FM_MAT_DISNEY = 1, FM_MAT_SPECULAR_TRANSMISSION = 2 };
+enum { FM_MAT_LAMBERT = 0, FM_MAT_DISNEY = 1, FM_MAT_SPECULAR_TRANSMISSION = 2, FM_MAT_SPECULAR_REFLECTION = 3 };
 enum {
     FM_D_BASECOLOR = 0, FM_D_METALLIC, FM_D_SPECULAR, FM_D_SPECULAR_TINT,
     FM_D_ROUGHNESS, FM_D_ANISOTROPIC, FM_D_SHEEN, FM_D_SHEEN_TINT,
@@ -93,10 +93,18 @@ typedef struct fm_material_desc {
     int32_t type;              /* FM_MAT_* */
     int32_t thin;              /* Disney `thin` */
     fm_texture_desc tex[FM_D_NPARAM]; /* Lambert uses tex[0] = basecolor */
-    float reflect_color[3];    /* specular_transmission */
+    float reflect_color[3];    /* specular_transmission; specular_reflection's colour */
     float refract_color[3];
     float eta_i;
     float eta_t;
+    /* specular_reflection: a conductor mirror (CreateSpecularReflection,
+     * specular_reflection.cpp:46-65, ConductorFresnel fresnel.h:70-110) with
+     * per-channel indices.  No scene-JSON material type makes one
+     * (scene_loader.h:177-200 warns and skips it): it is reachable from code,
+     * as in the reference. */
+    float cond_eta_i[3];
+    float cond_eta_t[3];
+    float cond_k[3];
 } fm_material_desc;
 
 /* Triangle mesh after OBJ parsing (tinyobj attrib_t + index_t triples).

@@ -41,6 +41,7 @@ def lib():
             "or_philox": (None, [P, P, P]),
             "or_abi_version": (C.c_int, []),
             "or_post_process": (None, [C.POINTER(A.fm_post_op), C.c_int, P, I64, P, P]),
+            "or_conductor_fr": (None, [P, P, P, P, I64, C.c_int, P]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -136,6 +137,16 @@ def philox(ctr, key):
     o = np.zeros(4, np.uint32)
     lib().or_philox(_p(c), _p(k), _p(o))
     return o
+
+
+def conductor_fr(eta_i, eta_t, k, cos_i, fp64=False):
+    """ConductorFresnel::CalFr per channel for an array of cosines: (n, 3)."""
+    f3 = lambda v: np.ascontiguousarray(np.broadcast_to(np.float32(v), (3,)))
+    a, b, c = f3(eta_i), f3(eta_t), f3(k)
+    cs = np.ascontiguousarray(cos_i, np.float64).ravel()
+    out = np.zeros((cs.size, 3), np.float64)
+    lib().or_conductor_fr(_p(a), _p(b), _p(c), _p(cs), cs.size, 1 if fp64 else 0, _p(out))
+    return out
 
 
 def post_process(ops, rgb):

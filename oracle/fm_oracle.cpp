@@ -119,6 +119,9 @@ template <typename R> struct Scene {
             o.refl = V3<R>(m.reflect_color[0], m.reflect_color[1], m.reflect_color[2]);
             o.refr = V3<R>(m.refract_color[0], m.refract_color[1], m.refract_color[2]);
             o.eta_i = m.eta_i; o.eta_t = m.eta_t;
+            o.c_eta_i = V3<R>(m.cond_eta_i[0], m.cond_eta_i[1], m.cond_eta_i[2]);
+            o.c_eta_t = V3<R>(m.cond_eta_t[0], m.cond_eta_t[1], m.cond_eta_t[2]);
+            o.c_k = V3<R>(m.cond_k[0], m.cond_k[1], m.cond_k[2]);
         }
         /* default_material = LambertDiffuse(LambertDiffuseConfig()): white */
         Material<R>& dm = materials[d.num_materials];
@@ -616,6 +619,21 @@ void or_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) { philox
 int or_abi_version(void) { return (int)FM_ABI_VERSION; }
 /* PostProcess::Process over the colour image in list order, then
  * Image::save_to_file's byte conversion (image.cpp:97-110). */
+/* ConductorFresnel (fresnel.h:70-110) for n cosines, per channel: out is
+ * 3 doubles per cosine (FP32 or FP64 arithmetic). */
+void or_conductor_fr(const float* eta_i, const float* eta_t, const float* k, const double* cos_i, int64_t n, int fp64,
+                     double* out) {
+    if (fp64) {
+        const ConductorFr<double> f(V3<double>(eta_i[0], eta_i[1], eta_i[2]), V3<double>(eta_t[0], eta_t[1], eta_t[2]),
+                                    V3<double>(k[0], k[1], k[2]));
+        for (int64_t i = 0; i < n; i++) { const V3<double> r = f.CalFr(cos_i[i]); out[3 * i] = r.x; out[3 * i + 1] = r.y; out[3 * i + 2] = r.z; }
+    } else {
+        const ConductorFr<float> f(V3<float>(eta_i[0], eta_i[1], eta_i[2]), V3<float>(eta_t[0], eta_t[1], eta_t[2]),
+                                   V3<float>(k[0], k[1], k[2]));
+        for (int64_t i = 0; i < n; i++) { const V3<float> r = f.CalFr((float)cos_i[i]); out[3 * i] = r.x; out[3 * i + 1] = r.y; out[3 * i + 2] = r.z; }
+    }
+}
+
 void or_post_process(const fm_post_op* ops, int n_ops, const float* in, int64_t n, float* out, uint8_t* out8) {
     for (int64_t i = 0; i < n; i++) {
         float c = in[i];

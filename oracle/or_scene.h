@@ -389,6 +389,36 @@ template <typename R> inline R DielectricFr(R eta_i, R eta_t, R cos_i) {
     return (para * para + perp * perp) * (R)0.5;
 }
 
+/* core/fresnel.h:70-110 ConductorFresnel: constructor terms and CalFr, per
+ * channel */
+template <typename R> struct ConductorFr {
+    V3<R> eta2, etak2;
+    ConductorFr(V3<R> eta_i, V3<R> eta_t, V3<R> k) {
+        eta2 = eta_t / eta_i;
+        eta2 = eta2 * eta2;
+        etak2 = k / eta_i;
+        etak2 = etak2 * etak2;
+    }
+    static R One(R e2, R k2, R cos_i) {
+        const R cos2 = cos_i * cos_i;
+        const R sin2 = std::max(R(0), R(1) - cos2);
+        const R t0 = e2 - k2 - sin2;
+        const R a2b2 = std::sqrt(t0 * t0 + R(4) * e2 * k2);
+        const R t1 = a2b2 + cos2;
+        const R a = std::sqrt(R(0.5) * (a2b2 + t0));
+        const R t2 = R(2) * cos_i * a;
+        const R rs = (t1 - t2) / (t1 + t2);
+        const R t3 = cos2 * a2b2 + sin2 * sin2;
+        const R t4 = t2 * sin2;
+        const R rp = rs * (t3 - t4) / (t3 + t4);
+        return (rp + rs) * R(0.5);
+    }
+    V3<R> CalFr(R cos_i) const {
+        if (cos_i <= R(0)) return V3<R>(R(0));
+        return V3<R>(One(eta2.x, etak2.x, cos_i), One(eta2.y, etak2.y, cos_i), One(eta2.z, etak2.z, cos_i));
+    }
+};
+
 /* src/material/material_common.cpp:112-125 */
 template <typename R> inline bool RefractDir(V3<R> wi, V3<R> n, R eta, V3<R>* out) {
     const R cos_i = dot(n, wi);
@@ -430,6 +460,26 @@ template <typename R> struct SpecTransBSDF : BSDF<R> {
         return o;
     }
     V3<R> GetAlbedo() const override { return refl; }
+};
+
+/* src/material/specular_reflection.cpp:6-44: conductor mirror, delta */
+template <typename R> struct SpecReflBSDF : BSDF<R> {
+    V3<R> base; ConductorFr<R> fresnel;
+    SpecReflBSDF(const HitPoint<R>& h, V3<R> c, const ConductorFr<R>& f) : BSDF<R>(h), base(c), fresnel(f) {}
+    V3<R> CalFuncLocal(V3<R>, V3<R>) const override { return V3<R>(R(0)); }
+    R PdfLocal(V3<R>, V3<R>) const override { return R(0); }
+    SampleInfo<R> SampleBSDF(V3<R> wo_w, V3<R>) const override {
+        const V3<R> wo = this->ToLocal(wo_w);
+        const V3<R> wi(-wo.x, -wo.y, wo.z);
+        const R ci = CosDir(wi);
+        SampleInfo<R> o;
+        o.f = fresnel.CalFr(ci) * base / V3<R>(std::abs(ci));
+        o.wi = this->ToWorld(wi);
+        o.pdf = R(1);
+        o.delta = true;
+        return o;
+    }
+    V3<R> GetAlbedo() const override { return base; }
 };
 
 /* src/material/material_common.cpp:7-90 (literal transcendental forms) */
@@ -715,9 +765,12 @@ template <typename R> struct Material {
     bool thin = false;
     Texture<R> tex[FM_D_NPARAM];
     V3<R> refl, refr; R eta_i = 0, eta_t = 0;
+    V3<R> c_eta_i, c_eta_t, c_k;   /* specular_reflection (conductor) */
     std::unique_ptr<BSDF<R>> CreateBSDF(const HitPoint<R>& h) const {
         if (type == FM_MAT_LAMBERT) return std::unique_ptr<BSDF<R>>(new LambertBSDF<R>(h, tex[0].Sample(h.uv)));
         if (type == FM_MAT_SPECULAR_TRANSMISSION) return std::unique_ptr<BSDF<R>>(new SpecTransBSDF<R>(h, refl, refr, eta_i, eta_t));
+        if (type == FM_MAT_SPECULAR_REFLECTION)
+            return std::unique_ptr<BSDF<R>>(new SpecReflBSDF<R>(h, refl, ConductorFr<R>(c_eta_i, c_eta_t, c_k)));
         /* disney.cpp:627-643: metallic from .z, roughness from .y, others .x */
         R p[15];
         const V3<R> bc = tex[FM_D_BASECOLOR].Sample(h.uv);

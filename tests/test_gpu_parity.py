@@ -190,11 +190,35 @@ def _bsdf_case(g, o32, o64, q):
 
 
 def test_lambert_and_dielectric_bsdf_parity(gpu_ctx):
+    """Lambert, dielectric, the conductor mirror (a material only code can
+    make, as in the reference) and the default material."""
     s = scenes.box_scene()
+    cond = s.add_conductor(color=[0.9, 0.7, 0.5], eta_i=[1.0, 1.0, 1.2])
     g, o32, o64 = gpu_ctx.scene(s), oracle.OracleScene(s), oracle.OracleScene(s, fp64=True)
     rng = np.random.default_rng(26)
     for mat in (0, 1, 3, -1):
         _bsdf_case(g, o32, o64, bsdf_queries(rng, 4096, mat))
+    _bsdf_case(g, o32, o64, bsdf_queries(np.random.default_rng(27), 4096, cond))
+
+
+@pytest.mark.parametrize("mode", [A.FM_MODE_MEGAKERNEL, A.FM_MODE_WAVEFRONT])
+def test_conductor_sphere_render_matches_oracle(gpu_ctx, mode):
+    """The box with its Lambert sphere made a copper mirror and the glass
+    sphere kept: both delta materials share the wavefront's specular
+    class.  Common random numbers: most pixels equal the FP32 oracle's to
+    1e-3, the means agree within 2 %."""
+    s = scenes.box_scene(width=32, height=32, spp=16)
+    cond = s.add_conductor(color=[0.95, 0.64, 0.54])
+    spheres = [o for o in s.objects if o["shape"] == "sphere"]
+    spheres[0]["material"] = cond
+    g = gpu_ctx.scene(s)
+    cg, ag, _, _, st = g.render(seed=7, mode=mode)
+    co, ao, _, _ = oracle.OracleScene(s).render(0, 0, 32, 32, spp=16, seed=7, threads=8)
+    close = np.abs(cg - co) <= 1e-3 * np.maximum(1.0, np.abs(co))
+    print("pixels equal to 1e-3:", close.mean(), "mean", cg.mean(), co.mean())
+    assert close.mean() > 0.9
+    assert abs(cg.mean() - co.mean()) < 0.02 * co.mean()
+    assert np.mean(np.abs(ag - ao) < 1e-5) > 0.95
 
 
 @pytest.mark.parametrize("seed", [0, 1, 2])

@@ -161,6 +161,62 @@ def test_lambert_and_dielectric_closed_forms():
     np.testing.assert_allclose(r["swi"], -q["ns"], atol=1e-6)
 
 
+def _fresnel_textbook(n, c):
+    """Unpolarised Fresnel reflectance from vacuum into index n (Hecht,
+    Optics, 4.34 / 4.40): the mean of the s and p squares."""
+    s_t = np.sqrt(1 - c * c) / n
+    ct = np.sqrt(1 - s_t * s_t)
+    rs = ((c - n * ct) / (c + n * ct)) ** 2
+    rp = ((n * c - ct) / (n * c + ct)) ** 2
+    return 0.5 * (rs + rp)
+
+
+def test_conductor_fresnel_closed_forms():
+    """ConductorFresnel (fresnel.h:70-110): with k = 0 it is the dielectric
+    Fresnel of index eta_t / eta_i at every angle; at normal incidence it is
+    ((n - 1)^2 + k^2) / ((n + 1)^2 + k^2) with n, k relative to eta_i; it
+    rises to 1 at grazing incidence and is 0 for cos <= 0.  FP32 follows
+    FP64 to 1e-5."""
+    c = np.linspace(0.02, 1.0, 200)
+    for n in (1.33, 1.5, 2.4):
+        got = oracle.conductor_fr(1.0, n, 0.0, c, fp64=True)
+        want = _fresnel_textbook(float(np.float32(n)), c)    # the indices travel as float32
+        np.testing.assert_allclose(got, np.repeat(want[:, None], 3, 1), rtol=1e-12, atol=1e-14)
+    eta_i, eta_t, k = [1.0, 1.2, 1.0], [0.2004, 0.924, 1.1022], [3.9129, 2.4528, 2.1421]   # copper in air, and a denser medium
+    f1 = oracle.conductor_fr(eta_i, eta_t, k, [1.0], fp64=True)[0]
+    n, kk = np.float32(eta_t).astype(np.float64) / np.float32(eta_i), np.float32(k).astype(np.float64) / np.float32(eta_i)
+    np.testing.assert_allclose(f1, ((n - 1) ** 2 + kk ** 2) / ((n + 1) ** 2 + kk ** 2), rtol=1e-12)
+    g = oracle.conductor_fr(eta_i, eta_t, k, [1e-7, 1e-4], fp64=True)
+    assert np.all(g > 0.999) and np.all(g <= 1.0 + 1e-12)
+    assert np.all(oracle.conductor_fr(eta_i, eta_t, k, [0.0, -0.3], fp64=True) == 0.0)
+    np.testing.assert_allclose(oracle.conductor_fr(eta_i, eta_t, k, c), oracle.conductor_fr(eta_i, eta_t, k, c, fp64=True),
+                               rtol=1e-5, atol=1e-6)
+
+
+def test_conductor_mirror_is_a_delta_reflection():
+    """SpecularReflectionBSDF (specular_reflection.cpp:18-42): f and pdf are
+    0; a sample is the mirror direction with pdf 1, marked delta, and
+    f = Fr(cos) colour / |cos|; the albedo is the colour."""
+    s = SceneDesc()
+    col = [0.9, 0.7, 0.5]
+    m = s.add_conductor(color=col)
+    o = oracle.OracleScene(s, fp64=True)
+    rng = np.random.default_rng(7)
+    q = bsdf_queries(rng, 256, m)
+    r = o.bsdf(q)
+    assert np.all(r["f"] == 0) and np.all(r["pdf"] == 0)
+    assert np.all(r["sdelta"] == 1) and np.all(r["spdf"] == 1)
+    n = q["ns"].astype(np.float64)
+    wo = unit(q["wo"].astype(np.float64))
+    cos = np.sum(wo * n, 1)
+    np.testing.assert_allclose(r["swi"], 2 * cos[:, None] * n - wo, atol=2e-6)
+    up = cos > 1e-3
+    fr = np.array([oracle.conductor_fr(1.0, [0.2, 0.92, 1.1], [3.9, 2.45, 2.14], [c], fp64=True)[0] for c in cos[up]])
+    np.testing.assert_allclose(r["sf"][up], fr * np.array(col) / cos[up][:, None], rtol=2e-5)
+    assert np.all(r["sf"][cos < 0] == 0)
+    np.testing.assert_allclose(r["albedo"], np.tile(col, (256, 1)), rtol=1e-7)
+
+
 def _disney_scene(**params):
     s = SceneDesc()
     s.add_disney(**params)
