@@ -1,13 +1,14 @@
 /*
- * Path-replay adjoint kernels (device/dadjoint.h): one thread per (pixel,
- * sample) of a persistent grid; the first kernel replays the forward path
- * (render_path) and stores its radiance, the second replays it again and
- * accumulates parameter gradients.  Both ask for two waves per SIMD
- * (amdgpu_waves_per_eu(2)): at one wave the walk has no latency hiding and
- * ran 1.7x slower, even though two waves spill a few hundred bytes.  Material
- * gradients collect in LDS per block (FP32 atomics, num_materials x 6) and
- * reach the FP64 global sums once per block; env texel gradients go to FP64
- * global sums directly (they are spread over many texels).
+ * Path-replay adjoint kernel (device/dadjoint.h): one thread per (pixel,
+ * sample) of a persistent grid walks the path again from its radiance
+ * (A.totals, written per path by a wavefront run of the same Philox streams
+ * just before) and accumulates parameter gradients.  It asks for two waves
+ * per SIMD (amdgpu_waves_per_eu(2)): at one wave the walk has no latency
+ * hiding and ran 1.7x slower, even though two waves spill a few hundred
+ * bytes.  Material gradients collect in LDS per block (FP32 atomics,
+ * num_materials x 6) and reach the FP64 global sums once per block; env
+ * texel gradients go to FP64 global sums directly (they are spread over
+ * many texels).
  */
 #include "launch.h"
 #include "device/dadjoint.h"
@@ -16,35 +17,7 @@ using namespace fmd;
 
 namespace {
 
-/* The forward replay: each work item's path radiance (render_path), the
- * `total` the adjoint walk starts from.  A separate kernel, so that neither
- * carries the other's registers. */
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) k_adjoint_fwd(fm_dscene S, fm::AdjointArgs A) {
-    extern __shared__ uint32_t lds[];
-    uint32_t* stk = lds + threadIdx.x;
-    const int stride = blockDim.x;
-    PathCfg cfg;
-    cfg.depth = A.cfg_raw.depth; cfg.direct = A.cfg_raw.direct; cfg.rr_depth = A.cfg_raw.rr_depth; cfg.rr_coef = A.cfg_raw.rr_coef;
-    const int64_t npix = (int64_t)A.W * A.H;
-    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < A.total_items; w += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t s = w / npix;
-        const int64_t pix = w - s * npix;
-        const v3 g = mk(A.dl_dimg[3 * pix], A.dl_dimg[3 * pix + 1], A.dl_dimg[3 * pix + 2]);
-        if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f) continue;
-        const int row = (int)(pix / A.W), i = (int)(pix - (int64_t)row * A.W);
-        const int j = A.H - 1 - row;
-        Rng rng;
-        rng.init(A.seed, (uint32_t)pix, (uint32_t)(A.sample_offset + s), 0u);
-        const float fx = rng.next(), fy = rng.next();
-        const float lx = rng.next(), ly = rng.next();
-        const float px = ((float)i + fx) / (float)A.W, py = ((float)j + fy) / (float)A.H;
-        const Ray cr = camera_ray(S.camera, px, py, lx, ly);
-        const PathOut po = render_path<false>(S, cfg, cr, rng, stk, stride);
-        A.totals[3 * w] = po.color.x; A.totals[3 * w + 1] = po.color.y; A.totals[3 * w + 2] = po.color.z;
-    }
-}
-
-/* The adjoint walk of every work item from its replayed total. */
+/* The adjoint walk of every work item from its path radiance. */
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) k_adjoint(fm_dscene S, fm::AdjointArgs A, int stack_levels) {
     extern __shared__ uint32_t lds[];
     uint32_t* stk = lds + threadIdx.x;
@@ -91,7 +64,6 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) k
 namespace fm {
 
 hipError_t launch_adjoint(const fm_dscene& S, const AdjointArgs& A, int block, int grid, int stack_levels, hipStream_t st) {
-    hipLaunchKernelGGL(k_adjoint_fwd, dim3(grid), dim3(block), (size_t)stack_levels * block * sizeof(uint32_t), st, S, A);
     size_t lds = (size_t)stack_levels * block * sizeof(uint32_t);
     if (A.lds_grad) lds += (size_t)S.num_materials * AD_NPARAM * sizeof(float);
     hipLaunchKernelGGL(k_adjoint, dim3(grid), dim3(block), lds, st, S, A, stack_levels);

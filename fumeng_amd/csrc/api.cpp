@@ -126,7 +126,8 @@ struct fm_ctx {
     DevBuf wf_passes, wf_units, wf_cams, wf_pass_idx;   /* wavefront tables of the last run */
     DevBuf batch_film;                                  /* FP64 sums of fm_render_batch jobs */
     DevBuf adj_in, adj_gmat, adj_genv;                  /* adjoint: image gradient, FP64 parameter gradients */
-    DevBuf adj_tot;                                     /* adjoint: replayed path radiance per work item */
+    DevBuf adj_tot;                                     /* adjoint: path radiance per work item */
+    DevBuf adj_film;                                    /* adjoint: FP64 sums of its forward run (unused) */
     std::vector<hipEvent_t> trace_ev;  /* pairs around wavefront trace launches */
     std::vector<hipEvent_t> pass_ev;   /* end of each wavefront pass */
 };
@@ -348,7 +349,8 @@ hipEvent_t* event_pool(std::vector<hipEvent_t>& pool, size_t n) {
  * every pass, directly or (FM_RENDER_GRAPH) as one captured pass graph
  * replayed per pass. */
 void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int rr_depth, float rr_coef, int block,
-            int grid, int chunk, int flags, WfRunOut& R) {
+            int grid, int chunk, int flags, WfRunOut& R, float* path_out = nullptr) {
+    if (path_out && jobs.size() != 1) throw std::invalid_argument("per-path output needs exactly one job");
     const int NL = s->d.num_lights;
     const int64_t per_path = 100 + 64 * (1 + (int64_t)D) + 48 * (int64_t)D + 57 * (int64_t)D * std::max(NL, 1);
     const int64_t budget = std::max<int64_t>(1, std::min<int64_t>((int64_t)1 << 26, (int64_t)24e9 / per_path));
@@ -424,6 +426,8 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
     }
     W_.counts = B.counts.as<unsigned>();
     W_.counters = ctx->counters.as<unsigned long long>();
+    W_.path_out = path_out;
+    W_.path_s0 = path_out ? jobs[0].sample_offset : 0;
     fm::WfLaunch L{};
     L.max_paths = cap; L.max_pix = max_pix;
     L.block_trace = block; L.block_shade = block;
@@ -1037,6 +1041,18 @@ int fm_render_adjoint(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, const
         int grid = p->grid_blocks > 0 ? p->grid_blocks : ctx->num_cus * fm::adjoint_blocks_per_cu(block, lds) * 2;
         const int64_t need = (A.total_items + block - 1) / block;
         if (grid > need) grid = (int)need;
+        /* each work item's radiance: a wavefront run of the same paths (the
+         * (pixel, sample) Philox streams of the walk) writes it per path */
+        WfJob job{};
+        job.cam = s->d.camera;
+        job.W = A.W; job.H = A.H; job.spp = A.spp; job.depth = std::max(A.cfg_raw.depth, 1);
+        job.sample_offset = A.sample_offset; job.seed = A.seed;
+        if (ctx->adj_film.n < (size_t)npix * 12 * sizeof(double)) ctx->adj_film.alloc((size_t)npix * 12 * sizeof(double));
+        HIPCHECK(fm::launch_zero_f64(ctx->adj_film.as<double>(), npix * 12, st));
+        job.film = ctx->adj_film.as<double>();
+        WfRunOut fwd;
+        wf_run(ctx, s, {job}, A.cfg_raw.direct, A.cfg_raw.rr_depth, A.cfg_raw.rr_coef, 0, 0, 0, 0, fwd, A.totals);
+        HIPCHECK(fm::launch_zero_f64((double*)ctx->counters.p, 4, st));
         HIPCHECK(hipEventRecord(ctx->ev0, st));
         HIPCHECK(fm::launch_adjoint(s->d, A, block, std::max(grid, 1), levels, st));
         HIPCHECK(hipEventRecord(ctx->ev1, st));
@@ -1050,7 +1066,7 @@ int fm_render_adjoint(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, const
             HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
             unsigned long long c[4];
             HIPCHECK(hipMemcpy(c, A.counters, sizeof c, hipMemcpyDeviceToHost));
-            stats->kernel_ms = ms;
+            stats->kernel_ms = fwd.kernel_ms + ms;
             stats->paths = (int64_t)c[0];
         }
         return FM_OK;

@@ -85,6 +85,8 @@ struct WfArgs {
     int* q_cls[2];     /* WQ_NCLS segments of P path ids */
     unsigned* counts;  /* [(n_iter + 1) * WQ_N * WQ_CNT_STRIDE] */
     unsigned long long* counters;
+    float* path_out;   /* if set (one job): each path's radiance, 3 floats at ((sample - path_s0) npix + pixel) */
+    int path_s0;
 };
 
 struct WfLaunch {
@@ -121,8 +123,9 @@ struct AdjointArgs {
     int env_w, env_h;
     int lds_grad;             /* accumulate material gradients in LDS first */
     unsigned long long* counters;
-    float* totals;            /* path radiance per work item (3 floats), from the forward replay */
+    float* totals;            /* path radiance per work item (3 floats), from a wavefront run of the same paths */
 };
+/* The adjoint walk of every work item (A.totals filled beforehand). */
 hipError_t launch_adjoint(const fm_dscene& S, const AdjointArgs& A, int block, int grid, int stack_levels, hipStream_t st);
 int adjoint_blocks_per_cu(int block, size_t lds);
 

@@ -500,6 +500,10 @@ __global__ void __launch_bounds__(256) k_wf_film(WfArgs A) {
         for (int j = 0; j < U.k; j++) {
             const int64_t p = U.path_base + (int64_t)j * U.npix + pix;
             const v3 c = xyz(reinterpret_cast<const float4*>(A.st_color)[p]);
+            if (A.path_out) {
+                float* o = A.path_out + 3 * ((int64_t)(U.sample_base + j - A.path_s0) * U.npix + pix);
+                o[0] = c.x; o[1] = c.y; o[2] = c.z;
+            }
             cs += c;
             as += xyz(reinterpret_cast<const float4*>(A.out_alb)[p]);
             ns += xyz(reinterpret_cast<const float4*>(A.out_nrm)[p]);
