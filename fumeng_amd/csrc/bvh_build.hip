@@ -99,7 +99,13 @@ __device__ inline uint64_t spread21(uint32_t v) {
 __global__ void k_bounds_init(int* cb, int* max_level) {
     if (threadIdx.x < 3) cb[threadIdx.x] = 0x7fffffff;
     else if (threadIdx.x < 6) cb[threadIdx.x] = (int)0x80000000;
-    else if (threadIdx.x == 6) *max_level = 0;
+    else if (threadIdx.x < 9) max_level[threadIdx.x - 6] = 0;   /* deepest level, nodes written, slots written */
+}
+
+/* adds the number of lanes with `pred` to *counter, one atomic per wave */
+__device__ inline void wave_count(int* counter, bool pred) {
+    const uint64_t b = __ballot(pred);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(counter, (int)__popcll(b));
 }
 
 /* centroid min / max; NaN centroids are skipped by fminf / fmaxf */
@@ -262,6 +268,7 @@ __global__ void __launch_bounds__(kBlock) k_top_down(const BNode* __restrict__ n
     if (t >= count) return;
     const int k = lo + t;
     const int4 me = info[k];
+    wave_count(max_level + 1, me.x >= 0 && me.x < n_out);
     const BNode& n = nodes[k];
     const BNode& A = nodes[n.a];
     const BNode& B = nodes[n.b];
@@ -285,11 +292,13 @@ __global__ void __launch_bounds__(kBlock) k_top_down(const BNode* __restrict__ n
 }
 
 __global__ void __launch_bounds__(kBlock) k_leaf_slots(const BNode* __restrict__ nodes, int n, const int4* __restrict__ info,
-                                                      int32_t* __restrict__ slot_src) {
+                                                      int32_t* __restrict__ slot_src, int* __restrict__ counts) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+    if (k >= n) return;   /* lanes past n leave; the ballot below counts the others (lane 0 among them) */
     const int s = info[k].y;
-    if (s >= 0 && s < n) slot_src[s] = nodes[k].a;
+    const bool ok = s >= 0 && s < n;
+    if (ok) slot_src[s] = nodes[k].a;
+    wave_count(counts, ok);
 }
 
 __global__ void __launch_bounds__(kBlock) k_gather4(const float4* __restrict__ src, float4* __restrict__ dst,
@@ -341,7 +350,7 @@ hipError_t bvh_build_device(const float* boxes, int64_t n64, BvhBuildOut& out, h
     TRY(T.get(&keys[0], n)); TRY(T.get(&keys[1], n));
     TRY(T.get(&vals[0], n)); TRY(T.get(&vals[1], n));
     TRY(T.get(&cb, 8));
-    TRY(T.get(&max_level, 1));
+    TRY(T.get(&max_level, 3));
     size_t sort_bytes = 0, scan_bytes = 0;
     TRY(rocprim::radix_sort_pairs(nullptr, sort_bytes, keys[0], keys[1], vals[0], vals[1], (size_t)n, 0u, 63u, st));
     TRY(rocprim::inclusive_scan(nullptr, scan_bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr, (size_t)n,
@@ -404,13 +413,18 @@ hipError_t bvh_build_device(const float* boxes, int64_t n64, BvhBuildOut& out, h
     k_root_info<<<1, 1, 0, st>>>(nodes, root, info);
     for (size_t t = iters.size(); t-- > 0;)
         k_top_down<<<blocks(iters[t].second), kBlock, 0, st>>>(nodes, iters[t].first, iters[t].second, info, dn, n_inner, max_level);
-    k_leaf_slots<<<blocks(n), kBlock, 0, st>>>(nodes, n, info, slot_src);
+    k_leaf_slots<<<blocks(n), kBlock, 0, st>>>(nodes, n, info, slot_src, max_level + 2);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(e);
-    int lvl = 0;
-    e = hipMemcpyAsync(&lvl, max_level, sizeof lvl, hipMemcpyDeviceToHost, st);
+    int chk[3] = {0, 0, 0};   /* deepest level, node records written, slots written */
+    e = hipMemcpyAsync(chk, max_level, sizeof chk, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail(e);
+    if (chk[1] != n_inner || chk[2] != n) {   /* every node and slot written exactly where expected */
+        out.error = "inconsistent flattened tree";
+        return fail(hipErrorUnknown);
+    }
+    const int lvl = chk[0];
 
     out.nodes = dn;
     out.slot_src = slot_src;
