@@ -321,7 +321,8 @@ def main():
             bytes_per_ray = (cst.node_visits * float(info.trav_node_bytes) + cst.prim_tests * 48.0) / cst.rays
             achieved = bytes_per_ray * rays / (trace_ms * 1e-3) / 1e9
             default_run = (args.freq, args.spp, args.block, args.grid, args.chunk) == (224, 256, 0, 0, 0)
-            kname = ("k_wf_trace<false, %d>" % info.trav_width) if mode == _abi.FM_MODE_WAVEFRONT else "k_render_mega<false>"
+            kname = (("k_wf_trace<false, %d, %s>" % (info.trav_width, "true" if st.trace_short_stack else "false"))
+                     if mode == _abi.FM_MODE_WAVEFRONT else "k_render_mega<false>")
             traffic = profiled_traffic(kname) if default_run else None
             roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kname,

@@ -84,6 +84,12 @@ class Context:
         b = A.BUILDERS[builder] if isinstance(builder, str) else int(builder)
         _check(lib().fm_set_bvh_builder(self.h, b), self.h)
 
+    def set_trace_stack(self, form):
+        """Stack form of the wavefront trace kernel: "auto" (default), "full"
+        (all entries in LDS) or "short" (16 in LDS, the rest spilled to
+        HBM).  Results do not depend on it."""
+        _check(lib().fm_set_trace_stack(self.h, A.TRACE_STACKS[form] if isinstance(form, str) else int(form)), self.h)
+
     def post_process(self, ops, rgb):
         """The reference's output step on the device (fm_post_process):
         returns (post-processed float image, uint8 image).  ops: list of

@@ -6,6 +6,7 @@ FM_ACCEL_EMBREE = 3
 FM_BUILDER_HOST = 0
 FM_BUILDER_DEVICE = 1
 BUILDERS = {"host": FM_BUILDER_HOST, "device": FM_BUILDER_DEVICE}
+TRACE_STACKS = {"auto": 0, "full": 1, "short": 2}
 
 FM_OK = 0
 FM_ERR_INVALID = -1
@@ -141,7 +142,8 @@ class fm_film(C.Structure):
 
 class fm_render_stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("paths", C.c_int64), ("rays", C.c_int64),
-                ("node_visits", C.c_int64), ("prim_tests", C.c_int64), ("trace_ms", C.c_double)]
+                ("node_visits", C.c_int64), ("prim_tests", C.c_int64), ("trace_ms", C.c_double),
+                ("trace_block", C.c_int32), ("trace_short_stack", C.c_int32)]
 
 
 class fm_job(C.Structure):
@@ -211,6 +213,7 @@ EXPORTS = {
     "fm_set_counting": (C.c_int, [C.c_void_p, C.c_int]),
     "fm_set_bvh_width": (C.c_int, [C.c_void_p, C.c_int32]),
     "fm_set_bvh_builder": (C.c_int, [C.c_void_p, C.c_int32]),
+    "fm_set_trace_stack": (C.c_int, [C.c_void_p, C.c_int32]),
     "fm_scene_traversal": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64]),
     "fm_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(fm_render_params), C.c_int32,
                               C.POINTER(fm_tune_result), C.c_void_p, C.c_int32]),

@@ -64,6 +64,7 @@ struct WfBuffers {
     DevBuf ray_o, ray_d, hit_a, hit_b, st_beta, st_color, st_flags, out_alb, out_nrm;
     DevBuf mis_f, mis_beta, mis_ng, sh_o, sh_d, sh_c, sh_vis;
     DevBuf q_ext[2], q_sh[2], q_cls[2], counts;
+    DevBuf spill;   /* short traversal stacks' spill entries */
     /* capacities in elements: paths, ray slots, MisBSDF records, shadow records, iterations */
     int64_t cap_p = 0, cap_r = 0, cap_pd = 0, cap_pdl = 0, cap_it = 0;
     void ensure(int64_t P, int D, int NL, int depth) {
@@ -102,7 +103,7 @@ struct WfBuffers {
         int64_t n = 0;
         for (const DevBuf* b : {&ray_o, &ray_d, &hit_a, &hit_b, &st_beta, &st_color, &st_flags, &out_alb, &out_nrm, &mis_f,
                                 &mis_beta, &mis_ng, &sh_o, &sh_d, &sh_c, &sh_vis, &q_ext[0], &q_ext[1], &q_sh[0], &q_sh[1],
-                                &q_cls[0], &q_cls[1], &counts})
+                                &q_cls[0], &q_cls[1], &counts, &spill})
             n += (int64_t)b->n;
         return n;
     }
@@ -121,6 +122,7 @@ struct fm_ctx {
     int counting = 0;
     int bvh_width = 2;        /* traversal tree width of scenes created next */
     int bvh_builder = FM_BUILDER_HOST;   /* traversal tree builder of scenes created next */
+    int trace_stack = FM_TRACE_STACK_AUTO;   /* wavefront trace kernel's stack form */
     std::set<fm_scene*> scenes;  /* live scenes, released by fm_shutdown */
     WfBuffers wf;
     DevBuf wf_passes, wf_units, wf_cams, wf_pass_idx;   /* wavefront tables of the last run */
@@ -331,6 +333,7 @@ struct WfJob {
 struct WfRunOut {
     double kernel_ms = 0.0, trace_ms = 0.0;
     int passes = 0, units = 0;
+    int trace_block = 0, short_stack = 0;
     std::vector<double> job_done_ms;
 };
 
@@ -446,7 +449,16 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
         L.lights = (env && !area && !other) ? fm::LK_ENV : ((!env && !other) ? fm::LK_AREA : fm::LK_ANY);
     }
     L.width = s->hs.trav_width;
+    L.stack_form = ctx->trace_stack;
     fm::wavefront_grids(ctx->num_cus, std::max(1, s->hs.trav_stack), L);
+    if (L.short_stack) {
+        /* a column of spill entries per lane of the persistent trace grid */
+        const int cap = std::max(1, std::max(1, s->hs.trav_stack) - FM_SHORT_STACK);
+        const size_t need = (size_t)L.grid_trace * (size_t)L.block_trace * (size_t)cap * sizeof(uint32_t);
+        if (ctx->wf.spill.n < need) ctx->wf.spill.alloc(need);
+        W_.spill = ctx->wf.spill.as<uint32_t>();
+        W_.spill_cap = cap;
+    }
 
     const bool count = ctx->counting != 0;
     const bool timed = (flags & FM_RENDER_TIME_TRACE) != 0;
@@ -492,6 +504,8 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
         }
     R.passes = (int)npass;
     R.units = (int)units.size();
+    R.trace_block = L.block_trace;
+    R.short_stack = L.short_stack ? 1 : 0;
     R.job_done_ms.assign(jobs.size(), 0.0);
     for (size_t j = 0; j < jobs.size(); j++)
         if (last_pass[j] >= 0) {
@@ -715,6 +729,14 @@ int fm_set_bvh_width(fm_ctx* ctx, int32_t width) {
     return FM_OK;
 }
 
+int fm_set_trace_stack(fm_ctx* ctx, int32_t form) {
+    if (!ctx) return fail(nullptr, FM_ERR_INVALID, "NULL ctx");
+    if (form != FM_TRACE_STACK_AUTO && form != FM_TRACE_STACK_FULL && form != FM_TRACE_STACK_SHORT)
+        return fail(ctx, FM_ERR_INVALID, "unknown trace stack form");
+    ctx->trace_stack = form;
+    return FM_OK;
+}
+
 int fm_set_bvh_builder(fm_ctx* ctx, int32_t builder) {
     if (!ctx) return fail(nullptr, FM_ERR_INVALID, "NULL ctx");
     if (builder != FM_BUILDER_HOST && builder != FM_BUILDER_DEVICE) return fail(ctx, FM_ERR_INVALID, "unknown BVH builder");
@@ -745,6 +767,7 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
         const int64_t npix = (int64_t)W * H;
         const bool wave = p->mode == FM_MODE_WAVEFRONT;
         double trace_ms = 0.0;
+        int trace_block = 0, short_stack = 0;
         /* megakernel: samples per work item (the wavefront sizes its passes
          * from a memory budget unless chunk caps the samples per pass) */
         A.chunk = p->chunk > 0 ? std::min(p->chunk, spp) : std::min(spp, 16);
@@ -789,6 +812,8 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
             wf_run(ctx, s, {job}, A.cfg_raw.direct, A.cfg_raw.rr_depth, A.cfg_raw.rr_coef, p->block_size > 0 ? block : 0, p->grid_blocks,
                    p->chunk > 0 ? std::min(p->chunk, spp) : 0, p->flags, R);
             trace_ms = R.trace_ms;
+            trace_block = R.trace_block;
+            short_stack = R.short_stack;
         }
         HIPCHECK(hipStreamSynchronize(ctx->stream));
         ctx->film_spp += spp;
@@ -803,6 +828,8 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
             stats->rays = (int64_t)c[1];
             stats->node_visits = (int64_t)c[2];
             stats->prim_tests = (int64_t)c[3];
+            stats->trace_block = trace_block;
+            stats->trace_short_stack = short_stack;
             if (p->flags & FM_RENDER_TIME_TRACE) stats->trace_ms = wave ? trace_ms : ms;
         }
         if (film && (film->color || film->albedo || film->normal || film->color_sum || film->sumsq)) return fm_film_read(ctx, film);

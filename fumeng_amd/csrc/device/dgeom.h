@@ -277,6 +277,54 @@ struct Cnt { uint32_t nodes = 0, prims = 0; };
  * closest hit does not depend on the order (order-free tie rule). */
 #define FM_TRAV_NONE 0x7fffffff   /* no node / no pending leaf (leaf refs are negative) */
 
+/* A lane's traversal stack.  Full form: entry i at stk[i * stride] in LDS,
+ * as many entries as the tree needs.  Short form: the newest FM_SHORT_STACK
+ * entries stay in LDS as a ring (entry i in slot i mod FM_SHORT_STACK), the
+ * older ones move to the lane's column of a global spill array (entry i at
+ * spill[i * sstride], i < scap), so that deep trees do not cost LDS
+ * occupancy.  The traversal order, and with it every result, is the same in
+ * both forms. */
+template <bool SHORT> struct TravStack;
+template <> struct TravStack<false> {
+    uint32_t* stk;
+    int stride;
+    int sp;   /* depth times stride: no multiply per push / pop */
+    FMD TravStack(uint32_t* s, int st) : stk(s), stride(st), sp(0) {}
+    FMD void push(uint32_t v) { stk[sp] = v; sp += stride; }
+    FMD int32_t pop() {
+        if (sp == 0) return FM_TRAV_NONE;
+        sp -= stride;
+        return (int32_t)stk[sp];
+    }
+};
+template <> struct TravStack<true> {
+    uint32_t* stk;
+    int stride;
+    uint32_t* spill;
+    uint32_t sstride;
+    int scap;
+    int depth, ns;   /* entries on the stack, of which the oldest ns are in the spill array */
+    FMD TravStack(uint32_t* s, int st, uint32_t* sp, uint32_t sst, int cap)
+        : stk(s), stride(st), spill(sp), sstride(sst), scap(cap), depth(0), ns(0) {}
+    FMD void push(uint32_t v) {
+        if (depth - ns == FM_SHORT_STACK) {   /* ring full: its oldest entry (index ns) moves out */
+            if (ns < scap) spill[(size_t)ns * sstride] = stk[(ns & (FM_SHORT_STACK - 1)) * stride];
+            ns++;
+        }
+        stk[(depth & (FM_SHORT_STACK - 1)) * stride] = v;
+        depth++;
+    }
+    FMD int32_t pop() {
+        if (depth == 0) return FM_TRAV_NONE;
+        depth--;
+        if (depth < ns) {   /* the ring is empty: the newest spilled entry */
+            ns--;
+            return ns < scap ? (int32_t)spill[(size_t)ns * sstride] : FM_TRAV_NONE;
+        }
+        return (int32_t)stk[(depth & (FM_SHORT_STACK - 1)) * stride];
+    }
+};
+
 /* Node records through a buffer resource: 32-bit byte offsets (one shift
  * per node instead of 64-bit address arithmetic) and hardware range checks
  * (a read past the array returns zeros instead of faulting). */
@@ -292,19 +340,14 @@ FMD int4 buf_load4i(__amdgpu_buffer_rsrc_t r, uint32_t off) {
     return make_int4((int)v[0], (int)v[1], (int)v[2], (int)v[3]);
 }
 
-template <bool FAST, bool ANY, bool COUNT>
-FMD bool traverse_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& h, uint32_t* stk, int stride, Cnt& cn) {
+template <bool FAST, bool ANY, bool COUNT, bool SHORT>
+FMD bool traverse_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& h, TravStack<SHORT>& stack, Cnt& cn) {
     const __amdgpu_buffer_rsrc_t nrs = buf_rsrc(S.nodes, S.node_bytes);
     float tn;
     if (!box_hit<FAST>(S.root_box, r, rp, ANY ? r.tmax : h.t, tn)) return false;
     int32_t node = S.root_ref, leaf = FM_TRAV_NONE;
     if (node < 0) { leaf = node; node = FM_TRAV_NONE; }
-    int sp = 0;   /* stack depth times stride: no multiply per push / pop */
-    auto pop = [&]() -> int32_t {
-        if (sp == 0) return FM_TRAV_NONE;
-        sp -= stride;
-        return (int32_t)stk[sp];
-    };
+    auto pop = [&]() -> int32_t { return stack.pop(); };
     while (node != FM_TRAV_NONE || leaf != FM_TRAV_NONE) {
         while (node != FM_TRAV_NONE && node >= 0) {
             if (COUNT) cn.nodes++;
@@ -317,8 +360,7 @@ FMD bool traverse_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& 
             const bool h1 = box_hit6<FAST>(q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r, rp, tmax, t1);
             if (h0 && h1) {
                 const bool first0 = ANY || t0 <= t1;
-                stk[sp] = (uint32_t)(first0 ? cr.y : cr.x);
-                sp += stride;
+                stack.push((uint32_t)(first0 ? cr.y : cr.x));
                 node = first0 ? cr.x : cr.y;
             } else if (h0) {
                 node = cr.x;
@@ -362,18 +404,13 @@ FMD void sort2u(uint32_t& a, uint32_t& b) {
     a = lo; b = hi;
 }
 
-template <bool FAST, bool ANY, bool COUNT>
-FMD bool traverse4_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& h, uint32_t* stk, int stride, Cnt& cn) {
+template <bool FAST, bool ANY, bool COUNT, bool SHORT>
+FMD bool traverse4_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& h, TravStack<SHORT>& stack, Cnt& cn) {
     float tn;
     if (!box_hit<FAST>(S.root_box, r, rp, ANY ? r.tmax : h.t, tn)) return false;
     int32_t node = S.root_ref, leaf = FM_TRAV_NONE;
     if (node < 0) { leaf = node; node = FM_TRAV_NONE; }
-    int sp = 0;   /* stack depth times stride: no multiply per push / pop */
-    auto pop = [&]() -> int32_t {
-        if (sp == 0) return FM_TRAV_NONE;
-        sp -= stride;
-        return (int32_t)stk[sp];
-    };
+    auto pop = [&]() -> int32_t { return stack.pop(); };
     constexpr uint32_t MISS = 0xffffffffu;
     while (node != FM_TRAV_NONE || leaf != FM_TRAV_NONE) {
         while (node != FM_TRAV_NONE && node >= 0) {
@@ -398,9 +435,9 @@ FMD bool traverse4_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit&
             if (k0 == MISS) {
                 node = pop();
             } else {
-                if (k3 != MISS) { stk[sp] = (uint32_t)child_of4(cr, k3); sp += stride; }
-                if (k2 != MISS) { stk[sp] = (uint32_t)child_of4(cr, k2); sp += stride; }
-                if (k1 != MISS) { stk[sp] = (uint32_t)child_of4(cr, k1); sp += stride; }
+                if (k3 != MISS) stack.push((uint32_t)child_of4(cr, k3));
+                if (k2 != MISS) stack.push((uint32_t)child_of4(cr, k2));
+                if (k1 != MISS) stack.push((uint32_t)child_of4(cr, k1));
                 node = child_of4(cr, k0);
             }
             if (node < 0 && leaf == FM_TRAV_NONE) { leaf = node; node = pop(); }
@@ -425,9 +462,9 @@ FMD bool traverse4_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit&
 }
 
 /* WIDTH = 2 or 4 compiles one traversal only (the caller knows the scene's
- * width); 0 branches on S.width. */
-template <bool COUNT = false, int WIDTH = 0>
-FMD Hit trace_closest(const fm_dscene& S, const Ray& r, uint32_t* stk, int stride, Cnt* cn = nullptr) {
+ * width); 0 branches on S.width.  The stack is the caller's (TravStack). */
+template <bool COUNT = false, int WIDTH = 0, bool SHORT>
+FMD Hit trace_closest_on(const fm_dscene& S, const Ray& r, TravStack<SHORT>& stack, Cnt* cn = nullptr) {
     Hit h;
     h.t = r.tmax; h.slot = -1; h.kind = -1; h.tie = 0; h.half = 0; h.b0 = h.b1 = h.b2 = 0.0f; h.info = 0;
     if (S.num_slots == 0) return h;
@@ -435,17 +472,17 @@ FMD Hit trace_closest(const fm_dscene& S, const Ray& r, uint32_t* stk, int strid
     Cnt dummy;
     Cnt& c = COUNT ? *cn : dummy;
     if (WIDTH == 4 || (WIDTH == 0 && S.width == 4)) {
-        if (rp.fast) traverse4_impl<true, false, COUNT>(S, r, rp, h, stk, stride, c);
-        else traverse4_impl<false, false, COUNT>(S, r, rp, h, stk, stride, c);
+        if (rp.fast) traverse4_impl<true, false, COUNT>(S, r, rp, h, stack, c);
+        else traverse4_impl<false, false, COUNT>(S, r, rp, h, stack, c);
     } else {
-        if (rp.fast) traverse_impl<true, false, COUNT>(S, r, rp, h, stk, stride, c);
-        else traverse_impl<false, false, COUNT>(S, r, rp, h, stk, stride, c);
+        if (rp.fast) traverse_impl<true, false, COUNT>(S, r, rp, h, stack, c);
+        else traverse_impl<false, false, COUNT>(S, r, rp, h, stack, c);
     }
     return h;
 }
 
-template <bool COUNT = false, int WIDTH = 0>
-FMD bool trace_any(const fm_dscene& S, const Ray& r, uint32_t* stk, int stride, Cnt* cn = nullptr) {
+template <bool COUNT = false, int WIDTH = 0, bool SHORT>
+FMD bool trace_any_on(const fm_dscene& S, const Ray& r, TravStack<SHORT>& stack, Cnt* cn = nullptr) {
     if (S.num_slots == 0) return false;
     const RayPre rp = precompute(r, S);
     Cnt dummy;
@@ -453,10 +490,22 @@ FMD bool trace_any(const fm_dscene& S, const Ray& r, uint32_t* stk, int stride, 
     Hit h;
     h.t = r.tmax; h.slot = -1; h.kind = -1; h.tie = 0; h.half = 0; h.b0 = h.b1 = h.b2 = 0.0f; h.info = 0;
     if (WIDTH == 4 || (WIDTH == 0 && S.width == 4))
-        return rp.fast ? traverse4_impl<true, true, COUNT>(S, r, rp, h, stk, stride, c)
-                       : traverse4_impl<false, true, COUNT>(S, r, rp, h, stk, stride, c);
-    return rp.fast ? traverse_impl<true, true, COUNT>(S, r, rp, h, stk, stride, c)
-                   : traverse_impl<false, true, COUNT>(S, r, rp, h, stk, stride, c);
+        return rp.fast ? traverse4_impl<true, true, COUNT>(S, r, rp, h, stack, c)
+                       : traverse4_impl<false, true, COUNT>(S, r, rp, h, stack, c);
+    return rp.fast ? traverse_impl<true, true, COUNT>(S, r, rp, h, stack, c)
+                   : traverse_impl<false, true, COUNT>(S, r, rp, h, stack, c);
+}
+
+/* The same with the full LDS stack at stk[i * stride]. */
+template <bool COUNT = false, int WIDTH = 0>
+FMD Hit trace_closest(const fm_dscene& S, const Ray& r, uint32_t* stk, int stride, Cnt* cn = nullptr) {
+    TravStack<false> st(stk, stride);
+    return trace_closest_on<COUNT, WIDTH>(S, r, st, cn);
+}
+template <bool COUNT = false, int WIDTH = 0>
+FMD bool trace_any(const fm_dscene& S, const Ray& r, uint32_t* stk, int stride, Cnt* cn = nullptr) {
+    TravStack<false> st(stk, stride);
+    return trace_any_on<COUNT, WIDTH>(S, r, st, cn);
 }
 
 /* Hit point as the geometry's GetIntersect fills it (triangle.cpp:211-248,

@@ -37,6 +37,7 @@
 
 #define FM_STACK_DEPTH 32   /* depth limit of the binary traversal trees */
 #define FM_STACK_MAX 64     /* LDS traversal stack entries per lane, at most */
+#define FM_SHORT_STACK 16   /* LDS entries per lane of the short (spilling) traversal stack; a power of two */
 
 /* Traversal trees are two-wide (fm_dnode) or four-wide (fm_dnode4). */
 #define FM_NODE4_EMPTY 0x7fffffff   /* unused child of a four-wide node */

@@ -87,6 +87,8 @@ struct WfArgs {
     unsigned long long* counters;
     float* path_out;   /* if set (one job): each path's radiance, 3 floats at ((sample - path_s0) npix + pixel) */
     int path_s0;
+    uint32_t* spill;   /* short traversal stacks: spill_cap entries per trace lane (entry i of lane g at i * lanes + g) */
+    int spill_cap;
 };
 
 struct WfLaunch {
@@ -95,6 +97,8 @@ struct WfLaunch {
     int block_shade, grid_shade;   /* grid_shade > 0 overrides every class */
     int grid_class[WQ_NCLS];       /* persistent grid per shading class (its own occupancy) */
     int stack_levels;
+    int stack_form;                /* 0: whichever keeps more trace waves resident, 1: full LDS stack, 2: short stack */
+    bool short_stack;              /* the trace kernel's stack form (set by wavefront_grids) */
     bool has_class[WQ_NCLS];
     int lights;                    /* LK_* */
     int width;                     /* traversal tree width of the scene */

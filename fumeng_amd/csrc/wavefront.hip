@@ -192,14 +192,24 @@ __global__ void __launch_bounds__(256) k_wf_camera(fm_dscene S, WfArgs A) {
 
 /* ---- trace: closest hits + classification, and shadow any hits ----------- */
 
+template <bool SHORT>
+FMD TravStack<SHORT> make_stack(uint32_t* stk, int stride, uint32_t* spill, uint32_t lanes, int cap) {
+    if constexpr (SHORT) return TravStack<true>(stk, stride, spill, lanes, cap);
+    else return TravStack<false>(stk, stride);
+}
+
 /* One launch per iteration covers both queues: items [0, n_ext) are ray
  * slots for the closest hit, [n_ext, n_ext + n_sh) shadow records for the
  * any hit, so the two share the machine (and the latency of a short tail
  * iteration is paid once). */
-template <bool COUNT, int WIDTH>
-__global__ void __launch_bounds__(256) k_wf_trace(fm_dscene S, WfArgs A) {
+template <bool COUNT, int WIDTH, bool SHORT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SHORT ? 6 : 1))) k_wf_trace(fm_dscene S, WfArgs A) {
     uint32_t* stk = wf_stack();
     const int stride = blockDim.x;
+    /* SHORT: FM_SHORT_STACK entries per lane in LDS, the rest in this lane's
+     * column of A.spill (one column per lane of the persistent grid) */
+    uint32_t* const spill = SHORT ? A.spill + grid_id() : nullptr;
+    const uint32_t lanes = (uint32_t)(gridDim.x * blockDim.x);
     const int it = A.iter;
     const unsigned n_ext = *cnt(A, it, fm::WQ_EXT);
     const unsigned n_sh = *cnt(A, it, fm::WQ_SH);
@@ -221,7 +231,8 @@ __global__ void __launch_bounds__(256) k_wf_trace(fm_dscene S, WfArgs A) {
             if (i < (int64_t)n_ext) {
                 const int r = q[i];
                 const Ray tr = load_ray(A, r);
-                const Hit h = trace_closest<COUNT, WIDTH>(S, tr, stk, stride, &cn);
+                TravStack<SHORT> ts = make_stack<SHORT>(stk, stride, spill, lanes, A.spill_cap);
+                const Hit h = trace_closest_on<COUNT, WIDTH>(S, tr, ts, &cn);
                 reinterpret_cast<float4*>(A.hit_a)[r] =
                     make_float4(h.t, __int_as_float(h.slot), __int_as_float((h.kind & 0xff) | (h.half << 8)), h.b0);
                 reinterpret_cast<float2*>(A.hit_b)[r] = make_float2(h.b1, h.b2);
@@ -242,7 +253,8 @@ __global__ void __launch_bounds__(256) k_wf_trace(fm_dscene S, WfArgs A) {
                 const float4 d = reinterpret_cast<const float4*>(A.sh_d)[k];
                 Ray tr;
                 tr.o = xyz(o); tr.tmin = o.w; tr.d = xyz(d); tr.tmax = d.w;
-                const bool occluded = trace_any<COUNT, WIDTH>(S, tr, stk, stride, &cn);
+                TravStack<SHORT> ts = make_stack<SHORT>(stk, stride, spill, lanes, A.spill_cap);
+                const bool occluded = trace_any_on<COUNT, WIDTH>(S, tr, ts, &cn);
                 A.sh_vis[k] = occluded ? 1 : 2;
             }
         }
@@ -543,27 +555,38 @@ void launch_shades(const fm_dscene& S, const WfArgs& A, const WfLaunch& L, hipSt
 
 void wavefront_grids(int num_cus, int stack_levels, WfLaunch& L) {
     L.stack_levels = stack_levels;
-    auto trace_blocks = [&](int block) {
-        const size_t lds = (size_t)stack_levels * block * sizeof(uint32_t);
-        return L.width == 4 ? occupancy(k_wf_trace<false, 4>, block, lds) : occupancy(k_wf_trace<false, 2>, block, lds);
+    auto trace_blocks = [&](bool sh, int block) {
+        const size_t lds = (size_t)(sh ? FM_SHORT_STACK : stack_levels) * block * sizeof(uint32_t);
+        if (L.width == 4)
+            return sh ? occupancy(k_wf_trace<false, 4, true>, block, lds) : occupancy(k_wf_trace<false, 4, false>, block, lds);
+        return sh ? occupancy(k_wf_trace<false, 2, true>, block, lds) : occupancy(k_wf_trace<false, 2, false>, block, lds);
     };
-    if (L.block_trace <= 0) {
-        /* The block (256 or 128) that keeps the most trace waves resident
-         * per CU: its LDS stacks grow with the block (and with the tree's
-         * depth), registers cap the waves; equal counts keep the larger
-         * block.  One-wave blocks are left out: with deep trees the
-         * occupancy query counts more of them than run at once, and a
-         * persistent grid whose last blocks wait for a free slot runs their
-         * whole share after the rest (1.5x the trace time, measured). */
-        int best = 256, waves = 0;
-        for (int b : {256, 128}) {
-            const int w = trace_blocks(b) * (b / 64);
-            if (w > waves) { waves = w; best = b; }
+    /* The stack form and block (256 or 128 unless given) that keep the most
+     * trace waves resident per CU: full LDS stacks grow with the block and
+     * the tree's depth, short ones hold FM_SHORT_STACK entries and spill the
+     * rest to HBM; registers cap the waves.  Equal counts keep the full form,
+     * then the larger block.  One-wave blocks are left out: with deep trees
+     * the occupancy query counts more of them than run at once, and a
+     * persistent grid whose last blocks wait for a free slot runs their
+     * whole share after the rest (1.5x the trace time, measured). */
+    {
+        int best_b = L.block_trace > 0 ? L.block_trace : 256, waves = 0;
+        bool best_s = L.stack_form == 2;
+        for (int f = 0; f < 2; f++) {
+            const bool sh = f == 1;
+            if ((L.stack_form == 1 && sh) || (L.stack_form == 2 && !sh)) continue;
+            for (int b : {256, 128}) {
+                const int blk = L.block_trace > 0 ? L.block_trace : b;
+                const int w = trace_blocks(sh, blk) * (blk / 64);
+                if (w > waves) { waves = w; best_b = blk; best_s = sh; }
+                if (L.block_trace > 0) break;
+            }
         }
-        L.block_trace = best;
+        L.block_trace = best_b;
+        L.short_stack = best_s;
     }
     if (L.block_shade <= 0) L.block_shade = 256;
-    if (L.grid_trace <= 0) L.grid_trace = num_cus * trace_blocks(L.block_trace);
+    if (L.grid_trace <= 0) L.grid_trace = num_cus * trace_blocks(L.short_stack, L.block_trace);
     const int g = L.grid_shade;
     auto occ4 = [&](auto k0, auto k1, auto k2, auto k3) {
         L.grid_class[WC_END] = g > 0 ? g : num_cus * occupancy(k0, L.block_shade, 0);
@@ -596,18 +619,29 @@ void launch_shades(const fm_dscene& S, const WfArgs& A, const WfLaunch& L, hipSt
 
 hipError_t launch_wavefront_pass(const fm_dscene& S, WfArgs A, const WfLaunch& L, bool count, hipStream_t st,
                                  hipEvent_t* ev) {
-    const size_t lds = (size_t)L.stack_levels * L.block_trace * sizeof(uint32_t);
+    const size_t lds = (size_t)(L.short_stack ? FM_SHORT_STACK : L.stack_levels) * L.block_trace * sizeof(uint32_t);
     const int cam_grid = (int)std::min<int64_t>((L.max_paths + 255) / 256, (int64_t)L.grid_shade * 4);
     hipLaunchKernelGGL(k_wf_camera, dim3(std::max(cam_grid, 1)), dim3(256), 0, st, S, A);
     for (int it = 0; it < A.n_iter; it++) {
         A.iter = it;
         if (ev) hipEventRecord(ev[2 * it], st);
+        const dim3 g(L.grid_trace), b(L.block_trace);
         if (L.width == 4) {
-            if (count) hipLaunchKernelGGL((k_wf_trace<true, 4>), dim3(L.grid_trace), dim3(L.block_trace), lds, st, S, A);
-            else hipLaunchKernelGGL((k_wf_trace<false, 4>), dim3(L.grid_trace), dim3(L.block_trace), lds, st, S, A);
+            if (L.short_stack) {
+                if (count) hipLaunchKernelGGL((k_wf_trace<true, 4, true>), g, b, lds, st, S, A);
+                else hipLaunchKernelGGL((k_wf_trace<false, 4, true>), g, b, lds, st, S, A);
+            } else {
+                if (count) hipLaunchKernelGGL((k_wf_trace<true, 4, false>), g, b, lds, st, S, A);
+                else hipLaunchKernelGGL((k_wf_trace<false, 4, false>), g, b, lds, st, S, A);
+            }
         } else {
-            if (count) hipLaunchKernelGGL((k_wf_trace<true, 2>), dim3(L.grid_trace), dim3(L.block_trace), lds, st, S, A);
-            else hipLaunchKernelGGL((k_wf_trace<false, 2>), dim3(L.grid_trace), dim3(L.block_trace), lds, st, S, A);
+            if (L.short_stack) {
+                if (count) hipLaunchKernelGGL((k_wf_trace<true, 2, true>), g, b, lds, st, S, A);
+                else hipLaunchKernelGGL((k_wf_trace<false, 2, true>), g, b, lds, st, S, A);
+            } else {
+                if (count) hipLaunchKernelGGL((k_wf_trace<true, 2, false>), g, b, lds, st, S, A);
+                else hipLaunchKernelGGL((k_wf_trace<false, 2, false>), g, b, lds, st, S, A);
+            }
         }
         if (ev) hipEventRecord(ev[2 * it + 1], st);
         launch_shades(S, A, L, st);

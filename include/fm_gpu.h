@@ -294,6 +294,8 @@ typedef struct fm_render_stats {
     double trace_ms;         /* with FM_RENDER_TIME_TRACE: summed device time of the
                                 traversal launches (wavefront trace kernels; the
                                 whole megakernel) */
+    int32_t trace_block;     /* wavefront: the trace kernel's block size */
+    int32_t trace_short_stack;  /* wavefront: 1 if it ran with the short stack */
 } fm_render_stats;
 
 /* Renders into the context's device film (FP64 sums) and, when `film` has
@@ -422,6 +424,15 @@ enum {
  * No counterpart in the reference (BVHAggregate::Build runs on the host,
  * bvh_aggregate.cpp:38-79). */
 int fm_set_bvh_builder(fm_ctx* ctx, int32_t builder);
+/* Stack form of the wavefront trace kernel.  FM_TRACE_STACK_FULL keeps
+ * every entry a traversal needs in LDS (LDS grows with the tree's depth and
+ * can cap the resident waves); FM_TRACE_STACK_SHORT keeps the newest 16 in
+ * LDS and spills older ones to HBM; FM_TRACE_STACK_AUTO (default) takes the
+ * form that keeps more waves resident.  Results do not depend on it.  No
+ * counterpart in the reference (its traversal recurses on the host stack,
+ * bvh_aggregate.cpp:101-123). */
+enum { FM_TRACE_STACK_AUTO = 0, FM_TRACE_STACK_FULL = 1, FM_TRACE_STACK_SHORT = 2 };
+int fm_set_trace_stack(fm_ctx* ctx, int32_t form);
 /* Copies the device's traversal tree: num_trav_nodes nodes of
  * trav_node_bytes each into `nodes` (max_bytes bytes at most), and the
  * geometry index at each slot into slot_geometry (max_slots at most).

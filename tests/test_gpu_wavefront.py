@@ -141,3 +141,37 @@ def test_textured_many_light_scene_matches_oracle(gpu_ctx, mode):
     assert close.mean() > 0.9
     assert abs(cg.mean() - co.mean()) < 0.03 * co.mean()
     assert np.mean(np.abs(ag - ao) < 1e-4) > 0.97
+
+
+@pytest.mark.parametrize("width", [2, 4])
+def test_short_trace_stack_gives_the_same_films(gpu_ctx, width):
+    """The trace kernel's short stack (16 entries in LDS, older ones spilled
+    to HBM) against the full LDS stack, on trees deeper than 16 entries so
+    that rays spill: the traversal order is the same, so films, albedo and
+    ray counts are bit-identical, and so is a render whose trace uses
+    one-wave blocks (the largest spill area)."""
+    s = scenes.icosphere_scene(freq=40, width=64, height=48, spp=4, env_size=(256, 128), n_materials=8)
+    try:
+        gpu_ctx.set_bvh_width(width)
+        g = gpu_ctx.scene(s)
+    finally:
+        gpu_ctx.set_bvh_width(2)
+    assert g.info().trav_stack > 16
+    out = {}
+    try:
+        for form in ("full", "short"):
+            gpu_ctx.set_trace_stack(form)
+            out[form] = g.render(seed=5, mode=A.FM_MODE_WAVEFRONT)
+            assert out[form][4].trace_short_stack == (1 if form == "short" else 0)
+        gpu_ctx.set_trace_stack("short")
+        small = g.render(seed=5, mode=A.FM_MODE_WAVEFRONT, block_size=64)
+        gpu_ctx.set_trace_stack("auto")
+        auto = g.render(seed=5, mode=A.FM_MODE_WAVEFRONT)
+    finally:
+        gpu_ctx.set_trace_stack("auto")
+    for other in (out["short"], small, auto):
+        for a, b in zip(other[:4], out["full"][:4]):
+            assert np.array_equal(a, b, equal_nan=True)
+        assert other[4].rays == out["full"][4].rays
+    with pytest.raises(Exception):
+        gpu_ctx.set_trace_stack(7)
