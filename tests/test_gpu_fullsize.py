@@ -1,0 +1,77 @@
+"""GPU: the bench's scenes at their full geometry, through size-independent
+properties (the oracle cannot render them in test time).
+
+  forms     the wavefront and megakernel forms run the same estimator with
+            the same random numbers: at least 99 % of the pixels equal to
+            1e-4 (relative, floor 1) and ray counts equal to 1e-4
+  trees     the device-built (PLOC) and host-built (SAH) traversal trees give
+            the same film except where a path met a ray with
+            0 < |d.z| < 1e-2 (the reference's truncated shear makes its hit
+            depend on the boxes visited): at least 98 % of the pixels equal,
+            image means within 0.5 %
+  sanity    finite, positive images; every path counted
+
+Scenes: the displaced icosphere at frequency 224 (1,003,522 geometries,
+64 Disney materials, 2048x1024 env) at 1920x1080 and 16 spp; the stress
+scene (9,800,012 geometries, 200 textures of 2048^2, 10 area lights) at
+480x270 and 8 spp.
+"""
+import numpy as np
+import pytest
+
+from fumeng_amd import _abi as A
+from fumeng_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _close(a, b, rtol):
+    return np.abs(a - b) <= rtol * np.maximum(1.0, np.abs(b))
+
+
+def _forms_agree(g, **kw):
+    w = g.render(mode=A.FM_MODE_WAVEFRONT, **kw)
+    m = g.render(mode=A.FM_MODE_MEGAKERNEL, **kw)
+    frac = np.mean(np.all(_close(w[0], m[0], 1e-4), axis=-1))
+    print("forms: %.5f of pixels equal to 1e-4; rays %d vs %d" % (frac, w[4].rays, m[4].rays))
+    assert frac >= 0.99
+    assert abs(w[4].rays - m[4].rays) <= 1e-4 * m[4].rays
+    H, W = w[0].shape[:2]
+    assert w[4].paths == m[4].paths == H * W * kw["spp"]
+    fin = np.isfinite(w[0]).all(axis=-1)
+    assert fin.mean() > 0.999 and w[0][fin].mean() > 0
+    return w
+
+
+def _trees_agree(gpu_ctx, desc, ref, **kw):
+    try:
+        gpu_ctx.set_bvh_builder("device")
+        gd = gpu_ctx.scene(desc)
+    finally:
+        gpu_ctx.set_bvh_builder("host")
+    d = gd.render(mode=A.FM_MODE_WAVEFRONT, **kw)
+    same = np.mean(np.all(d[0] == ref[0], axis=-1) | np.all(np.isnan(d[0]) & np.isnan(ref[0]), axis=-1))
+    fin = np.isfinite(d[0]).all(axis=-1) & np.isfinite(ref[0]).all(axis=-1)
+    m0, m1 = d[0][fin].mean(), ref[0][fin].mean()
+    print("trees: %.5f of pixels equal, means %.6g vs %.6g" % (same, m0, m1))
+    assert same >= 0.98
+    assert abs(m0 - m1) <= 0.005 * m1
+    gd.close()
+
+
+def test_icosphere_at_full_geometry(gpu_ctx):
+    desc = scenes.icosphere_scene(spp=16)
+    g = gpu_ctx.scene(desc)
+    assert g.info().num_geometries == 1003522
+    ref = _forms_agree(g, seed=3, spp=16)
+    _trees_agree(gpu_ctx, desc, ref, seed=3, spp=16)
+    g.close()
+
+
+def test_stress_scene_at_full_geometry(gpu_ctx):
+    desc = scenes.stress_scene(width=480, height=270, spp=8)
+    g = gpu_ctx.scene(desc)
+    assert g.info().num_geometries == 9800012
+    ref = _forms_agree(g, seed=4, spp=8)
+    _trees_agree(gpu_ctx, desc, ref, seed=4, spp=8)
+    g.close()
