@@ -158,7 +158,7 @@ def test_adjoint_against_fp64_reference_differences(gpu_ctx):
 
 def test_torch_optimisation_lowers_the_loss(gpu_ctx):
     """Adam on the Disney parameters of a perturbed start, gradients from the
-    adjoint through torch.autograd; the start and end parameters are judged
+    adjoint through torch.autograd; the start and end parameters are scored
     by a 1024-spp L2 loss with common random numbers (single 64-spp losses
     are too noisy under the sun to compare)."""
     torch = pytest.importorskip("torch")
@@ -174,12 +174,12 @@ def test_torch_optimisation_lowers_the_loss(gpu_ctx):
     env_t = torch.from_numpy(env)
     opt = torch.optim.Adam([theta], lr=0.04)
 
-    def judged(params):
+    def scored(params):
         g.set_params(params, env)
         return float(np.mean((g.render(spp=1024, seed=77, mode=WF)[0] - target) ** 2))
 
     nan_rows = torch.full_like(theta, float("nan"))
-    start = judged(np.where(np.isfinite(mp), start_mp, np.nan).astype(np.float32))
+    start = scored(np.where(np.isfinite(mp), start_mp, np.nan).astype(np.float32))
     for it in range(20):
         opt.zero_grad()
         params = torch.where(mask > 0, theta, nan_rows)
@@ -190,7 +190,7 @@ def test_torch_optimisation_lowers_the_loss(gpu_ctx):
         opt.step()
         with torch.no_grad():
             theta.clamp_(0.0, 1.0)
-    end = judged(np.where(np.isfinite(mp), theta.detach().numpy(), np.nan).astype(np.float32))
+    end = scored(np.where(np.isfinite(mp), theta.detach().numpy(), np.nan).astype(np.float32))
     d0 = np.nanmean(np.abs(start_mp - mp))
     d1 = np.nanmean(np.abs(np.where(np.isfinite(mp), theta.detach().numpy(), np.nan) - mp))
     print("1024-spp loss %.5g -> %.5g; mean |theta - target| %.4f -> %.4f" % (start, end, d0, d1))
