@@ -2,10 +2,10 @@
  * Path-replay adjoint kernel (device/dadjoint.h): one thread per (pixel,
  * sample) of a persistent grid walks the path again from its radiance
  * (A.totals, written per path by a wavefront run of the same Philox streams
- * just before) and accumulates parameter gradients.  It asks for two waves
- * per SIMD (amdgpu_waves_per_eu(2)): at one wave the walk has no latency
- * hiding and ran 1.7x slower, even though two waves spill a few hundred
- * bytes.  Material gradients collect in LDS per block (FP32 atomics,
+ * just before) and accumulates parameter gradients.  It asks for four waves
+ * per SIMD (amdgpu_waves_per_eu(4)): the walk is latency-bound, and although
+ * the register cap spills some 770 bytes per lane it ran 1.7x faster at two
+ * waves than at one and another 10 % faster at four.  Material gradients collect in LDS per block (FP32 atomics,
  * num_materials x 6) and reach the FP64 global sums once per block; env
  * texel gradients go to FP64 global sums directly (they are spread over
  * many texels).
@@ -18,7 +18,7 @@ using namespace fmd;
 namespace {
 
 /* The adjoint walk of every work item from its path radiance. */
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) k_adjoint(fm_dscene S, fm::AdjointArgs A, int stack_levels) {
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k_adjoint(fm_dscene S, fm::AdjointArgs A, int stack_levels) {
     extern __shared__ uint32_t lds[];
     uint32_t* stk = lds + threadIdx.x;
     const int stride = blockDim.x;
