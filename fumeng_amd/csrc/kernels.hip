@@ -120,8 +120,13 @@ __global__ void k_env(fm_dscene S, const float* __restrict__ dirs, int64_t n, fl
     rgb[3 * i] = c.x; rgb[3 * i + 1] = c.y; rgb[3 * i + 2] = c.z;
 }
 
+/* The whole path state is live across both traversal sites. Uncapped, it
+ * takes 256 VGPRs + 142 AGPRs (1 wave per SIMD); capped at 2 waves the
+ * compiler spills 628 B per lane to scratch around the traces and the frame
+ * runs 1.46x faster (latency, not bandwidth, bounds this kernel). A cap of
+ * 3 waves spills 972 B and gains nothing more. */
 template <bool COUNT>
-__global__ void __launch_bounds__(256) k_render_mega(fm_dscene S, fm::RenderArgs A) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_render_mega(fm_dscene S, fm::RenderArgs A) {
     uint32_t* stk = lane_stack();
     const int stride = blockDim.x;
     const int64_t npix = (int64_t)A.W * A.H;
