@@ -116,7 +116,7 @@ def adjoint_leg(fumeng_amd, ctx, iters=6):
             env_t.clamp_(min=0.0)
         if it > 0:                       # the first iteration is the warm-up
             times.append(time.perf_counter() - t0)
-        losses.append(float(loss))
+        losses.append(float(loss.detach()))
     sc.close()
     res = {"workload": "inverse rendering, 100,822 geometries, 64 Disney materials, 512x256 env, 256x256, "
                        "32 + 32 spp, depth 6", "params": int(mask.sum().item()) + env.size,
