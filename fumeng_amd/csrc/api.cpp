@@ -683,10 +683,10 @@ int fm_autotune(fm_ctx* ctx, fm_scene* s, const fm_render_params* base, int32_t 
     std::vector<fm_tune_trial> cand;
     {
         fm_tune_trial t{};
-        t.mode = p.mode; t.chunk = wave ? 0 : 8;
+        t.mode = p.mode; t.chunk = wave ? 0 : 4;
         cand.push_back(t);
     }
-    const std::vector<int> chunks = wave ? std::vector<int>{0, std::max(1, p.spp / 4)} : std::vector<int>{4, 8, 16, 64};
+    const std::vector<int> chunks = wave ? std::vector<int>{0, std::max(1, p.spp / 4)} : std::vector<int>{2, 4, 8, 16, 64};
     for (int block : {64, 128, 256})
         for (int fixed_grid : {0, 1})
             for (int chunk : chunks) {
@@ -770,7 +770,7 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
         int trace_block = 0, short_stack = 0;
         /* megakernel: samples per work item (the wavefront sizes its passes
          * from a memory budget unless chunk caps the samples per pass) */
-        A.chunk = p->chunk > 0 ? std::min(p->chunk, spp) : std::min(spp, 8);
+        A.chunk = p->chunk > 0 ? std::min(p->chunk, spp) : std::min(spp, 4);
         const int64_t nchunks = (spp + A.chunk - 1) / A.chunk;
         A.total_items = npix * nchunks;
         /* film */

@@ -396,7 +396,7 @@ typedef struct fm_tune_result {
  * 64 / 128 / 256; the grid each kernel's occupancy gives, or a fixed one
  * (wavefront: 2048 threads per CU; megakernel: twice the occupancy grid);
  * for the wavefront form passes of the whole trial or of a quarter of it
- * (megakernel: chunks of 4, 8, 16 or 64 samples).  Returns the fastest by device time; `trials` (may be
+ * (megakernel: chunks of 2, 4, 8, 16 or 64 samples).  Returns the fastest by device time; `trials` (may be
  * NULL) receives up to max_trials of them.  Overwrites the device film.  No
  * counterpart in the reference (its thread count is a JSON setting,
  * renderer.h:70-77). */
