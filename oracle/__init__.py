@@ -42,6 +42,8 @@ def lib():
             "or_abi_version": (C.c_int, []),
             "or_post_process": (None, [C.POINTER(A.fm_post_op), C.c_int, P, I64, P, P]),
             "or_conductor_fr": (None, [P, P, P, P, I64, C.c_int, P]),
+            "or_mat_func": (None, [C.c_int, C.c_int, P, I64, P]),
+            "or_disney": (None, [P, C.c_int, C.c_int, C.c_int, P, I64, P]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -161,3 +163,53 @@ def post_process(ops, rgb):
     out8 = np.empty(rgb.shape, np.uint8)
     lib().or_post_process(arr, len(ops), rgb.ctypes.data, rgb.size, out.ctypes.data, out8.ctypes.data)
     return out, out8
+
+
+MAT_FUNCS = {"gtr2": 0, "gtr1": 1, "lambda": 2, "smith_sep": 3, "sample_gtr2": 4, "sample_gtr1": 5,
+             "dielectric": 6, "smith_aniso": 7}
+
+
+def mat_func(name, args, fp64=True):
+    """One of the reference's microfacet / Fresnel helpers
+    (material_common.cpp:7-125, fresnel.h:16-68) on rows of arguments:
+    returns (n, 8) doubles (scalar results in column 0, vectors in 0..2)."""
+    a = np.asarray(args, np.float64)
+    a = a.reshape(-1, a.shape[-1])
+    buf = np.zeros((a.shape[0], 9), np.float64)
+    buf[:, :a.shape[1]] = a
+    out = np.zeros((a.shape[0], 8), np.float64)
+    lib().or_mat_func(MAT_FUNCS[name], 1 if fp64 else 0, buf.ctypes.data, buf.shape[0], out.ctypes.data)
+    return out
+
+
+DISNEY_PARTS = {"f": 0, "pdf": 1, "diffuse": 2, "specular": 3, "clearcoat": 4, "transmission": 5, "fake_ss": 6,
+                "lambert_trans": 7, "pdf_specular": 8, "pdf_clearcoat": 9, "pdf_transmission": 10,
+                "pdf_diffuse": 11, "pdf_lambert_trans": 12, "sample": 13}
+DISNEY_KEYS = ["metallic", "specular", "specular_tint", "roughness", "anisotropic", "sheen", "sheen_tint",
+               "clearcoat", "clearcoat_gloss", "spec_trans", "spec_trans_roughness", "diff_trans", "flatness", "ior"]
+DISNEY_DEFAULTS = {"specular": 0.5, "ior": 1.5}   # material.h:36-76; the rest default to 0
+
+
+def disney(part, wo, wi=None, sample=None, base=(0.0, 0.0, 0.0), thin=False, fp64=True, **params):
+    """DisneyBSDF (disney.cpp:11-579) in its shading frame: `part` is the whole
+    BSDF ("f", "pdf", "sample") or one lobe.  wo, wi: (n, 3) local directions.
+    Returns (n, 8) doubles: f rgb in 0..2, pdf in 3 (sample: wi in 4..6,
+    delta in 7)."""
+    mat = np.zeros(17, np.float64)
+    mat[:3] = base
+    for k, name in enumerate(DISNEY_KEYS):
+        mat[3 + k] = params.pop(name, DISNEY_DEFAULTS.get(name, 0.0))
+    if params:
+        raise TypeError("unknown Disney parameters: %s" % sorted(params))
+    wo = np.asarray(wo, np.float64).reshape(-1, 3)
+    n = wo.shape[0]
+    buf = np.zeros((n, 9), np.float64)
+    buf[:, 0:3] = wo
+    if wi is not None:
+        buf[:, 3:6] = np.broadcast_to(np.asarray(wi, np.float64).reshape(-1, 3), (n, 3))
+    if sample is not None:
+        buf[:, 6:9] = np.broadcast_to(np.asarray(sample, np.float64).reshape(-1, 3), (n, 3))
+    out = np.zeros((n, 8), np.float64)
+    lib().or_disney(mat.ctypes.data, 1 if thin else 0, DISNEY_PARTS[part], 1 if fp64 else 0, buf.ctypes.data, n,
+                    out.ctypes.data)
+    return out

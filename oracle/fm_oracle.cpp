@@ -527,6 +527,91 @@ template <typename R> static int render_t(or_handle* hd, const fm_render_params*
 
 #define DISPATCH(fn, ...) (h->fp64 ? fn<double>(h, __VA_ARGS__) : fn<float>(h, __VA_ARGS__))
 
+/* ---- probes of the BSDF building blocks, for the tests that pin this
+ * restatement against closed forms and invariants.  Inputs are 9 doubles
+ * per query, outputs 8 doubles per query. */
+enum {
+    MF_GTR2 = 0,        /* in: wh xyz, ax, ay             out: D */
+    MF_GTR1 = 1,        /* in: wh xyz, alpha              out: D */
+    MF_LAMBDA = 2,      /* in: w xyz, ax, ay              out: SmithLambda */
+    MF_SMITH_SEP = 3,   /* in: w xyz, alpha               out: SmithGGXSeprate */
+    MF_SAMPLE_GTR2 = 4, /* in: s.x, s.y, ax, ay           out: wh xyz */
+    MF_SAMPLE_GTR1 = 5, /* in: s.x, s.y, alpha            out: wh xyz */
+    MF_DIELECTRIC = 6,  /* in: eta_i, eta_t, cos_i        out: fr */
+    MF_SMITH_ANISO = 7  /* in: wo xyz, wi xyz, ax, ay     out: G */
+};
+template <typename R> static void mat_func_t(int which, const double* in, int64_t n, double* out) {
+    for (int64_t i = 0; i < n; i++) {
+        const double* a = in + 9 * i;
+        double* o = out + 8 * i;
+        const V3<R> w((R)a[0], (R)a[1], (R)a[2]);
+        V3<R> r(R(0));
+        switch (which) {
+            case MF_GTR2: o[0] = GTR2Aniso(w, (R)a[3], (R)a[4]); break;
+            case MF_GTR1: o[0] = GTR1(w, (R)a[3]); break;
+            case MF_LAMBDA: o[0] = SmithLambda(w, (R)a[3], (R)a[4]); break;
+            case MF_SMITH_SEP: o[0] = SmithGGXSep(w, (R)a[3]); break;
+            case MF_SAMPLE_GTR2: r = SampleGTR2((R)a[2], (R)a[3], V2<R>((R)a[0], (R)a[1])); break;
+            case MF_SAMPLE_GTR1: r = SampleGTR1((R)a[2], V2<R>((R)a[0], (R)a[1])); break;
+            case MF_DIELECTRIC: o[0] = DielectricFr((R)a[0], (R)a[1], (R)a[2]); break;
+            case MF_SMITH_ANISO:
+                o[0] = SmithGGXAniso(w, V3<R>((R)a[3], (R)a[4], (R)a[5]), (R)a[6], (R)a[7]); break;
+            default: break;
+        }
+        if (which == MF_SAMPLE_GTR2 || which == MF_SAMPLE_GTR1) { o[0] = r.x; o[1] = r.y; o[2] = r.z; }
+    }
+}
+
+/* Disney BSDF in its shading frame (ng = ns = z, ss = x): the whole BSDF and
+ * each lobe.  mat = baseColor rgb then the 14 scalar parameters in
+ * FM_D_METALLIC..FM_D_IOR order.  The BSDF is made per query from wo, so the
+ * lobe weights are those of that wo. */
+enum {
+    DL_F = 0, DL_PDF, DL_DIFFUSE, DL_SPECULAR, DL_CLEARCOAT, DL_TRANSMISSION, DL_FAKE_SS, DL_LAMBERT_TRANS,
+    DL_PDF_SPECULAR, DL_PDF_CLEARCOAT, DL_PDF_TRANSMISSION, DL_PDF_DIFFUSE, DL_PDF_LAMBERT_TRANS,
+    DL_SAMPLE  /* in: wo, -, s xyz; out: f rgb, pdf, wi xyz, delta */
+};
+template <typename R> static void disney_t(const double* mat, int thin, int which, const double* in, int64_t n, double* out) {
+    R p[15];
+    for (int k = 1; k < 15; k++) p[k] = (R)mat[2 + k];
+    const V3<R> base((R)mat[0], (R)mat[1], (R)mat[2]);
+    for (int64_t i = 0; i < n; i++) {
+        const double* a = in + 9 * i;
+        double* o = out + 8 * i;
+        HitPoint<R> h;
+        h.ng = h.ns = V3<R>(R(0), R(0), R(1));
+        h.ss = V3<R>(R(1), R(0), R(0));
+        h.wo = V3<R>((R)a[0], (R)a[1], (R)a[2]);
+        const DisneyBSDF<R> b(h, base, p, thin != 0);
+        const V3<R> wo = h.wo, wi((R)a[3], (R)a[4], (R)a[5]);
+        V3<R> f(R(0));
+        R s = 0;
+        switch (which) {
+            case DL_F: f = b.CalFuncLocal(wo, wi); break;
+            case DL_PDF: s = b.PdfLocal(wo, wi); break;
+            case DL_DIFFUSE: f = b.DiffEval(wo, wi); break;
+            case DL_SPECULAR: f = b.SpecEval(wo, wi); break;
+            case DL_CLEARCOAT: f = b.CCEval(wo, wi); break;
+            case DL_TRANSMISSION: f = b.MTEval(wo, wi, b.tax, b.tay, b.base); break;
+            case DL_FAKE_SS: f = b.FakeSSEval(wo, wi); break;
+            case DL_LAMBERT_TRANS: f = b.LTEval(wo, wi); break;
+            case DL_PDF_SPECULAR: s = b.SpecPdf(wo, wi); break;
+            case DL_PDF_CLEARCOAT: s = b.CCPdf(wo, wi); break;
+            case DL_PDF_TRANSMISSION: s = b.MTPdf(wo, wi); break;
+            case DL_PDF_DIFFUSE: s = b.DiffPdf(wo, wi); break;
+            case DL_PDF_LAMBERT_TRANS: s = b.LTPdf(wo, wi); break;
+            case DL_SAMPLE: {
+                const SampleInfo<R> si = b.SampleBSDF(wo, V3<R>((R)a[6], (R)a[7], (R)a[8]));
+                o[0] = si.f.x; o[1] = si.f.y; o[2] = si.f.z; o[3] = si.pdf;
+                o[4] = si.wi.x; o[5] = si.wi.y; o[6] = si.wi.z; o[7] = si.delta ? 1 : 0;
+                continue;
+            }
+            default: break;
+        }
+        o[0] = f.x; o[1] = f.y; o[2] = f.z; o[3] = s;
+    }
+}
+
 /* ---- output step (FP32 build: real = float) ----
  * ToneMapping::FilmicCore / Filmic (post/tonemapping.cpp:10-31),
  * SrgbToLinear / LinearToSrgb (core/utils.h:120-132), RealToUInt8
@@ -616,6 +701,12 @@ int or_render(void* p, const fm_render_params* prm, int x0, int y0, int w, int h
     or_handle* h = (or_handle*)p; return DISPATCH(render_t, prm, x0, y0, w, hh, nthreads, color, albedo, normal, sumsq);
 }
 void or_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) { philox4x32_10(ctr, key, out); }
+void or_mat_func(int which, int fp64, const double* in, int64_t n, double* out) {
+    if (fp64) mat_func_t<double>(which, in, n, out); else mat_func_t<float>(which, in, n, out);
+}
+void or_disney(const double* mat, int thin, int which, int fp64, const double* in, int64_t n, double* out) {
+    if (fp64) disney_t<double>(mat, thin, which, in, n, out); else disney_t<float>(mat, thin, which, in, n, out);
+}
 int or_abi_version(void) { return (int)FM_ABI_VERSION; }
 /* PostProcess::Process over the colour image in list order, then
  * Image::save_to_file's byte conversion (image.cpp:97-110). */
