@@ -32,6 +32,7 @@ FM_LIGHT_ENV, FM_LIGHT_DIRECTIONAL = 0, 1
 FM_CAMERA_PIN_HOLE, FM_CAMERA_THIN_LENS = 0, 1
 FM_ACCEL_BVH, FM_ACCEL_NAIVE, FM_ACCEL_BVH_REFERENCE_TREE = 0, 1, 2
 FM_MODE_MEGAKERNEL, FM_MODE_WAVEFRONT = 0, 1
+FM_NOALLOC_MAX_PIXELS = 262144   # include/fm_gpu.h
 FM_RENDER_TIME_TRACE = 1
 FM_RENDER_GRAPH = 2
 FM_DPARAM_PER_MATERIAL = 6
@@ -208,6 +209,8 @@ EXPORTS = {
     "fm_render": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(fm_render_params), C.POINTER(fm_film),
                             C.POINTER(fm_render_stats)]),
     "fm_film_read": (C.c_int, [C.c_void_p, C.POINTER(fm_film)]),
+    "fm_render_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(fm_render_params), C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.POINTER(fm_render_stats)]),
     "fm_render_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(fm_job), C.c_int, C.POINTER(fm_render_params),
                                   C.POINTER(C.c_double), C.POINTER(fm_batch_stats)]),
     "fm_set_counting": (C.c_int, [C.c_void_p, C.c_int]),

@@ -16,7 +16,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <set>
 #include <string>
 #include <utility>
@@ -130,6 +132,7 @@ struct fm_ctx {
     DevBuf adj_in, adj_gmat, adj_genv;                  /* adjoint: image gradient, FP64 parameter gradients */
     DevBuf adj_tot;                                     /* adjoint: path radiance per work item */
     DevBuf adj_film;                                    /* adjoint: FP64 sums of its forward run (unused) */
+    double* noalloc = nullptr;         /* this device's fm_render_noalloc working array */
     std::vector<hipEvent_t> trace_ev;  /* pairs around wavefront trace launches */
     std::vector<hipEvent_t> pass_ev;   /* end of each wavefront pass */
 };
@@ -527,6 +530,20 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
         }
 }
 
+/* Device address of fm_render_noalloc's static working array, looked up
+ * once per device (with that device current) and kept for the process. */
+double* noalloc_array(int device) {
+    static std::mutex mu;
+    static std::map<int, double*> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(device);
+    if (it != addr.end()) return it->second;
+    double* p = nullptr;
+    HIPCHECK(fm::noalloc_scratch(&p));
+    addr[device] = p;
+    return p;
+}
+
 }  // namespace
 
 extern "C" {
@@ -546,6 +563,7 @@ int fm_init(int device, fm_ctx** out) {
         HIPCHECK(hipEventCreate(&c->ev1));
         HIPCHECK(hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device));
         c->counters.alloc(4 * sizeof(unsigned long long));
+        c->noalloc = noalloc_array(device);
         return FM_OK;
     });
     if (rc != FM_OK) {
@@ -865,6 +883,70 @@ int fm_film_read(fm_ctx* ctx, fm_film* film) {
             if (film->normal) film->normal[k] = (float)(h[(size_t)(2 * n + k)] * inv);
             if (film->color_sum) film->color_sum[k] = h[(size_t)k];
             if (film->sumsq) film->sumsq[k] = h[(size_t)(3 * n + k)];
+        }
+        return FM_OK;
+    });
+}
+
+int fm_render_noalloc(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, void* stream, float* color, float* albedo,
+                      float* normal, fm_render_stats* stats) {
+    if (!ctx || !s || !p) return fail(ctx, FM_ERR_INVALID, "NULL argument");
+    return guarded(ctx, [&]() -> int {
+        const fm_renderer_desc& rd = s->hs.renderer;
+        const int W = p->width > 0 ? p->width : rd.width;
+        const int H = p->height > 0 ? p->height : rd.height;
+        const int spp = p->spp > 0 ? p->spp : rd.spp;
+        if (W <= 0 || H <= 0 || spp <= 0) return fail(ctx, FM_ERR_INVALID, "width, height and spp must be positive");
+        if ((int64_t)W * H > FM_NOALLOC_MAX_PIXELS)
+            return fail(ctx, FM_ERR_INVALID, "fm_render_noalloc: width * height exceeds FM_NOALLOC_MAX_PIXELS (" +
+                                                 std::to_string(FM_NOALLOC_MAX_PIXELS) + ")");
+        if (p->mode != FM_MODE_MEGAKERNEL) return fail(ctx, FM_ERR_INVALID, "fm_render_noalloc renders the megakernel form only");
+        if (p->accumulate) return fail(ctx, FM_ERR_INVALID, "fm_render_noalloc does not accumulate");
+        if (!ctx->noalloc) return fail(ctx, FM_ERR_INVALID, "no working array on this device");
+        const int block = p->block_size > 0 ? p->block_size : 256;
+        if (block % 64 != 0 || block > 256) return fail(ctx, FM_ERR_INVALID, "block_size must be a multiple of 64, at most 256");
+        fm::RenderArgs A{};
+        A.W = W; A.H = H; A.spp = spp;
+        A.cfg_raw.depth = p->depth > 0 ? p->depth : rd.depth;
+        A.cfg_raw.direct = p->direct_loop > 0 ? p->direct_loop : rd.direct_loop;
+        A.cfg_raw.rr_depth = p->rr_depth >= 0 ? p->rr_depth : rd.rr_depth;
+        A.cfg_raw.rr_coef = p->rr_coef > 0 ? p->rr_coef : rd.rr_coef;
+        if (A.cfg_raw.direct <= 0) return fail(ctx, FM_ERR_INVALID, "direct_loop must be positive");
+        A.seed = p->seed;
+        A.sample_offset = p->sample_offset;
+        const int64_t npix = (int64_t)W * H;
+        A.chunk = p->chunk > 0 ? std::min(p->chunk, spp) : std::min(spp, 4);
+        const int64_t nchunks = (spp + A.chunk - 1) / A.chunk;
+        A.total_items = npix * nchunks;
+        A.atomic = nchunks > 1 ? 1 : 0;
+        HIPCHECK(hipSetDevice(ctx->device));
+        hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+        /* counters, then the film: both cleared by a fill kernel (no memset node) */
+        double* work = ctx->noalloc;
+        double* film = work + fm::NOALLOC_COUNTER_DOUBLES;
+        A.counters = (unsigned long long*)work;
+        A.color_sum = film; A.albedo_sum = film + 3 * npix; A.normal_sum = film + 6 * npix; A.sumsq = film + 9 * npix;
+        HIPCHECK(fm::launch_zero_f64(work, fm::NOALLOC_COUNTER_DOUBLES + 12 * npix, st));
+        int grid = p->grid_blocks;
+        if (grid <= 0) grid = ctx->num_cus * std::max(1, fm::mega_blocks_per_cu(s->d, block)) * 4;
+        grid = (int)std::min<int64_t>(grid, (A.total_items + block - 1) / block);
+        if (stats) HIPCHECK(hipEventRecord(ctx->ev0, st));
+        HIPCHECK(fm::launch_render_mega(s->d, A, block, grid, ctx->counting != 0, st));
+        if (stats) HIPCHECK(hipEventRecord(ctx->ev1, st));
+        if (color || albedo || normal) HIPCHECK(fm::launch_film_mean(film, npix, 1.0 / (double)spp, color, albedo, normal, st));
+        if (stats) {
+            HIPCHECK(hipStreamSynchronize(st));
+            memset(stats, 0, sizeof(*stats));
+            float ms = 0.0f;
+            HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            unsigned long long c[4];
+            HIPCHECK(hipMemcpy(c, work, sizeof c, hipMemcpyDeviceToHost));
+            stats->kernel_ms = ms;
+            stats->paths = (int64_t)c[0];
+            stats->rays = (int64_t)c[1];
+            stats->node_visits = (int64_t)c[2];
+            stats->prim_tests = (int64_t)c[3];
+            if (p->flags & FM_RENDER_TIME_TRACE) stats->trace_ms = ms;
         }
         return FM_OK;
     });

@@ -213,6 +213,17 @@ __global__ void k_post(fm::PostOps ops, const float* __restrict__ in, int64_t n,
     }
 }
 
+/* Means over spp of the FP64 film sums into float images (any output may
+ * be null): the device side of fm_film_read for fm_render_noalloc. */
+__global__ void k_film_mean(const double* __restrict__ film, int64_t n3, double inv, float* __restrict__ color,
+                            float* __restrict__ albedo, float* __restrict__ normal) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n3; k += (int64_t)gridDim.x * blockDim.x) {
+        if (color) color[k] = (float)(film[k] * inv);
+        if (albedo) albedo[k] = (float)(film[n3 + k] * inv);
+        if (normal) normal[k] = (float)(film[2 * n3 + k] * inv);
+    }
+}
+
 __global__ void k_zero_f64(double* p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0.0;
 }
@@ -220,6 +231,14 @@ __global__ void k_zero_f64(double* p, int64_t n) {
 inline int blocks_for(int64_t n, int bs) { return (int)std::min<int64_t>((n + bs - 1) / bs, 1 << 20); }
 
 }  // namespace
+
+/* Working memory of fm_render_noalloc: the render counters (the first
+ * FM_NOALLOC_COUNTER_DOUBLES entries) and the FP64 film sums (colour,
+ * albedo, normal, sum of squares: 12 doubles per pixel) of at most
+ * FM_NOALLOC_MAX_PIXELS pixels, 25.2 MB.  Every GPU the library runs on
+ * holds its own copy until the process ends.  Host code never uses this
+ * name as an address: fm::noalloc_scratch looks the device address up. */
+static __device__ __align__(256) double scratch[fm::NOALLOC_COUNTER_DOUBLES + 12 * (size_t)FM_NOALLOC_MAX_PIXELS];
 
 namespace fm {
 
@@ -249,6 +268,18 @@ hipError_t launch_env(const fm_dscene& S, const float* dirs, int64_t n, float* p
 hipError_t launch_post(const PostOps& ops, const float* in, int64_t n, float* out, unsigned char* out8, hipStream_t st) {
     hipLaunchKernelGGL(k_post, dim3(blocks_for(n, 256)), dim3(256), 0, st, ops, in, n, out, out8);
     return hipGetLastError();
+}
+hipError_t launch_film_mean(const double* film, int64_t npix, double inv, float* color, float* albedo, float* normal,
+                            hipStream_t st) {
+    hipLaunchKernelGGL(k_film_mean, dim3(blocks_for(3 * npix, 256)), dim3(256), 0, st, film, 3 * npix, inv, color, albedo,
+                       normal);
+    return hipGetLastError();
+}
+hipError_t noalloc_scratch(double** out) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    *out = e == hipSuccess ? (double*)p : nullptr;
+    return e;
 }
 hipError_t launch_zero_f64(double* p, int64_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_f64, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, n);

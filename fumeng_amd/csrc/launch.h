@@ -159,6 +159,15 @@ hipError_t launch_bsdf(const fm_dscene& S, const fm_bsdf_query* q, int64_t n, fm
 hipError_t launch_light(const fm_dscene& S, const fm_light_query* q, int64_t n, fm_light_result* out, hipStream_t st);
 hipError_t launch_env(const fm_dscene& S, const float* dirs, int64_t n, float* pdf, float* rgb, hipStream_t st);
 hipError_t launch_zero_f64(double* p, int64_t n, hipStream_t st);
+/* color/albedo/normal[k] = film sums * inv for the 3 npix channels of each
+ * image (film laid out as in RenderArgs: colour, albedo, normal, sumsq). */
+hipError_t launch_film_mean(const double* film, int64_t npix, double inv, float* color, float* albedo, float* normal,
+                            hipStream_t st);
+/* Doubles at the start of fm_render_noalloc's static working array that
+ * hold the render counters (four u64, padded to 256 bytes). */
+enum { NOALLOC_COUNTER_DOUBLES = 32 };
+/* Device address of that array on the current device (hipGetSymbolAddress). */
+hipError_t noalloc_scratch(double** out);
 struct PostOps {
     int n;
     fm_post_op op[FM_MAX_POST_OPS];

@@ -305,6 +305,25 @@ int fm_render(fm_ctx* ctx, fm_scene* scene, const fm_render_params* params,
 /* Copies the current device film (means over the samples rendered so far). */
 int fm_film_read(fm_ctx* ctx, fm_film* film);
 
+/* Largest frame (width * height) of fm_render_noalloc. */
+#define FM_NOALLOC_MAX_PIXELS 262144
+/* fm_render with the megakernel form for callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, hosts that
+ * forbid hipMalloc.  No counterpart in the reference.  The call allocates
+ * nothing: its FP64 film sums and counters live in one static device array
+ * per GPU (25.2 MB, held until the process ends), set up by fm_init.
+ * Inputs are the scene and params (mode FM_MODE_MEGAKERNEL, accumulate 0,
+ * width * height <= FM_NOALLOC_MAX_PIXELS; anything else is FM_ERR_INVALID);
+ * outputs are caller-owned DEVICE arrays of width * height * 3 floats (means
+ * over spp, row 0 = top; any may be NULL), written on `stream` (a
+ * hipStream_t; NULL = the context's stream).  With stats == NULL the call
+ * only enqueues work (it can be captured into a graph; synchronize the
+ * stream before reading the outputs); with stats it waits and fills them.
+ * Every call on a GPU shares that GPU's array, which keeps the last call's
+ * contents: calls that overlap on one GPU race. */
+int fm_render_noalloc(fm_ctx* ctx, fm_scene* scene, const fm_render_params* params, void* stream, float* color,
+                      float* albedo, float* normal, fm_render_stats* stats);
+
 /* ---- output step -------------------------------------------------------- */
 
 /* The scene's post-process list; *n_ops receives its length (at most
