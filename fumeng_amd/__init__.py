@@ -84,6 +84,10 @@ class Context:
         b = A.BUILDERS[builder] if isinstance(builder, str) else int(builder)
         _check(lib().fm_set_bvh_builder(self.h, b), self.h)
 
+    def set_path_budget(self, nbytes=0):
+        """Bytes of wavefront path state per pass (fm_set_path_budget; 0 = default)."""
+        _check(lib().fm_set_path_budget(self.h, int(nbytes)), self.h)
+
     def set_trace_stack(self, form):
         """Stack form of the wavefront trace kernel: "auto" (default), "full"
         (all entries in LDS) or "short" (16 in LDS, the rest spilled to

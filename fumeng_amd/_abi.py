@@ -209,6 +209,7 @@ EXPORTS = {
     "fm_render": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(fm_render_params), C.POINTER(fm_film),
                             C.POINTER(fm_render_stats)]),
     "fm_film_read": (C.c_int, [C.c_void_p, C.POINTER(fm_film)]),
+    "fm_set_path_budget": (C.c_int, [C.c_void_p, C.c_int64]),
     "fm_render_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(fm_render_params), C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.POINTER(fm_render_stats)]),
     "fm_render_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(fm_job), C.c_int, C.POINTER(fm_render_params),

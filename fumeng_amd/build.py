@@ -19,7 +19,14 @@ CLI = os.path.join(HERE, "bin", "fumeng_gpu")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = "gfx950"
 
-HIP_FLAGS = ["-O3", "--offload-arch=" + ARCH, "-std=c++17", "-fPIC",
+# -ffp-contract=off: no a*b+c is fused unless the code asks for fmaf, as in
+# the reference's x86-64 build.  Fused products round differently, and the
+# reference's truncated shear axis (triangle.cpp:13-18) makes the triangle
+# test ill-conditioned for rays with a small z component, so one ulp in a ray
+# can change a hit.  Unfused, renders with common random numbers equal the
+# oracle's on more pixels (box: 98.7 % -> 99.4 %), the two forms equal each
+# other on all of them, and the frame is no slower (494.8 -> 487.8 ms).
+HIP_FLAGS = ["-O3", "--offload-arch=" + ARCH, "-std=c++17", "-fPIC", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
              "-I" + os.path.join(ROCM, "include"), "-D__HIP_PLATFORM_AMD__"]

@@ -125,6 +125,7 @@ struct fm_ctx {
     int bvh_width = 2;        /* traversal tree width of scenes created next */
     int bvh_builder = FM_BUILDER_HOST;   /* traversal tree builder of scenes created next */
     int trace_stack = FM_TRACE_STACK_AUTO;   /* wavefront trace kernel's stack form */
+    int64_t path_budget = 0;  /* bytes of wavefront path state a pass may use (fixed per context) */
     std::set<fm_scene*> scenes;  /* live scenes, released by fm_shutdown */
     WfBuffers wf;
     DevBuf wf_passes, wf_units, wf_cams, wf_pass_idx;   /* wavefront tables of the last run */
@@ -359,14 +360,12 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
     if (path_out && jobs.size() != 1) throw std::invalid_argument("per-path output needs exactly one job");
     const int NL = s->d.num_lights;
     const int64_t per_path = 100 + 64 * (1 + (int64_t)D) + 48 * (int64_t)D + 57 * (int64_t)D * std::max(NL, 1);
-    /* Passes as large as a fixed 96 GB of path state allows (a third of the
-     * device's HBM): fewer passes mean fewer nearly empty tail iterations
-     * (1920x1080 at 256 spp: 8 passes took 522 ms, 2 took 493). The budget
-     * is a constant so that a frame's pass split, and with it the film's FP32
-     * sum order, does not depend on what else is allocated; the index bound
-     * keeps every queue entry inside int32. */
+    /* Passes as large as the context's path-state budget allows (96 GB on
+     * an idle device): fewer passes mean fewer nearly empty tail iterations
+     * (1920x1080 at 256 spp: 8 passes took 522 ms, 2 took 493).  The index
+     * bound keeps every queue entry inside int32. */
     const int64_t idx_cap = (((int64_t)1 << 31) - 1) / std::max<int64_t>(1 + D, (int64_t)D * std::max(NL, 1));
-    const int64_t budget = std::max<int64_t>(1, std::min<int64_t>(idx_cap, (int64_t)96e9 / per_path));
+    const int64_t budget = std::max<int64_t>(1, std::min<int64_t>(idx_cap, ctx->path_budget / per_path));
     std::vector<fm::WfUnit> units;
     std::vector<fm::WfPass> passes;
     std::vector<int> last_pass(jobs.size(), -1);
@@ -394,6 +393,11 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
             if (ps.n_units > 0) {
                 const int64_t fit = (budget - used) / npix;
                 if (fit <= 0) continue;
+                /* with a chunk, a unit that does not fit whole waits for the
+                 * next pass: units are then chunk samples (the last one the
+                 * remainder) in a batch as alone, and so are the FP32 sums
+                 * the film adds */
+                if (chunk > 0 && fit < k) continue;
                 k = std::min(k, fit);
             }
             fm::WfUnit u{};
@@ -530,6 +534,16 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
         }
 }
 
+/* Wavefront path state a pass may hold: 96 GB (a third of a 288 GB
+ * device), or half the memory free when the context is created if that is
+ * less.  It is fixed per context, so a frame's pass split, and with it the
+ * film's FP32 sum order, does not change from call to call. */
+int64_t default_path_budget() {
+    size_t free_b = 0, total_b = 0;
+    HIPCHECK(hipMemGetInfo(&free_b, &total_b));
+    return std::max<int64_t>((int64_t)1 << 26, std::min<int64_t>((int64_t)96e9, (int64_t)(free_b / 2)));
+}
+
 /* Device address of fm_render_noalloc's static working array, looked up
  * once per device (with that device current) and kept for the process. */
 double* noalloc_array(int device) {
@@ -564,6 +578,7 @@ int fm_init(int device, fm_ctx** out) {
         HIPCHECK(hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device));
         c->counters.alloc(4 * sizeof(unsigned long long));
         c->noalloc = noalloc_array(device);
+        c->path_budget = default_path_budget();
         return FM_OK;
     });
     if (rc != FM_OK) {
@@ -767,6 +782,16 @@ int fm_set_trace_stack(fm_ctx* ctx, int32_t form) {
     return FM_OK;
 }
 
+int fm_set_path_budget(fm_ctx* ctx, int64_t bytes) {
+    if (!ctx) return fail(nullptr, FM_ERR_INVALID, "NULL ctx");
+    if (bytes < 0) return fail(ctx, FM_ERR_INVALID, "budget must be >= 0");
+    return guarded(ctx, [&]() -> int {
+        HIPCHECK(hipSetDevice(ctx->device));
+        ctx->path_budget = bytes > 0 ? bytes : default_path_budget();
+        return FM_OK;
+    });
+}
+
 int fm_set_bvh_builder(fm_ctx* ctx, int32_t builder) {
     if (!ctx) return fail(nullptr, FM_ERR_INVALID, "NULL ctx");
     if (builder != FM_BUILDER_HOST && builder != FM_BUILDER_DEVICE) return fail(ctx, FM_ERR_INVALID, "unknown BVH builder");
@@ -821,7 +846,13 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
         HIPCHECK(fm::launch_zero_f64((double*)A.counters, 4, ctx->stream));  /* a fill kernel, not a memset node */
         const int block = p->block_size > 0 ? p->block_size : 256;
         if (block % 64 != 0 || block > 256) return fail(ctx, FM_ERR_INVALID, "block_size must be a multiple of 64, at most 256");
-        if (!wave) {
+        /* depth <= 0: the reference's bounce loop (path_tracing.cpp:42, for
+         * idx_depth = 1 .. m_depth) never runs and every sample is black */
+        const bool black = A.cfg_raw.depth <= 0;
+        if (black) {
+            HIPCHECK(hipEventRecord(ctx->ev0, ctx->stream));
+            HIPCHECK(hipEventRecord(ctx->ev1, ctx->stream));
+        } else if (!wave) {
             int grid = p->grid_blocks;
             if (grid <= 0) {
                 int per_cu = fm::mega_blocks_per_cu(s->d, block);
@@ -836,7 +867,7 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
         } else {
             WfJob job{};
             job.cam = s->d.camera;
-            job.W = W; job.H = H; job.spp = spp; job.depth = std::max(A.cfg_raw.depth, 1);
+            job.W = W; job.H = H; job.spp = spp; job.depth = A.cfg_raw.depth;
             job.sample_offset = A.sample_offset; job.seed = A.seed; job.film = base;
             WfRunOut R;
             wf_run(ctx, s, {job}, A.cfg_raw.direct, A.cfg_raw.rr_depth, A.cfg_raw.rr_coef, p->block_size > 0 ? block : 0, p->grid_blocks,
@@ -854,7 +885,7 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
             unsigned long long c[4];
             HIPCHECK(hipMemcpy(c, A.counters, sizeof c, hipMemcpyDeviceToHost));
             stats->kernel_ms = ms;
-            stats->paths = (int64_t)c[0];
+            stats->paths = black ? npix * spp : (int64_t)c[0];
             stats->rays = (int64_t)c[1];
             stats->node_visits = (int64_t)c[2];
             stats->prim_tests = (int64_t)c[3];
@@ -930,8 +961,9 @@ int fm_render_noalloc(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, void*
         int grid = p->grid_blocks;
         if (grid <= 0) grid = ctx->num_cus * std::max(1, fm::mega_blocks_per_cu(s->d, block)) * 4;
         grid = (int)std::min<int64_t>(grid, (A.total_items + block - 1) / block);
+        const bool black = A.cfg_raw.depth <= 0;   /* no bounce runs (path_tracing.cpp:42) */
         if (stats) HIPCHECK(hipEventRecord(ctx->ev0, st));
-        HIPCHECK(fm::launch_render_mega(s->d, A, block, grid, ctx->counting != 0, st));
+        if (!black) HIPCHECK(fm::launch_render_mega(s->d, A, block, grid, ctx->counting != 0, st));
         if (stats) HIPCHECK(hipEventRecord(ctx->ev1, st));
         if (color || albedo || normal) HIPCHECK(fm::launch_film_mean(film, npix, 1.0 / (double)spp, color, albedo, normal, st));
         if (stats) {
@@ -942,7 +974,7 @@ int fm_render_noalloc(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, void*
             unsigned long long c[4];
             HIPCHECK(hipMemcpy(c, work, sizeof c, hipMemcpyDeviceToHost));
             stats->kernel_ms = ms;
-            stats->paths = (int64_t)c[0];
+            stats->paths = black ? npix * spp : (int64_t)c[0];
             stats->rays = (int64_t)c[1];
             stats->node_visits = (int64_t)c[2];
             stats->prim_tests = (int64_t)c[3];
@@ -989,9 +1021,20 @@ int fm_render_batch(fm_ctx* ctx, fm_scene* s, const fm_job* jobs, int n, const f
         for (int i = 0; i < n; i++) wj[(size_t)i].film = film + off[(size_t)i];
         HIPCHECK(fm::launch_zero_f64(film, off[(size_t)n], ctx->stream));
         HIPCHECK(fm::launch_zero_f64((double*)ctx->counters.p, 4, ctx->stream));
+        /* a job of depth <= 0 keeps its zeroed film (no bounce runs,
+         * path_tracing.cpp:42) and is left out of the run */
+        std::vector<WfJob> run;
+        std::vector<int> run_idx;
+        for (int i = 0; i < n; i++)
+            if (wj[(size_t)i].depth > 0) { run.push_back(wj[(size_t)i]); run_idx.push_back(i); }
         WfRunOut R;
-        wf_run(ctx, s, wj, D, rr_depth, rr_coef, cp.block_size > 0 ? block : 0, cp.grid_blocks, cp.chunk, cp.flags, R);
-        if (job_done_ms) for (int i = 0; i < n; i++) job_done_ms[i] = R.job_done_ms[(size_t)i];
+        R.job_done_ms.assign(run.size(), 0.0);
+        if (!run.empty())
+            wf_run(ctx, s, run, D, rr_depth, rr_coef, cp.block_size > 0 ? block : 0, cp.grid_blocks, cp.chunk, cp.flags, R);
+        if (job_done_ms) {
+            for (int i = 0; i < n; i++) job_done_ms[i] = 0.0;
+            for (size_t r = 0; r < run.size(); r++) job_done_ms[run_idx[r]] = R.job_done_ms[r];
+        }
         if (stats) {
             memset(stats, 0, sizeof(*stats));
             unsigned long long c[4];
@@ -1164,9 +1207,16 @@ int fm_render_adjoint(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, const
         if (grid > need) grid = (int)need;
         /* each work item's radiance: a wavefront run of the same paths (the
          * (pixel, sample) Philox streams of the walk) writes it per path */
+        if (A.cfg_raw.depth <= 0) {   /* black image (path_tracing.cpp:42): every gradient is 0 */
+            HIPCHECK(hipStreamSynchronize(st));
+            if (g_material) memset(g_material, 0, (size_t)nm * FM_DPARAM_PER_MATERIAL * sizeof(double));
+            if (g_env && nenv > 0) memset(g_env, 0, (size_t)nenv * sizeof(double));
+            if (stats) { memset(stats, 0, sizeof(*stats)); stats->paths = A.total_items; }
+            return FM_OK;
+        }
         WfJob job{};
         job.cam = s->d.camera;
-        job.W = A.W; job.H = A.H; job.spp = A.spp; job.depth = std::max(A.cfg_raw.depth, 1);
+        job.W = A.W; job.H = A.H; job.spp = A.spp; job.depth = A.cfg_raw.depth;
         job.sample_offset = A.sample_offset; job.seed = A.seed;
         if (ctx->adj_film.n < (size_t)npix * 12 * sizeof(double)) ctx->adj_film.alloc((size_t)npix * 12 * sizeof(double));
         HIPCHECK(fm::launch_zero_f64(ctx->adj_film.as<double>(), npix * 12, st));

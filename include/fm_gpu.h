@@ -362,8 +362,9 @@ typedef struct fm_batch_stats {
 } fm_batch_stats;
 
 /* Renders n jobs in one sequence of wavefront passes: each pass packs
- * samples of several jobs into one set of path queues (jobs in order, about
- * 64 M paths per pass), so renders of uneven size, spp and depth share the
+ * samples of several jobs into one set of path queues (jobs in order, as
+ * many paths per pass as the context's path budget holds, see
+ * fm_set_path_budget), so renders of uneven size, spp and depth share the
  * device.  `common` (may be NULL) gives direct_loop, rr_depth, rr_coef,
  * block_size, grid_blocks, chunk (max samples of a job per pass) and flags
  * (FM_RENDER_GRAPH replays one captured pass graph).  job_done_ms (may be
@@ -430,6 +431,12 @@ int fm_set_counting(fm_ctx* ctx, int enabled);
  * hit does not depend on the width (ties are resolved by reference slot).  No counterpart in the reference (whose BVHAggregate
  * is binary, bvh_aggregate.cpp:38-79): a device traversal parameter. */
 int fm_set_bvh_width(fm_ctx* ctx, int32_t width);
+/* Bytes of wavefront path state one pass may hold on this context (0 =
+ * the default: 96 GB, or half the device memory free at fm_init if that is
+ * less).  A frame that needs more is rendered in several passes of equal
+ * sample counts; results do not depend on the split except for the order of
+ * FP32 additions in the film.  No counterpart in the reference. */
+int fm_set_path_budget(fm_ctx* ctx, int64_t bytes);
 /* Traversal tree builders of fm_set_bvh_builder. */
 enum {
     FM_BUILDER_HOST = 0,    /* binned SAH on the host build threads (default) */

@@ -171,3 +171,19 @@ def mesh_light_scene(accelerator="embree", width=16, height=16, spp=64, depth=3)
     m = s.add_mesh(pos, np.array(idx))
     s.add_mesh_object(m, emittance=[6.0, 5.0, 4.0])
     return s
+
+
+def crn_sample_agreement(g, o, width, height, spp, seed, mode, tol=1e-3, threads=8):
+    """Common random numbers, sample by sample: the fraction of (pixel,
+    sample) radiances of the GPU render `g` that equal the oracle's `o` to
+    `tol` relative (both NaN counts as equal).  Each sample is rendered as a
+    1-spp frame at its own sample offset, so the figure does not depend on
+    spp the way a per-pixel mean does."""
+    ok = []
+    for k in range(spp):
+        cg = g.render(width=width, height=height, spp=1, seed=seed, mode=mode, sample_offset=k)[0]
+        co = o.render(0, 0, width, height, spp=1, seed=seed, sample_offset=k, width=width, height=height,
+                      threads=threads)[0]
+        same = np.abs(cg - co) <= tol * np.maximum(1.0, np.abs(co))
+        ok.append(np.all(same | (np.isnan(cg) & np.isnan(co)), axis=2))
+    return float(np.mean(ok))

@@ -10,6 +10,7 @@ every replay gives the direct launches' film, bit for bit.
 import numpy as np
 import pytest
 
+import oracle
 from fumeng_amd import _abi as A
 from fumeng_amd import FumengError, scenes
 from helpers import mesh_scene
@@ -108,3 +109,92 @@ def test_autotune_picks_a_timed_configuration_that_renders_the_same_image(gpu_ct
         got = g.render(seed=3, mode=mode, block_size=b["block_size"], grid_blocks=b["grid_blocks"], chunk=b["chunk"])
         for a, b_ in zip(got[:3], ref[:3]):
             assert np.allclose(a, b_, rtol=1e-6, atol=1e-7, equal_nan=True)
+
+
+def test_lowered_path_budget_splits_a_frame_into_equal_passes(gpu_ctx):
+    """With chunk 0 a frame larger than the path-state budget runs in passes
+    of equal sample counts: 24 spp under a budget of 7 samples' worth of
+    paths gives 4 passes of 6, bit for bit the film of an explicit chunk of
+    6.  (Box scene: one light, one direct loop, 333 B of path state per path.)"""
+    g = gpu_ctx.scene(scenes.box_scene(width=64, height=48, spp=24))
+    per_path = 100 + 64 * 2 + 48 + 57
+    try:
+        gpu_ctx.set_path_budget(7 * 64 * 48 * per_path)
+        films, _, st = g.render_batch([{"camera": None, "seed": 3}], chunk=0)
+        assert st.passes == 4 and st.units == 4
+        wf = g.render(seed=3, mode=A.FM_MODE_WAVEFRONT, chunk=0)
+    finally:
+        gpu_ctx.set_path_budget(0)
+    ref, _, st6 = g.render_batch([{"camera": None, "seed": 3}], chunk=6)
+    assert st6.passes == 4
+    for a, b in zip(films[0], ref[0]):
+        assert np.array_equal(a, b, equal_nan=True)
+    wf6 = g.render(seed=3, mode=A.FM_MODE_WAVEFRONT, chunk=6)
+    for k in range(3):
+        assert np.array_equal(wf[k], wf6[k], equal_nan=True)
+
+
+def test_depth_zero_renders_black_like_the_reference(gpu_ctx):
+    """The reference's bounce loop runs for idx_depth = 1 .. depth
+    (path_tracing.cpp:42): depth 0 returns black colour, albedo and normal,
+    camera misses included; so does every form here, and a depth-0 job of a
+    batch leaves the other jobs as they render alone."""
+    s = scenes.box_scene(width=16, height=16, spp=4)
+    s.renderer["depth"] = 0
+    ref = oracle.OracleScene(s).render(0, 0, 16, 16, spp=4, seed=2)
+    assert all((r == 0).all() for r in ref[:3])
+    g = gpu_ctx.scene(s)
+    for mode in (A.FM_MODE_MEGAKERNEL, A.FM_MODE_WAVEFRONT):
+        out = g.render(seed=2, mode=mode)
+        assert out[4].paths == 16 * 16 * 4
+        for k in range(3):
+            assert (out[k] == 0).all(), (mode, k)
+    jobs = [{"camera": None, "seed": 2}, {"camera": None, "seed": 2, "depth": 3}]
+    films, done, _ = g.render_batch(jobs, chunk=4)
+    assert all((f == 0).all() for f in films[0][:3]) and done[0] == 0.0
+    solo, _, _ = g.render_batch(jobs[1:], chunk=4)
+    assert np.array_equal(films[1][0], solo[0][0]) and films[1][0].mean() > 0
+
+
+def test_config4_uneven_box_batch(gpu_ctx):
+    """The batched-jobs workload at full size: 64 renders of the box scene
+    from seeded cameras on a radius-4 sphere, r x r with r ~ U{128..1024},
+    spp ~ U{16..256}, depth ~ U{4..16} (4.12e9 paths), in one batch of
+    wavefront passes with the captured pass graph.  With units of 16
+    samples, four seeded jobs give bit for bit the films they give alone;
+    on 16x16 crops of two jobs, sample by sample, the radiances equal the
+    oracle's with common random numbers.  The per-frame loop this replaces is Renderer::DrawFrame
+    (renderer.h:84-144), one process per render."""
+    s = scenes.box_scene()
+    g = gpu_ctx.scene(s)
+    jobs = scenes.batch_jobs(64)
+    films, done, st = g.render_batch(jobs, chunk=16, graph=True)
+    total = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
+    print("config 4: %d paths, %d passes, %d units, %.1f ms" % (st.paths, st.passes, st.units, st.kernel_ms))
+    assert st.paths == total
+    assert np.all(done > 0) and np.all(done <= st.kernel_ms * 1.001)
+    assert all(np.isfinite(f[0]).mean() > 0.999 for f in films)
+    # cameras behind a wall see only its unlit outside; those through the open side see the light
+    lit = [i for i, f in enumerate(films) if np.nanmean(f[0]) > 0.01]
+    assert len(lit) >= 8, len(lit)
+    rng = np.random.default_rng(64)
+    for i in rng.choice(64, 4, replace=False):
+        solo, _, _ = g.render_batch([jobs[i]], chunk=16, graph=True)
+        for a, b in zip(films[i], solo[0]):
+            assert np.array_equal(a, b, equal_nan=True), i
+    # common random numbers sample by sample: the job's first 16 samples as
+    # 1-spp jobs at their sample offsets, against the oracle on a 16x16 crop
+    for i in rng.choice(lit, 2, replace=False):
+        j = jobs[i]
+        js = scenes.box_scene(width=j["width"], height=j["height"], spp=j["spp"], depth=j["depth"])
+        js.camera = dict(j["camera"])
+        o = oracle.OracleScene(js)
+        x0, y0 = j["width"] // 2 - 8, j["height"] // 2 - 8
+        ones, _, _ = g.render_batch([dict(j, spp=1, sample_offset=k) for k in range(16)], chunk=1)
+        ok = []
+        for k in range(16):
+            ref = o.render(x0, y0, 16, 16, spp=1, seed=j["seed"], sample_offset=k, threads=8)[0]
+            crop = ones[k][0][y0:y0 + 16, x0:x0 + 16]
+            ok.append(np.all(np.abs(crop - ref) <= 1e-3 * np.maximum(1.0, np.abs(ref)), axis=2))
+        print("job", i, "crop samples equal to 1e-3:", np.mean(ok))
+        assert np.mean(ok) >= 0.995, (i, np.mean(ok))

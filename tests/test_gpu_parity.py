@@ -9,9 +9,17 @@ Bars (stated per test):
   hit fill    pos / normals / uv within 1e-5 x max(1, |value|)
   BSDF        f / pdf / sampled direction and values: error against the FP64
               oracle no worse than the FP32 oracle's (median, p99, max); the
-              device uses algebraic microfacet terms instead of acos/atan2/tan
-  render      common random numbers: most pixels equal to the FP32 oracle
-              within 1e-3, image means within statistical agreement
+              device uses algebraic microfacet terms instead of acos/atan2/tan;
+              f and pdf on given directions within 1e-5 relative on >= 99.9 %
+              of well-conditioned queries
+  render      common random numbers: >= 99.5 % of (pixel, sample) radiances
+              equal to the FP32 oracle's within 1e-3 (1-spp frames), >= 99 %
+              of pixels of the box scene's 16-sample means; image means
+              within statistical agreement.  What differs is one-ulp noise
+              of the transcendentals (neither the device's nor glibc's float
+              sin/cos/acos/atan2 are correctly rounded) that the reference's
+              truncated shear axis (triangle.cpp:13-18) can turn into another
+              hit for rays with a small z component
 """
 import numpy as np
 import pytest
@@ -20,7 +28,7 @@ import oracle
 from fumeng_amd import _abi as A
 from fumeng_amd import scenes
 
-from helpers import bsdf_queries, mesh_scene, random_sphere_dirs, scene_rays, unit
+from helpers import bsdf_queries, crn_sample_agreement, mesh_scene, random_sphere_dirs, scene_rays, unit
 
 pytestmark = pytest.mark.gpu
 
@@ -177,9 +185,24 @@ def _no_worse_than_fp32(g, a32, a64, floor, name):
     assert eg.max() <= max(4 * er.max(), 1e-3), (name, "max", eg.max(), er.max())
 
 
+def _within_1e5_where_well_conditioned(g, a32, a64, floor, name):
+    """The north-star bar: BSDF values within 1e-5 relative of the FP64
+    oracle on at least 99.9 % of the well-conditioned queries (those where
+    the FP32 oracle itself is within 1e-6 of FP64); the rest are logged."""
+    r32 = _rel(a32, a64, floor).ravel()
+    rg = _rel(g, a64, floor).ravel()
+    wc = r32 <= 1e-6
+    ok = rg[wc] <= 1e-5
+    print("%s: %.2f%% well-conditioned, %.4f%% of them within 1e-5 (worst %.2e); ill-conditioned: %d, worst %.2e"
+          % (name, 100 * wc.mean(), 100 * ok.mean(), rg[wc].max(initial=0), (~wc).sum(), rg[~wc].max(initial=0)))
+    assert ok.mean() >= 0.999, (name, ok.mean())
+
+
 def _check_bsdf(rg, ro32, ro64):
     for k in ("f", "pdf", "sf", "spdf"):
         _no_worse_than_fp32(rg[k], ro32[k], ro64[k], 1e-4, k)
+    for k in ("f", "pdf"):   # on given directions; sampled values also carry the direction's rounding
+        _within_1e5_where_well_conditioned(rg[k], ro32[k], ro64[k], 1e-4, k)
     _no_worse_than_fp32(rg["swi"], ro32["swi"], ro64["swi"], 1.0, "swi")
     assert np.array_equal(rg["sdelta"], ro32["sdelta"])
     assert np.all(_rel(rg["albedo"], ro32["albedo"], 1e-6) < 1e-6)
@@ -213,10 +236,13 @@ def test_conductor_sphere_render_matches_oracle(gpu_ctx, mode):
     spheres[0]["material"] = cond
     g = gpu_ctx.scene(s)
     cg, ag, _, _, st = g.render(seed=7, mode=mode)
-    co, ao, _, _ = oracle.OracleScene(s).render(0, 0, 32, 32, spp=16, seed=7, threads=8)
+    o = oracle.OracleScene(s)
+    co, ao, _, _ = o.render(0, 0, 32, 32, spp=16, seed=7, threads=8)
     close = np.abs(cg - co) <= 1e-3 * np.maximum(1.0, np.abs(co))
-    print("pixels equal to 1e-3:", close.mean(), "mean", cg.mean(), co.mean())
-    assert close.mean() > 0.9
+    fs = crn_sample_agreement(g, o, 32, 32, 16, 7, mode)
+    print("pixels equal to 1e-3:", close.mean(), "samples:", fs, "mean", cg.mean(), co.mean())
+    assert fs >= 0.995
+    assert close.mean() >= 0.99
     assert abs(cg.mean() - co.mean()) < 0.02 * co.mean()
     assert np.mean(np.abs(ag - ao) < 1e-5) > 0.95
 
@@ -298,10 +324,13 @@ def test_box_render_matches_oracle_with_common_random_numbers(gpu_ctx):
     s = scenes.box_scene(width=32, height=32, spp=16)
     g = gpu_ctx.scene(s)
     cg, ag, ng, qg, st = g.render(seed=7)
-    co, ao, no, qo = oracle.OracleScene(s).render(0, 0, 32, 32, spp=16, seed=7, threads=8)
+    o = oracle.OracleScene(s)
+    co, ao, no, qo = o.render(0, 0, 32, 32, spp=16, seed=7, threads=8)
     close = np.abs(cg - co) <= 1e-3 * np.maximum(1.0, np.abs(co))
-    print("pixels equal to 1e-3:", close.mean(), "mean", cg.mean(), co.mean())
-    assert close.mean() > 0.9
+    fs = crn_sample_agreement(g, o, 32, 32, 16, 7, A.FM_MODE_MEGAKERNEL)
+    print("pixels equal to 1e-3:", close.mean(), "samples:", fs, "mean", cg.mean(), co.mean())
+    assert fs >= 0.995
+    assert close.mean() >= 0.99
     assert abs(cg.mean() - co.mean()) < 0.02 * co.mean()
     assert np.mean(np.abs(ag - ao) < 1e-5) > 0.95
     assert st.paths == 32 * 32 * 16
@@ -363,7 +392,8 @@ def test_embree_mesh_light_parity(gpu_ctx):
     for mode in (A.FM_MODE_WAVEFRONT, A.FM_MODE_MEGAKERNEL):
         c = g.render(seed=4, mode=mode)[0]
         close = np.abs(c - ref) <= 1e-3 * np.maximum(1.0, np.abs(ref))
-        assert close.mean() > 0.9, (mode, close.mean())
+        print(mode, "pixels equal to 1e-3:", close.mean())
+        assert close.mean() >= 0.99, (mode, close.mean())
     b = gpu_ctx.scene(mesh_light_scene("bvh", width=24, height=24, spp=64))
     assert b.info().num_lights == 12
     e_img, _, _, e_sq, _ = g.render(seed=9, spp=1024, mode=A.FM_MODE_WAVEFRONT)
@@ -371,3 +401,39 @@ def test_embree_mesh_light_parity(gpu_ctx):
     var = lambda c, sq: (np.maximum(sq / 1024 - c.astype(np.float64) ** 2, 0) / 1024).sum() / c.size ** 2
     z = (e_img.mean() - b_img.mean()) / np.sqrt(var(e_img, e_sq) + var(b_img, b_sq))
     assert abs(z) < 4, z
+
+
+def test_directional_light_parity(gpu_ctx):
+    """DirectionalLight (directional_light.cpp:11-31): a delta light with
+    pdf 1, wi = -direction and an infinite distance; MisLight's delta branch
+    adds f L |cos| with no MIS weight (path_tracing.cpp:122-161).  Light
+    samples equal the oracle's; with an env light beside it, renders with
+    common random numbers equal the oracle's in both forms."""
+    s = mesh_scene(freq=8)
+    s.add_env(s.add_image(scenes.sky_env(64, 32, 5)), factor=0.5)
+    ldir = unit([0.3, 0.5, -1.0])
+    s.add_directional([2.0, 1.8, 1.5], ldir)
+    s.renderer.update(width=24, height=24, spp=16, depth=4)
+    g, o = gpu_ctx.scene(s), oracle.OracleScene(s)
+    nl = g.info().num_lights
+    assert nl == o.info()["lights"]
+    rng = np.random.default_rng(63)
+    n = 2048
+    q = np.zeros(n, dtype=np.dtype(A.fm_light_query))
+    q["light"] = nl - 1
+    q["pos"] = rng.uniform(-1, 1, (n, 3))
+    q["sample"] = rng.random((n, 3))
+    rg, ro = g.light_sample(q), o.light_sample(q)
+    np.testing.assert_allclose(rg["wi"], np.broadcast_to(-ldir, (n, 3)), atol=1e-6)
+    for k in ("wi", "radiance", "ref_pos"):
+        assert np.array_equal(rg[k], ro[k]), k
+    assert (rg["pdf"] == 1).all() and (ro["pdf"] == 1).all()
+    assert np.all(np.abs(rg["light_pos"] - ro["light_pos"]) <= 1e-6 * np.abs(ro["light_pos"]).max())
+    ref = o.render(0, 0, 24, 24, spp=16, seed=6, threads=8)[0]
+    for mode in (A.FM_MODE_MEGAKERNEL, A.FM_MODE_WAVEFRONT):
+        c = g.render(seed=6, mode=mode)[0]
+        close = np.abs(c - ref) <= 1e-3 * np.maximum(1.0, np.abs(ref))
+        fs = crn_sample_agreement(g, o, 24, 24, 16, 6, mode)
+        print(mode, "pixels equal to 1e-3:", close.mean(), "samples:", fs, "mean", c.mean(), ref.mean())
+        assert fs >= 0.995, (mode, fs)
+        assert abs(c.mean() - ref.mean()) < 0.02 * ref.mean()

@@ -4,9 +4,9 @@ It runs the megakernel's estimator with the same random-number dimensions,
 so, given the same samples per pass (chunk), a pixel gets the same paths in
 both forms.  Bars:
   film        per pixel |wavefront - megakernel| <= 1e-5 x max(1, |value|)
-              on all but a few pixels (a path may take another branch if the
-              two kernels round one operation differently), and the image
-              means agree to 1e-4
+              on >= 99.9 % of pixels (both are built without contracted
+              multiply-adds, so they round alike), and the image means agree
+              to 1e-4
   counters    same number of paths; ray counts within 0.5 %
   oracle      common random numbers against the FP32 oracle, as for the
               megakernel
@@ -17,7 +17,7 @@ import pytest
 import oracle
 from fumeng_amd import _abi as A
 from fumeng_amd import scenes
-from helpers import mesh_scene, unit
+from helpers import crn_sample_agreement, mesh_scene, unit
 
 pytestmark = pytest.mark.gpu
 
@@ -25,7 +25,7 @@ WF = A.FM_MODE_WAVEFRONT
 MK = A.FM_MODE_MEGAKERNEL
 
 
-def _same_film(a, b, name, frac=0.995):
+def _same_film(a, b, name, frac=0.999):
     for x, y, what in zip(a[:3], b[:3], ("color", "albedo", "normal")):
         close = np.abs(x - y) <= 1e-5 * np.maximum(1.0, np.abs(x))
         print("%s %s: pixels equal to 1e-5: %.5f, means %.6g %.6g" % (name, what, close.mean(), x.mean(), y.mean()))
@@ -81,10 +81,13 @@ def test_wavefront_matches_oracle_with_common_random_numbers(gpu_ctx):
     s = scenes.box_scene(width=32, height=32, spp=16)
     g = gpu_ctx.scene(s)
     cg, ag, ng, qg, st = g.render(seed=7, mode=WF)
-    co, ao, no, qo = oracle.OracleScene(s).render(0, 0, 32, 32, spp=16, seed=7, threads=8)
+    o = oracle.OracleScene(s)
+    co, ao, no, qo = o.render(0, 0, 32, 32, spp=16, seed=7, threads=8)
     close = np.abs(cg - co) <= 1e-3 * np.maximum(1.0, np.abs(co))
-    print("pixels equal to 1e-3:", close.mean(), "mean", cg.mean(), co.mean())
-    assert close.mean() > 0.9
+    fs = crn_sample_agreement(g, o, 32, 32, 16, 7, WF)
+    print("pixels equal to 1e-3:", close.mean(), "samples:", fs, "mean", cg.mean(), co.mean())
+    assert fs >= 0.995
+    assert close.mean() >= 0.99
     assert abs(cg.mean() - co.mean()) < 0.02 * co.mean()
     assert np.mean(np.abs(ag - ao) < 1e-5) > 0.95
 
@@ -135,10 +138,12 @@ def test_textured_many_light_scene_matches_oracle(gpu_ctx, mode):
     g = gpu_ctx.scene(s)
     assert g.info().num_lights == 10
     cg, ag, ng, qg, st = g.render(seed=3, mode=mode)
-    co, ao, no, qo = oracle.OracleScene(s).render(0, 0, 32, 18, spp=8, seed=3, threads=8)
+    o = oracle.OracleScene(s)
+    co, ao, no, qo = o.render(0, 0, 32, 18, spp=8, seed=3, threads=8)
     close = np.abs(cg - co) <= 1e-3 * np.maximum(1.0, np.abs(co))
-    print("pixels equal to 1e-3:", close.mean(), "mean", cg.mean(), co.mean())
-    assert close.mean() > 0.9
+    fs = crn_sample_agreement(g, o, 32, 18, 8, 3, mode)
+    print("pixels equal to 1e-3:", close.mean(), "samples:", fs, "mean", cg.mean(), co.mean())
+    assert fs >= 0.995
     assert abs(cg.mean() - co.mean()) < 0.03 * co.mean()
     assert np.mean(np.abs(ag - ao) < 1e-4) > 0.97
 
