@@ -226,7 +226,7 @@ def main():
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--mode", choices=["megakernel", "wavefront"], default="wavefront")
-    ap.add_argument("--bvh-width", type=int, choices=[2, 4], default=2,
+    ap.add_argument("--bvh-width", type=int, choices=[2, 4, 8], default=2,
                     help="traversal tree width (the closest hit does not depend on it)")
     ap.add_argument("--bvh-builder", choices=["host", "device"], default="host",
                     help="traversal tree builder: binned SAH on the host threads or PLOC on the GPU")

@@ -148,16 +148,20 @@ class Scene:
 
     def autotune(self, trial_spp=16, mode=A.FM_MODE_WAVEFRONT, width=0, height=0, depth=0, seed=1):
         """fm_autotune: returns {"best": {...}, "baseline": {...}, "trials": [...]}
-        with block_size / grid_blocks / chunk / kernel_ms per configuration;
-        pass best's block_size, grid_blocks and chunk to render()."""
+        with bvh_width / block_size / grid_blocks / chunk / kernel_ms and the
+        trial's film_hash / film_mean per configuration; the scene is left at
+        best's tree width; pass best's block_size, grid_blocks and chunk to
+        render()."""
         p = A.fm_render_params(width=width, height=height, depth=depth, rr_depth=-1, mode=mode, seed=seed)
         res = A.fm_tune_result()
-        arr = (A.fm_tune_trial * 64)()
-        _check(lib().fm_autotune(self.ctx.h, self.h, C.byref(p), int(trial_spp), C.byref(res), arr, 64), self.ctx.h)
-        conv = lambda t: {"block_size": t.block_size, "grid_blocks": t.grid_blocks, "chunk": t.chunk,
-                          "kernel_ms": t.kernel_ms}
+        cap = 256
+        arr = (A.fm_tune_trial * cap)()
+        _check(lib().fm_autotune(self.ctx.h, self.h, C.byref(p), int(trial_spp), C.byref(res), arr, cap), self.ctx.h)
+        conv = lambda t: {"bvh_width": t.bvh_width, "block_size": t.block_size, "grid_blocks": t.grid_blocks,
+                          "chunk": t.chunk, "kernel_ms": t.kernel_ms, "film_hash": t.film_hash,
+                          "film_mean": t.film_mean}
         return {"best": conv(res.best), "baseline": conv(res.baseline), "trial_spp": res.trial_spp,
-                "trials": [conv(arr[i]) for i in range(min(res.num_trials, 64))]}
+                "num_trials": res.num_trials, "trials": [conv(arr[i]) for i in range(min(res.num_trials, cap))]}
 
     def traversal(self):
         """(nodes, slot_geometry): the device's traversal tree (fm_dnode

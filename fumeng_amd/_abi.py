@@ -98,7 +98,8 @@ class fm_post_op(C.Structure):
 
 class fm_tune_trial(C.Structure):
     _fields_ = [("mode", C.c_int32), ("block_size", C.c_int32), ("grid_blocks", C.c_int32), ("chunk", C.c_int32),
-                ("kernel_ms", C.c_double)]
+                ("kernel_ms", C.c_double), ("bvh_width", C.c_int32), ("pad_", C.c_int32), ("film_hash", C.c_uint64),
+                ("film_mean", C.c_double)]
 
 
 class fm_tune_result(C.Structure):

@@ -141,7 +141,7 @@ struct fm_ctx {
 struct fm_scene {
     fm_ctx* ctx = nullptr;
     fmh::HostScene hs;        /* host copy (kept for info and tests) */
-    DevBuf nodes, nodes4, prims, meta, tri_sh, rects, spheres, materials, lights, mesh_tris, envs, images, texels, env_tables,
+    DevBuf nodes, nodes4, nodes8, prims, meta, tri_sh, rects, spheres, materials, lights, mesh_tris, envs, images, texels, env_tables,
         lut;
     fm_dscene d{};
     int64_t device_bytes = 0;
@@ -172,7 +172,7 @@ template <typename V> uint64_t fnv1a(const V& v, uint64_t h = 146959810393466560
 }
 
 void set_trav_hash(fmh::HostScene& hs) {
-    hs.trav_hash = hs.device_tree ? 0 : fnv1a(hs.prims, fnv1a(hs.nodes4, fnv1a(hs.nodes)));
+    hs.trav_hash = hs.device_tree ? 0 : fnv1a(hs.prims, fnv1a(hs.nodes8, fnv1a(hs.nodes4, fnv1a(hs.nodes))));
 }
 
 int fail(fm_ctx* ctx, int code, const std::string& msg) {
@@ -205,6 +205,7 @@ void upload_scene(fm_scene* s) {
     fmh::HostScene& hs = s->hs;
     s->nodes.upload(hs.nodes, st);
     s->nodes4.upload(hs.nodes4, st);
+    s->nodes8.upload(hs.nodes8, st);
     s->prims.upload(hs.prims, st);
     s->meta.upload(hs.meta, st);
     s->tri_sh.upload(hs.tri_sh, st);
@@ -222,6 +223,7 @@ void upload_scene(fm_scene* s) {
     fm_dscene& d = s->d;
     d.nodes = s->nodes.as<fm_dnode>();
     d.nodes4 = s->nodes4.as<fm_dnode4>();
+    d.nodes8 = s->nodes8.as<fm_dnode8>();
     d.width = hs.trav_width;
     d.stack_levels = hs.trav_stack;
     d.node_bytes = (uint32_t)(hs.nodes.size() * sizeof(fm_dnode));
@@ -248,7 +250,7 @@ void upload_scene(fm_scene* s) {
     memcpy(d.root_box, hs.root_box, sizeof d.root_box);
     d.camera = hs.camera;
     s->device_bytes = 0;
-    for (DevBuf* b : {&s->nodes, &s->nodes4, &s->prims, &s->meta, &s->tri_sh, &s->rects, &s->spheres, &s->materials, &s->lights, &s->mesh_tris,
+    for (DevBuf* b : {&s->nodes, &s->nodes4, &s->nodes8, &s->prims, &s->meta, &s->tri_sh, &s->rects, &s->spheres, &s->materials, &s->lights, &s->mesh_tris,
                       &s->envs, &s->images, &s->texels, &s->env_tables, &s->lut})
         s->device_bytes += (int64_t)b->n;
     s->upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -257,6 +259,28 @@ void upload_scene(fm_scene* s) {
 /* Device-built traversal tree (FM_BUILDER_DEVICE): the slot records were
  * uploaded in geometry order; the builder makes the tree and the slot order,
  * and the records are gathered into it. */
+/* Re-collapses a host-built scene's traversal tree to width W in place and
+ * uploads the wide nodes (the binary nodes and the slots stay). */
+void set_scene_width(fm_scene* s, int W) {
+    fmh::HostScene& hs = s->hs;
+    if (hs.trav_width == W) return;
+    if (hs.device_tree) throw fmh::BuildError(FM_ERR_UNSUPPORTED, "device-built traversal trees are two-wide only");
+    fmh::collapse_traversal(hs, W);
+    hipStream_t st = s->ctx->stream;
+    s->nodes4.upload(hs.nodes4, st);
+    s->nodes8.upload(hs.nodes8, st);
+    HIPCHECK(hipStreamSynchronize(st));
+    s->d.nodes4 = s->nodes4.as<fm_dnode4>();
+    s->d.nodes8 = s->nodes8.as<fm_dnode8>();
+    s->d.width = hs.trav_width;
+    s->d.stack_levels = hs.trav_stack;
+    set_trav_hash(hs);
+    s->device_bytes = 0;
+    for (DevBuf* b : {&s->nodes, &s->nodes4, &s->nodes8, &s->prims, &s->meta, &s->tri_sh, &s->rects, &s->spheres, &s->materials,
+                      &s->lights, &s->mesh_tris, &s->envs, &s->images, &s->texels, &s->env_tables, &s->lut})
+        s->device_bytes += (int64_t)b->n;
+}
+
 void build_device_tree(fm_scene* s) {
     fmh::HostScene& hs = s->hs;
     const int64_t G = hs.num_geoms;
@@ -303,7 +327,7 @@ void build_device_tree(fm_scene* s) {
     d.meta = s->meta.as<int32_t>();
     d.tri_sh = s->tri_sh.as<float>();
     s->device_bytes = 0;
-    for (DevBuf* b : {&s->nodes, &s->nodes4, &s->prims, &s->meta, &s->tri_sh, &s->rects, &s->spheres, &s->materials, &s->lights, &s->mesh_tris,
+    for (DevBuf* b : {&s->nodes, &s->nodes4, &s->nodes8, &s->prims, &s->meta, &s->tri_sh, &s->rects, &s->spheres, &s->materials, &s->lights, &s->mesh_tris,
                       &s->envs, &s->images, &s->texels, &s->env_tables, &s->lut})
         s->device_bytes += (int64_t)b->n;
     hs.device_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -662,8 +686,10 @@ static void fill_info(const fmh::HostScene& hs, int64_t device_bytes, double upl
     }
     o->trav_width = hs.trav_width;
     o->trav_stack = hs.trav_stack;
-    o->num_trav_nodes = hs.device_tree ? hs.device_nodes : (int64_t)(hs.trav_width == 4 ? hs.nodes4.size() : hs.nodes.size());
-    o->trav_node_bytes = hs.trav_width == 4 ? (int32_t)sizeof(fm_dnode4) : (int32_t)sizeof(fm_dnode);
+    o->num_trav_nodes = hs.device_tree ? hs.device_nodes
+                        : (int64_t)(hs.trav_width == 8 ? hs.nodes8.size() : (hs.trav_width == 4 ? hs.nodes4.size() : hs.nodes.size()));
+    o->trav_node_bytes = hs.trav_width == 8 ? (int32_t)sizeof(fm_dnode8)
+                         : (hs.trav_width == 4 ? (int32_t)sizeof(fm_dnode4) : (int32_t)sizeof(fm_dnode));
     o->build_threads = hs.report.threads;
     o->build_ms_geometry = hs.report.geometry_ms;
     o->build_ms_ref_bvh = hs.report.ref_bvh_ms;
@@ -671,7 +697,7 @@ static void fill_info(const fmh::HostScene& hs, int64_t device_bytes, double upl
     o->build_ms_tables = hs.report.tables_ms;
     o->build_ms_total = hs.report.total_ms;
     o->upload_ms = upload_ms;
-    o->host_bytes = vbytes(hs.ref_nodes) + vbytes(hs.order) + vbytes(hs.nodes) + vbytes(hs.nodes4) + vbytes(hs.prims) +
+    o->host_bytes = vbytes(hs.ref_nodes) + vbytes(hs.order) + vbytes(hs.nodes) + vbytes(hs.nodes4) + vbytes(hs.nodes8) + vbytes(hs.prims) +
                     vbytes(hs.meta) + vbytes(hs.tri_sh) + vbytes(hs.rects) + vbytes(hs.spheres) + vbytes(hs.materials) +
                     vbytes(hs.lights) + vbytes(hs.mesh_tris) + vbytes(hs.envs) + vbytes(hs.images) + vbytes(hs.texels) + vbytes(hs.env_tables) +
                     vbytes(hs.srgb_lut);
@@ -686,11 +712,11 @@ int fm_scene_traversal(const fm_scene* s, void* nodes, int64_t max_bytes, int32_
     return guarded(s->ctx, [&]() -> int {
         HIPCHECK(hipSetDevice(s->ctx->device));
         const fmh::HostScene& hs = s->hs;
-        const bool w4 = s->d.width == 4;
-        const int64_t bytes = w4 ? vbytes(hs.nodes4) : (int64_t)s->d.node_bytes;
+        const int w = s->d.width;
+        const int64_t bytes = w == 8 ? vbytes(hs.nodes8) : (w == 4 ? vbytes(hs.nodes4) : (int64_t)s->d.node_bytes);
+        const void* src = w == 8 ? s->nodes8.p : (w == 4 ? s->nodes4.p : s->nodes.p);
         if (nodes && bytes > 0 && max_bytes > 0)
-            HIPCHECK(hipMemcpyAsync(nodes, w4 ? s->nodes4.p : s->nodes.p, (size_t)std::min(bytes, max_bytes), hipMemcpyDeviceToHost,
-                                    s->ctx->stream));
+            HIPCHECK(hipMemcpyAsync(nodes, src, (size_t)std::min(bytes, max_bytes), hipMemcpyDeviceToHost, s->ctx->stream));
         if (slot_geometry && hs.num_geoms > 0 && max_slots > 0) {
             std::vector<int32_t> meta((size_t)hs.num_geoms * 4);
             HIPCHECK(hipMemcpyAsync(meta.data(), s->meta.p, meta.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s->ctx->stream));
@@ -724,52 +750,79 @@ int fm_autotune(fm_ctx* ctx, fm_scene* s, const fm_render_params* base, int32_t 
     p.flags = 0;
     p.sample_offset = 0;
     const bool wave = p.mode == FM_MODE_WAVEFRONT;
-    /* the defaults first, then every other combination */
+    const int width0 = s->hs.trav_width;
+    /* the tree widths to try: every width for host-built trees (they are
+     * collapses of one binary tree with the same slots), the scene's own
+     * otherwise */
+    std::vector<int> widths{width0};
+    if (!s->hs.device_tree)
+        for (int w : {2, 4, 8}) if (w != width0) widths.push_back(w);
+    /* per width: the defaults first, then every other launch combination */
     std::vector<fm_tune_trial> cand;
-    {
-        fm_tune_trial t{};
-        t.mode = p.mode; t.chunk = wave ? 0 : 4;
-        cand.push_back(t);
-    }
     const std::vector<int> chunks = wave ? std::vector<int>{0, std::max(1, p.spp / 4)} : std::vector<int>{2, 4, 8, 16, 64};
-    for (int block : {64, 128, 256})
-        for (int fixed_grid : {0, 1})
-            for (int chunk : chunks) {
-                fm_tune_trial t{};
-                t.mode = p.mode; t.block_size = block; t.chunk = chunk;
-                if (fixed_grid)
-                    t.grid_blocks = wave ? ctx->num_cus * (2048 / block)
-                                         : ctx->num_cus * std::max(1, fm::mega_blocks_per_cu(s->d, block)) * 8;
-                cand.push_back(t);
+    for (int w : widths) {
+        fm_tune_trial t{};
+        t.mode = p.mode; t.chunk = wave ? 0 : 4; t.bvh_width = w;
+        cand.push_back(t);
+        for (int block : {64, 128, 256})
+            for (int fixed_grid : {0, 1})
+                for (int chunk : chunks) {
+                    fm_tune_trial u{};
+                    u.mode = p.mode; u.block_size = block; u.chunk = chunk; u.bvh_width = w;
+                    if (fixed_grid)
+                        u.grid_blocks = wave ? ctx->num_cus * (2048 / block)
+                                             : ctx->num_cus * std::max(1, fm::mega_blocks_per_cu(s->d, block)) * 8;
+                    cand.push_back(u);
+                }
+    }
+    int rc = guarded(ctx, [&]() -> int {
+        fm_render_stats st;
+        std::vector<double> film;
+        int cur_w = -1;
+        size_t best = 0;
+        for (size_t i = 0; i < cand.size(); i++) {
+            fm_tune_trial& t = cand[i];
+            if (t.bvh_width != cur_w) {
+                HIPCHECK(hipSetDevice(ctx->device));
+                set_scene_width(s, t.bvh_width);
+                cur_w = t.bvh_width;
+                /* a warm-up with the defaults sizes the buffers for this tree */
+                fm_render_params q = p;
+                q.block_size = 0; q.grid_blocks = 0; q.chunk = t.chunk;
+                const int r = fm_render(ctx, s, &q, nullptr, &st);
+                if (r != FM_OK) return r;
             }
-    fm_render_stats st;
-    {   /* a warm-up with the defaults sizes the buffers */
-        fm_render_params q = p;
-        q.block_size = 0; q.grid_blocks = 0; q.chunk = cand[0].chunk;
-        const int rc = fm_render(ctx, s, &q, nullptr, &st);
-        if (rc != FM_OK) return rc;
-    }
-    size_t best = 0;
-    for (size_t i = 0; i < cand.size(); i++) {
-        fm_tune_trial& t = cand[i];
-        fm_render_params q = p;
-        q.block_size = t.block_size; q.grid_blocks = t.grid_blocks; q.chunk = t.chunk;
-        const int rc = fm_render(ctx, s, &q, nullptr, &st);
-        if (rc != FM_OK) return rc;
-        t.kernel_ms = st.kernel_ms;
-        if (t.kernel_ms < cand[best].kernel_ms) best = i;
-    }
-    out->baseline = cand[0];
-    out->best = cand[best];
-    out->num_trials = (int32_t)cand.size();
-    out->trial_spp = p.spp;
-    for (int32_t i = 0; i < max_trials && i < (int32_t)cand.size(); i++) trials[i] = cand[(size_t)i];
-    return FM_OK;
+            fm_render_params q = p;
+            q.block_size = t.block_size; q.grid_blocks = t.grid_blocks; q.chunk = t.chunk;
+            const int r = fm_render(ctx, s, &q, nullptr, &st);
+            if (r != FM_OK) return r;
+            t.kernel_ms = st.kernel_ms;
+            /* what the trial rendered: FNV-1a of the FP64 colour sums and their mean */
+            const int64_t n = (int64_t)ctx->film_w * ctx->film_h * 3;
+            film.resize((size_t)n);
+            HIPCHECK(hipMemcpy(film.data(), ctx->film.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+            t.film_hash = fnv1a(film);
+            double sum = 0.0;
+            for (double v : film) sum += v;
+            t.film_mean = n > 0 ? sum / (double)n / (double)p.spp : 0.0;
+            if (t.kernel_ms < cand[best].kernel_ms) best = i;
+        }
+        /* the scene keeps the fastest trial's tree width */
+        set_scene_width(s, cand[best].bvh_width);
+        out->baseline = cand[0];
+        out->best = cand[best];
+        out->num_trials = (int32_t)cand.size();
+        out->trial_spp = p.spp;
+        for (int32_t i = 0; i < max_trials && i < (int32_t)cand.size(); i++) trials[i] = cand[(size_t)i];
+        return FM_OK;
+    });
+    if (rc != FM_OK && s->hs.trav_width != width0) (void)guarded(ctx, [&]() -> int { set_scene_width(s, width0); return FM_OK; });
+    return rc;
 }
 
 int fm_set_bvh_width(fm_ctx* ctx, int32_t width) {
     if (!ctx) return fail(nullptr, FM_ERR_INVALID, "NULL ctx");
-    if (width != 2 && width != 4) return fail(ctx, FM_ERR_INVALID, "BVH width must be 2 or 4");
+    if (width != 2 && width != 4 && width != 8) return fail(ctx, FM_ERR_INVALID, "BVH width must be 2, 4 or 8");
     ctx->bvh_width = width;
     return FM_OK;
 }
@@ -1368,7 +1421,7 @@ int fm_host_build_bvh(const fm_scene_desc* desc, int64_t* num_nodes, float* boxe
 
 int fm_host_scene_info(const fm_scene_desc* desc, int32_t bvh_width, fm_scene_info* out) {
     if (!desc || !out) return fail(nullptr, FM_ERR_INVALID, "NULL argument");
-    if (bvh_width != 2 && bvh_width != 4) return fail(nullptr, FM_ERR_INVALID, "BVH width must be 2 or 4");
+    if (bvh_width != 2 && bvh_width != 4 && bvh_width != 8) return fail(nullptr, FM_ERR_INVALID, "BVH width must be 2, 4 or 8");
     return guarded(nullptr, [&]() -> int {
         fmh::HostScene hs;
         hs.trav_width = bvh_width;

@@ -461,8 +461,87 @@ FMD bool traverse4_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit&
     return false;
 }
 
-/* WIDTH = 2 or 4 compiles one traversal only (the caller knows the scene's
- * width); 0 branches on S.width.  The stack is the caller's (TravStack). */
+/* The same traversal over eight-wide nodes: the children's boxes are tested
+ * in two groups of four (one float4 per bound plane and group), the entered
+ * ones are ordered by one 8-key sorting network (Batcher's 19 comparators;
+ * key = distance bits with the child index in the three low bits), the
+ * nearest is entered and the others are pushed far to near. */
+FMD uint32_t wide_key8(bool hit, float t, uint32_t i) { return hit ? (__float_as_uint(t) & ~7u) | i : 0xffffffffu; }
+
+template <bool FAST, bool ANY, bool COUNT, bool SHORT>
+FMD bool traverse8_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& h, TravStack<SHORT>& stack, Cnt& cn) {
+    float tn;
+    if (!box_hit<FAST>(S.root_box, r, rp, ANY ? r.tmax : h.t, tn)) return false;
+    int32_t node = S.root_ref, leaf = FM_TRAV_NONE;
+    if (node < 0) { leaf = node; node = FM_TRAV_NONE; }
+    auto pop = [&]() -> int32_t { return stack.pop(); };
+    constexpr uint32_t MISS = 0xffffffffu;
+    while (node != FM_TRAV_NONE || leaf != FM_TRAV_NONE) {
+        while (node != FM_TRAV_NONE && node >= 0) {
+            if (COUNT) cn.nodes++;
+            const float4* nd = reinterpret_cast<const float4*>(S.nodes8 + node);
+            const int4 ca = reinterpret_cast<const int4*>(nd)[12], cb = reinterpret_cast<const int4*>(nd)[13];
+            const float tmax = ANY ? r.tmax : h.t;
+            uint32_t k[8];
+#pragma unroll
+            for (int g = 0; g < 2; g++) {
+                /* planes lo_x, hi_x, lo_y, hi_y, lo_z, hi_z: 8 floats each, lanes 4g..4g+3 */
+                const float4 lx = nd[0 + g], hx = nd[2 + g], ly = nd[4 + g], hy = nd[6 + g], lz = nd[8 + g], hz = nd[10 + g];
+                const int4 cr = g ? cb : ca;
+                float t0, t1, t2, t3;
+                const bool b0 = box_hit6<FAST>(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, r, rp, tmax, t0) && cr.x != FM_NODE4_EMPTY;
+                const bool b1 = box_hit6<FAST>(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, r, rp, tmax, t1) && cr.y != FM_NODE4_EMPTY;
+                const bool b2 = box_hit6<FAST>(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, r, rp, tmax, t2) && cr.z != FM_NODE4_EMPTY;
+                const bool b3 = box_hit6<FAST>(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, r, rp, tmax, t3) && cr.w != FM_NODE4_EMPTY;
+                k[4 * g + 0] = wide_key8(b0, t0, 4u * g + 0u);
+                k[4 * g + 1] = wide_key8(b1, t1, 4u * g + 1u);
+                k[4 * g + 2] = wide_key8(b2, t2, 4u * g + 2u);
+                k[4 * g + 3] = wide_key8(b3, t3, 4u * g + 3u);
+            }
+            sort2u(k[0], k[2]); sort2u(k[1], k[3]); sort2u(k[4], k[6]); sort2u(k[5], k[7]);
+            sort2u(k[0], k[4]); sort2u(k[1], k[5]); sort2u(k[2], k[6]); sort2u(k[3], k[7]);
+            sort2u(k[0], k[1]); sort2u(k[2], k[3]); sort2u(k[4], k[5]); sort2u(k[6], k[7]);
+            sort2u(k[2], k[4]); sort2u(k[3], k[5]);
+            sort2u(k[1], k[4]); sort2u(k[3], k[6]);
+            sort2u(k[1], k[2]); sort2u(k[3], k[4]); sort2u(k[5], k[6]);
+            auto child = [&](uint32_t key) -> int32_t {
+                const uint32_t i = key & 7u;
+                const int4 c = i < 4u ? ca : cb;
+                const uint32_t j = i & 3u;
+                return j == 0u ? c.x : (j == 1u ? c.y : (j == 2u ? c.z : c.w));
+            };
+            if (k[0] == MISS) {
+                node = pop();
+            } else {
+#pragma unroll
+                for (int q = 7; q >= 1; q--)
+                    if (k[q] != MISS) stack.push((uint32_t)child(k[q]));
+                node = child(k[0]);
+            }
+            if (node < 0 && leaf == FM_TRAV_NONE) { leaf = node; node = pop(); }
+            if (__all(leaf != FM_TRAV_NONE)) break;
+        }
+        while (leaf != FM_TRAV_NONE) {
+            const int32_t start = FM_LEAF_START(leaf), count = FM_LEAF_COUNT(leaf);
+            if (ANY) {
+                for (int q = 0; q < count; q++) {
+                    if (COUNT) cn.prims++;
+                    if (test_slot_any(S, r, rp, start + q)) return true;
+                }
+            } else {
+                if (COUNT) cn.prims += (uint32_t)count;
+                for (int q = 0; q < count; q++) test_slot_closest(S, r, rp, start + q, h);
+            }
+            if (node < 0) { leaf = node; node = pop(); }
+            else leaf = FM_TRAV_NONE;
+        }
+    }
+    return false;
+}
+
+/* WIDTH = 2, 4 or 8 compiles one traversal only (the caller knows the
+ * scene's width); 0 branches on S.width.  The stack is the caller's
+ * (TravStack). */
 template <bool COUNT = false, int WIDTH = 0, bool SHORT>
 FMD Hit trace_closest_on(const fm_dscene& S, const Ray& r, TravStack<SHORT>& stack, Cnt* cn = nullptr) {
     Hit h;
@@ -471,7 +550,10 @@ FMD Hit trace_closest_on(const fm_dscene& S, const Ray& r, TravStack<SHORT>& sta
     const RayPre rp = precompute(r, S);
     Cnt dummy;
     Cnt& c = COUNT ? *cn : dummy;
-    if (WIDTH == 4 || (WIDTH == 0 && S.width == 4)) {
+    if (WIDTH == 8 || (WIDTH == 0 && S.width == 8)) {
+        if (rp.fast) traverse8_impl<true, false, COUNT>(S, r, rp, h, stack, c);
+        else traverse8_impl<false, false, COUNT>(S, r, rp, h, stack, c);
+    } else if (WIDTH == 4 || (WIDTH == 0 && S.width == 4)) {
         if (rp.fast) traverse4_impl<true, false, COUNT>(S, r, rp, h, stack, c);
         else traverse4_impl<false, false, COUNT>(S, r, rp, h, stack, c);
     } else {
@@ -489,6 +571,9 @@ FMD bool trace_any_on(const fm_dscene& S, const Ray& r, TravStack<SHORT>& stack,
     Cnt& c = COUNT ? *cn : dummy;
     Hit h;
     h.t = r.tmax; h.slot = -1; h.kind = -1; h.tie = 0; h.half = 0; h.b0 = h.b1 = h.b2 = 0.0f; h.info = 0;
+    if (WIDTH == 8 || (WIDTH == 0 && S.width == 8))
+        return rp.fast ? traverse8_impl<true, true, COUNT>(S, r, rp, h, stack, c)
+                       : traverse8_impl<false, true, COUNT>(S, r, rp, h, stack, c);
     if (WIDTH == 4 || (WIDTH == 0 && S.width == 4))
         return rp.fast ? traverse4_impl<true, true, COUNT>(S, r, rp, h, stack, c)
                        : traverse4_impl<false, true, COUNT>(S, r, rp, h, stack, c);

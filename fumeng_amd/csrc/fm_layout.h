@@ -39,8 +39,9 @@
 #define FM_STACK_MAX 64     /* LDS traversal stack entries per lane, at most */
 #define FM_SHORT_STACK 16   /* LDS entries per lane of the short (spilling) traversal stack; a power of two */
 
-/* Traversal trees are two-wide (fm_dnode) or four-wide (fm_dnode4). */
-#define FM_NODE4_EMPTY 0x7fffffff   /* unused child of a four-wide node */
+/* Traversal trees are two-wide (fm_dnode), four-wide (fm_dnode4) or
+ * eight-wide (fm_dnode8). */
+#define FM_NODE4_EMPTY 0x7fffffff   /* unused child of a four- or eight-wide node */
 
 typedef struct fm_dnode {
     float b[12];     /* c0 min xyz, c0 max xyz, c1 min xyz, c1 max xyz */
@@ -55,6 +56,14 @@ typedef struct fm_dnode4 {
     int32_t child[4];
     int32_t pad[4];
 } fm_dnode4;         /* 128 bytes */
+
+/* Eight child boxes as a structure of arrays (two float4 per bound plane,
+ * lanes 0-7 = children 0-7), then the eight child refs. */
+typedef struct fm_dnode8 {
+    float lo_x[8], hi_x[8], lo_y[8], hi_y[8], lo_z[8], hi_z[8];
+    int32_t child[8];
+    int32_t pad[8];
+} fm_dnode8;         /* 256 bytes */
 
 typedef struct fm_drect {
     float pa[4], pb[4], pc[4], pd[4]; /* xyz + pad */
@@ -135,6 +144,7 @@ typedef struct fm_dcamera {
 typedef struct fm_dscene {
     const fm_dnode* nodes;       /* width 2 */
     const fm_dnode4* nodes4;     /* width 4 */
+    const fm_dnode8* nodes8;     /* width 8 */
     const float* prims;          /* float4 x 3 per slot */
     const int32_t* meta;         /* int4 per slot */
     const float* tri_sh;         /* float4 x 4 per slot */
@@ -155,7 +165,7 @@ typedef struct fm_dscene {
     int32_t env;                 /* env table index or -1 */
     int32_t naive;               /* brute force in object order */
     int32_t num_materials;
-    int32_t width;               /* traversal tree: 2 or 4 */
+    int32_t width;               /* traversal tree: 2, 4 or 8 */
     float root_box[6];
     int32_t stack_levels;        /* LDS stack entries a traversal needs */
     uint32_t node_bytes;         /* size of nodes (two-wide), < 4 GiB */

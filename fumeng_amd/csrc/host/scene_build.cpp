@@ -623,6 +623,80 @@ void make_camera(const fm_camera_desc& c, fm_dcamera& dc) {
     memcpy(dc.c2w, c2w.m, sizeof dc.c2w);
 }
 
+/* The traversal tree of width W from the binary one in hs.nodes (pre-order,
+ * node 0 the root): a wide node starts from the two children of a binary
+ * node and opens its inner child of largest surface area until it holds W
+ * children or only leaves.  The slots stay where they are.  The traversal
+ * pushes at most (children - 1) refs per node on the way down, so the stack
+ * never holds more than the largest sum of (children - 1) over a
+ * root-to-leaf path (one per binary level for W = 2). */
+void collapse_traversal(HostScene& hs, int W) {
+    if (W != 2 && W != 4 && W != 8) throw BuildError(FM_ERR_INVALID, "traversal tree width must be 2, 4 or 8");
+    hs.nodes4.clear();
+    hs.nodes8.clear();
+    hs.trav_width = W;
+    if (W == 2 || hs.nodes.empty()) {
+        hs.trav_stack = hs.nodes.empty() ? 1 : std::max(1, hs.trav_depth - 1);
+        if (hs.trav_stack > FM_STACK_MAX) throw BuildError(FM_ERR_UNSUPPORTED, "traversal needs more stack than FM_STACK_MAX");
+        return;
+    }
+    struct Child { int32_t ref; const float* box; };
+    auto area = [](const float* b) {
+        const double dx = (double)b[3] - b[0], dy = (double)b[4] - b[1], dz = (double)b[5] - b[2];
+        return dx * dy + dy * dz + dz * dx;
+    };
+    const std::vector<fm_dnode>& bin = hs.nodes;
+    std::function<int32_t(int32_t, int&)> make_wide = [&](int32_t bi, int& need) -> int32_t {
+        Child c[8];
+        c[0] = Child{bin[(size_t)bi].child[0], &bin[(size_t)bi].b[0]};
+        c[1] = Child{bin[(size_t)bi].child[1], &bin[(size_t)bi].b[6]};
+        int n = 2;
+        while (n < W) {
+            int best = -1;
+            double ba = -1.0;
+            for (int k = 0; k < n; k++)
+                if (c[k].ref >= 0 && area(c[k].box) > ba) { ba = area(c[k].box); best = k; }
+            if (best < 0) break;
+            const fm_dnode& x = bin[(size_t)c[best].ref];
+            c[best] = Child{x.child[0], &x.b[0]};
+            c[n++] = Child{x.child[1], &x.b[6]};
+        }
+        int32_t id;
+        if (W == 4) { id = (int32_t)hs.nodes4.size(); hs.nodes4.emplace_back(); }
+        else { id = (int32_t)hs.nodes8.size(); hs.nodes8.emplace_back(); }
+        int32_t refs[8];
+        for (int k = 0; k < 8; k++) refs[k] = FM_NODE4_EMPTY;
+        int below = 0;
+        for (int k = 0; k < n; k++) {
+            if (c[k].ref < 0) { refs[k] = c[k].ref; continue; }
+            int nk = 0;
+            refs[k] = make_wide(c[k].ref, nk);
+            below = std::max(below, nk);
+        }
+        float *lo_x, *hi_x, *lo_y, *hi_y, *lo_z, *hi_z;
+        int32_t* child;
+        if (W == 4) {
+            fm_dnode4& o = hs.nodes4[(size_t)id];
+            lo_x = o.lo_x; hi_x = o.hi_x; lo_y = o.lo_y; hi_y = o.hi_y; lo_z = o.lo_z; hi_z = o.hi_z; child = o.child;
+        } else {
+            fm_dnode8& o = hs.nodes8[(size_t)id];
+            lo_x = o.lo_x; hi_x = o.hi_x; lo_y = o.lo_y; hi_y = o.hi_y; lo_z = o.lo_z; hi_z = o.hi_z; child = o.child;
+        }
+        for (int k = 0; k < W; k++) {
+            child[k] = refs[k];
+            const float* b = k < n ? c[k].box : nullptr;
+            lo_x[k] = b ? b[0] : 0.0f; lo_y[k] = b ? b[1] : 0.0f; lo_z[k] = b ? b[2] : 0.0f;
+            hi_x[k] = b ? b[3] : 0.0f; hi_y[k] = b ? b[4] : 0.0f; hi_z[k] = b ? b[5] : 0.0f;
+        }
+        need = (n - 1) + below;
+        return id;
+    };
+    int need = 0;
+    make_wide(0, need);
+    hs.trav_stack = std::max(1, need);
+    if (hs.trav_stack > FM_STACK_MAX) throw BuildError(FM_ERR_UNSUPPORTED, "traversal needs more stack than FM_STACK_MAX");
+}
+
 void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
     using clk = std::chrono::steady_clock;
     const clk::time_point t_start = clk::now();
@@ -852,7 +926,8 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
     std::vector<TNode> tn;
     hs.device_tree = hs.builder == FM_BUILDER_DEVICE && d.accelerator != FM_ACCEL_BVH_REFERENCE_TREE;
     if (hs.builder != FM_BUILDER_HOST && hs.builder != FM_BUILDER_DEVICE) throw BuildError(FM_ERR_INVALID, "unknown BVH builder");
-    if (hs.trav_width != 2 && hs.trav_width != 4) throw BuildError(FM_ERR_INVALID, "traversal tree width must be 2 or 4");
+    if (hs.trav_width != 2 && hs.trav_width != 4 && hs.trav_width != 8)
+        throw BuildError(FM_ERR_INVALID, "traversal tree width must be 2, 4 or 8");
     if (hs.device_tree && hs.trav_width != 2) throw BuildError(FM_ERR_UNSUPPORTED, "the device builder makes two-wide trees only");
     if (hs.device_tree) {
         /* slots in geometry order; the device builds the tree and permutes them */
@@ -875,6 +950,7 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
         hs.leaf_boxes.swap(tight);
         hs.nodes.clear();
         hs.nodes4.clear();
+        hs.nodes8.clear();
         hs.trav_depth = 0;
         hs.trav_stack = 1;
         hs.root_ref = 0;
@@ -919,80 +995,29 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
             if (n.leaf) return FM_LEAF_REF(n.a, n.b - n.a);
             return dev_id[(size_t)i];
         };
+        /* the binary tree is kept whatever the width: the wider trees are
+         * collapses of it with the same slot order (collapse_traversal) */
         hs.nodes.clear();
         hs.nodes4.clear();
-        if (hs.trav_width == 2) {
-            hs.nodes.assign((size_t)nint, fm_dnode{});
-            for (size_t i = 0; i < tn.size(); i++) {
-                const TNode& n = tn[i];
-                if (n.leaf) continue;
-                fm_dnode& o = hs.nodes[(size_t)dev_id[i]];
-                const float* a = &node_boxes[(size_t)n.a * 6];
-                const float* b = &node_boxes[(size_t)n.b * 6];
-                for (int k = 0; k < 6; k++) { o.b[k] = a[k]; o.b[6 + k] = b[k]; }
-                o.child[0] = ref_of(n.a);
-                o.child[1] = ref_of(n.b);
-            }
-            if (hs.nodes.size() * sizeof(fm_dnode) >= ((size_t)1 << 32))
-                throw BuildError(FM_ERR_UNSUPPORTED, "two-wide traversal tree of 4 GiB or more");
-            /* one entry pushed per inner node on the way down */
-            hs.trav_stack = std::max(1, tree_depth - 1);
-        } else if (G > 0 && !tn[0].leaf) {
-            /* Four-wide tree collapsed from the binary one: a node starts from
-             * the two children of a binary node and opens its inner child of
-             * largest surface area until it holds four children or only leaves.
-             * The traversal pushes at most (children - 1) refs per node on the
-             * way down, so the stack never holds more than the largest sum of
-             * (children - 1) over a root-to-leaf path. */
-            auto area = [&](int64_t i) {
-                const float* b = &node_boxes[(size_t)i * 6];
-                const double dx = (double)b[3] - b[0], dy = (double)b[4] - b[1], dz = (double)b[5] - b[2];
-                return dx * dy + dy * dz + dz * dx;
-            };
-            std::function<int32_t(int64_t, int&)> make4 = [&](int64_t bin, int& need) -> int32_t {
-                int64_t c[4] = {tn[(size_t)bin].a, tn[(size_t)bin].b, -1, -1};
-                int n = 2;
-                while (n < 4) {
-                    int best = -1;
-                    double ba = -1.0;
-                    for (int k = 0; k < n; k++)
-                        if (!tn[(size_t)c[k]].leaf && area(c[k]) > ba) { ba = area(c[k]); best = k; }
-                    if (best < 0) break;
-                    const int64_t x = c[best];
-                    c[best] = tn[(size_t)x].a;
-                    c[n++] = tn[(size_t)x].b;
-                }
-                const int32_t id = (int32_t)hs.nodes4.size();
-                hs.nodes4.emplace_back();
-                int32_t refs[4] = {FM_NODE4_EMPTY, FM_NODE4_EMPTY, FM_NODE4_EMPTY, FM_NODE4_EMPTY};
-                int below = 0;
-                for (int k = 0; k < n; k++) {
-                    if (tn[(size_t)c[k]].leaf) { refs[k] = ref_of(c[k]); continue; }
-                    int nk = 0;
-                    refs[k] = make4(c[k], nk);
-                    below = std::max(below, nk);
-                }
-                fm_dnode4& o = hs.nodes4[(size_t)id];
-                for (int k = 0; k < 4; k++) {
-                    o.child[k] = refs[k];
-                    const float* b = k < n ? &node_boxes[(size_t)c[k] * 6] : nullptr;
-                    o.lo_x[k] = b ? b[0] : 0.0f; o.lo_y[k] = b ? b[1] : 0.0f; o.lo_z[k] = b ? b[2] : 0.0f;
-                    o.hi_x[k] = b ? b[3] : 0.0f; o.hi_y[k] = b ? b[4] : 0.0f; o.hi_z[k] = b ? b[5] : 0.0f;
-                }
-                need = (n - 1) + below;
-                return id;
-            };
-            int need = 0;
-            make4(0, need);
-            hs.trav_stack = std::max(1, need);
-        } else {
-            hs.trav_stack = 1;
+        hs.nodes8.clear();
+        hs.nodes.assign((size_t)nint, fm_dnode{});
+        for (size_t i = 0; i < tn.size(); i++) {
+            const TNode& n = tn[i];
+            if (n.leaf) continue;
+            fm_dnode& o = hs.nodes[(size_t)dev_id[i]];
+            const float* a = &node_boxes[(size_t)n.a * 6];
+            const float* b = &node_boxes[(size_t)n.b * 6];
+            for (int k = 0; k < 6; k++) { o.b[k] = a[k]; o.b[6 + k] = b[k]; }
+            o.child[0] = ref_of(n.a);
+            o.child[1] = ref_of(n.b);
         }
-        if (hs.trav_stack > FM_STACK_MAX) throw BuildError(FM_ERR_UNSUPPORTED, "traversal needs more stack than FM_STACK_MAX");
+        if (hs.nodes.size() * sizeof(fm_dnode) >= ((size_t)1 << 32))
+            throw BuildError(FM_ERR_UNSUPPORTED, "two-wide traversal tree of 4 GiB or more");
         if (G > 0) {
-            hs.root_ref = hs.trav_width == 4 && !tn[0].leaf ? 0 : ref_of(0);
+            hs.root_ref = ref_of(0);   /* node 0 of every width when the root is inner */
             for (int k = 0; k < 6; k++) hs.root_box[k] = node_boxes[(size_t)k];
         }
+        collapse_traversal(hs, hs.trav_width);
     }
 
     lap(hs.report.trav_tree_ms);

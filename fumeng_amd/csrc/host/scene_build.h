@@ -40,7 +40,7 @@ struct HostScene {
     std::vector<int32_t> order;         /* geometry index per leaf slot */
     int32_t bvh_depth = 0;          /* reference tree */
     int32_t trav_depth = 0;         /* traversal tree (binary form) */
-    int32_t trav_width = 2;         /* set before the build: 2 or 4 */
+    int32_t trav_width = 2;         /* set before the build: 2, 4 or 8 */
     int32_t builder = FM_BUILDER_HOST;  /* set before the build: FM_BUILDER_* */
     bool device_tree = false;       /* the traversal tree is left to the device builder: slots
                                        are in geometry order, leaf_boxes holds their bounds */
@@ -54,6 +54,7 @@ struct HostScene {
 
     std::vector<fm_dnode> nodes;        /* width 2 */
     std::vector<fm_dnode4> nodes4;      /* width 4 */
+    std::vector<fm_dnode8> nodes8;      /* width 8 */
     std::vector<float> prims;            /* 12 floats per slot */
     std::vector<int32_t> meta;           /* 4 ints per slot */
     std::vector<float> tri_sh;           /* 16 floats per slot */
@@ -84,6 +85,11 @@ void build_env_tables(HostScene& hs, fm_denv& e, const std::function<void(int, i
 
 /* Device camera record for a camera description (per-job cameras). */
 void make_camera(const fm_camera_desc& c, fm_dcamera& dc);
+
+/* Re-collapses a host-built scene's traversal tree to width W (2, 4 or 8)
+ * from the binary tree in hs.nodes: sets nodes4 / nodes8, trav_width and
+ * trav_stack; the slots and their order do not change.  Throws BuildError. */
+void collapse_traversal(HostScene& hs, int W);
 
 /* Throws BuildError.  only_bvh stops after the reference BVH (host tests). */
 void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh = false);

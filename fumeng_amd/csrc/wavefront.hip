@@ -557,6 +557,8 @@ void wavefront_grids(int num_cus, int stack_levels, WfLaunch& L) {
     L.stack_levels = stack_levels;
     auto trace_blocks = [&](bool sh, int block) {
         const size_t lds = (size_t)(sh ? FM_SHORT_STACK : stack_levels) * block * sizeof(uint32_t);
+        if (L.width == 8)
+            return sh ? occupancy(k_wf_trace<false, 8, true>, block, lds) : occupancy(k_wf_trace<false, 8, false>, block, lds);
         if (L.width == 4)
             return sh ? occupancy(k_wf_trace<false, 4, true>, block, lds) : occupancy(k_wf_trace<false, 4, false>, block, lds);
         return sh ? occupancy(k_wf_trace<false, 2, true>, block, lds) : occupancy(k_wf_trace<false, 2, false>, block, lds);
@@ -626,7 +628,15 @@ hipError_t launch_wavefront_pass(const fm_dscene& S, WfArgs A, const WfLaunch& L
         A.iter = it;
         if (ev) hipEventRecord(ev[2 * it], st);
         const dim3 g(L.grid_trace), b(L.block_trace);
-        if (L.width == 4) {
+        if (L.width == 8) {
+            if (L.short_stack) {
+                if (count) hipLaunchKernelGGL((k_wf_trace<true, 8, true>), g, b, lds, st, S, A);
+                else hipLaunchKernelGGL((k_wf_trace<false, 8, true>), g, b, lds, st, S, A);
+            } else {
+                if (count) hipLaunchKernelGGL((k_wf_trace<true, 8, false>), g, b, lds, st, S, A);
+                else hipLaunchKernelGGL((k_wf_trace<false, 8, false>), g, b, lds, st, S, A);
+            }
+        } else if (L.width == 4) {
             if (L.short_stack) {
                 if (count) hipLaunchKernelGGL((k_wf_trace<true, 4, true>), g, b, lds, st, S, A);
                 else hipLaunchKernelGGL((k_wf_trace<false, 4, true>), g, b, lds, st, S, A);

@@ -232,10 +232,10 @@ typedef struct fm_scene_info {
     fm_renderer_desc renderer;
     fm_camera_desc camera;
     int32_t env_width, env_height;   /* first env light's image (0 if none) */
-    int32_t trav_width;       /* traversal tree the device walks: 2 or 4 wide */
+    int32_t trav_width;       /* traversal tree the device walks: 2, 4 or 8 wide */
     int32_t trav_stack;       /* stack entries per traversal (LDS) */
     int64_t num_trav_nodes;   /* its inner nodes */
-    int32_t trav_node_bytes;  /* bytes per node: 64 (2 wide) or 128 (4 wide) */
+    int32_t trav_node_bytes;  /* bytes per node: 64 (2 wide), 128 (4 wide) or 256 (8 wide) */
     int32_t build_threads;    /* host threads the build used */
     /* build report: wall time of the host build's phases and the upload */
     double build_ms_geometry, build_ms_ref_bvh, build_ms_trav_tree, build_ms_tables, build_ms_total, upload_ms;
@@ -400,10 +400,17 @@ int fm_render_adjoint(fm_ctx* ctx, fm_scene* scene, const fm_render_params* para
                       double* g_material, double* g_env, fm_render_stats* stats);
 /* ---- autotuner ------------------------------------------------------------ */
 
-/* One timed configuration of fm_autotune. */
+/* One timed configuration of fm_autotune, with what it rendered: FNV-1a of
+ * the FP64 colour sums (equal for trials that differ only in tree width,
+ * block or grid) and the image mean (equal up to the order of FP32
+ * additions for other chunks). */
 typedef struct fm_tune_trial {
     int32_t mode, block_size, grid_blocks, chunk;
     double kernel_ms;
+    int32_t bvh_width;
+    int32_t pad_;
+    uint64_t film_hash;
+    double film_mean;
 } fm_tune_trial;
 typedef struct fm_tune_result {
     fm_tune_trial best;        /* copy into fm_render_params */
@@ -412,11 +419,14 @@ typedef struct fm_tune_result {
     int32_t trial_spp;
 } fm_tune_result;
 /* Times short renders of `scene` (the frame of `base`, trial_spp samples,
- * <= 0: 16) for each launch configuration of the chosen form: block sizes
- * 64 / 128 / 256; the grid each kernel's occupancy gives, or a fixed one
- * (wavefront: 2048 threads per CU; megakernel: twice the occupancy grid);
- * for the wavefront form passes of the whole trial or of a quarter of it
- * (megakernel: chunks of 2, 4, 8, 16 or 64 samples).  Returns the fastest by device time; `trials` (may be
+ * <= 0: 16) for each traversal tree width (2, 4 and 8 for host-built trees,
+ * re-collapsed from the scene's binary tree; the scene's own width for
+ * device-built ones) and each launch configuration of the chosen form: block
+ * sizes 64 / 128 / 256; the grid each kernel's occupancy gives, or a fixed
+ * one (wavefront: 2048 threads per CU; megakernel: twice the occupancy
+ * grid); for the wavefront form passes of the whole trial or of a quarter of
+ * it (megakernel: chunks of 2, 4, 8, 16 or 64 samples).  Returns the fastest
+ * by device time and leaves the scene at its tree width; `trials` (may be
  * NULL) receives up to max_trials of them.  Overwrites the device film.  No
  * counterpart in the reference (its thread count is a JSON setting,
  * renderer.h:70-77). */
@@ -425,11 +435,12 @@ int fm_autotune(fm_ctx* ctx, fm_scene* scene, const fm_render_params* base, int3
 
 /* Enables the node / primitive counters in later renders (0 = off). */
 int fm_set_counting(fm_ctx* ctx, int enabled);
-/* Width (2 or 4, default 2) of the traversal tree of the scenes created on
- * this context afterwards.  The four-wide tree is the binary one with nodes
- * merged (half the node visits, twice the box tests per visit); the closest
- * hit does not depend on the width (ties are resolved by reference slot).  No counterpart in the reference (whose BVHAggregate
- * is binary, bvh_aggregate.cpp:38-79): a device traversal parameter. */
+/* Width (2, 4 or 8, default 2) of the traversal tree of the scenes created
+ * on this context afterwards.  The four- and eight-wide trees are the binary
+ * one with nodes merged (fewer node visits, more box tests per visit); the
+ * closest hit does not depend on the width (ties are resolved by reference
+ * slot).  No counterpart in the reference (whose BVHAggregate is binary,
+ * bvh_aggregate.cpp:38-79): a device traversal parameter. */
 int fm_set_bvh_width(fm_ctx* ctx, int32_t width);
 /* Bytes of wavefront path state one pass may hold on this context (0 =
  * the default: 96 GB, or half the device memory free at fm_init if that is

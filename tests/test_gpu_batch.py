@@ -94,16 +94,36 @@ def test_batch_rejects_bad_jobs(gpu_ctx):
     assert films == [] and st.paths == 0
 
 
-def test_autotune_picks_a_timed_configuration_that_renders_the_same_image(gpu_ctx):
-    """fm_autotune times every launch configuration once; its best is no
-    slower than the defaults and renders the same film (launch shape and
-    pass size do not change paths; passes only regroup the FP64 sums)."""
-    s = scenes.box_scene(width=64, height=48, spp=8)
+def test_autotune_sweeps_tree_width_and_launch_shape_over_one_image(gpu_ctx):
+    """fm_autotune on the box scene (the batched-jobs workload's scene):
+    every trial is timed; the trials cover tree widths 2, 4 and 8 (the
+    scene's tree re-collapsed per width) and each width's launch shapes; the
+    best is the measured minimum and the scene is left at its width; every
+    trial renders the same image: bit for bit where only the width, block or
+    grid differs from the defaults, and to the order of FP32 film
+    additions where the pass size (wavefront) or work item (megakernel)
+    differs."""
+    s = scenes.box_scene(width=128, height=96, spp=8)
     g = gpu_ctx.scene(s)
     for mode in (A.FM_MODE_WAVEFRONT, A.FM_MODE_MEGAKERNEL):
+        w0 = g.info().trav_width   # the previous call may have left another width
         r = g.autotune(trial_spp=8, mode=mode)
-        assert r["best"]["kernel_ms"] <= r["baseline"]["kernel_ms"] and len(r["trials"]) >= 7
-        assert all(t["kernel_ms"] > 0 for t in r["trials"])
+        tr = r["trials"]
+        assert len(tr) == r["num_trials"]
+        assert {t["bvh_width"] for t in tr} == {2, 4, 8}
+        assert all(t["kernel_ms"] > 0 for t in tr)
+        best = min(tr, key=lambda t: t["kernel_ms"])
+        assert r["best"]["kernel_ms"] == best["kernel_ms"] and r["best"]["bvh_width"] == best["bvh_width"]
+        assert g.info().trav_width == best["bvh_width"]
+        base = r["baseline"]
+        assert base["bvh_width"] == w0 and base["block_size"] == 0 and base["grid_blocks"] == 0
+        for t in tr:
+            if t["chunk"] == base["chunk"]:
+                assert t["film_hash"] == base["film_hash"], t
+            assert abs(t["film_mean"] - base["film_mean"]) <= 1e-6 * abs(base["film_mean"]), t
+        print("autotune mode %d: %d trials, best width %d block %d grid %d chunk %d: %.3f ms (defaults %.3f ms)"
+              % (mode, len(tr), best["bvh_width"], best["block_size"], best["grid_blocks"], best["chunk"],
+                 best["kernel_ms"], base["kernel_ms"]))
         b = r["best"]
         ref = g.render(seed=3, mode=mode)
         got = g.render(seed=3, mode=mode, block_size=b["block_size"], grid_blocks=b["grid_blocks"], chunk=b["chunk"])

@@ -113,23 +113,25 @@ def test_traversal_tree_does_not_change_the_closest_hit(gpu_ctx):
 
 
 def test_bvh_width_does_not_change_hits_or_images(gpu_ctx):
-    """Two-wide and four-wide traversal trees (the second merges nodes of the
-    first): equal hits for random and axis-aligned rays, closest and any,
-    and bit-identical wavefront and megakernel films."""
+    """Two-, four- and eight-wide traversal trees (the wider ones merge nodes
+    of the first): equal hits for random and axis-aligned rays, closest and
+    any, and bit-identical wavefront and megakernel films."""
     from fumeng_amd import FumengError
     s = mesh_scene(freq=20)
+    g = {}
     try:
-        gpu_ctx.set_bvh_width(2)
-        g2 = gpu_ctx.scene(s)
-        gpu_ctx.set_bvh_width(4)
-        g4 = gpu_ctx.scene(s)
+        for w in (2, 4, 8):
+            gpu_ctx.set_bvh_width(w)
+            g[w] = gpu_ctx.scene(s)
         with pytest.raises(FumengError):
             gpu_ctx.set_bvh_width(3)
     finally:
         gpu_ctx.set_bvh_width(2)
-    i2, i4 = g2.info(), g4.info()
+    i2, i4, i8 = g[2].info(), g[4].info(), g[8].info()
     assert (i2.trav_width, i2.trav_node_bytes, i4.trav_width, i4.trav_node_bytes) == (2, 64, 4, 128)
-    assert 0 < i4.num_trav_nodes < i2.num_trav_nodes and 1 <= i4.trav_stack <= 64
+    assert (i8.trav_width, i8.trav_node_bytes) == (8, 256)
+    assert 0 < i8.num_trav_nodes < i4.num_trav_nodes < i2.num_trav_nodes
+    assert 1 <= i4.trav_stack <= 64 and 1 <= i8.trav_stack <= 64
     rng = np.random.default_rng(31)
     r = scene_rays(rng, 20000)
     n = 3000
@@ -138,15 +140,19 @@ def test_bvh_width_does_not_change_hits_or_images(gpu_ctx):
     ax[np.arange(n), 4 + rng.integers(0, 3, n)] = rng.choice([-1.0, 1.0], n)
     ax[:, 7] = 3.4e38
     for rays in (r, ax):
-        h2, h4 = g2.trace(rays), g4.trace(rays)
-        assert np.array_equal(h2["geom"], h4["geom"])
-        assert np.array_equal(h2["t"], h4["t"])
-        assert np.array_equal(g2.trace(rays, True)["geom"], g4.trace(rays, True)["geom"])
+        h2 = g[2].trace(rays)
+        a2 = g[2].trace(rays, True)["geom"]
+        for w in (4, 8):
+            hw = g[w].trace(rays)
+            assert np.array_equal(h2["geom"], hw["geom"]), w
+            assert np.array_equal(h2["t"], hw["t"]), w
+            assert np.array_equal(a2, g[w].trace(rays, True)["geom"]), w
     for mode in (A.FM_MODE_WAVEFRONT, A.FM_MODE_MEGAKERNEL):
-        a = g2.render(width=48, height=32, spp=4, seed=9, mode=mode)
-        b = g4.render(width=48, height=32, spp=4, seed=9, mode=mode)
-        for x, y in zip(a[:4], b[:4]):
-            assert np.array_equal(x, y, equal_nan=True), mode
+        a = g[2].render(width=48, height=32, spp=4, seed=9, mode=mode)
+        for w in (4, 8):
+            b = g[w].render(width=48, height=32, spp=4, seed=9, mode=mode)
+            for x, y in zip(a[:4], b[:4]):
+                assert np.array_equal(x, y, equal_nan=True), (mode, w)
 
 
 def test_naive_accelerator_parity(gpu_ctx):

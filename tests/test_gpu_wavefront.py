@@ -148,7 +148,7 @@ def test_textured_many_light_scene_matches_oracle(gpu_ctx, mode):
     assert np.mean(np.abs(ag - ao) < 1e-4) > 0.97
 
 
-@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("width", [2, 4, 8])
 def test_short_trace_stack_gives_the_same_films(gpu_ctx, width):
     """The trace kernel's short stack (16 entries in LDS, older ones spilled
     to HBM) against the full LDS stack, on trees deeper than 16 entries so

@@ -83,21 +83,25 @@ def test_host_builder_equals_reference_topology(freq):
 
 @pytest.mark.parametrize("name", ["mesh", "box", "stress_small"])
 def test_host_build_of_both_traversal_widths(name):
-    """The whole host build (no device) for both tree widths: a two-wide
-    traversal pushes at most one entry per level (stack = depth - 1), the
-    four-wide tree merges nodes (fewer of them, each of 128 bytes) and its
-    exact stack bound stays within three entries per four-wide level."""
+    """The whole host build (no device) for every tree width: a two-wide
+    traversal pushes at most one entry per level (stack = depth - 1); the
+    four- and eight-wide trees merge nodes (fewer of them, each of 128 or
+    256 bytes); the four-wide stack bound stays within three entries per
+    two binary levels, the eight-wide one within seven per level."""
     s = {"mesh": lambda: mesh_scene(freq=20), "box": lambda: scenes.box_scene(width=16, height=16),
          "stress_small": lambda: scenes.stress_scene(n_spheres=6, freq=6, n_textures=2, tex_size=32, width=16,
                                                      height=16)}[name]()
     i2 = fumeng_amd.host_scene_info(s, 2)
     i4 = fumeng_amd.host_scene_info(s, 4)
-    assert i2.device_bytes == 0 and i2.num_geometries == i4.num_geometries > 0
+    i8 = fumeng_amd.host_scene_info(s, 8)
+    assert i2.device_bytes == 0 and i2.num_geometries == i4.num_geometries == i8.num_geometries > 0
     assert (i2.trav_width, i2.trav_node_bytes, i4.trav_width, i4.trav_node_bytes) == (2, 64, 4, 128)
+    assert (i8.trav_width, i8.trav_node_bytes) == (8, 256)
     assert i2.num_trav_nodes == i2.num_geometries - 1 or i2.num_trav_nodes < i2.num_geometries
-    assert 0 < i4.num_trav_nodes <= i2.num_trav_nodes
+    assert 0 < i8.num_trav_nodes <= i4.num_trav_nodes <= i2.num_trav_nodes
     depth = i2.trav_stack + 1
     assert 1 <= i4.trav_stack <= 3 * ((depth + 1) // 2)
+    assert 1 <= i8.trav_stack <= min(64, 7 * (depth - 1))   # every wide node on a path spans >= 1 binary level
     with pytest.raises(fumeng_amd.FumengError):
         fumeng_amd.host_scene_info(s, 3)
 
@@ -244,10 +248,10 @@ def test_threaded_builds_equal_sequential_builds(monkeypatch):
     slot records are the same as a one-thread build, on a scene large enough
     for the threaded levels (81,920 triangles)."""
     s = mesh_scene(freq=64, with_box=False, with_spheres=False)
-    par = [fumeng_amd.host_scene_info(s, w) for w in (2, 4)]
+    par = [fumeng_amd.host_scene_info(s, w) for w in (2, 4, 8)]
     ref_par = fumeng_amd.host_bvh(s)
     monkeypatch.setenv("FUMENG_BUILD_THREADS", "1")
-    seq = [fumeng_amd.host_scene_info(s, w) for w in (2, 4)]
+    seq = [fumeng_amd.host_scene_info(s, w) for w in (2, 4, 8)]
     ref_seq = fumeng_amd.host_bvh(s)
     assert seq[0].build_threads == 1 and par[0].build_threads >= 1
     for a, b in zip(par, seq):
