@@ -25,10 +25,19 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def host_threads():
+    """The host cores this process may use: its CPU affinity, or the thread
+    count the host grants it (OMP_NUM_THREADS) if that is lower."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(cap))) if cap.isdigit() and int(cap) > 0 else max(1, n)
+
+
 def cpu_baseline(desc, W, H, spp, seconds_target=15.0):
-    """Oracle FP32 path tracer on a centred crop, all host cores (<= 16)."""
+    """Oracle FP32 path tracer on a centred crop, on every host core the
+    process may use (host_threads)."""
     import oracle
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_threads()
     o = oracle.OracleScene(desc)
     # calibrate on a tiny crop, then size the sample to about seconds_target
     cw, ch, cs = 16, 16, 4
@@ -42,8 +51,70 @@ def cpu_baseline(desc, W, H, spp, seconds_target=15.0):
     o.render(W // 2 - side // 2, H // 2 - side // 2, side, side, spp=16, seed=12, threads=threads)
     dt = time.perf_counter() - t0
     paths = side * side * 16
+    visible = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
-            "sample": "%dx%d centred crop of the same frame at 16 spp (%d paths, %.1f s)" % (side, side, paths, dt)}
+            "sample": "%dx%d centred crop of the same frame at 16 spp (%d paths, %.1f s; %d threads, %s cores in "
+                      "the process's affinity)" % (side, side, paths, dt, threads, visible)}
+
+
+# Algorithmic FP32 operations (add, sub, mul, div, sqrt, min, max; a fused
+# multiply-add counts 2; compares and selects 0) of the reference formulas
+# the traversal and shading run, counted from the restated code:
+FLOPS_RAY_SETUP = 3        # 1 / dir (bvh_aggregate.cpp:101-123)
+FLOPS_BOX = 24             # slab test of one child box: 6 sub, 6 mul, 6 min/max, 6 for the t_min / t_max clamps (bbox.h:59-69)
+FLOPS_TRIANGLE = 48        # watertight test: 9 translate, 3 div shear, 12 shear, 9 edges, 2 det, 3 scale, 1 div, 6 t, 3 bc
+                           # (triangle.cpp:25-94)
+FLOPS_RECTANGLE = 2 * FLOPS_TRIANGLE   # two triangle tests when the first rejects (rectangle.cpp:42-85)
+FLOPS_SPHERE = 70          # world-to-object point (27) and vector (15), quadratic setup (18), root (10) (sphere.cpp:91-129)
+FLOPS_VERTEX = 280         # one path vertex of the box scene: hit fill 20, BSDF frame 41, rectangle light sample 45,
+                           # Lambert eval + pdf 30, two MIS weights 40, two BSDF samples 80, RR + throughput 24
+
+
+def box_leg(fumeng_amd, ctx, steps=3):
+    """The open-box workload: 5 Lambert walls, a 0.5x0.5 area light, a
+    Lambert and a glass sphere; pin-hole 40 degrees, 512x512, 64 spp,
+    depth 8, Russian roulette at every vertex.  Both forms, plus the VALU
+    roofline of the megakernel (the plain form): algorithmic FLOPs per ray
+    from the op-count table above and the counting variant's node visits
+    and primitive tests by kind, over the kernel's HIP-event duration.
+    Vertices are taken as rays / 3 (one shadow, one MisBSDF and one
+    continuation ray per vertex with one light)."""
+    from fumeng_amd import scenes, _abi
+    desc = scenes.box_scene()
+    W, H, spp = desc.renderer["width"], desc.renderer["height"], desc.renderer["spp"]
+    sc = ctx.scene(desc)
+    res = {"workload": "open box, 5 Lambert walls + 0.5x0.5 light + Lambert and glass spheres, pin-hole 40 deg, "
+                       "512x512, 64 spp, depth 8, rr_depth 3"}
+    for name, mode in (("megakernel", _abi.FM_MODE_MEGAKERNEL), ("wavefront", _abi.FM_MODE_WAVEFRONT)):
+        sc.render(seed=3, spp=4, read=False, mode=mode)   # warm-up
+        ms, rays = [], 0
+        t0 = time.perf_counter()
+        for i in range(steps):
+            _, _, _, _, st = sc.render(seed=1 + i, read=False, mode=mode)
+            ms.append(st.kernel_ms)
+            rays += st.rays
+        wall = time.perf_counter() - t0
+        res[name] = {"mpaths_per_s": W * H * spp * steps / wall / 1e6, "mrays_per_s": rays / wall / 1e6,
+                     "ms_per_frame": wall * 1e3 / steps, "kernel_ms": float(sum(ms) / len(ms)), "rays": rays // steps}
+        log("box %s: %.1f Mpaths/s, %.1f ms per frame" % (name, res[name]["mpaths_per_s"], res[name]["ms_per_frame"]))
+    fumeng_amd.lib().fm_set_counting(ctx.h, 1)
+    _, _, _, _, cst = sc.render(seed=2, spp=8, read=False, mode=_abi.FM_MODE_MEGAKERNEL)
+    fumeng_amd.lib().fm_set_counting(ctx.h, 0)
+    sc.close()
+    r = float(cst.rays)
+    tris = cst.prim_tests - cst.sphere_tests - cst.rect_tests
+    per_ray = (FLOPS_RAY_SETUP + 2 * FLOPS_BOX * cst.node_visits / r + FLOPS_TRIANGLE * tris / r
+               + FLOPS_RECTANGLE * cst.rect_tests / r + FLOPS_SPHERE * cst.sphere_tests / r + FLOPS_VERTEX / 3.0)
+    mk = res["megakernel"]
+    achieved = per_ray * mk["rays"] / (mk["kernel_ms"] * 1e-3) / 1e12
+    res["roofline_valu"] = {"bound": "valu", "kernel": "k_render_mega<false>", "achieved": achieved,
+                            "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                            "flops_per_ray": per_ray, "nodes_per_ray": cst.node_visits / r,
+                            "rect_tests_per_ray": cst.rect_tests / r, "sphere_tests_per_ray": cst.sphere_tests / r,
+                            "triangle_tests_per_ray": tris / r}
+    log("box megakernel VALU roofline: %.0f FLOP per ray, %.2f TFLOP/s = %.4f of %.1f"
+        % (per_ray, achieved, achieved / FP32_PEAK_TFLOPS, FP32_PEAK_TFLOPS))
+    return res
 
 
 def batch_leg(fumeng_amd, ctx):
@@ -186,17 +257,24 @@ def stress_leg(fumeng_amd, ctx):
     return res
 
 
-def profiled_traffic(kernel):
-    """HBM bytes per launch of `kernel` on the default workload, from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by a
-    1 GiB copy); None when no summary names the kernel."""
+def profiled(tag, kernel):
+    """A figure of `kernel` on the default workload from the committed
+    rocprofv3 summaries under profiles/: lines "<tag>: <kernel> <value>".
+    bench-traffic = HBM bytes per launch (FETCH_SIZE + WRITE_SIZE scaled by a
+    1 GiB copy), bench-l2-hit = L2 hit rate (TCC_HIT / (TCC_HIT + TCC_MISS)),
+    bench-valu-busy = share of cycles with VALU instructions issued,
+    bench-launch-us = average launch duration in the counter run's trace.
+    None when no summary names the kernel."""
     import glob
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         for line in open(path):
-            if line.startswith("bench-traffic:"):
-                name, _, count = line[len("bench-traffic:"):].strip().rpartition(" ")
-                if name == kernel and count.isdigit():
-                    return int(count)
+            if line.startswith(tag + ":"):
+                name, _, value = line[len(tag) + 1:].strip().rpartition(" ")
+                if name == kernel:
+                    try:
+                        return float(value)
+                    except ValueError:
+                        return None
     return None
 
 
@@ -236,8 +314,12 @@ def main():
                     help="skip the inverse-rendering (adjoint) leg")
     ap.add_argument("--no-batch", dest="batch", action="store_false",
                     help="skip the batched-jobs leg (64 uneven renders of the box scene)")
+    ap.add_argument("--no-box", dest="box", action="store_false",
+                    help="skip the open-box leg (512x512, 64 spp, both forms, VALU roofline)")
     ap.add_argument("--no-compare", dest="compare", action="store_false",
                     help="skip the one-step run of the other form")
+    ap.add_argument("--box-only", action="store_true",
+                    help="only run the open-box leg and print its JSON (profiling of that workload)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -252,6 +334,11 @@ def main():
     import fumeng_amd
     from fumeng_amd import scenes
     from fumeng_amd import _abi as _abi_mod
+    if args.box_only:
+        ctx = fumeng_amd.Context(0)
+        print(json.dumps(box_leg(fumeng_amd, ctx)))
+        ctx.close()
+        return
 
     t0 = time.perf_counter()
     desc = scenes.icosphere_scene(freq=args.freq, spp=args.spp)
@@ -323,14 +410,35 @@ def main():
             default_run = (args.freq, args.spp, args.block, args.grid, args.chunk) == (224, 256, 0, 0, 0)
             kname = (("k_wf_trace<false, %d, %s>" % (info.trav_width, "true" if st.trace_short_stack else "false"))
                      if mode == _abi.FM_MODE_WAVEFRONT else "k_render_mega<false>")
-            traffic = profiled_traffic(kname) if default_run else None
+            traffic = profiled("bench-traffic", kname) if default_run else None
+            traffic = int(traffic) if traffic is not None else None
+            launch_ms = trace_ms / args.steps / max(1, 2 * (12 + 1))   # 2 passes x 13 iterations per frame
+            # `achieved` / `frac` rate the bytes the traversal requests; the
+            # measured HBM rate (counter traffic per launch over the launch
+            # time) and the L2 hit rate say how much of that reaches HBM
+            measured = None
+            if traffic is not None:
+                hbm_gbs = traffic / (launch_ms * 1e-3) / 1e9
+                measured = {"hbm_gbs": hbm_gbs, "hbm_frac": hbm_gbs / HBM_PEAK_GBS,
+                            "traffic_over_requested": traffic / (bytes_per_ray * rays / args.steps / 26.0),
+                            "l2_hit": profiled("bench-l2-hit", kname),
+                            "valu_busy": profiled("bench-valu-busy", kname)}
             roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kname,
+                        "achieved_is": "requested (algorithmic) bytes per second",
+                        "measured": measured,
                         "bytes_per_ray": bytes_per_ray, "kernel_ms_per_step": trace_ms / args.steps,
                         "nodes_per_ray": cst.node_visits / cst.rays, "prims_per_ray": cst.prim_tests / cst.rays,
                         "node_bytes": info.trav_node_bytes}
     except Exception as e:  # counting is diagnostic; the throughput stands without it
         log("counting pass failed: %s" % e)
+
+    box = None
+    if args.box:
+        try:
+            box = box_leg(fumeng_amd, ctx)
+        except Exception as e:
+            log("box leg failed: %s" % e)
 
     batch = None
     if args.batch:
@@ -367,8 +475,9 @@ def main():
                                   "thin lens, %dx%d, %d spp, depth 12, %s" % (20 * args.freq ** 2, W, H, spp, args.mode),
                       "mode": args.mode, "width": W, "height": H, "spp": spp, "depth": 12,
                       "bvh_width": args.bvh_width, "bvh_builder": args.bvh_builder},
-           "mrays_per_s": mrays, "paths": paths, "rays": rays, "other_form": other, "batch": batch, "adjoint": adjoint, "stress": stress,
-           "roofline": roofline, "cpu_baseline": cpu}
+           "mrays_per_s": mrays, "paths": paths, "rays": rays, "other_form": other, "box": box, "batch": batch, "adjoint": adjoint,
+           "stress": stress, "roofline": roofline,
+           "roofline_valu": box["roofline_valu"] if box else None, "cpu_baseline": cpu}
     print(json.dumps(out))
     scene.close()
     ctx.close()

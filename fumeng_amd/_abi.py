@@ -145,7 +145,8 @@ class fm_film(C.Structure):
 class fm_render_stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("paths", C.c_int64), ("rays", C.c_int64),
                 ("node_visits", C.c_int64), ("prim_tests", C.c_int64), ("trace_ms", C.c_double),
-                ("trace_block", C.c_int32), ("trace_short_stack", C.c_int32)]
+                ("trace_block", C.c_int32), ("trace_short_stack", C.c_int32), ("sphere_tests", C.c_int64),
+                ("rect_tests", C.c_int64)]
 
 
 class fm_job(C.Structure):

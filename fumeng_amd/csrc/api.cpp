@@ -600,7 +600,7 @@ int fm_init(int device, fm_ctx** out) {
         HIPCHECK(hipEventCreate(&c->ev0));
         HIPCHECK(hipEventCreate(&c->ev1));
         HIPCHECK(hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device));
-        c->counters.alloc(4 * sizeof(unsigned long long));
+        c->counters.alloc(8 * sizeof(unsigned long long));
         c->noalloc = noalloc_array(device);
         c->path_budget = default_path_budget();
         return FM_OK;
@@ -896,7 +896,7 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
         A.color_sum = base; A.albedo_sum = base + 3 * npix; A.normal_sum = base + 6 * npix; A.sumsq = base + 9 * npix;
         A.atomic = (nchunks > 1) ? 1 : 0;
         A.counters = ctx->counters.as<unsigned long long>();
-        HIPCHECK(fm::launch_zero_f64((double*)A.counters, 4, ctx->stream));  /* a fill kernel, not a memset node */
+        HIPCHECK(fm::launch_zero_f64((double*)A.counters, 8, ctx->stream));  /* a fill kernel, not a memset node */
         const int block = p->block_size > 0 ? p->block_size : 256;
         if (block % 64 != 0 || block > 256) return fail(ctx, FM_ERR_INVALID, "block_size must be a multiple of 64, at most 256");
         /* depth <= 0: the reference's bounce loop (path_tracing.cpp:42, for
@@ -935,13 +935,15 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
             memset(stats, 0, sizeof(*stats));
             float ms = 0.0f;
             HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-            unsigned long long c[4];
+            unsigned long long c[8];
             HIPCHECK(hipMemcpy(c, A.counters, sizeof c, hipMemcpyDeviceToHost));
             stats->kernel_ms = ms;
             stats->paths = black ? npix * spp : (int64_t)c[0];
             stats->rays = (int64_t)c[1];
             stats->node_visits = (int64_t)c[2];
             stats->prim_tests = (int64_t)c[3];
+            stats->sphere_tests = (int64_t)c[4];
+            stats->rect_tests = (int64_t)c[5];
             stats->trace_block = trace_block;
             stats->trace_short_stack = short_stack;
             if (p->flags & FM_RENDER_TIME_TRACE) stats->trace_ms = wave ? trace_ms : ms;
@@ -1024,13 +1026,15 @@ int fm_render_noalloc(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, void*
             memset(stats, 0, sizeof(*stats));
             float ms = 0.0f;
             HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-            unsigned long long c[4];
+            unsigned long long c[8];
             HIPCHECK(hipMemcpy(c, work, sizeof c, hipMemcpyDeviceToHost));
             stats->kernel_ms = ms;
             stats->paths = black ? npix * spp : (int64_t)c[0];
             stats->rays = (int64_t)c[1];
             stats->node_visits = (int64_t)c[2];
             stats->prim_tests = (int64_t)c[3];
+            stats->sphere_tests = (int64_t)c[4];
+            stats->rect_tests = (int64_t)c[5];
             if (p->flags & FM_RENDER_TIME_TRACE) stats->trace_ms = ms;
         }
         return FM_OK;
@@ -1073,7 +1077,7 @@ int fm_render_batch(fm_ctx* ctx, fm_scene* s, const fm_job* jobs, int n, const f
         double* film = ctx->batch_film.as<double>();
         for (int i = 0; i < n; i++) wj[(size_t)i].film = film + off[(size_t)i];
         HIPCHECK(fm::launch_zero_f64(film, off[(size_t)n], ctx->stream));
-        HIPCHECK(fm::launch_zero_f64((double*)ctx->counters.p, 4, ctx->stream));
+        HIPCHECK(fm::launch_zero_f64((double*)ctx->counters.p, 8, ctx->stream));
         /* a job of depth <= 0 keeps its zeroed film (no bounce runs,
          * path_tracing.cpp:42) and is left out of the run */
         std::vector<WfJob> run;
@@ -1090,7 +1094,7 @@ int fm_render_batch(fm_ctx* ctx, fm_scene* s, const fm_job* jobs, int n, const f
         }
         if (stats) {
             memset(stats, 0, sizeof(*stats));
-            unsigned long long c[4];
+            unsigned long long c[8];
             HIPCHECK(hipMemcpy(c, ctx->counters.p, sizeof c, hipMemcpyDeviceToHost));
             stats->kernel_ms = R.kernel_ms;
             stats->paths = (int64_t)c[0];
@@ -1244,7 +1248,7 @@ int fm_render_adjoint(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, const
             if (ctx->adj_genv.n < (size_t)nenv * sizeof(double)) ctx->adj_genv.alloc((size_t)nenv * sizeof(double));
             HIPCHECK(fm::launch_zero_f64(ctx->adj_genv.as<double>(), nenv, st));
         }
-        HIPCHECK(fm::launch_zero_f64((double*)ctx->counters.p, 4, st));
+        HIPCHECK(fm::launch_zero_f64((double*)ctx->counters.p, 8, st));
         A.dl_dimg = ctx->adj_in.as<float>();
         A.g_mat = ctx->adj_gmat.as<double>();
         A.g_env = nenv > 0 ? ctx->adj_genv.as<double>() : nullptr;
@@ -1276,7 +1280,7 @@ int fm_render_adjoint(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, const
         job.film = ctx->adj_film.as<double>();
         WfRunOut fwd;
         wf_run(ctx, s, {job}, A.cfg_raw.direct, A.cfg_raw.rr_depth, A.cfg_raw.rr_coef, 0, 0, 0, 0, fwd, A.totals);
-        HIPCHECK(fm::launch_zero_f64((double*)ctx->counters.p, 4, st));
+        HIPCHECK(fm::launch_zero_f64((double*)ctx->counters.p, 8, st));
         HIPCHECK(hipEventRecord(ctx->ev0, st));
         HIPCHECK(fm::launch_adjoint(s->d, A, block, std::max(grid, 1), levels, st));
         HIPCHECK(hipEventRecord(ctx->ev1, st));
@@ -1288,7 +1292,7 @@ int fm_render_adjoint(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, const
             memset(stats, 0, sizeof(*stats));
             float ms = 0.0f;
             HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-            unsigned long long c[4];
+            unsigned long long c[8];
             HIPCHECK(hipMemcpy(c, A.counters, sizeof c, hipMemcpyDeviceToHost));
             stats->kernel_ms = fwd.kernel_ms + ms;
             stats->paths = (int64_t)c[0];

@@ -268,7 +268,16 @@ FMD bool test_slot_any(const fm_dscene& S, const Ray& r, const RayPre& rp, int32
 }
 
 /* Work counters of the counting variant (node visits, primitive tests). */
-struct Cnt { uint32_t nodes = 0, prims = 0; };
+struct Cnt { uint32_t nodes = 0, prims = 0, spheres = 0, rects = 0; };
+/* counting variant: one primitive test at slot p, by kind (a rectangle is
+ * two triangle tests, a sphere one quadratic) */
+template <bool COUNT> FMD void count_slot(const fm_dscene& S, int32_t p, Cnt& cn) {
+    if (!COUNT) return;
+    const int kind = FM_PRIM_KIND(__float_as_int(reinterpret_cast<const float4*>(S.prims)[3 * (size_t)p].w));
+    cn.prims++;
+    cn.spheres += kind == FM_PRIM_SPHERE ? 1u : 0u;
+    cn.rects += kind == FM_PRIM_RECT ? 1u : 0u;
+}
 
 /* Traversal with a per-thread stack in LDS, stack[level * stride + lane],
  * in the speculative "while-while" form of Aila & Laine (HPG 2009): a lane
@@ -376,12 +385,11 @@ FMD bool traverse_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& 
             const int32_t start = FM_LEAF_START(leaf), count = FM_LEAF_COUNT(leaf);
             if (ANY) {
                 for (int k = 0; k < count; k++) {
-                    if (COUNT) cn.prims++;
+                    count_slot<COUNT>(S, start + k, cn);
                     if (test_slot_any(S, r, rp, start + k)) return true;
                 }
             } else {
-                if (COUNT) cn.prims += (uint32_t)count;
-                for (int k = 0; k < count; k++) test_slot_closest(S, r, rp, start + k, h);
+                for (int k = 0; k < count; k++) { count_slot<COUNT>(S, start + k, cn); test_slot_closest(S, r, rp, start + k, h); }
             }
             if (node < 0) { leaf = node; node = pop(); }
             else leaf = FM_TRAV_NONE;
@@ -447,12 +455,11 @@ FMD bool traverse4_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit&
             const int32_t start = FM_LEAF_START(leaf), count = FM_LEAF_COUNT(leaf);
             if (ANY) {
                 for (int k = 0; k < count; k++) {
-                    if (COUNT) cn.prims++;
+                    count_slot<COUNT>(S, start + k, cn);
                     if (test_slot_any(S, r, rp, start + k)) return true;
                 }
             } else {
-                if (COUNT) cn.prims += (uint32_t)count;
-                for (int k = 0; k < count; k++) test_slot_closest(S, r, rp, start + k, h);
+                for (int k = 0; k < count; k++) { count_slot<COUNT>(S, start + k, cn); test_slot_closest(S, r, rp, start + k, h); }
             }
             if (node < 0) { leaf = node; node = pop(); }
             else leaf = FM_TRAV_NONE;
@@ -525,12 +532,11 @@ FMD bool traverse8_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit&
             const int32_t start = FM_LEAF_START(leaf), count = FM_LEAF_COUNT(leaf);
             if (ANY) {
                 for (int q = 0; q < count; q++) {
-                    if (COUNT) cn.prims++;
+                    count_slot<COUNT>(S, start + q, cn);
                     if (test_slot_any(S, r, rp, start + q)) return true;
                 }
             } else {
-                if (COUNT) cn.prims += (uint32_t)count;
-                for (int q = 0; q < count; q++) test_slot_closest(S, r, rp, start + q, h);
+                for (int q = 0; q < count; q++) { count_slot<COUNT>(S, start + q, cn); test_slot_closest(S, r, rp, start + q, h); }
             }
             if (node < 0) { leaf = node; node = pop(); }
             else leaf = FM_TRAV_NONE;

@@ -177,6 +177,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     if (COUNT) {
         wave_add_u64(A.counters + 2, cn.nodes);
         wave_add_u64(A.counters + 3, cn.prims);
+        wave_add_u64(A.counters + 4, cn.spheres);
+        wave_add_u64(A.counters + 5, cn.rects);
     }
 }
 

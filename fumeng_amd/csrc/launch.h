@@ -20,7 +20,8 @@ struct RenderArgs {
     double* albedo_sum;
     double* normal_sum;
     double* sumsq;
-    unsigned long long* counters;   /* [0] paths, [1] rays, [2] node visits, [3] primitive tests */
+    unsigned long long* counters;   /* [0] paths, [1] rays, [2] node visits, [3] primitive tests, [4] of them
+                                       spheres, [5] rectangles (8 entries) */
 };
 
 /* Wavefront form.  Per-iteration queue counters, [iteration][WQ_*]. */

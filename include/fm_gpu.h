@@ -296,6 +296,8 @@ typedef struct fm_render_stats {
                                 whole megakernel) */
     int32_t trace_block;     /* wavefront: the trace kernel's block size */
     int32_t trace_short_stack;  /* wavefront: 1 if it ran with the short stack */
+    int64_t sphere_tests;    /* of prim_tests (counting only): spheres */
+    int64_t rect_tests;      /* of prim_tests (counting only): rectangles (two triangle tests each) */
 } fm_render_stats;
 
 /* Renders into the context's device film (FP64 sums) and, when `film` has
