@@ -238,6 +238,11 @@ def stress_leg(fumeng_amd, ctx):
         % (dbuild, dinfo.build_ms_total, dinfo.build_ms_device, dinfo.build_iterations, dinfo.num_trav_nodes,
            dinfo.trav_stack, ddt * 1e3, dst.trace_ms))
     info, build, st, dt = build_and_frame("host")
+    files = None
+    try:
+        files = stress_from_files(desc, info)
+    except Exception as e:
+        log("stress from files failed: %s" % e)
     del desc
     res = {"workload": "stress: 400 displaced spheres (9,800,012 geometries), 200 textures 2048^2, 10 rect lights, "
                        "1920x1080, 128 spp, depth 10", "mpaths_per_s": W * H * spp / dt / 1e6,
@@ -247,13 +252,65 @@ def stress_leg(fumeng_amd, ctx):
                             "reference_bvh_ms": info.build_ms_ref_bvh, "traversal_tree_ms": info.build_ms_trav_tree,
                             "tables_ms": info.build_ms_tables, "host_total_ms": info.build_ms_total,
                             "upload_ms": info.upload_ms, "host_gb": info.host_bytes / 1e9},
-           "trav_nodes": info.num_trav_nodes, "trav_stack": info.trav_stack, "device_builder": device}
+           "trav_nodes": info.num_trav_nodes, "trav_stack": info.trav_stack, "device_builder": device,
+           "from_files": files}
     log("stress: %.1f Mpaths/s, %.1f Mrays/s (%.0f ms per frame), built in %.1f s, %.2f GB in HBM"
         % (res["mpaths_per_s"], res["mrays_per_s"], dt * 1e3, build, res["device_gb"]))
     log("stress build on %d threads: geometry %.0f ms, reference BVH %.0f ms, traversal tree %.0f ms, tables %.0f ms, "
         "upload %.0f ms; host copy %.2f GB" % (info.build_threads, info.build_ms_geometry, info.build_ms_ref_bvh,
                                                info.build_ms_trav_tree, info.build_ms_tables, info.upload_ms,
                                                info.host_bytes / 1e9))
+    return res
+
+
+LOAD_CHILD = r"""
+import json, resource, sys, time
+sys.path.insert(0, sys.argv[1])
+import fumeng_amd
+t0 = time.perf_counter()
+ctx = fumeng_amd.Context(0)
+t1 = time.perf_counter()
+sc = ctx.load(sys.argv[2])
+t2 = time.perf_counter()
+i = sc.info()
+print(json.dumps({"load_build_upload_s": t2 - t1, "context_s": t1 - t0, "geometries": i.num_geometries,
+                  "trav_hash": str(i.trav_hash), "device_bytes": i.device_bytes,
+                  "build_ms_total": i.build_ms_total, "upload_ms": i.upload_ms,
+                  "peak_host_gb": resource.getrusage(resource.RUSAGE_SELF).ru_maxrss / 1e6}))
+sc.close()
+ctx.close()
+"""
+
+
+def stress_from_files(desc, info):
+    """The stress scene end to end from files: SceneDesc.write (scene JSON,
+    400 OBJ meshes, 200 TGA textures) into a temporary directory, then a
+    child process loads it with fm_scene_load (the project's JSON / OBJ /
+    TGA readers with the reference loader's semantics) and builds and
+    uploads it; its wall time and peak host memory are reported, and its
+    traversal hash (tree and slot records) must equal the in-memory build's."""
+    import shutil
+    import subprocess
+    import tempfile
+    d = tempfile.mkdtemp(prefix="fm_stress_")
+    try:
+        t0 = time.perf_counter()
+        path = desc.write(d)
+        write_s = time.perf_counter() - t0
+        size = sum(os.path.getsize(os.path.join(d, f)) for f in os.listdir(d))
+        r = subprocess.run([sys.executable, "-c", LOAD_CHILD, ROOT, path], stdout=subprocess.PIPE,
+                           stderr=subprocess.PIPE, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(r.stderr[-2000:])
+        res = json.loads(r.stdout.strip().splitlines()[-1])
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    res.update({"write_s": write_s, "files_gb": size / 1e9, "files": len(os.listdir(d)) if os.path.exists(d) else None,
+                "same_scene": res["trav_hash"] == str(info.trav_hash) and res["geometries"] == info.num_geometries})
+    res.pop("files")
+    log("stress from files: %.2f GB written in %.1f s; loaded, built and uploaded in %.1f s by a child process "
+        "(peak host memory %.1f GB); same traversal tree and slots as the in-memory build: %s"
+        % (res["files_gb"], write_s, res["load_build_upload_s"], res["peak_host_gb"], res["same_scene"]))
     return res
 
 

@@ -84,6 +84,12 @@ class Context:
         b = A.BUILDERS[builder] if isinstance(builder, str) else int(builder)
         _check(lib().fm_set_bvh_builder(self.h, b), self.h)
 
+    def set_shear_axis(self, rule="truncated"):
+        """Triangle-test axis rule of scenes created afterwards
+        (fm_set_shear_axis): "truncated" (abs resolved to the integer
+        overload, the default) or "float"."""
+        _check(lib().fm_set_shear_axis(self.h, A.SHEAR_AXIS[rule]), self.h)
+
     def set_path_budget(self, nbytes=0):
         """Bytes of wavefront path state per pass (fm_set_path_budget; 0 = default)."""
         _check(lib().fm_set_path_budget(self.h, int(nbytes)), self.h)

@@ -33,6 +33,7 @@ FM_CAMERA_PIN_HOLE, FM_CAMERA_THIN_LENS = 0, 1
 FM_ACCEL_BVH, FM_ACCEL_NAIVE, FM_ACCEL_BVH_REFERENCE_TREE = 0, 1, 2
 FM_MODE_MEGAKERNEL, FM_MODE_WAVEFRONT = 0, 1
 FM_NOALLOC_MAX_PIXELS = 262144   # include/fm_gpu.h
+SHEAR_AXIS = {"truncated": 0, "float": 1}
 FM_RENDER_TIME_TRACE = 1
 FM_RENDER_GRAPH = 2
 FM_DPARAM_PER_MATERIAL = 6
@@ -212,6 +213,7 @@ EXPORTS = {
                             C.POINTER(fm_render_stats)]),
     "fm_film_read": (C.c_int, [C.c_void_p, C.POINTER(fm_film)]),
     "fm_set_path_budget": (C.c_int, [C.c_void_p, C.c_int64]),
+    "fm_set_shear_axis": (C.c_int, [C.c_void_p, C.c_int32]),
     "fm_render_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(fm_render_params), C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.POINTER(fm_render_stats)]),
     "fm_render_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(fm_job), C.c_int, C.POINTER(fm_render_params),

@@ -126,6 +126,7 @@ struct fm_ctx {
     int bvh_builder = FM_BUILDER_HOST;   /* traversal tree builder of scenes created next */
     int trace_stack = FM_TRACE_STACK_AUTO;   /* wavefront trace kernel's stack form */
     int64_t path_budget = 0;  /* bytes of wavefront path state a pass may use (fixed per context) */
+    int shear_axis = FM_SHEAR_AXIS_TRUNCATED;   /* triangle test's axis rule of scenes created next */
     std::set<fm_scene*> scenes;  /* live scenes, released by fm_shutdown */
     WfBuffers wf;
     DevBuf wf_passes, wf_units, wf_cams, wf_pass_idx;   /* wavefront tables of the last run */
@@ -645,6 +646,7 @@ int fm_scene_create(fm_ctx* ctx, const fm_scene_desc* desc, fm_scene** out) {
         fmh::build_host_scene(*desc, s->hs);
         set_trav_hash(s->hs);
         upload_scene(s.get());
+        s->d.shear_float_abs = ctx->shear_axis == FM_SHEAR_AXIS_FLOAT ? 1 : 0;
         if (s->hs.device_tree) build_device_tree(s.get());
         *out = s.release();
         ctx->scenes.insert(*out);
@@ -843,6 +845,13 @@ int fm_set_path_budget(fm_ctx* ctx, int64_t bytes) {
         ctx->path_budget = bytes > 0 ? bytes : default_path_budget();
         return FM_OK;
     });
+}
+
+int fm_set_shear_axis(fm_ctx* ctx, int32_t rule) {
+    if (!ctx) return fail(nullptr, FM_ERR_INVALID, "NULL ctx");
+    if (rule != FM_SHEAR_AXIS_TRUNCATED && rule != FM_SHEAR_AXIS_FLOAT) return fail(ctx, FM_ERR_INVALID, "unknown shear axis rule");
+    ctx->shear_axis = rule;
+    return FM_OK;
 }
 
 int fm_set_bvh_builder(fm_ctx* ctx, int32_t builder) {

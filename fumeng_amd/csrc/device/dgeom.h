@@ -59,10 +59,19 @@ FMD int trunc_iabs(float x) { const int i = (int)x; return i < 0 ? -i : i; }
 FMD RayPre precompute(const Ray& r, const fm_dscene& S) {
 #pragma clang fp contract(off)
     RayPre p;
-    const int a0 = trunc_iabs(r.d.x), a1 = trunc_iabs(r.d.y), a2 = trunc_iabs(r.d.z);
+    /* GetMaxAxis (triangle.cpp:13-18): unqualified abs of a float, the
+     * integer overload (truncating) as libstdc++ builds resolve it, or the
+     * float overload (S.shear_float_abs) */
     int az;
-    if (a0 > a1 && a0 > a2) az = 0;
-    else az = a1 > a2 ? 1 : 2;
+    if (S.shear_float_abs) {
+        const float a0 = fabsf(r.d.x), a1 = fabsf(r.d.y), a2 = fabsf(r.d.z);
+        if (a0 > a1 && a0 > a2) az = 0;
+        else az = a1 > a2 ? 1 : 2;
+    } else {
+        const int a0 = trunc_iabs(r.d.x), a1 = trunc_iabs(r.d.y), a2 = trunc_iabs(r.d.z);
+        if (a0 > a1 && a0 > a2) az = 0;
+        else az = a1 > a2 ? 1 : 2;
+    }
     p.kz = az;
     p.kx = (az + 1) % 3;
     p.ky = (p.kx + 1) % 3;

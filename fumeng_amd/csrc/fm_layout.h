@@ -164,6 +164,7 @@ typedef struct fm_dscene {
     int32_t env_light;           /* light index of the first env light or -1 */
     int32_t env;                 /* env table index or -1 */
     int32_t naive;               /* brute force in object order */
+    int32_t shear_float_abs;     /* triangle test's axis choice: 0 truncating integer abs, 1 float abs */
     int32_t num_materials;
     int32_t width;               /* traversal tree: 2, 4 or 8 */
     float root_box[6];

@@ -377,28 +377,31 @@ def write_tga(path, img):
 
 
 def write_obj(path, mesh):
-    lines = []
+    """OBJ text: v / vn / vt records (%.9g round-trips float32) and faces
+    with 1-based v/vt/vn indices, written a block of rows at a time."""
     p = mesh["positions"]
-    lines.append("\n".join("v %.9g %.9g %.9g" % tuple(r) for r in p))
-    if mesh["normals"].size:
-        lines.append("\n".join("vn %.9g %.9g %.9g" % tuple(r) for r in mesh["normals"]))
-    if mesh["texcoords"].size:
-        lines.append("\n".join("vt %.9g %.9g" % tuple(r) for r in mesh["texcoords"]))
-    idx = mesh["indices"] + 1
-    faces = []
-    for t in idx:
-        corners = []
-        for c in range(3):
-            v, n, tt = t[3 * c], t[3 * c + 1], t[3 * c + 2]
-            if n > 0 and tt > 0:
-                corners.append("%d/%d/%d" % (v, tt, n))
-            elif n > 0:
-                corners.append("%d//%d" % (v, n))
-            elif tt > 0:
-                corners.append("%d/%d" % (v, tt))
-            else:
-                corners.append("%d" % v)
-        faces.append("f " + " ".join(corners))
-    lines.append("\n".join(faces))
+    idx = np.asarray(mesh["indices"], np.int64) + 1
     with open(path, "w") as f:
-        f.write("\n".join(l for l in lines if l) + "\n")
+        if len(p):
+            np.savetxt(f, np.asarray(p, np.float64), fmt="v %.9g %.9g %.9g")
+        if mesh["normals"].size:
+            np.savetxt(f, np.asarray(mesh["normals"], np.float64), fmt="vn %.9g %.9g %.9g")
+        if mesh["texcoords"].size:
+            np.savetxt(f, np.asarray(mesh["texcoords"], np.float64), fmt="vt %.9g %.9g")
+        if not len(idx):
+            return
+        v, n, t = idx[:, 0::3], idx[:, 1::3], idx[:, 2::3]
+        has_n, has_t = (n > 0).all(1), (t > 0).all(1)
+        for hn, ht, fmt in ((True, True, "%d/%d/%d"), (True, False, "%d//%d"), (False, True, "%d/%d"),
+                            (False, False, "%d")):
+            rows = np.flatnonzero((has_n == hn) & (has_t == ht))
+            if not len(rows):
+                continue
+            cols = [v[rows]] + ([t[rows]] if ht else []) + ([n[rows]] if hn else [])
+            k = len(cols)
+            arr = np.stack(cols, 2).reshape(len(rows), 3 * k)   # per corner: v[, vt][, vn]
+            if len(rows) == len(idx):
+                np.savetxt(f, arr, fmt="f " + " ".join([fmt] * 3))
+            else:   # mixed faces keep their order: write them one by one
+                for r, a in zip(rows, arr):
+                    f.write("f " + " ".join([fmt] * 3) % tuple(a) + "\n")

@@ -450,6 +450,18 @@ int fm_set_bvh_width(fm_ctx* ctx, int32_t width);
  * sample counts; results do not depend on the split except for the order of
  * FP32 additions in the film.  No counterpart in the reference. */
 int fm_set_path_budget(fm_ctx* ctx, int64_t bytes);
+/* The triangle test's shear axis (GetMaxAxis, triangle.cpp:13-18) calls
+ * unqualified abs() on floats.  Built with libstdc++ and the reference's
+ * includes that resolves to ::abs(int): every component with |d| < 1
+ * truncates to 0 and the axis is z for nearly every ray
+ * (FM_SHEAR_AXIS_TRUNCATED, the default; the FP32 reference then darkens
+ * pixel lines where rays have a near-zero z component).  A toolchain that
+ * resolves it to the float overload picks the largest |d| component
+ * (FM_SHEAR_AXIS_FLOAT).  The rule applies to the scenes created on this
+ * context afterwards.  No counterpart in the reference: it selects which
+ * reference build is matched. */
+enum { FM_SHEAR_AXIS_TRUNCATED = 0, FM_SHEAR_AXIS_FLOAT = 1 };
+int fm_set_shear_axis(fm_ctx* ctx, int32_t rule);
 /* Traversal tree builders of fm_set_bvh_builder. */
 enum {
     FM_BUILDER_HOST = 0,    /* binned SAH on the host build threads (default) */

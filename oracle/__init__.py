@@ -44,6 +44,7 @@ def lib():
             "or_conductor_fr": (None, [P, P, P, P, I64, C.c_int, P]),
             "or_mat_func": (None, [C.c_int, C.c_int, P, I64, P]),
             "or_disney": (None, [P, C.c_int, C.c_int, C.c_int, P, I64, P]),
+            "or_set_shear_axis": (None, [C.c_int]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -213,3 +214,10 @@ def disney(part, wo, wi=None, sample=None, base=(0.0, 0.0, 0.0), thin=False, fp6
     lib().or_disney(mat.ctypes.data, 1 if thin else 0, DISNEY_PARTS[part], 1 if fp64 else 0, buf.ctypes.data, n,
                     out.ctypes.data)
     return out
+
+
+def set_shear_axis(rule):
+    """GetMaxAxis reading for every oracle scene of this process:
+    "truncated" (abs resolved to the integer overload, the default) or
+    "float" (the float overload)."""
+    lib().or_set_shear_axis(1 if rule == "float" else 0)

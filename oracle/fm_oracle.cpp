@@ -708,6 +708,7 @@ void or_disney(const double* mat, int thin, int which, int fp64, const double* i
     if (fp64) disney_t<double>(mat, thin, which, in, n, out); else disney_t<float>(mat, thin, which, in, n, out);
 }
 int or_abi_version(void) { return (int)FM_ABI_VERSION; }
+void or_set_shear_axis(int float_abs) { shear_float_abs() = float_abs != 0; }
 /* PostProcess::Process over the colour image in list order, then
  * Image::save_to_file's byte conversion (image.cpp:97-110). */
 /* ConductorFresnel (fresnel.h:70-110) for n cosines, per channel: out is

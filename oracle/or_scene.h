@@ -64,7 +64,14 @@ template <typename R> struct HitPoint {
  * floats, which with libstdc++ and the reference's includes resolves to
  * ::abs(int) and truncates toward zero; restated as compiled. */
 inline int trunc_abs(double x) { int i = (int)x; return i < 0 ? -i : i; }
+/* the float-overload reading (a toolchain that resolves abs(float) to the
+ * float overload); off by default, set per process by or_set_shear_axis */
+inline bool& shear_float_abs() { static bool v = false; return v; }
 template <typename R> inline int GetMaxAxis(V3<R> v) {
+    if (shear_float_abs()) {
+        if (std::abs(v[0]) > std::abs(v[1]) && std::abs(v[0]) > std::abs(v[2])) return 0;
+        return std::abs(v[1]) > std::abs(v[2]) ? 1 : 2;
+    }
     if (trunc_abs(v[0]) > trunc_abs(v[1]) && trunc_abs(v[0]) > trunc_abs(v[2])) return 0;
     return trunc_abs(v[1]) > trunc_abs(v[2]) ? 1 : 2;
 }

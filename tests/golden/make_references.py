@@ -14,6 +14,20 @@ Run from the repository root:  python tests/golden/make_references.py
 [fp64_box32 fp32_ico48x27 ...]  (no argument: all four)
 (about 2 minutes on 8 host cores).  The FP32 renders use another seed than
 the FP64 ones, so the two are independent estimates.
+
+Pixel-subset references of the full-size workloads (`px` fixtures) hold a
+seeded subset of pixels of the real frames, rendered one pixel at a time by
+the FP64 oracle on the host threads:
+  fp64_ico1080_px      the 1,003,520-triangle Disney / HDR-env frame
+                       (1920x1080, depth 12): 32 icosphere, 8 ground and 8 sky
+                       pixels at 16384 spp (about 10 minutes on 8 cores)
+  fp64_stress1080_px   the 9.8 M-triangle textured stress frame (10 lights,
+                       depth 10): 6 sphere and 2 ground pixels at 512 spp
+                       (the oracle's reference-tree traversal runs about 3
+                       paths per second per core there; about an hour)
+The pixels are chosen by what the centre ray of a seeded candidate pixel
+hits.  Name them explicitly to make them (python
+tests/golden/make_references.py fp64_ico1080_px).
 """
 import os
 import sys
@@ -26,16 +40,98 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 import oracle  # noqa: E402
-from helpers import convergence_scene, scene_signature  # noqa: E402
+from helpers import convergence_scene, px_scene, scene_signature  # noqa: E402
 
 SPP = 16384
 SEEDS = {"fp64": 1001, "fp32": 1002}
 NAMES = ("box32", "ico48x27")
+# per full-size frame: pixels whose centre ray hits the mesh / the ground /
+# nothing, the subset seed and the samples per pixel
+PX_SCENES = {"ico1080": dict(mesh=32, ground=8, sky=8, seed=2, spp=16384),
+             "stress1080": dict(mesh=6, ground=2, sky=0, seed=3, spp=512)}
+
+
+def _geometry_counts(s):
+    n = []
+    for o in s.objects:
+        n.append(int(s.meshes[o["mesh"]]["indices"].shape[0]) if o["shape"] == "triangle_mesh" else 1)
+    return np.cumsum(n)
+
+
+def pixel_subset(name, s, o):
+    """Seeded pixels stratified by what the oracle's closest hit of the pixel
+    centre's camera ray is: a mesh face, the ground (the object whose
+    material is named "ground"), or nothing."""
+    cfg = PX_SCENES[name]
+    W, H = s.renderer["width"], s.renderer["height"]
+    rng = np.random.default_rng(cfg["seed"])
+    cand = rng.choice(W * H, 4000, replace=False)
+    rows, cols = cand // W, cand % W
+    j = H - 1 - rows
+    ndc = np.stack([(cols + 0.5) / W, (j + 0.5) / H, np.full(len(cand), 0.5), np.full(len(cand), 0.5)], 1)
+    rays = o.camera_rays(ndc)
+    r = np.zeros((len(cand), 8), np.float32)
+    r[:, 0:3], r[:, 3], r[:, 4:7], r[:, 7] = rays["ori"], rays["t_min"], rays["dir"], rays["t_max"]
+    geom = o.trace(r)["geom"]
+    cum = _geometry_counts(s)
+    obj = np.searchsorted(cum, geom, side="right")
+    ground = [k for k, ob in enumerate(s.objects) if s.material_names[ob["material"]] == "ground"][0]
+    kind = np.where(geom < 0, "sky", np.where(obj == ground, "ground", np.where(obj < ground, "mesh", "other")))
+    pick = []
+    for k in ("mesh", "ground", "sky"):
+        pick += list(np.flatnonzero(kind == k)[:cfg[k]])
+    pick = np.sort(np.array(pick, np.int64))
+    return rows[pick], cols[pick], kind[pick]
+
+
+def make_pixel_reference(name):
+    import threading
+    s = px_scene(name)
+    W, H = s.renderer["width"], s.renderer["height"]
+    spp = PX_SCENES[name]["spp"]
+    t0 = time.time()
+    o = oracle.OracleScene(s, fp64=True)
+    print("%s: oracle scene built in %.1f s" % (name, time.time() - t0), flush=True)
+    rows, cols, kinds = pixel_subset(name, s, o)
+    print("%s: %d pixels (%s)" % (name, len(rows), ", ".join("%d %s" % (np.sum(kinds == k), k)
+                                                                  for k in ("mesh", "ground", "sky"))), flush=True)
+    out = {k: np.zeros((len(rows), 3)) for k in ("mean", "sumsq", "albedo", "normal")}
+    next_i = [0]
+    lock = threading.Lock()
+
+    def work():
+        while True:
+            with lock:
+                i = next_i[0]
+                next_i[0] += 1
+            if i >= len(rows):
+                return
+            t = time.time()
+            c, a, nn, q = o.render(int(cols[i]), int(rows[i]), 1, 1, spp=spp, seed=SEEDS["fp64"], width=W, height=H,
+                                   threads=1)
+            out["mean"][i], out["albedo"][i], out["normal"][i], out["sumsq"][i] = c[0, 0], a[0, 0], nn[0, 0], q[0, 0]
+            print("  pixel %d (%d, %d): %.1f s" % (i, rows[i], cols[i], time.time() - t), flush=True)
+
+    th = [threading.Thread(target=work) for _ in range(min(16, os.cpu_count() or 1))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fp64_%s_px.npz" % name)
+    np.savez_compressed(path, rows=rows, cols=cols, kinds=kinds.astype("U6"), spp=np.int64(spp),
+                        seed=np.int64(SEEDS["fp64"]), width=np.int64(W), height=np.int64(H),
+                        signature=np.float64(scene_signature(s)), **out)
+    print("%s: %d pixels x %d spp in %.1f s -> %s" % (name, len(rows), spp, time.time() - t0, path))
 
 
 def main():
     threads = min(16, os.cpu_count() or 1)
     only = sys.argv[1:]
+    for name in PX_SCENES:
+        if "fp64_%s_px" % name in only:
+            make_pixel_reference(name)
+    if only and all("_px" in a for a in only):
+        return
     for prec, seed in SEEDS.items():
         for name in NAMES:
             if only and "%s_%s" % (prec, name) not in only:

@@ -84,6 +84,15 @@ def convergence_scene(name):
     raise KeyError(name)
 
 
+def px_scene(name):
+    """Full-size frames of the pixel-subset FP64 references (tests/golden)."""
+    if name == "ico1080":
+        return scenes.icosphere_scene()
+    if name == "stress1080":
+        return scenes.stress_scene()
+    raise KeyError(name)
+
+
 def scene_signature(s):
     """A number that changes when the generated geometry, images, materials
     or renderer settings change."""

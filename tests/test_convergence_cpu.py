@@ -59,3 +59,36 @@ def test_exact_zero_direction_component_gives_a_nan_sample():
 
 def _m(ref):
     return ref["mean"], ref["sumsq"], float(ref["spp"])
+
+
+def test_float_abs_reading_removes_the_truncated_axis_lines():
+    """With abs() resolved to the float overload (a toolchain other than
+    libstdc++'s, fm_set_shear_axis FM_SHEAR_AXIS_FLOAT) the shear axis is
+    the largest direction component: the FP32 box is then unbiased against
+    FP64 on the six pixel lines too, where under the integer reading it is
+    darker; the NaN sample of the exact-zero component is gone."""
+    s = convergence_scene("box32")
+    lines = np.zeros((32, 32), bool)
+    lines[[3, 15, 16, 28], :] = True
+    lines[:, [14, 15]] = True
+    spp = 4096
+    try:
+        oracle.set_shear_axis("float")
+        r32 = oracle.OracleScene(s).render(0, 0, 32, 32, spp=spp, seed=11, threads=8)
+        r64 = oracle.OracleScene(s, fp64=True).render(0, 0, 32, 32, spp=spp, seed=12, threads=8)
+        nan = oracle.OracleScene(s).render(29, 13, 1, 1, spp=1, seed=1001, sample_offset=7077)[0]
+    finally:
+        oracle.set_shear_axis("truncated")
+    ref = {"mean": r64[0], "sumsq": r64[3], "spp": np.int64(spp)}
+    st = convergence_stats(r32[0], r32[3], float(spp), ref)
+    z = st["z"].reshape(32, 32, 3)
+    print("box32 float abs, fp32 vs fp64: mean z %.3f, on the lines %.3f, |z| > 3: %.4f"
+          % (st["mean_z"], z[lines].mean(), st["frac_z3"]))
+    assert abs(st["mean_z"]) < 0.06 and st["frac_z3"] < 0.01
+    assert abs(z[lines].mean()) < 0.15
+    assert np.all(np.isfinite(nan))
+    # the integer reading at the same spp: the lines are darker in FP32
+    t32 = oracle.OracleScene(s).render(0, 0, 32, 32, spp=spp, seed=11, threads=8)
+    t64 = oracle.OracleScene(s, fp64=True).render(0, 0, 32, 32, spp=spp, seed=12, threads=8)
+    zt = convergence_stats(t32[0], t32[3], float(spp), {"mean": t64[0], "sumsq": t64[3], "spp": np.int64(spp)})["z"]
+    assert zt.reshape(32, 32, 3)[lines].mean() < -0.5
