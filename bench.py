@@ -145,19 +145,21 @@ def adjoint_leg(fumeng_amd, ctx, iters=6):
     geometries), 64 Disney materials, 512x256 env, 256x256, depth 6.  Adam
     on all 384 material parameters and 393,216 env texel values from a
     perturbed start towards a target rendered from the hidden values; one
-    iteration = set parameters (env tables rebuilt in FP64) + 32-spp forward
-    render + 32-spp path-replay adjoint + optimiser step."""
+    iteration = set parameters (env tables rebuilt in FP64 on the GPU) +
+    32-spp forward render + 32-spp path-replay adjoint + optimiser step, with
+    the parameters, image and gradients as tensors in HBM."""
     import numpy as np
     import torch
     from fumeng_amd import scenes, _abi
     from fumeng_amd.torch_ext import render
     sc = ctx.scene(scenes.inverse_scene())
     mp, env = sc.params()
-    target = torch.from_numpy(sc.render(spp=256, seed=5, mode=_abi.FM_MODE_WAVEFRONT)[0])
+    dev = torch.device("cuda", ctx.device)
+    target = torch.from_numpy(sc.render(spp=256, seed=5, mode=_abi.FM_MODE_WAVEFRONT)[0]).to(dev)
     start_mp, start_env = scenes.perturb_params(mp, env, seed=5)
-    mask = torch.from_numpy(np.isfinite(mp).astype(np.float32))
-    theta = torch.tensor(np.nan_to_num(start_mp, nan=0.0), requires_grad=True)
-    env_t = torch.tensor(start_env, requires_grad=True)
+    mask = torch.from_numpy(np.isfinite(mp).astype(np.float32)).to(dev)
+    theta = torch.tensor(np.nan_to_num(start_mp, nan=0.0), device=dev, requires_grad=True)
+    env_t = torch.tensor(start_env, device=dev, requires_grad=True)
     opt = torch.optim.Adam([{"params": [theta], "lr": 0.03}, {"params": [env_t], "lr": 0.03 * float(env.mean())}])
     nan_rows = torch.full_like(theta, float("nan"))
     times, losses, fwd_ms, adj_ms = [], [], [], []
@@ -174,6 +176,7 @@ def adjoint_leg(fumeng_amd, ctx, iters=6):
         return out
     sc.render, sc.adjoint = timed_render, timed_adjoint
     for it in range(iters + 1):
+        torch.cuda.synchronize(dev)
         t0 = time.perf_counter()
         opt.zero_grad()
         params = torch.where(mask > 0, theta, nan_rows)
@@ -185,6 +188,7 @@ def adjoint_leg(fumeng_amd, ctx, iters=6):
         with torch.no_grad():
             theta.clamp_(0.0, 1.0)
             env_t.clamp_(min=0.0)
+        torch.cuda.synchronize(dev)
         if it > 0:                       # the first iteration is the warm-up
             times.append(time.perf_counter() - t0)
         losses.append(float(loss.detach()))

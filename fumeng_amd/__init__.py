@@ -58,6 +58,7 @@ class Context:
         h = C.c_void_p()
         _check(lib().fm_init(device, C.byref(h)))
         self.h = h
+        self.device = device
         self._scenes = weakref.WeakSet()
 
     def close(self):
@@ -74,7 +75,7 @@ class Context:
             pass
 
     def set_bvh_width(self, width):
-        """Traversal tree width (2, the default, or 4) of the scenes created afterwards."""
+        """Traversal tree width (2, the default, 4 or 8) of the scenes created afterwards."""
         _check(lib().fm_set_bvh_width(self.h, int(width)), self.h)
 
     def set_bvh_builder(self, builder):
@@ -204,6 +205,7 @@ class Scene:
         info = self.info()
         W = width or info.renderer.width
         H = height or info.renderer.height
+        self.last_render_shape = (H, W)
         st = A.fm_render_stats()
         film = A.fm_film()
         out = None
@@ -274,6 +276,29 @@ class Scene:
         env = None if env_texels is None else np.ascontiguousarray(env_texels, np.float32)
         _check(lib().fm_params_set(self.ctx.h, self.h, _ptr(mp) if mp is not None else None,
                                    _ptr(env) if env is not None else None), self.ctx.h)
+
+    def set_params_device(self, material_params=0, env_texels=0):
+        """fm_params_set_device: device addresses (ints; 0 = leave as is) of
+        float32 arrays on the context's GPU, (num_materials, 6) and (H, W, 3),
+        ready when called."""
+        _check(lib().fm_params_set_device(self.ctx.h, self.h, material_params or None, env_texels or None), self.ctx.h)
+
+    def film_read_device(self, color=0, albedo=0, normal=0):
+        """fm_film_read_device: the last render's means into device arrays
+        (addresses of float32 (H, W, 3) arrays on the context's GPU; 0 = skip)."""
+        _check(lib().fm_film_read_device(self.ctx.h, color or None, albedo or None, normal or None), self.ctx.h)
+
+    def adjoint_device(self, dl_dimg, g_material, g_env, spp=0, seed=1, sample_offset=0, depth=0, width=0, height=0,
+                       rr_depth=-1, rr_coef=0.0, direct_loop=0, grid_blocks=0):
+        """fm_render_adjoint_device: dl_dimg (float32 (H, W, 3)), g_material
+        (float64 (num_materials, 6)) and g_env (float64 (H, W, 3)) are device
+        addresses on the context's GPU (0 = no output); returns the stats."""
+        p = A.fm_render_params(width=width, height=height, spp=spp, depth=depth, seed=seed, sample_offset=sample_offset,
+                               rr_depth=rr_depth, rr_coef=rr_coef, direct_loop=direct_loop, grid_blocks=grid_blocks)
+        st = A.fm_render_stats()
+        _check(lib().fm_render_adjoint_device(self.ctx.h, self.h, C.byref(p), dl_dimg, g_material or None,
+                                              g_env or None, C.byref(st)), self.ctx.h)
+        return st
 
     def adjoint(self, dl_dimg, spp=0, seed=1, sample_offset=0, depth=0, width=0, height=0, rr_depth=-1, rr_coef=0.0,
                 direct_loop=0, grid_blocks=0):

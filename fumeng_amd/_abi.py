@@ -212,6 +212,7 @@ EXPORTS = {
     "fm_render": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(fm_render_params), C.POINTER(fm_film),
                             C.POINTER(fm_render_stats)]),
     "fm_film_read": (C.c_int, [C.c_void_p, C.POINTER(fm_film)]),
+    "fm_film_read_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fm_set_path_budget": (C.c_int, [C.c_void_p, C.c_int64]),
     "fm_set_shear_axis": (C.c_int, [C.c_void_p, C.c_int32]),
     "fm_render_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(fm_render_params), C.c_void_p, C.c_void_p,
@@ -231,8 +232,11 @@ EXPORTS = {
     "fm_host_scene_info": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(fm_scene_info)]),
     "fm_params_get": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fm_params_set": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fm_params_set_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fm_render_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(fm_render_params), C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.POINTER(fm_render_stats)]),
+    "fm_render_adjoint_device": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(fm_render_params), C.c_void_p, C.c_void_p,
+                                           C.c_void_p, C.POINTER(fm_render_stats)]),
     "fm_trace": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p]),
     "fm_camera_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "fm_bsdf_eval": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),

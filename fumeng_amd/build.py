@@ -32,7 +32,7 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
              "-I" + os.path.join(ROCM, "include"), "-D__HIP_PLATFORM_AMD__"]
 
 HOST_SOURCES = ["api.cpp", "host/loader.cpp", "host/scene_build.cpp"]
-HIP_SOURCES = ["kernels.hip", "wavefront.hip", "adjoint.hip", "bvh_build.hip"]
+HIP_SOURCES = ["kernels.hip", "wavefront.hip", "adjoint.hip", "bvh_build.hip", "envtab.hip"]
 # wavefront.hip aggregates its queue atomics per wave itself and issues
 # those of several queues back to back; the compiler's own atomic
 # aggregation would put a wait after each of them.

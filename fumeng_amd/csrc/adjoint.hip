@@ -28,7 +28,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k
         for (int k = threadIdx.x; k < ng; k += blockDim.x) gl[k] = 0.0f;
     __syncthreads();
     AdSink K;
-    K.lds = gl; K.g_mat = A.g_mat; K.g_env = A.g_env; K.env_w = A.env_w; K.env_h = A.env_h;
+    K.lds = gl; K.g_mat = A.g_mat; K.g_env = A.g_env; K.env_cell = A.env_cell; K.env_w = A.env_w; K.env_h = A.env_h;
     PathCfg cfg;
     cfg.depth = A.cfg_raw.depth; cfg.direct = A.cfg_raw.direct; cfg.rr_depth = A.cfg_raw.rr_depth; cfg.rr_coef = A.cfg_raw.rr_coef;
     const int64_t npix = (int64_t)A.W * A.H;

@@ -133,6 +133,7 @@ struct fm_ctx {
     DevBuf batch_film;                                  /* FP64 sums of fm_render_batch jobs */
     DevBuf adj_in, adj_gmat, adj_genv;                  /* adjoint: image gradient, FP64 parameter gradients */
     DevBuf adj_tot;                                     /* adjoint: path radiance per work item */
+    DevBuf adj_cell;                                    /* adjoint: env table derivatives per cell (2 x texels), rows (2H + 1) */
     DevBuf adj_film;                                    /* adjoint: FP64 sums of its forward run (unused) */
     double* noalloc = nullptr;         /* this device's fm_render_noalloc working array */
     std::vector<hipEvent_t> trace_ev;  /* pairs around wavefront trace launches */
@@ -143,7 +144,9 @@ struct fm_scene {
     fm_ctx* ctx = nullptr;
     fmh::HostScene hs;        /* host copy (kept for info and tests) */
     DevBuf nodes, nodes4, nodes8, prims, meta, tri_sh, rects, spheres, materials, lights, mesh_tris, envs, images, texels, env_tables,
-        lut;
+        lut, env_aux;
+    DevBuf env_work;          /* device env-table rebuild: W x H + H doubles, status int */
+    bool env_stale = false;   /* the env image, its tables and records changed on the device only */
     fm_dscene d{};
     int64_t device_bytes = 0;
     double upload_ms = 0.0;
@@ -219,6 +222,7 @@ void upload_scene(fm_scene* s) {
     s->images.upload(hs.images, st);
     s->texels.upload(hs.texels, st);
     s->env_tables.upload(hs.env_tables, st);
+    s->env_aux.upload(hs.env_aux, st);
     s->lut.upload(hs.srgb_lut, st);
     HIPCHECK(hipStreamSynchronize(st));
     fm_dscene& d = s->d;
@@ -240,6 +244,7 @@ void upload_scene(fm_scene* s) {
     d.images = s->images.as<fm_dimage>();
     d.texels = s->texels.as<unsigned char>();
     d.env_tables = s->env_tables.as<float>();
+    d.env_aux = s->env_aux.as<double>();
     d.srgb_lut = s->lut.as<float>();
     d.root_ref = hs.root_ref;
     d.num_slots = (int32_t)hs.num_geoms;
@@ -252,7 +257,7 @@ void upload_scene(fm_scene* s) {
     d.camera = hs.camera;
     s->device_bytes = 0;
     for (DevBuf* b : {&s->nodes, &s->nodes4, &s->nodes8, &s->prims, &s->meta, &s->tri_sh, &s->rects, &s->spheres, &s->materials, &s->lights, &s->mesh_tris,
-                      &s->envs, &s->images, &s->texels, &s->env_tables, &s->lut})
+                      &s->envs, &s->images, &s->texels, &s->env_tables, &s->lut, &s->env_aux})
         s->device_bytes += (int64_t)b->n;
     s->upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -278,7 +283,7 @@ void set_scene_width(fm_scene* s, int W) {
     set_trav_hash(hs);
     s->device_bytes = 0;
     for (DevBuf* b : {&s->nodes, &s->nodes4, &s->nodes8, &s->prims, &s->meta, &s->tri_sh, &s->rects, &s->spheres, &s->materials,
-                      &s->lights, &s->mesh_tris, &s->envs, &s->images, &s->texels, &s->env_tables, &s->lut})
+                      &s->lights, &s->mesh_tris, &s->envs, &s->images, &s->texels, &s->env_tables, &s->lut, &s->env_aux})
         s->device_bytes += (int64_t)b->n;
 }
 
@@ -329,7 +334,7 @@ void build_device_tree(fm_scene* s) {
     d.tri_sh = s->tri_sh.as<float>();
     s->device_bytes = 0;
     for (DevBuf* b : {&s->nodes, &s->nodes4, &s->nodes8, &s->prims, &s->meta, &s->tri_sh, &s->rects, &s->spheres, &s->materials, &s->lights, &s->mesh_tris,
-                      &s->envs, &s->images, &s->texels, &s->env_tables, &s->lut})
+                      &s->envs, &s->images, &s->texels, &s->env_tables, &s->lut, &s->env_aux})
         s->device_bytes += (int64_t)b->n;
     hs.device_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -983,6 +988,18 @@ int fm_film_read(fm_ctx* ctx, fm_film* film) {
     });
 }
 
+int fm_film_read_device(fm_ctx* ctx, float* color, float* albedo, float* normal) {
+    if (!ctx) return fail(ctx, FM_ERR_INVALID, "NULL argument");
+    return guarded(ctx, [&]() -> int {
+        if (!ctx->film.p || ctx->film_spp <= 0) return fail(ctx, FM_ERR_INVALID, "nothing rendered yet");
+        HIPCHECK(hipSetDevice(ctx->device));
+        HIPCHECK(fm::launch_film_mean(ctx->film.as<double>(), (int64_t)ctx->film_w * ctx->film_h, 1.0 / (double)ctx->film_spp, color,
+                                      albedo, normal, ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+        return FM_OK;
+    });
+}
+
 int fm_render_noalloc(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, void* stream, float* color, float* albedo,
                       float* normal, fm_render_stats* stats) {
     if (!ctx || !s || !p) return fail(ctx, FM_ERR_INVALID, "NULL argument");
@@ -1154,45 +1171,95 @@ const fm_dimage* env_image(const fm_scene* s) {
     return im.format == 0 ? &im : nullptr;
 }
 
-}  // namespace
-
-int fm_params_get(fm_ctx* ctx, fm_scene* s, float* material_params, float* env_texels) {
-    if (!ctx || !s) return fail(ctx, FM_ERR_INVALID, "NULL argument");
-    return guarded(ctx, [&]() -> int {
-        const auto& M = s->hs.materials;
-        if (material_params)
-            for (size_t m = 0; m < M.size(); m++)
-                for (int p = 0; p < FM_DPARAM_PER_MATERIAL; p++) {
-                    int slot, ch;
-                    dparam_slot(p, slot, ch);
-                    const bool ok = M[m].type == FM_MAT_DISNEY && M[m].tex[slot].type == FM_TEX_CONSTANT;
-                    material_params[m * FM_DPARAM_PER_MATERIAL + p] = ok ? M[m].tex[slot].value[ch] : NAN;
-                }
-        if (env_texels) {
-            const fm_dimage* im = env_image(s);
-            if (!im) return fail(ctx, FM_ERR_UNSUPPORTED, "no env light with a linear float image");
-            const float* t = reinterpret_cast<const float*>(s->hs.texels.data() + im->offset);
-            for (int64_t k = 0; k < (int64_t)im->w * im->h; k++)
-                for (int c = 0; c < 3; c++) env_texels[3 * k + c] = t[4 * k + c];
-        }
-        return FM_OK;
-    });
+/* The differentiable env (the first env light) when its image holds linear
+ * float texels and its tables have the image's size; else null. */
+const fm_dimage* diff_env_image(const fm_scene* s) {
+    const fm_dimage* im = env_image(s);
+    if (!im) return nullptr;
+    const fm_denv& e = s->hs.envs[(size_t)s->hs.env];
+    return e.W == im->w && e.H == im->h ? im : nullptr;
 }
 
-int fm_params_set(fm_ctx* ctx, fm_scene* s, const float* material_params, const float* env_texels) {
+/* Device work space of the env-table rebuild: W x H doubles (luminance), H
+ * doubles (row sums), a status word, then W x H x 3 floats (staged texels). */
+struct EnvWork { double* lum; double* rs; int* status; float* stage; };
+EnvWork env_work(fm_scene* s) {
+    const fm_denv& e = s->hs.envs[(size_t)s->hs.env];
+    const size_t n = (size_t)e.W * e.H;
+    const size_t need = (n + e.H + 2) * sizeof(double) + n * 3 * sizeof(float);
+    if (s->env_work.n < need) s->env_work.alloc(need);
+    EnvWork w;
+    w.lum = s->env_work.as<double>();
+    w.rs = w.lum + n;
+    w.status = reinterpret_cast<int*>(w.rs + e.H);
+    w.stage = reinterpret_cast<float*>(w.rs + e.H + 2);
+    return w;
+}
+
+/* New texels (rgb: H x W x 3 on the device) into the differentiable env's
+ * image, and its sampling tables rebuilt on the device in FP64. */
+void env_rebuild_device(fm_ctx* ctx, fm_scene* s, const float* rgb, const EnvWork& w) {
+    fmh::HostScene& hs = s->hs;
+    const fm_dimage* im = diff_env_image(s);
+    fm::EnvTableArgs A{};
+    A.env = hs.envs[(size_t)hs.env];
+    A.env_record = s->envs.as<fm_denv>() + hs.env;
+    A.rgb = rgb;
+    A.tex4 = reinterpret_cast<float*>((char*)s->texels.p + im->offset);
+    A.tables = s->env_tables.as<float>();
+    A.aux = s->env_aux.as<double>() + A.env.off_aux;
+    A.lum = w.lum;
+    A.rs = w.rs;
+    A.status = w.status;
+    HIPCHECK(fm::launch_env_tables(A, ctx->stream));
+    int status = 0;
+    HIPCHECK(hipMemcpyAsync(&status, w.status, sizeof status, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    s->env_stale = true;
+    if (status) throw fmh::BuildError(FM_ERR_INVALID, "env map has zero luminance (invalid 1d distribution)");
+}
+
+/* Brings the host copy of the env image, tables and records up to date
+ * after device-side rebuilds. */
+void env_refresh_host(fm_ctx* ctx, fm_scene* s) {
+    if (!s->env_stale) return;
+    fmh::HostScene& hs = s->hs;
+    hipStream_t st = ctx->stream;
+    if (const fm_dimage* im = env_image(s))
+        HIPCHECK(hipMemcpyAsync(hs.texels.data() + im->offset, (char*)s->texels.p + im->offset, (size_t)im->w * im->h * 16,
+                                hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(hs.env_tables.data(), s->env_tables.p, hs.env_tables.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(hs.env_aux.data(), s->env_aux.p, hs.env_aux.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(hs.envs.data(), s->envs.p, hs.envs.size() * sizeof(fm_denv), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    s->env_stale = false;
+}
+
+int params_set(fm_ctx* ctx, fm_scene* s, const float* material_params, const float* env_texels, bool device) {
     if (!ctx || !s) return fail(ctx, FM_ERR_INVALID, "NULL argument");
     return guarded(ctx, [&]() -> int {
         HIPCHECK(hipSetDevice(ctx->device));
         fmh::HostScene& hs = s->hs;
         hipStream_t st = ctx->stream;
+        if (env_texels && !diff_env_image(s))
+            return fail(ctx, FM_ERR_UNSUPPORTED, "no env light with a linear float image of its tables' size");
         if (material_params) {
+            const size_t nv = hs.materials.size() * FM_DPARAM_PER_MATERIAL;
+            std::vector<float> staged;
+            const float* mp = material_params;
+            if (device) {
+                staged.resize(nv);
+                HIPCHECK(hipMemcpyAsync(staged.data(), material_params, nv * sizeof(float), hipMemcpyDeviceToHost, st));
+                HIPCHECK(hipStreamSynchronize(st));
+                mp = staged.data();
+            }
             for (size_t m = 0; m < hs.materials.size(); m++) {
                 fm_dmaterial& M = hs.materials[m];
                 if (M.type != FM_MAT_DISNEY) continue;
                 for (int p = 0; p < FM_DPARAM_PER_MATERIAL; p++) {
                     int slot, ch;
                     dparam_slot(p, slot, ch);
-                    const float v = material_params[m * FM_DPARAM_PER_MATERIAL + p];
+                    const float v = mp[m * FM_DPARAM_PER_MATERIAL + p];
                     if (M.tex[slot].type != FM_TEX_CONSTANT || v != v) continue;
                     if (slot == FM_D_BASECOLOR) M.tex[slot].value[ch] = v;
                     else for (int c = 0; c < 3; c++) M.tex[slot].value[c] = v;
@@ -1202,30 +1269,22 @@ int fm_params_set(fm_ctx* ctx, fm_scene* s, const float* material_params, const 
                                     hipMemcpyHostToDevice, st));
         }
         if (env_texels) {
-            const fm_dimage* im = env_image(s);
-            if (!im) return fail(ctx, FM_ERR_UNSUPPORTED, "no env light with a linear float image");
-            float* t = reinterpret_cast<float*>(hs.texels.data() + im->offset);
-            const int64_t n = (int64_t)im->w * im->h;
-            for (int64_t k = 0; k < n; k++)
-                for (int c = 0; c < 3; c++) t[4 * k + c] = env_texels[3 * k + c];
-            fm_denv& e = hs.envs[(size_t)hs.env];
-            const int W = e.W;
-            fmh::build_env_tables(hs, e, [&](int u, int v, double rgb[3]) {
-                const float* q = t + 4 * ((size_t)v * W + (size_t)u);
-                for (int c = 0; c < 3; c++) rgb[c] = q[c];
-            }, false);
-            HIPCHECK(hipMemcpyAsync((char*)s->texels.p + im->offset, t, (size_t)n * 16, hipMemcpyHostToDevice, st));
-            HIPCHECK(hipMemcpyAsync(s->env_tables.p, hs.env_tables.data(), hs.env_tables.size() * sizeof(float),
-                                    hipMemcpyHostToDevice, st));
-            HIPCHECK(hipMemcpyAsync(s->envs.p, hs.envs.data(), hs.envs.size() * sizeof(fm_denv), hipMemcpyHostToDevice, st));
+            const fm_dimage* im = diff_env_image(s);
+            const EnvWork w = env_work(s);
+            const float* rgb = env_texels;
+            if (!device) {
+                HIPCHECK(hipMemcpyAsync(w.stage, env_texels, (size_t)im->w * im->h * 3 * sizeof(float), hipMemcpyHostToDevice, st));
+                rgb = w.stage;
+            }
+            env_rebuild_device(ctx, s, rgb, w);
         }
         HIPCHECK(hipStreamSynchronize(st));
         return FM_OK;
     });
 }
 
-int fm_render_adjoint(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, const float* dl_dimg, double* g_material,
-                      double* g_env, fm_render_stats* stats) {
+int render_adjoint(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, const float* dl_dimg, double* g_material, double* g_env,
+                   fm_render_stats* stats, bool device) {
     if (!ctx || !s || !p || !dl_dimg) return fail(ctx, FM_ERR_INVALID, "NULL argument");
     return guarded(ctx, [&]() -> int {
         const fm_renderer_desc& rd = s->hs.renderer;
@@ -1248,19 +1307,41 @@ int fm_render_adjoint(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, const
         const int nm = s->d.num_materials;
         const fm_dimage* im = env_image(s);
         const int64_t nenv = im ? (int64_t)im->w * im->h * 3 : 0;
-        ctx->adj_in.alloc((size_t)npix * 3 * sizeof(float));
-        HIPCHECK(hipMemcpyAsync(ctx->adj_in.p, dl_dimg, (size_t)npix * 3 * sizeof(float), hipMemcpyHostToDevice, st));
-        if (ctx->adj_gmat.n < (size_t)nm * FM_DPARAM_PER_MATERIAL * sizeof(double))
-            ctx->adj_gmat.alloc((size_t)nm * FM_DPARAM_PER_MATERIAL * sizeof(double));
-        HIPCHECK(fm::launch_zero_f64(ctx->adj_gmat.as<double>(), (int64_t)nm * FM_DPARAM_PER_MATERIAL, st));
+        if (device) {
+            A.dl_dimg = dl_dimg;
+        } else {
+            ctx->adj_in.alloc((size_t)npix * 3 * sizeof(float));
+            HIPCHECK(hipMemcpyAsync(ctx->adj_in.p, dl_dimg, (size_t)npix * 3 * sizeof(float), hipMemcpyHostToDevice, st));
+            A.dl_dimg = ctx->adj_in.as<float>();
+        }
+        /* gradients: FP64 sums in the caller's device arrays, or in the context's */
+        if (device && g_material) {
+            A.g_mat = g_material;
+        } else {
+            if (ctx->adj_gmat.n < (size_t)nm * FM_DPARAM_PER_MATERIAL * sizeof(double))
+                ctx->adj_gmat.alloc((size_t)nm * FM_DPARAM_PER_MATERIAL * sizeof(double));
+            A.g_mat = ctx->adj_gmat.as<double>();
+        }
+        HIPCHECK(fm::launch_zero_f64(A.g_mat, (int64_t)nm * FM_DPARAM_PER_MATERIAL, st));
         if (nenv > 0) {
-            if (ctx->adj_genv.n < (size_t)nenv * sizeof(double)) ctx->adj_genv.alloc((size_t)nenv * sizeof(double));
-            HIPCHECK(fm::launch_zero_f64(ctx->adj_genv.as<double>(), nenv, st));
+            if (device && g_env) {
+                A.g_env = g_env;
+            } else {
+                if (ctx->adj_genv.n < (size_t)nenv * sizeof(double)) ctx->adj_genv.alloc((size_t)nenv * sizeof(double));
+                A.g_env = ctx->adj_genv.as<double>();
+            }
+            HIPCHECK(fm::launch_zero_f64(A.g_env, nenv, st));
+        }
+        /* table derivatives per env cell (the env's tables have the image's size) */
+        const fm_dimage* dim = diff_env_image(s);
+        const int64_t ncell = dim ? (int64_t)dim->w * dim->h : 0;
+        if (ncell > 0) {
+            const size_t need = (size_t)(2 * ncell + 2 * dim->h + 1) * sizeof(double);
+            if (ctx->adj_cell.n < need) ctx->adj_cell.alloc(need);
+            HIPCHECK(fm::launch_zero_f64(ctx->adj_cell.as<double>(), 2 * ncell, st));
+            A.env_cell = ctx->adj_cell.as<double>();
         }
         HIPCHECK(fm::launch_zero_f64((double*)ctx->counters.p, 8, st));
-        A.dl_dimg = ctx->adj_in.as<float>();
-        A.g_mat = ctx->adj_gmat.as<double>();
-        A.g_env = nenv > 0 ? ctx->adj_genv.as<double>() : nullptr;
         A.env_w = im ? im->w : 0; A.env_h = im ? im->h : 0;
         A.lds_grad = (int64_t)nm * FM_DPARAM_PER_MATERIAL <= 4096 ? 1 : 0;
         A.counters = ctx->counters.as<unsigned long long>();
@@ -1275,8 +1356,10 @@ int fm_render_adjoint(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, const
          * (pixel, sample) Philox streams of the walk) writes it per path */
         if (A.cfg_raw.depth <= 0) {   /* black image (path_tracing.cpp:42): every gradient is 0 */
             HIPCHECK(hipStreamSynchronize(st));
-            if (g_material) memset(g_material, 0, (size_t)nm * FM_DPARAM_PER_MATERIAL * sizeof(double));
-            if (g_env && nenv > 0) memset(g_env, 0, (size_t)nenv * sizeof(double));
+            if (!device) {
+                if (g_material) memset(g_material, 0, (size_t)nm * FM_DPARAM_PER_MATERIAL * sizeof(double));
+                if (g_env && nenv > 0) memset(g_env, 0, (size_t)nenv * sizeof(double));
+            }
             if (stats) { memset(stats, 0, sizeof(*stats)); stats->paths = A.total_items; }
             return FM_OK;
         }
@@ -1292,11 +1375,15 @@ int fm_render_adjoint(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, const
         HIPCHECK(fm::launch_zero_f64((double*)ctx->counters.p, 8, st));
         HIPCHECK(hipEventRecord(ctx->ev0, st));
         HIPCHECK(fm::launch_adjoint(s->d, A, block, std::max(grid, 1), levels, st));
+        if (ncell > 0)
+            HIPCHECK(fm::launch_env_table_grad(A.env_cell, A.env_cell + 2 * ncell,
+                                               reinterpret_cast<const float*>((const char*)s->texels.p + dim->offset), dim->w, dim->h,
+                                               s->env_aux.as<double>() + s->hs.envs[(size_t)s->hs.env].off_aux, A.g_env, st));
         HIPCHECK(hipEventRecord(ctx->ev1, st));
         HIPCHECK(hipStreamSynchronize(st));
-        if (g_material)
+        if (!device && g_material)
             HIPCHECK(hipMemcpy(g_material, A.g_mat, (size_t)nm * FM_DPARAM_PER_MATERIAL * sizeof(double), hipMemcpyDeviceToHost));
-        if (g_env && nenv > 0) HIPCHECK(hipMemcpy(g_env, A.g_env, (size_t)nenv * sizeof(double), hipMemcpyDeviceToHost));
+        if (!device && g_env && nenv > 0) HIPCHECK(hipMemcpy(g_env, A.g_env, (size_t)nenv * sizeof(double), hipMemcpyDeviceToHost));
         if (stats) {
             memset(stats, 0, sizeof(*stats));
             float ms = 0.0f;
@@ -1308,6 +1395,51 @@ int fm_render_adjoint(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, const
         }
         return FM_OK;
     });
+}
+
+}  // namespace
+
+int fm_params_get(fm_ctx* ctx, fm_scene* s, float* material_params, float* env_texels) {
+    if (!ctx || !s) return fail(ctx, FM_ERR_INVALID, "NULL argument");
+    return guarded(ctx, [&]() -> int {
+        const auto& M = s->hs.materials;
+        if (material_params)
+            for (size_t m = 0; m < M.size(); m++)
+                for (int p = 0; p < FM_DPARAM_PER_MATERIAL; p++) {
+                    int slot, ch;
+                    dparam_slot(p, slot, ch);
+                    const bool ok = M[m].type == FM_MAT_DISNEY && M[m].tex[slot].type == FM_TEX_CONSTANT;
+                    material_params[m * FM_DPARAM_PER_MATERIAL + p] = ok ? M[m].tex[slot].value[ch] : NAN;
+                }
+        if (env_texels) {
+            const fm_dimage* im = env_image(s);
+            if (!im) return fail(ctx, FM_ERR_UNSUPPORTED, "no env light with a linear float image");
+            HIPCHECK(hipSetDevice(ctx->device));
+            env_refresh_host(ctx, s);
+            const float* t = reinterpret_cast<const float*>(s->hs.texels.data() + im->offset);
+            for (int64_t k = 0; k < (int64_t)im->w * im->h; k++)
+                for (int c = 0; c < 3; c++) env_texels[3 * k + c] = t[4 * k + c];
+        }
+        return FM_OK;
+    });
+}
+
+int fm_params_set(fm_ctx* ctx, fm_scene* s, const float* material_params, const float* env_texels) {
+    return params_set(ctx, s, material_params, env_texels, false);
+}
+
+int fm_params_set_device(fm_ctx* ctx, fm_scene* s, const float* material_params, const float* env_texels) {
+    return params_set(ctx, s, material_params, env_texels, true);
+}
+
+int fm_render_adjoint(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, const float* dl_dimg, double* g_material,
+                      double* g_env, fm_render_stats* stats) {
+    return render_adjoint(ctx, s, p, dl_dimg, g_material, g_env, stats, false);
+}
+
+int fm_render_adjoint_device(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, const float* dl_dimg, double* g_material,
+                             double* g_env, fm_render_stats* stats) {
+    return render_adjoint(ctx, s, p, dl_dimg, g_material, g_env, stats, true);
 }
 
 int fm_trace(fm_ctx* ctx, fm_scene* s, const fm_ray* rays, int64_t n, int any_hit, fm_hit* hits) {

@@ -24,8 +24,11 @@
  *   continuation     (d f / f) rem  +  rem d ln(q/p)
  * where the MIS weights' derivatives come from the BSDF pdf's.  Russian
  * roulette is constant; no parameter moves a visibility boundary.  Env
- * texel gradients differentiate the emitted radiance with the env sampling
- * tables held fixed.  d f, d p and d ln(q/p) are central differences of the
+ * texel gradients differentiate the emitted radiance and, through the
+ * sampling tables, both env densities (they enter the MIS weights, which
+ * the reference's Distribution2D::Pdf keeps from summing to one): each
+ * sample adds d(estimate)/d ln(density) to its env cell, and
+ * launch_env_table_grad maps the cells to texels.  d f, d p and d ln(q/p) are central differences of the
  * same FP32 evaluations at the path's directions (smooth in the
  * parameters); materials with transmission lobes get the d f terms only.
  */
@@ -102,8 +105,31 @@ struct AdSink {
     float* lds;        /* num_materials * AD_NPARAM floats of this block, or null */
     double* g_mat;     /* num_materials * AD_NPARAM */
     double* g_env;     /* env texels * 3, or null */
+    double* env_cell;  /* 2 x env texels (light samples, then escaped BSDF rays), or null */
     int env_w, env_h;
 };
+
+/* The env cell (row-major texel index) of direction dir, as env_pdf finds it. */
+FMD int env_cell_of(const fm_denv& E, v3 dir) {
+    const v3 d = m3_apply(E.w2l, dir);
+    const float z = clampf(d.z, -1.0f, 1.0f);
+    const float theta = acosf(z);
+    float phi = atan2f(d.y, d.x);
+    if (phi < 0) phi += 2.0f * FM_PI;
+    const float u = phi * FM_INV_PI * 0.5f, v = theta * FM_INV_PI;
+    const int iu = min((int)(u * (float)E.W), E.W - 1);
+    const int iv = min((int)(v * (float)E.H), E.H - 1);
+    return iv * E.W + iu;
+}
+
+/* d(estimate) / d ln(density) of one sample in env cell `cell`: strategy 0
+ * (light samples: the density the sample was drawn with) or 1 (escaped BSDF
+ * rays: the pdf of their MIS weight).  The texel gradients follow once all
+ * paths are in (launch_env_table_grad). */
+FMD void add_env_cell(const AdSink& K, int strategy, int cell, float v) {
+    if (!K.env_cell || v == 0.0f || !(v == v)) return;
+    atomicAdd(K.env_cell + (size_t)strategy * K.env_w * K.env_h + cell, (double)v);
+}
 
 FMD void add_param(const AdSink& K, int mat, int p, float v) {
     if (v == 0.0f || !(v == v)) return;
@@ -198,7 +224,12 @@ FMD void adjoint_walk(const fm_dscene& S, const PathCfg& cfg, Ray ray, Rng& rng,
                     const float kw = delta ? 0.0f : dot(g, beta * ff) / ls.pdf / dl * (-2.0f * ls.pdf * ls.pdf * bp / (den * den));
                     add_material_grad(K, b, sf.material, sf.wo, wo_l, b.local(ls.wi), g * beta * ls.radiance * s, kw, 0.0f);
                 }
-                if (L.kind == FM_DLIGHT_ENV && L.aux == S.env) add_env_grad(S, K, S.envs[L.aux], -ls.wi, g * beta * f * s);
+                if (L.kind == FM_DLIGHT_ENV && L.aux == S.env) {
+                    add_env_grad(S, K, S.envs[L.aux], -ls.wi, g * beta * f * s);
+                    /* the weight w = p_l^2 / (p_l^2 + p^2): d w / d ln p_l = 2 w (1 - w) */
+                    if (K.env_cell && w < 1.0f)
+                        add_env_cell(K, 0, env_cell_of(S.envs[L.aux], ls.wi), dot(g, beta * ff) / ls.pdf / dl * (2.0f * w * (1.0f - w)));
+                }
             }
             BsdfSample ms = b.sample(sf.wo, rng.next3());
             if (length(ms.f) < FM_EPS_PDF || ms.pdf < FM_EPS_PDF) continue;
@@ -222,6 +253,9 @@ FMD void adjoint_walk(const fm_dscene& S, const PathCfg& cfg, Ray ray, Rng& rng,
                                       u * (2.0f * ms.pdf * lp * lp / (den * den)), u * w);
                 }
                 add_env_grad(S, K, E, -mr.d, g * beta * ms.f * s);
+                /* w = p^2 / (p^2 + p_e^2): d w / d ln p_e = -2 w (1 - w) */
+                if (K.env_cell && !ms.delta && w < 1.0f)
+                    add_env_cell(K, 1, env_cell_of(E, mr.d), dot(g, beta * f) / ms.pdf / dl * (-2.0f * w * (1.0f - w)));
             } else {
                 const Surf ls = fill_hit(S, mr, mh);
                 if (ls.light < 0) continue;

@@ -40,6 +40,7 @@ struct RayPre {
     v3 inv;          /* 1 / d per component */
     v3 clo, chi;     /* fast path: t of a lo / hi plane x is fma(x, inv, -c) */
     bool fast;       /* no zero direction component and finite constants */
+    bool solo;       /* ill-conditioned triangle test: traverse without the wave (trav_break) */
 };
 
 FMD int trunc_iabs(float x) { const int i = (int)x; return i < 0 ? -i : i; }
@@ -96,6 +97,14 @@ FMD RayPre precompute(const Ray& r, const fm_dscene& S) {
                              fmaxf(fabsf(p.chi.y), fabsf(p.chi.z)));
     p.fast = r.d.x != 0.0f && r.d.y != 0.0f && r.d.z != 0.0f && cmax < INFINITY &&
              fabsf(p.inv.x) < INFINITY && fabsf(p.inv.y) < INFINITY && fabsf(p.inv.z) < INFINITY;
+    /* Sheared along a component 64 times smaller than another (the
+     * truncated axis rule shears along z for every |d| < 1), the FP32
+     * triangle test can report false hits away from the triangle, outside
+     * its leaf's box; whether one stands then depends on whether its leaf is
+     * reached before a closer hit culls the box, i.e. on the visit order.
+     * (Hits that depend on the tree, checked against the oracle's, were all
+     * on rays sheared along a component 100 times smaller.) */
+    p.solo = fabsf(dz) * 64.0f < fmaxf(fabsf(dx), fabsf(dy));
     return p;
 }
 
@@ -292,8 +301,21 @@ template <bool COUNT> FMD void count_slot(const fm_dscene& S, int32_t p, Cnt& cn
  * in the speculative "while-while" form of Aila & Laine (HPG 2009): a lane
  * that reaches a leaf postpones it and keeps descending until every active
  * lane of the wave holds a leaf, then the leaves are tested together; the
- * closest hit does not depend on the order (order-free tie rule). */
+ * closest hit does not depend on the order (order-free tie rule) as long as
+ * every hit lies in its leaf's box, which solo rays (trav_break) may break. */
 #define FM_TRAV_NONE 0x7fffffff   /* no node / no pending leaf (leaf refs are negative) */
+
+/* End of a descent.  A lane holding a leaf keeps descending until every
+ * lane of the wave holds one, except a solo ray (RayPre::solo), which tests
+ * each leaf as soon as it reaches it: its visit order then depends on the
+ * ray alone, not on which rays share its wave (that changes from run to run
+ * in the wavefront form, whose queues are filled by atomics), so the hit it
+ * reports, false ones included, is the same in every run and in both forms.
+ * The other rays' closest hits do not depend on the order. */
+FMD bool trav_break(const RayPre& rp, int32_t leaf) {
+    const bool all = __all(leaf != FM_TRAV_NONE || rp.solo);
+    return rp.solo ? leaf != FM_TRAV_NONE : all;
+}
 
 /* A lane's traversal stack.  Full form: entry i at stk[i * stride] in LDS,
  * as many entries as the tree needs.  Short form: the newest FM_SHORT_STACK
@@ -388,7 +410,7 @@ FMD bool traverse_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& 
                 node = pop();
             }
             if (node < 0 && leaf == FM_TRAV_NONE) { leaf = node; node = pop(); }
-            if (__all(leaf != FM_TRAV_NONE)) break;
+            if (trav_break(rp, leaf)) break;
         }
         while (leaf != FM_TRAV_NONE) {
             const int32_t start = FM_LEAF_START(leaf), count = FM_LEAF_COUNT(leaf);
@@ -458,7 +480,7 @@ FMD bool traverse4_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit&
                 node = child_of4(cr, k0);
             }
             if (node < 0 && leaf == FM_TRAV_NONE) { leaf = node; node = pop(); }
-            if (__all(leaf != FM_TRAV_NONE)) break;
+            if (trav_break(rp, leaf)) break;
         }
         while (leaf != FM_TRAV_NONE) {
             const int32_t start = FM_LEAF_START(leaf), count = FM_LEAF_COUNT(leaf);
@@ -535,7 +557,7 @@ FMD bool traverse8_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit&
                 node = child(k[0]);
             }
             if (node < 0 && leaf == FM_TRAV_NONE) { leaf = node; node = pop(); }
-            if (__all(leaf != FM_TRAV_NONE)) break;
+            if (trav_break(rp, leaf)) break;
         }
         while (leaf != FM_TRAV_NONE) {
             const int32_t start = FM_LEAF_START(leaf), count = FM_LEAF_COUNT(leaf);

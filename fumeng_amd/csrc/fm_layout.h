@@ -126,6 +126,8 @@ typedef struct fm_denv {
     int64_t off_marg_guide; /* int32 bits, guide_m + 1: last row with cdf <= g / guide_m */
     int64_t off_cond_guide; /* int32 bits, H * (guide_c + 1): the same per row */
     int32_t guide_m, guide_c;  /* powers of two */
+    int64_t off_aux;        /* doubles into env_aux: sin(theta) of each row's texel centres (H),
+                               each row's raw sum of luminance x sin(theta) (H), their total (1) */
 } fm_denv;
 
 typedef struct fm_dimage {
@@ -157,6 +159,7 @@ typedef struct fm_dscene {
     const fm_dimage* images;
     const unsigned char* texels;
     const float* env_tables;
+    const double* env_aux;       /* FP64 row facts of the env tables (fm_denv.off_aux) */
     const float* srgb_lut;       /* 256 entries */
     int32_t root_ref;            /* node index or leaf ref; ignored if num_slots == 0 */
     int32_t num_slots;

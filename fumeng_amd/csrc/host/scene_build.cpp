@@ -505,16 +505,28 @@ struct SahBuilder {
  * env_importance_sampler.h:28-95, with Distribution1D / 2D of
  * distribution_{1d,2d}.cpp), built in FP64 from the texels and stored FP32:
  * luminance x sin(theta) at texel centres through the bilinear sampler,
- * normalised to mean 1; conditional and marginal CDFs; guide tables.
+ * normalised to mean 1; conditional and marginal CDFs; guide tables.  The
+ * FP64 sums run row by row (each row's sum in texel order, then the rows'
+ * sums in row order), the order the device rebuild (launch_env_tables)
+ * follows, so both give the same tables bit for bit.  Also keeps, in
+ * env_aux, sin(theta) per row, the raw row sums and their total (the
+ * adjoint's table derivatives and the device rebuild read them).
  * fresh = false rewrites the tables at the offsets already in e (same size:
  * new texel values of the same image). */
 void build_env_tables(HostScene& hs, fm_denv& e, const std::function<void(int, int, double*)>& texel, bool fresh) {
     const int W = e.W, H = e.H;
     std::vector<double> lum((size_t)W * H);
+    if (fresh) {
+        e.off_aux = (int64_t)hs.env_aux.size();
+        hs.env_aux.resize(hs.env_aux.size() + 2 * (size_t)H + 1);
+    }
+    double* aux = hs.env_aux.data() + e.off_aux;
     double total = 0.0;
     for (int j = 0; j < H; j++) {
         const double v = ((double)j + 0.5) / H;
         const double st = std::sin(kPId * v);
+        aux[j] = st;
+        double row = 0.0;
         for (int i = 0; i < W; i++) {
             const double u = ((double)i + 0.5) / W;
             const double ru = u * W, rv = v * H;
@@ -529,9 +541,12 @@ void build_env_tables(HostScene& hs, fm_denv& e, const std::function<void(int, i
             for (int c = 0; c < 3; c++) rgb[c] = (a[c] * (1 - pu) + b[c] * pu) * (1 - pv) + (c3[c] * (1 - pu) + d3[c] * pu) * pv;
             const double L = (0.2126 * rgb[0] + 0.7152 * rgb[1] + 0.0722 * rgb[2]) * st;
             lum[(size_t)j * W + i] = L;
-            total += L;
+            row += L;
         }
+        aux[H + j] = row;
+        total += row;
     }
+    aux[2 * H] = total;
     if (!(total > 0.0)) throw BuildError(FM_ERR_INVALID, "env map has zero luminance (invalid 1d distribution)");
     const double scale = (double)W * (double)H / total;
     std::vector<float>& T = hs.env_tables;
@@ -1182,6 +1197,7 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
         hs.lights.push_back(L);
     }
     if (hs.env_tables.empty()) hs.env_tables.resize(16);
+    if (hs.env_aux.empty()) hs.env_aux.resize(1);
 
     /* ---- camera (pin_hole_camera.cpp:5-16) ---- */
     make_camera(d.camera, hs.camera);

@@ -67,6 +67,7 @@ struct HostScene {
     std::vector<fm_dimage> images;
     std::vector<unsigned char> texels;
     std::vector<float> env_tables;
+    std::vector<double> env_aux;         /* per env: fm_denv.off_aux */
     std::vector<float> srgb_lut;
     int32_t root_ref = 0;
     float root_box[6] = {0, 0, 0, 0, 0, 0};

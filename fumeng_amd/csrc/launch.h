@@ -125,6 +125,7 @@ struct AdjointArgs {
     const float* dl_dimg;     /* H x W x 3, row 0 = top */
     double* g_mat;            /* num_materials x 6 */
     double* g_env;            /* env texels x 3, or null */
+    double* env_cell;         /* 2 x env texels, zeroed (table derivatives per env cell), or null */
     int env_w, env_h;
     int lds_grad;             /* accumulate material gradients in LDS first */
     unsigned long long* counters;
@@ -133,6 +134,25 @@ struct AdjointArgs {
 /* The adjoint walk of every work item (A.totals filled beforehand). */
 hipError_t launch_adjoint(const fm_dscene& S, const AdjointArgs& A, int block, int grid, int stack_levels, hipStream_t st);
 int adjoint_blocks_per_cu(int block, size_t lds);
+
+/* Device FP64 rebuild of one float env image's sampling tables (envtab.hip),
+ * bit-identical to the host build. */
+struct EnvTableArgs {
+    fm_denv env;              /* host copy: sizes and table offsets */
+    fm_denv* env_record;      /* the env's device record (marg_sum is written) */
+    const float* rgb;         /* new texels, H x W x 3 (device), or null: rebuild from tex4 */
+    float* tex4;              /* the env image's float4 texels (device) */
+    float* tables;            /* device env tables */
+    double* aux;              /* device env_aux + env.off_aux (sin theta per row is read; row sums, total written) */
+    double* lum;              /* W * H doubles of work space */
+    double* rs;               /* H doubles of work space */
+    int* status;              /* set to 1 if the map has zero luminance, else 0 */
+};
+hipError_t launch_env_tables(const EnvTableArgs& A, hipStream_t st);
+/* Adds the table derivatives of the adjoint's env cell sums (2 x W x H) to
+ * g_env (W x H x 3); rows is 2H + 1 doubles of work space. */
+hipError_t launch_env_table_grad(const double* cell, double* rows, const float* tex4, int W, int H, const double* aux, double* g_env,
+                                 hipStream_t st);
 
 /* Device traversal-tree build (bvh_build.hip) over n primitive boxes (6
  * floats each: min xyz, max xyz, in primitive order).  On success the

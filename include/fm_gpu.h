@@ -306,6 +306,11 @@ int fm_render(fm_ctx* ctx, fm_scene* scene, const fm_render_params* params,
               fm_film* film, fm_render_stats* stats);
 /* Copies the current device film (means over the samples rendered so far). */
 int fm_film_read(fm_ctx* ctx, fm_film* film);
+/* fm_film_read into DEVICE arrays of the context's GPU (height x width x 3
+ * floats each, any may be NULL), for callers whose images stay in HBM (the
+ * PyTorch extension).  Blocking.  No counterpart in the reference, which
+ * keeps its RenderTarget on the host (core/renderer.h:84-144). */
+int fm_film_read_device(fm_ctx* ctx, float* color, float* albedo, float* normal);
 
 /* Largest frame (width * height) of fm_render_noalloc. */
 #define FM_NOALLOC_MAX_PIXELS 262144
@@ -386,20 +391,34 @@ int fm_render_batch(fm_ctx* ctx, fm_scene* scene, const fm_job* jobs, int n, con
  * first env light's linear float image, height x width x 3 (row 0 first). */
 enum { FM_DPARAM_PER_MATERIAL = 6 };
 int fm_params_get(fm_ctx* ctx, fm_scene* scene, float* material_params, float* env_texels);
-/* Either pointer may be NULL.  New env texels rebuild the env sampling
- * tables (FP64, on the host) and upload them. */
+/* Either pointer may be NULL.  New env texels are copied to the device and
+ * the env sampling tables rebuilt there in FP64 (the host build's
+ * arithmetic in its order: the same FP32 tables bit for bit;
+ * EnvImportanceSampler, env_importance_sampler.h:28-95).  A map of zero
+ * luminance is FM_ERR_INVALID (its texels are then set, its tables not). */
 int fm_params_set(fm_ctx* ctx, fm_scene* scene, const float* material_params, const float* env_texels);
+/* fm_params_set with both arrays in DEVICE memory of the context's GPU,
+ * ready when the call is made (the PyTorch extension passes its tensors). */
+int fm_params_set_device(fm_ctx* ctx, fm_scene* scene, const float* material_params, const float* env_texels);
 
 /* Path-replay adjoint (no counterpart in the reference, which only renders
  * forward).  For the paths of `params` (seed, spp, sample_offset, sizes,
  * depth as in fm_render) and dl_dimg = dL/d(mean image), height x width x 3
  * (row 0 = top), writes dL/d(material parameters) (num_materials x 6) and
  * dL/d(env texels) (env height x width x 3) as FP64 sums; either output may
- * be NULL.  Each path is recomputed from its random numbers, pdfs and MIS
- * weights are held fixed; the BSDF's parameter derivatives are central
- * differences of its FP32 value at the path's directions. */
+ * be NULL.  Each path is recomputed from its random numbers; the gradient
+ * is that of the estimator's expectation: the BSDF value, the MIS weights
+ * through the BSDF pdf, the density of the folded reflection lobes, the
+ * env radiance, and the env sampling tables through both env densities.
+ * The BSDF's parameter derivatives are central differences of its FP32
+ * value at the path's directions. */
 int fm_render_adjoint(fm_ctx* ctx, fm_scene* scene, const fm_render_params* params, const float* dl_dimg,
                       double* g_material, double* g_env, fm_render_stats* stats);
+/* fm_render_adjoint with dl_dimg and the outputs in DEVICE memory of the
+ * context's GPU (dl_dimg ready when the call is made; outputs written, as
+ * FP64 sums, when it returns). */
+int fm_render_adjoint_device(fm_ctx* ctx, fm_scene* scene, const fm_render_params* params, const float* dl_dimg,
+                             double* g_material, double* g_env, fm_render_stats* stats);
 /* ---- autotuner ------------------------------------------------------------ */
 
 /* One timed configuration of fm_autotune, with what it rendered: FNV-1a of

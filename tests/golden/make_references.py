@@ -107,9 +107,16 @@ def make_pixel_reference(name):
             if i >= len(rows):
                 return
             t = time.time()
-            c, a, nn, q = o.render(int(cols[i]), int(rows[i]), 1, 1, spp=spp, seed=SEEDS["fp64"], width=W, height=H,
-                                   threads=1)
-            out["mean"][i], out["albedo"][i], out["normal"][i], out["sumsq"][i] = c[0, 0], a[0, 0], nn[0, 0], q[0, 0]
+            saved = os.path.join(CACHE, "%s_%d_%d_%d.npy" % (name, i, rows[i], cols[i])) if CACHE else None
+            if saved and os.path.exists(saved):
+                v = np.load(saved)
+            else:
+                c, a, nn, q = o.render(int(cols[i]), int(rows[i]), 1, 1, spp=spp, seed=SEEDS["fp64"], width=W, height=H,
+                                       threads=1)
+                v = np.stack([c[0, 0], a[0, 0], nn[0, 0], q[0, 0]]).astype(np.float64)
+                if saved:
+                    np.save(saved, v)
+            out["mean"][i], out["albedo"][i], out["normal"][i], out["sumsq"][i] = v[0], v[1], v[2], v[3]
             print("  pixel %d (%d, %d): %.1f s" % (i, rows[i], cols[i], time.time() - t), flush=True)
 
     th = [threading.Thread(target=work) for _ in range(min(16, os.cpu_count() or 1))]
@@ -124,9 +131,20 @@ def make_pixel_reference(name):
     print("%s: %d pixels x %d spp in %.1f s -> %s" % (name, len(rows), spp, time.time() - t0, path))
 
 
+CACHE = None
+
+
 def main():
+    global CACHE
     threads = min(16, os.cpu_count() or 1)
     only = sys.argv[1:]
+    if "--cache" in only:
+        # --cache DIR: keep each finished pixel of a pixel-subset reference
+        # there and reuse it, so an interrupted run resumes
+        k = only.index("--cache")
+        CACHE = only[k + 1]
+        os.makedirs(CACHE, exist_ok=True)
+        only = only[:k] + only[k + 2:]
     for name in PX_SCENES:
         if "fp64_%s_px" % name in only:
             make_pixel_reference(name)
