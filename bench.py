@@ -163,7 +163,7 @@ def adjoint_leg(fumeng_amd, ctx, iters=6):
     opt = torch.optim.Adam([{"params": [theta], "lr": 0.03}, {"params": [env_t], "lr": 0.03 * float(env.mean())}])
     nan_rows = torch.full_like(theta, float("nan"))
     times, losses, fwd_ms, adj_ms = [], [], [], []
-    orig_render, orig_adjoint = sc.render, sc.adjoint
+    orig_render, orig_adjoint = sc.render, sc.adjoint_device
 
     def timed_render(**kw):
         out = orig_render(**kw)
@@ -172,9 +172,9 @@ def adjoint_leg(fumeng_amd, ctx, iters=6):
 
     def timed_adjoint(*a, **kw):
         out = orig_adjoint(*a, **kw)
-        adj_ms.append(out[2].kernel_ms)
+        adj_ms.append(out.kernel_ms)
         return out
-    sc.render, sc.adjoint = timed_render, timed_adjoint
+    sc.render, sc.adjoint_device = timed_render, timed_adjoint
     for it in range(iters + 1):
         torch.cuda.synchronize(dev)
         t0 = time.perf_counter()
@@ -201,6 +201,18 @@ def adjoint_leg(fumeng_amd, ctx, iters=6):
     log("adjoint: %.1f ms/iter (forward kernel %.1f ms, adjoint kernel %.1f ms), loss %.4g -> %.4g"
         % (res["ms_per_iter"], res["forward_kernel_ms"], res["adjoint_kernel_ms"], losses[0], losses[-1]))
     return res
+
+
+def adjoint_child():
+    """The adjoint leg in a child process that imports torch before the
+    library: its tensors live in HBM, which needs torch's HIP runtime to be
+    the process's only one (this process loaded the library's first)."""
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--adjoint-only"], stdout=subprocess.PIPE, text=True,
+                       timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError("adjoint child exited with %d" % r.returncode)
+    return json.loads(r.stdout.strip().splitlines()[-1])["adjoint"]
 
 
 def stress_leg(fumeng_amd, ctx):
@@ -379,6 +391,8 @@ def main():
                     help="skip the open-box leg (512x512, 64 spp, both forms, VALU roofline)")
     ap.add_argument("--no-compare", dest="compare", action="store_false",
                     help="skip the one-step run of the other form")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the inverse-rendering leg and print its result (the main run starts it as a child)")
     ap.add_argument("--box-only", action="store_true",
                     help="only run the open-box leg and print its JSON (profiling of that workload)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -392,6 +406,14 @@ def main():
               "sample range; see the design notes)", file=sys.stderr)
         sys.exit(2)
 
+    if args.adjoint_only:
+        import torch
+        torch.ones(1, device="cuda")   # torch's HIP runtime first: the library then shares it
+        import fumeng_amd
+        ctx = fumeng_amd.Context(0)
+        print(json.dumps({"adjoint": adjoint_leg(fumeng_amd, ctx)}))
+        ctx.close()
+        return
     import fumeng_amd
     from fumeng_amd import scenes
     from fumeng_amd import _abi as _abi_mod
@@ -518,7 +540,7 @@ def main():
     adjoint = None
     if args.adjoint:
         try:
-            adjoint = adjoint_leg(fumeng_amd, ctx)
+            adjoint = adjoint_child()
         except Exception as e:
             log("adjoint leg failed: %s" % e)
 

@@ -136,6 +136,7 @@ class Context:
 class Scene:
     def __init__(self, ctx, h):
         self.ctx, self.h = ctx, h
+        self.params_generation = 0   # counts set_params / set_params_device calls
 
     def close(self):
         if self.h and self.ctx.h:
@@ -276,12 +277,14 @@ class Scene:
         env = None if env_texels is None else np.ascontiguousarray(env_texels, np.float32)
         _check(lib().fm_params_set(self.ctx.h, self.h, _ptr(mp) if mp is not None else None,
                                    _ptr(env) if env is not None else None), self.ctx.h)
+        self.params_generation += 1
 
     def set_params_device(self, material_params=0, env_texels=0):
         """fm_params_set_device: device addresses (ints; 0 = leave as is) of
         float32 arrays on the context's GPU, (num_materials, 6) and (H, W, 3),
         ready when called."""
         _check(lib().fm_params_set_device(self.ctx.h, self.h, material_params or None, env_texels or None), self.ctx.h)
+        self.params_generation += 1
 
     def film_read_device(self, color=0, albedo=0, normal=0):
         """fm_film_read_device: the last render's means into device arrays

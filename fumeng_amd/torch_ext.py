@@ -12,6 +12,13 @@ the image gradient and writes the image and the gradients through device
 pointers (fm_params_set_device, fm_film_read_device,
 fm_render_adjoint_device), after torch's current stream has finished
 producing them.  CPU tensors go through host memory.
+
+Device tensors need one HIP runtime in the process.  torch ships its own
+libamdhip64 and asks for it by file name; the library asks for the soname
+and takes whichever runtime is already loaded.  So import torch before the
+library is first used (before the first fumeng_amd.Context): the library
+then runs on torch's runtime.  Loaded the other way round, the process holds
+two runtimes and torch finds no GPU.
 """
 import numpy as np
 import torch
@@ -39,6 +46,7 @@ class RenderFunction(torch.autograd.Function):
             env = None if env_texels is None else env_texels.detach().to(torch.float32).contiguous().clone()
             _ready(mp.device)
             scene.set_params_device(mp.data_ptr(), env.data_ptr() if env is not None else 0)
+            ctx.generation = scene.params_generation
             scene.render(spp=spp, seed=seed, mode=mode, read=False)
             H, W = scene.last_render_shape
             color = torch.empty((H, W, 3), dtype=torch.float32, device=mp.device)
@@ -62,7 +70,8 @@ class RenderFunction(torch.autograd.Function):
             ge = torch.empty(env.shape, dtype=torch.float64, device=mp.device) if env is not None else None
             _ready(mp.device)
             # the scene may have been set to other values since this forward pass
-            scene.set_params_device(mp.data_ptr(), env.data_ptr() if env is not None else 0)
+            if scene.params_generation != ctx.generation:
+                scene.set_params_device(mp.data_ptr(), env.data_ptr() if env is not None else 0)
             scene.adjoint_device(g.data_ptr(), gm.data_ptr(), ge.data_ptr() if ge is not None else 0,
                                  spp=ctx.spp_adjoint, seed=ctx.seed_adjoint, width=g.shape[1], height=g.shape[0])
             return gm.to(torch.float32), (ge.to(torch.float32) if ge is not None else None), None, None, None, None, None, None
