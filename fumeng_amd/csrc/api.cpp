@@ -67,10 +67,13 @@ struct WfBuffers {
     DevBuf mis_f, mis_beta, mis_ng, sh_o, sh_d, sh_c, sh_vis;
     DevBuf q_ext[2], q_sh[2], q_cls[2], counts;
     DevBuf spill;   /* short traversal stacks' spill entries */
-    /* capacities in elements: paths, ray slots, MisBSDF records, shadow records, iterations */
-    int64_t cap_p = 0, cap_r = 0, cap_pd = 0, cap_pdl = 0, cap_it = 0;
+    /* capacities in elements: paths, ray slots, MisBSDF records, shadow records, any-hit queue entries,
+     * iterations */
+    int64_t cap_p = 0, cap_r = 0, cap_pd = 0, cap_pdl = 0, cap_qs = 0, cap_it = 0;
     void ensure(int64_t P, int D, int NL, int depth) {
-        const int64_t PD = P * D, R = P + PD, PDL = PD * NL, IT = depth + 2;
+        /* the any-hit queue holds the shadow records and, in scenes lit by
+         * the environment alone, the MisBSDF rays */
+        const int64_t PD = P * D, R = P + PD, PDL = PD * NL, QS = PDL + PD, IT = depth + 2;
         /* a group's capacity is 0 while it is being reallocated, so an
          * allocation failure never leaves a smaller buffer behind a larger one */
         if (P > cap_p) {
@@ -96,8 +99,12 @@ struct WfBuffers {
             cap_pdl = 0;
             for (DevBuf* b : {&sh_o, &sh_d, &sh_c}) b->alloc((size_t)PDL * 16);
             sh_vis.alloc((size_t)PDL);
-            for (DevBuf& b : q_sh) b.alloc((size_t)PDL * 4);
             cap_pdl = PDL;
+        }
+        if (QS > cap_qs) {
+            cap_qs = 0;
+            for (DevBuf& b : q_sh) b.alloc((size_t)QS * 4);
+            cap_qs = QS;
         }
         if (IT > cap_it) { cap_it = 0; counts.alloc((size_t)IT * fm::WQ_N * fm::WQ_CNT_STRIDE * 4); cap_it = IT; }
     }
@@ -137,7 +144,8 @@ struct fm_ctx {
     DevBuf adj_film;                                    /* adjoint: FP64 sums of its forward run (unused) */
     double* noalloc = nullptr;         /* this device's fm_render_noalloc working array */
     std::vector<hipEvent_t> trace_ev;  /* pairs around wavefront trace launches */
-    std::vector<hipEvent_t> pass_ev;   /* end of each wavefront pass */
+    DevBuf wf_stamps;                  /* device wall clock before the first and after each wavefront pass */
+    int wall_khz = 0;                  /* its rate */
 };
 
 struct fm_scene {
@@ -383,18 +391,20 @@ hipEvent_t* event_pool(std::vector<hipEvent_t>& pool, size_t n) {
 /* Wavefront run: packs the jobs' samples into passes of about `budget`
  * paths (jobs in order, at most one unit per job per pass, `chunk` > 0 caps
  * the samples of a unit), uploads the pass / unit / camera tables and runs
- * every pass, directly or (FM_RENDER_GRAPH) as one captured pass graph
- * replayed per pass. */
+ * every pass, directly or (FM_RENDER_GRAPH) captured as one graph of all
+ * the passes and launched once.  A wall-clock stamp kernel before the first
+ * pass and after each one gives the jobs' completion times in either
+ * form. */
 void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int rr_depth, float rr_coef, int block,
             int grid, int chunk, int flags, WfRunOut& R, float* path_out = nullptr) {
     if (path_out && jobs.size() != 1) throw std::invalid_argument("per-path output needs exactly one job");
     const int NL = s->d.num_lights;
-    const int64_t per_path = 100 + 64 * (1 + (int64_t)D) + 48 * (int64_t)D + 57 * (int64_t)D * std::max(NL, 1);
+    const int64_t per_path = 100 + 64 * (1 + (int64_t)D) + 56 * (int64_t)D + 57 * (int64_t)D * std::max(NL, 1);
     /* Passes as large as the context's path-state budget allows (96 GB on
      * an idle device): fewer passes mean fewer nearly empty tail iterations
      * (1920x1080 at 256 spp: 8 passes took 522 ms, 2 took 493).  The index
      * bound keeps every queue entry inside int32. */
-    const int64_t idx_cap = (((int64_t)1 << 31) - 1) / std::max<int64_t>(1 + D, (int64_t)D * std::max(NL, 1));
+    const int64_t idx_cap = (((int64_t)1 << 31) - 1) / std::max<int64_t>(1 + D, (int64_t)D * (std::max(NL, 1) + 1));
     const int64_t budget = std::max<int64_t>(1, std::min<int64_t>(idx_cap, ctx->path_budget / per_path));
     std::vector<fm::WfUnit> units;
     std::vector<fm::WfPass> passes;
@@ -447,7 +457,7 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
         cap = std::max(cap, used); max_pix = std::max(max_pix, pix);
     }
     if (passes.empty()) return;
-    if (cap * (1 + D) >= ((int64_t)1 << 31) || cap * D * std::max(NL, 1) >= ((int64_t)1 << 31) ||
+    if (cap * (1 + D) >= ((int64_t)1 << 31) || cap * D * (std::max(NL, 1) + 1) >= ((int64_t)1 << 31) ||
         units.size() >= ((size_t)1 << 23))
         throw std::invalid_argument("wavefront pass too large: lower chunk (samples per pass)");
     ctx->wf.ensure(cap, D, NL, max_depth);
@@ -514,13 +524,24 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
     const bool graph = (flags & FM_RENDER_GRAPH) != 0 && !timed;
     const size_t npass = passes.size();
     hipEvent_t* tev = timed ? event_pool(ctx->trace_ev, npass * 2 * (size_t)W_.n_iter) : nullptr;
-    hipEvent_t* pev = event_pool(ctx->pass_ev, npass);
+    if (ctx->wf_stamps.n < (npass + 1) * sizeof(unsigned long long)) ctx->wf_stamps.alloc((npass + 1) * sizeof(unsigned long long));
+    unsigned long long* const stamps = ctx->wf_stamps.as<unsigned long long>();
+    /* the pass sequence: the same kernels every pass (the pass index lives
+     * on the device), each pass followed by its stamp */
+    auto issue = [&](hipEvent_t* ev) {
+        hipError_t e = fm::launch_stamp(stamps, 0, st);
+        for (size_t c = 0; c < npass && e == hipSuccess; c++) {
+            e = fm::launch_wavefront_pass(s->d, W_, L, count, st, ev ? ev + c * 2 * W_.n_iter : nullptr);
+            if (e == hipSuccess) e = fm::launch_stamp(stamps, (int)c + 1, st);
+        }
+        return e;
+    };
     HIPCHECK(fm::launch_set_pass(W_.pass_idx, 0, st));
     hipGraphExec_t exec = nullptr;
     if (graph) {
         hipGraph_t g = nullptr;
         HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        const hipError_t e1 = fm::launch_wavefront_pass(s->d, W_, L, count, st, nullptr);
+        const hipError_t e1 = issue(nullptr);
         const hipError_t e2 = hipStreamEndCapture(st, &g);
         if (e1 != hipSuccess || e2 != hipSuccess) {
             if (g) (void)hipGraphDestroy(g);
@@ -531,12 +552,7 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
         if (e3 != hipSuccess) throw HipError(std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
     }
     HIPCHECK(hipEventRecord(ctx->ev0, st));
-    hipError_t err = hipSuccess;
-    for (size_t c = 0; c < npass && err == hipSuccess; c++) {
-        if (graph) err = hipGraphLaunch(exec, st);
-        else err = fm::launch_wavefront_pass(s->d, W_, L, count, st, tev ? tev + c * 2 * W_.n_iter : nullptr);
-        if (err == hipSuccess) err = hipEventRecord(pev[c], st);
-    }
+    hipError_t err = graph ? hipGraphLaunch(exec, st) : issue(tev);
     if (err == hipSuccess) err = hipEventRecord(ctx->ev1, st);
     const hipError_t es = hipStreamSynchronize(st);
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -555,13 +571,12 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
     R.units = (int)units.size();
     R.trace_block = L.block_trace;
     R.short_stack = L.short_stack ? 1 : 0;
+    std::vector<unsigned long long> clk(npass + 1);
+    HIPCHECK(hipMemcpy(clk.data(), stamps, clk.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
     R.job_done_ms.assign(jobs.size(), 0.0);
     for (size_t j = 0; j < jobs.size(); j++)
-        if (last_pass[j] >= 0) {
-            float t = 0.0f;
-            HIPCHECK(hipEventElapsedTime(&t, ctx->ev0, pev[last_pass[j]]));
-            R.job_done_ms[j] = t;
-        }
+        if (last_pass[j] >= 0 && ctx->wall_khz > 0)
+            R.job_done_ms[j] = (double)(clk[last_pass[j] + 1] - clk[0]) / (double)ctx->wall_khz;
 }
 
 /* Wavefront path state a pass may hold: 96 GB (a third of a 288 GB
@@ -606,6 +621,7 @@ int fm_init(int device, fm_ctx** out) {
         HIPCHECK(hipEventCreate(&c->ev0));
         HIPCHECK(hipEventCreate(&c->ev1));
         HIPCHECK(hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device));
+        HIPCHECK(hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device));
         c->counters.alloc(8 * sizeof(unsigned long long));
         c->noalloc = noalloc_array(device);
         c->path_budget = default_path_budget();

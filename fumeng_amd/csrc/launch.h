@@ -113,6 +113,8 @@ hipError_t launch_wavefront_pass(const fm_dscene& S, WfArgs A, const WfLaunch& L
                                  hipEvent_t* ev = nullptr);
 /* *pass_idx = v (a kernel, so it can precede a graph launch). */
 hipError_t launch_set_pass(int* pass_idx, int v, hipStream_t st);
+/* writes the device wall clock into out[idx] (one kernel node) */
+hipError_t launch_stamp(unsigned long long* out, int idx, hipStream_t st);
 /* Persistent grid sizes for the wavefront kernels on this device. */
 void wavefront_grids(int num_cus, int stack_levels, WfLaunch& L);
 

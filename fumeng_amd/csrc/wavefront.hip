@@ -42,6 +42,9 @@ using fm::WfArgs;
 namespace {
 
 enum { F_PENDING = 1, F_CONT = 2 };
+/* An any-hit queue entry with this bit is a MisBSDF ray slot (scenes lit by
+ * the environment alone), without it a shadow record. */
+constexpr unsigned AH_RAY = 0x80000000u;
 
 __device__ __forceinline__ uint32_t* wf_stack() {
     extern __shared__ uint32_t fm_wf_lds[];
@@ -248,14 +251,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SHORT 
                 }
                 if (u == 0) { cls0 = cls; r0 = r; } else { cls1 = cls; r1 = r; }
             } else {
-                const int k = qs[i - (int64_t)n_ext];
-                const float4 o = reinterpret_cast<const float4*>(A.sh_o)[k];
-                const float4 d = reinterpret_cast<const float4*>(A.sh_d)[k];
+                /* a shadow record, or a MisBSDF ray of a scene lit by the
+                 * environment alone: only whether it escapes matters there,
+                 * and with t_min = 0 a closest hit exists exactly when an
+                 * any hit does (the same t range, primitive rules and box
+                 * culling until the first accepted hit) */
+                const unsigned e = (unsigned)qs[i - (int64_t)n_ext];
+                const bool mis = (e & AH_RAY) != 0u;
+                const unsigned k = e & ~AH_RAY;
+                const float4 o = reinterpret_cast<const float4*>(mis ? A.ray_o : A.sh_o)[k];
+                const float4 d = reinterpret_cast<const float4*>(mis ? A.ray_d : A.sh_d)[k];
                 Ray tr;
                 tr.o = xyz(o); tr.tmin = o.w; tr.d = xyz(d); tr.tmax = d.w;
                 TravStack<SHORT> ts = make_stack<SHORT>(stk, stride, spill, lanes, A.spill_cap);
                 const bool occluded = trace_any_on<COUNT, WIDTH>(S, tr, ts, &cn);
-                A.sh_vis[k] = occluded ? 1 : 2;
+                if (mis) reinterpret_cast<float4*>(A.hit_a)[k] = make_float4(0.0f, __int_as_float(occluded ? 0 : -1), 0.0f, 0.0f);
+                else A.sh_vis[k] = occluded ? 1 : 2;
             }
         }
         unsigned* const cc[fm::WQ_NCLS] = {cnt(A, it, fm::WQ_CLS), cnt(A, it, fm::WQ_CLS + 1), cnt(A, it, fm::WQ_CLS + 2),
@@ -333,15 +344,21 @@ FMD v3 resolve_direct(const fm_dscene& S, const WfArgs& A, int64_t p) {
 /* Every queue entry of one shade step of a wave with one round trip of
  * atomics: shadow records (lane bit pass * NL + light of sb), MisBSDF rays
  * (bit pass of mb), the continuation and the path end.  Entries of one
- * queue fill consecutive ranges in bit order. */
+ * queue fill consecutive ranges in bit order.  MIS_ANY (scenes lit by the
+ * environment alone): the MisBSDF rays go to the any-hit queue, after the
+ * shadow records, tagged AH_RAY. */
+template <bool MIS_ANY>
 FMD void push_step(const WfArgs& A, int nx, int64_t p, uint64_t sb, uint32_t mb, bool cont, bool end) {
     const int nsb = A.D * A.NL, nmb = A.D;
     const bool any_sh = __ballot(sb != 0ull) != 0ull, any_mis = __ballot(mb != 0u) != 0ull;
     unsigned t_sh = 0, t_ext = 0;
     if (any_sh)
         for (int b = 0; b < nsb; b++) t_sh += (unsigned)__popcll(__ballot((sb >> b) & 1ull));
+    unsigned t_mis = 0;
     if (any_mis)
-        for (int b = 0; b < nmb; b++) t_ext += (unsigned)__popcll(__ballot((mb >> b) & 1u));
+        for (int b = 0; b < nmb; b++) t_mis += (unsigned)__popcll(__ballot((mb >> b) & 1u));
+    if (MIS_ANY) t_sh += t_mis;
+    else t_ext += t_mis;
     const unsigned long long mc = __ballot(cont), me = __ballot(end);
     t_ext += (unsigned)__popcll(mc);
     const unsigned t_end = (unsigned)__popcll(me);
@@ -369,11 +386,14 @@ FMD void push_step(const WfArgs& A, int nx, int64_t p, uint64_t sb, uint32_t mb,
     }
     int* qe = A.q_ext[nx & 1];
     if (any_mis) {
+        int* const qm = MIS_ANY ? A.q_sh[nx & 1] : qe;
+        unsigned& bm = MIS_ANY ? b_sh : b_ext;
+        const unsigned tag = MIS_ANY ? AH_RAY : 0u;
         for (int b = 0; b < nmb; b++) {
             const bool on = (mb >> b) & 1u;
             const unsigned long long m = __ballot(on);
-            if (on) qe[b_ext + rank(m)] = (int)(A.P + p * nmb + b);
-            b_ext += (unsigned)__popcll(m);
+            if (on) qm[bm + rank(m)] = (int)((unsigned)(A.P + p * nmb + b) | tag);
+            bm += (unsigned)__popcll(m);
         }
     }
     if (cont) qe[b_ext + rank(mc)] = (int)p;
@@ -474,6 +494,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
                     reinterpret_cast<float4*>(A.mis_ng)[m] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 }
                 if (bits) mb |= (uint32_t)(ok ? 1 : 0) << pass;
+                else if (LK == fm::LK_ENV) wave_push(cnt(A, nx, fm::WQ_SH), A.q_sh[nx & 1], ok, (int)((unsigned)(A.P + m) | AH_RAY));
                 else wave_push(cnt(A, nx, fm::WQ_EXT), A.q_ext[nx & 1], ok, (int)(A.P + m));
                 any = any || ok;
             }
@@ -495,7 +516,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
             }
             dim_out = rng.dim;
         }
-        push_step(A, nx, p, sb, mb, !term, term && any);
+        push_step<LK == fm::LK_ENV>(A, nx, p, sb, mb, !term, term && any);
         reinterpret_cast<float4*>(A.st_beta)[p] = f4(beta, __int_as_float(depth));
         reinterpret_cast<float4*>(A.st_color)[p] = f4(color, __uint_as_float(dim_out));
         A.st_flags[p] = (fl & ~0xff) | (any ? F_PENDING : 0) | (term ? 0 : F_CONT);
@@ -540,6 +561,13 @@ __global__ void k_wf_next_pass(int* pass_idx) {
 }
 __global__ void k_wf_set_pass(int* pass_idx, int v) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *pass_idx = v;
+}
+/* Device wall clock (constant rate, hipDeviceAttributeWallClockRate) at
+ * this point of the stream, into out[idx]: a kernel node, so a captured
+ * graph can carry it (per-pass completion times of a batch launched as one
+ * graph). */
+__global__ void k_wf_stamp(unsigned long long* out, int idx) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[idx] = wall_clock64();
 }
 
 template <typename K>
@@ -666,6 +694,11 @@ hipError_t launch_wavefront_pass(const fm_dscene& S, WfArgs A, const WfLaunch& L
 
 hipError_t launch_set_pass(int* pass_idx, int v, hipStream_t st) {
     hipLaunchKernelGGL(k_wf_set_pass, dim3(1), dim3(64), 0, st, pass_idx, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_stamp(unsigned long long* out, int idx, hipStream_t st) {
+    hipLaunchKernelGGL(k_wf_stamp, dim3(1), dim3(64), 0, st, out, idx);
     return hipGetLastError();
 }
 

@@ -254,8 +254,8 @@ enum { FM_MODE_MEGAKERNEL = 0, FM_MODE_WAVEFRONT = 1 };
 /* fm_render_params.flags */
 enum {
     FM_RENDER_TIME_TRACE = 1,   /* time each traversal launch (fm_render_stats.trace_ms) */
-    FM_RENDER_GRAPH = 2         /* wavefront: capture one pass as a HIP graph, replay it per pass
-                                   (ignored with FM_RENDER_TIME_TRACE) */
+    FM_RENDER_GRAPH = 2         /* wavefront: capture every pass of the call as one HIP graph and
+                                   launch it once (ignored with FM_RENDER_TIME_TRACE) */
 };
 typedef struct fm_render_params {
     int32_t width, height, spp, depth;   /* <= 0: take the scene's renderer block */
@@ -374,9 +374,10 @@ typedef struct fm_batch_stats {
  * fm_set_path_budget), so renders of uneven size, spp and depth share the
  * device.  `common` (may be NULL) gives direct_loop, rr_depth, rr_coef,
  * block_size, grid_blocks, chunk (max samples of a job per pass) and flags
- * (FM_RENDER_GRAPH replays one captured pass graph).  job_done_ms (may be
- * NULL, n entries) receives each job's completion time: device time from
- * the start of the batch to the end of its last pass.  The replacement of
+ * (FM_RENDER_GRAPH: all the passes as one captured graph, one launch).
+ * job_done_ms (may be NULL, n entries) receives each job's completion time:
+ * device wall-clock time from the start of the batch to the end of its last
+ * pass.  The replacement of
  * the reference's one-render-per-process DrawFrame (core/renderer.h:84-144)
  * for many renders of one scene; it has no single counterpart there. */
 int fm_render_batch(fm_ctx* ctx, fm_scene* scene, const fm_job* jobs, int n, const fm_render_params* common,

@@ -3,9 +3,9 @@
 A job's paths depend only on its camera, resolution, seed, sample indices
 and depth, and its film sums samples in units of at most `chunk` samples
 per pass; so with the same chunk a job gives the same film alone or in a
-batch with other jobs, bit for bit.  A pass replayed from the captured HIP
-graph runs the same kernels with the same arguments as the direct launches:
-every replay gives the direct launches' film, bit for bit.
+batch with other jobs, bit for bit.  The captured HIP graph of a call's
+passes runs the same kernels with the same arguments as the direct
+launches: every graph launch gives the direct launches' film, bit for bit.
 """
 import numpy as np
 import pytest
@@ -40,7 +40,9 @@ def test_batch_jobs_equal_solo_renders(gpu_ctx):
     films, done, st = g.render_batch(JOBS, chunk=4)
     assert st.paths == sum(j["width"] * j["height"] * j["spp"] for j in JOBS)
     assert st.units >= len(JOBS) and st.passes >= 1
-    assert np.all(done > 0) and np.all(done <= st.kernel_ms * 1.001)
+    assert np.all(done > 0) and np.all(done <= st.kernel_ms * 1.001 + 0.05), (done.max(), st.kernel_ms)
+    # the last job ends with the batch's last pass
+    assert done.max() >= 0.95 * st.kernel_ms, (done.max(), st.kernel_ms)
     for i, j in enumerate(JOBS):
         solo, _, _ = g.render_batch([j], chunk=4)
         for a, b in zip(films[i], solo[0]):
@@ -135,9 +137,9 @@ def test_lowered_path_budget_splits_a_frame_into_equal_passes(gpu_ctx):
     """With chunk 0 a frame larger than the path-state budget runs in passes
     of equal sample counts: 24 spp under a budget of 7 samples' worth of
     paths gives 4 passes of 6, bit for bit the film of an explicit chunk of
-    6.  (Box scene: one light, one direct loop, 333 B of path state per path.)"""
+    6.  (Box scene: one light, one direct loop, 341 B of path state per path.)"""
     g = gpu_ctx.scene(scenes.box_scene(width=64, height=48, spp=24))
-    per_path = 100 + 64 * 2 + 48 + 57
+    per_path = 100 + 64 * 2 + 56 + 57
     try:
         gpu_ctx.set_path_budget(7 * 64 * 48 * per_path)
         films, _, st = g.render_batch([{"camera": None, "seed": 3}], chunk=0)
@@ -180,7 +182,7 @@ def test_config4_uneven_box_batch(gpu_ctx):
     """The batched-jobs workload at full size: 64 renders of the box scene
     from seeded cameras on a radius-4 sphere, r x r with r ~ U{128..1024},
     spp ~ U{16..256}, depth ~ U{4..16} (4.12e9 paths), in one batch of
-    wavefront passes with the captured pass graph.  With units of 16
+    wavefront passes captured as one graph and launched once.  With units of 16
     samples, four seeded jobs give bit for bit the films they give alone;
     on 16x16 crops of two jobs, sample by sample, the radiances equal the
     oracle's with common random numbers.  The per-frame loop this replaces is Renderer::DrawFrame
@@ -192,7 +194,9 @@ def test_config4_uneven_box_batch(gpu_ctx):
     total = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
     print("config 4: %d paths, %d passes, %d units, %.1f ms" % (st.paths, st.passes, st.units, st.kernel_ms))
     assert st.paths == total
-    assert np.all(done > 0) and np.all(done <= st.kernel_ms * 1.001)
+    assert np.all(done > 0) and np.all(done <= st.kernel_ms * 1.001 + 0.05), (done.max(), st.kernel_ms)
+    # the last job ends with the batch's last pass
+    assert done.max() >= 0.95 * st.kernel_ms, (done.max(), st.kernel_ms)
     assert all(np.isfinite(f[0]).mean() > 0.999 for f in films)
     # cameras behind a wall see only its unlit outside; those through the open side see the light
     lit = [i for i, f in enumerate(films) if np.nanmean(f[0]) > 0.01]
