@@ -66,6 +66,7 @@ struct WfBuffers {
     DevBuf ray_o, ray_d, hit_a, hit_b, st_beta, st_color, st_flags, out_alb, out_nrm;
     DevBuf mis_f, mis_beta, mis_ng, sh_o, sh_d, sh_c, sh_vis;
     DevBuf q_ext[2], q_sh[2], q_cls[2], counts;
+    DevBuf cls;     /* shading class per ray-queue position */
     DevBuf spill;   /* short traversal stacks' spill entries */
     /* capacities in elements: paths, ray slots, MisBSDF records, shadow records, any-hit queue entries,
      * iterations */
@@ -88,6 +89,7 @@ struct WfBuffers {
             for (DevBuf* b : {&ray_o, &ray_d, &hit_a}) b->alloc((size_t)R * 16);
             hit_b.alloc((size_t)R * 8);
             for (DevBuf& b : q_ext) b.alloc((size_t)R * 4);
+            cls.alloc((size_t)R);
             cap_r = R;
         }
         if (PD > cap_pd) {
@@ -112,7 +114,7 @@ struct WfBuffers {
         int64_t n = 0;
         for (const DevBuf* b : {&ray_o, &ray_d, &hit_a, &hit_b, &st_beta, &st_color, &st_flags, &out_alb, &out_nrm, &mis_f,
                                 &mis_beta, &mis_ng, &sh_o, &sh_d, &sh_c, &sh_vis, &q_ext[0], &q_ext[1], &q_sh[0], &q_sh[1],
-                                &q_cls[0], &q_cls[1], &counts, &spill})
+                                &q_cls[0], &q_cls[1], &counts, &spill, &cls})
             n += (int64_t)b->n;
         return n;
     }
@@ -399,7 +401,7 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
             int grid, int chunk, int flags, WfRunOut& R, float* path_out = nullptr) {
     if (path_out && jobs.size() != 1) throw std::invalid_argument("per-path output needs exactly one job");
     const int NL = s->d.num_lights;
-    const int64_t per_path = 100 + 64 * (1 + (int64_t)D) + 56 * (int64_t)D + 57 * (int64_t)D * std::max(NL, 1);
+    const int64_t per_path = 100 + 65 * (1 + (int64_t)D) + 56 * (int64_t)D + 57 * (int64_t)D * std::max(NL, 1);
     /* Passes as large as the context's path-state budget allows (96 GB on
      * an idle device): fewer passes mean fewer nearly empty tail iterations
      * (1920x1080 at 256 spp: 8 passes took 522 ms, 2 took 493).  The index
@@ -487,6 +489,7 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
         W_.q_ext[b] = B.q_ext[b].as<int>(); W_.q_sh[b] = B.q_sh[b].as<int>(); W_.q_cls[b] = B.q_cls[b].as<int>();
     }
     W_.counts = B.counts.as<unsigned>();
+    W_.cls = B.cls.as<unsigned char>();
     W_.counters = ctx->counters.as<unsigned long long>();
     W_.path_out = path_out;
     W_.path_s0 = path_out ? jobs[0].sample_offset : 0;

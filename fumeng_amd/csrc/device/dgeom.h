@@ -330,6 +330,7 @@ template <> struct TravStack<false> {
     int stride;
     int sp;   /* depth times stride: no multiply per push / pop */
     FMD TravStack(uint32_t* s, int st) : stk(s), stride(st), sp(0) {}
+    FMD void reset() { sp = 0; }
     FMD void push(uint32_t v) { stk[sp] = v; sp += stride; }
     FMD int32_t pop() {
         if (sp == 0) return FM_TRAV_NONE;
@@ -346,6 +347,7 @@ template <> struct TravStack<true> {
     int depth, ns;   /* entries on the stack, of which the oldest ns are in the spill array */
     FMD TravStack(uint32_t* s, int st, uint32_t* sp, uint32_t sst, int cap)
         : stk(s), stride(st), spill(sp), sstride(sst), scap(cap), depth(0), ns(0) {}
+    FMD void reset() { depth = 0; ns = 0; }
     FMD void push(uint32_t v) {
         if (depth - ns == FM_SHORT_STACK) {   /* ring full: its oldest entry (index ns) moves out */
             if (ns < scap) spill[(size_t)ns * sstride] = stk[(ns & (FM_SHORT_STACK - 1)) * stride];
@@ -380,60 +382,24 @@ FMD int4 buf_load4i(__amdgpu_buffer_rsrc_t r, uint32_t off) {
     return make_int4((int)v[0], (int)v[1], (int)v[2], (int)v[3]);
 }
 
-template <bool FAST, bool ANY, bool COUNT, bool SHORT>
-FMD bool traverse_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& h, TravStack<SHORT>& stack, Cnt& cn) {
-    const __amdgpu_buffer_rsrc_t nrs = buf_rsrc(S.nodes, S.node_bytes);
-    float tn;
-    if (!box_hit<FAST>(S.root_box, r, rp, ANY ? r.tmax : h.t, tn)) return false;
-    int32_t node = S.root_ref, leaf = FM_TRAV_NONE;
-    if (node < 0) { leaf = node; node = FM_TRAV_NONE; }
-    auto pop = [&]() -> int32_t { return stack.pop(); };
-    while (node != FM_TRAV_NONE || leaf != FM_TRAV_NONE) {
-        while (node != FM_TRAV_NONE && node >= 0) {
-            if (COUNT) cn.nodes++;
-            const uint32_t off = (uint32_t)node << 6;
-            const float4 q0 = buf_load4(nrs, off), q1 = buf_load4(nrs, off + 16u), q2 = buf_load4(nrs, off + 32u);
-            const int4 cr = buf_load4i(nrs, off + 48u);
-            const float tmax = ANY ? r.tmax : h.t;
-            float t0, t1;
-            const bool h0 = box_hit6<FAST>(q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, r, rp, tmax, t0);
-            const bool h1 = box_hit6<FAST>(q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r, rp, tmax, t1);
-            if (h0 && h1) {
-                const bool first0 = ANY || t0 <= t1;
-                stack.push((uint32_t)(first0 ? cr.y : cr.x));
-                node = first0 ? cr.x : cr.y;
-            } else if (h0) {
-                node = cr.x;
-            } else if (h1) {
-                node = cr.y;
-            } else {
-                node = pop();
-            }
-            if (node < 0 && leaf == FM_TRAV_NONE) { leaf = node; node = pop(); }
-            if (trav_break(rp, leaf)) break;
-        }
-        while (leaf != FM_TRAV_NONE) {
-            const int32_t start = FM_LEAF_START(leaf), count = FM_LEAF_COUNT(leaf);
-            if (ANY) {
-                for (int k = 0; k < count; k++) {
-                    count_slot<COUNT>(S, start + k, cn);
-                    if (test_slot_any(S, r, rp, start + k)) return true;
-                }
-            } else {
-                for (int k = 0; k < count; k++) { count_slot<COUNT>(S, start + k, cn); test_slot_closest(S, r, rp, start + k, h); }
-            }
-            if (node < 0) { leaf = node; node = pop(); }
-            else leaf = FM_TRAV_NONE;
-        }
-    }
-    return false;
+/* Box test forms: the exact std::min/max one, the one-FMA-per-plane one
+ * (rays without a zero direction component, RayPre::fast), or either
+ * chosen per lane (a caller whose lanes hold different rays). */
+enum { BOX_EXACT = 0, BOX_FAST = 1, BOX_LANE = 2 };
+template <int BM>
+FMD bool box_test(float x0, float y0, float z0, float x1, float y1, float z1, const Ray& r, const RayPre& rp, float tmax,
+                  float& tn) {
+    if constexpr (BM == BOX_LANE) return box_hit_any(x0, y0, z0, x1, y1, z1, r, rp, tmax, tn);
+    else return box_hit6<BM == BOX_FAST>(x0, y0, z0, x1, y1, z1, r, rp, tmax, tn);
 }
 
-/* The same traversal over four-wide nodes.  The children a ray enters are
- * ordered by entry distance with one 4-key sorting network (key = distance
- * bits with the child index in the two low bits; entry distances are >= 0,
- * so the unsigned order of the bits is their order); the nearest is entered
- * and the others are pushed far to near. */
+/* Four-wide nodes: the children a ray enters are ordered by entry distance
+ * with one 4-key sorting network (key = distance bits with the child index
+ * in the two low bits; entry distances are >= 0, so the unsigned order of
+ * the bits is their order).  Eight-wide nodes: the children's boxes are
+ * tested in two groups of four (one float4 per bound plane and group), the
+ * entered ones ordered by one 8-key sorting network (Batcher's 19
+ * comparators; the child index in the three low bits). */
 FMD int32_t child_of4(int4 c, uint32_t key) {
     const uint32_t i = key & 3u;
     return i == 0u ? c.x : (i == 1u ? c.y : (i == 2u ? c.z : c.w));
@@ -442,138 +408,148 @@ FMD void sort2u(uint32_t& a, uint32_t& b) {
     const uint32_t lo = min(a, b), hi = max(a, b);
     a = lo; b = hi;
 }
-
-template <bool FAST, bool ANY, bool COUNT, bool SHORT>
-FMD bool traverse4_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& h, TravStack<SHORT>& stack, Cnt& cn) {
-    float tn;
-    if (!box_hit<FAST>(S.root_box, r, rp, ANY ? r.tmax : h.t, tn)) return false;
-    int32_t node = S.root_ref, leaf = FM_TRAV_NONE;
-    if (node < 0) { leaf = node; node = FM_TRAV_NONE; }
-    auto pop = [&]() -> int32_t { return stack.pop(); };
-    constexpr uint32_t MISS = 0xffffffffu;
-    while (node != FM_TRAV_NONE || leaf != FM_TRAV_NONE) {
-        while (node != FM_TRAV_NONE && node >= 0) {
-            if (COUNT) cn.nodes++;
-            const float4* nd = reinterpret_cast<const float4*>(S.nodes4 + node);
-            const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
-            const int4 cr = reinterpret_cast<const int4*>(nd)[6];
-            const float tmax = ANY ? r.tmax : h.t;
-            /* all four tests unconditionally (children 0 and 1 always exist;
-             * an empty slot's box is masked after the test), so that the
-             * whole node is fetched at once */
-            float t0, t1, t2, t3;
-            const bool b0 = box_hit6<FAST>(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, r, rp, tmax, t0);
-            const bool b1 = box_hit6<FAST>(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, r, rp, tmax, t1);
-            const bool b2 = box_hit6<FAST>(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, r, rp, tmax, t2) && cr.z != FM_NODE4_EMPTY;
-            const bool b3 = box_hit6<FAST>(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, r, rp, tmax, t3) && cr.w != FM_NODE4_EMPTY;
-            uint32_t k0 = b0 ? (__float_as_uint(t0) & ~3u) | 0u : MISS;
-            uint32_t k1 = b1 ? (__float_as_uint(t1) & ~3u) | 1u : MISS;
-            uint32_t k2 = b2 ? (__float_as_uint(t2) & ~3u) | 2u : MISS;
-            uint32_t k3 = b3 ? (__float_as_uint(t3) & ~3u) | 3u : MISS;
-            sort2u(k0, k1); sort2u(k2, k3); sort2u(k0, k2); sort2u(k1, k3); sort2u(k1, k2);
-            if (k0 == MISS) {
-                node = pop();
-            } else {
-                if (k3 != MISS) stack.push((uint32_t)child_of4(cr, k3));
-                if (k2 != MISS) stack.push((uint32_t)child_of4(cr, k2));
-                if (k1 != MISS) stack.push((uint32_t)child_of4(cr, k1));
-                node = child_of4(cr, k0);
-            }
-            if (node < 0 && leaf == FM_TRAV_NONE) { leaf = node; node = pop(); }
-            if (trav_break(rp, leaf)) break;
-        }
-        while (leaf != FM_TRAV_NONE) {
-            const int32_t start = FM_LEAF_START(leaf), count = FM_LEAF_COUNT(leaf);
-            if (ANY) {
-                for (int k = 0; k < count; k++) {
-                    count_slot<COUNT>(S, start + k, cn);
-                    if (test_slot_any(S, r, rp, start + k)) return true;
-                }
-            } else {
-                for (int k = 0; k < count; k++) { count_slot<COUNT>(S, start + k, cn); test_slot_closest(S, r, rp, start + k, h); }
-            }
-            if (node < 0) { leaf = node; node = pop(); }
-            else leaf = FM_TRAV_NONE;
-        }
-    }
-    return false;
-}
-
-/* The same traversal over eight-wide nodes: the children's boxes are tested
- * in two groups of four (one float4 per bound plane and group), the entered
- * ones are ordered by one 8-key sorting network (Batcher's 19 comparators;
- * key = distance bits with the child index in the three low bits), the
- * nearest is entered and the others are pushed far to near. */
 FMD uint32_t wide_key8(bool hit, float t, uint32_t i) { return hit ? (__float_as_uint(t) & ~7u) | i : 0xffffffffu; }
 
-template <bool FAST, bool ANY, bool COUNT, bool SHORT>
-FMD bool traverse8_impl(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& h, TravStack<SHORT>& stack, Cnt& cn) {
-    float tn;
-    if (!box_hit<FAST>(S.root_box, r, rp, ANY ? r.tmax : h.t, tn)) return false;
-    int32_t node = S.root_ref, leaf = FM_TRAV_NONE;
-    if (node < 0) { leaf = node; node = FM_TRAV_NONE; }
-    auto pop = [&]() -> int32_t { return stack.pop(); };
+/* One inner node of a WIDTH-wide tree: tests its children's boxes against
+ * [t_min, tmax], pushes the entered children but the nearest (far to near;
+ * the any hit skips the ordering of two-wide nodes) and returns the
+ * nearest, or the popped entry if none is entered.  Two-wide records are
+ * read through a buffer resource (nrs). */
+template <int WIDTH, int BM, bool ANY, bool SHORT>
+FMD int32_t visit_node(const fm_dscene& S, __amdgpu_buffer_rsrc_t nrs, int32_t node, const Ray& r, const RayPre& rp,
+                       float tmax, TravStack<SHORT>& stack) {
     constexpr uint32_t MISS = 0xffffffffu;
-    while (node != FM_TRAV_NONE || leaf != FM_TRAV_NONE) {
-        while (node != FM_TRAV_NONE && node >= 0) {
-            if (COUNT) cn.nodes++;
-            const float4* nd = reinterpret_cast<const float4*>(S.nodes8 + node);
-            const int4 ca = reinterpret_cast<const int4*>(nd)[12], cb = reinterpret_cast<const int4*>(nd)[13];
-            const float tmax = ANY ? r.tmax : h.t;
-            uint32_t k[8];
-#pragma unroll
-            for (int g = 0; g < 2; g++) {
-                /* planes lo_x, hi_x, lo_y, hi_y, lo_z, hi_z: 8 floats each, lanes 4g..4g+3 */
-                const float4 lx = nd[0 + g], hx = nd[2 + g], ly = nd[4 + g], hy = nd[6 + g], lz = nd[8 + g], hz = nd[10 + g];
-                const int4 cr = g ? cb : ca;
-                float t0, t1, t2, t3;
-                const bool b0 = box_hit6<FAST>(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, r, rp, tmax, t0) && cr.x != FM_NODE4_EMPTY;
-                const bool b1 = box_hit6<FAST>(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, r, rp, tmax, t1) && cr.y != FM_NODE4_EMPTY;
-                const bool b2 = box_hit6<FAST>(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, r, rp, tmax, t2) && cr.z != FM_NODE4_EMPTY;
-                const bool b3 = box_hit6<FAST>(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, r, rp, tmax, t3) && cr.w != FM_NODE4_EMPTY;
-                k[4 * g + 0] = wide_key8(b0, t0, 4u * g + 0u);
-                k[4 * g + 1] = wide_key8(b1, t1, 4u * g + 1u);
-                k[4 * g + 2] = wide_key8(b2, t2, 4u * g + 2u);
-                k[4 * g + 3] = wide_key8(b3, t3, 4u * g + 3u);
-            }
-            sort2u(k[0], k[2]); sort2u(k[1], k[3]); sort2u(k[4], k[6]); sort2u(k[5], k[7]);
-            sort2u(k[0], k[4]); sort2u(k[1], k[5]); sort2u(k[2], k[6]); sort2u(k[3], k[7]);
-            sort2u(k[0], k[1]); sort2u(k[2], k[3]); sort2u(k[4], k[5]); sort2u(k[6], k[7]);
-            sort2u(k[2], k[4]); sort2u(k[3], k[5]);
-            sort2u(k[1], k[4]); sort2u(k[3], k[6]);
-            sort2u(k[1], k[2]); sort2u(k[3], k[4]); sort2u(k[5], k[6]);
-            auto child = [&](uint32_t key) -> int32_t {
-                const uint32_t i = key & 7u;
-                const int4 c = i < 4u ? ca : cb;
-                const uint32_t j = i & 3u;
-                return j == 0u ? c.x : (j == 1u ? c.y : (j == 2u ? c.z : c.w));
-            };
-            if (k[0] == MISS) {
-                node = pop();
-            } else {
-#pragma unroll
-                for (int q = 7; q >= 1; q--)
-                    if (k[q] != MISS) stack.push((uint32_t)child(k[q]));
-                node = child(k[0]);
-            }
-            if (node < 0 && leaf == FM_TRAV_NONE) { leaf = node; node = pop(); }
-            if (trav_break(rp, leaf)) break;
+    if constexpr (WIDTH == 2) {
+        const uint32_t off = (uint32_t)node << 6;
+        const float4 q0 = buf_load4(nrs, off), q1 = buf_load4(nrs, off + 16u), q2 = buf_load4(nrs, off + 32u);
+        const int4 cr = buf_load4i(nrs, off + 48u);
+        float t0, t1;
+        const bool h0 = box_test<BM>(q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, r, rp, tmax, t0);
+        const bool h1 = box_test<BM>(q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r, rp, tmax, t1);
+        if (h0 && h1) {
+            const bool first0 = ANY || t0 <= t1;
+            stack.push((uint32_t)(first0 ? cr.y : cr.x));
+            return first0 ? cr.x : cr.y;
         }
-        while (leaf != FM_TRAV_NONE) {
-            const int32_t start = FM_LEAF_START(leaf), count = FM_LEAF_COUNT(leaf);
-            if (ANY) {
-                for (int q = 0; q < count; q++) {
-                    count_slot<COUNT>(S, start + q, cn);
-                    if (test_slot_any(S, r, rp, start + q)) return true;
-                }
-            } else {
-                for (int q = 0; q < count; q++) { count_slot<COUNT>(S, start + q, cn); test_slot_closest(S, r, rp, start + q, h); }
-            }
-            if (node < 0) { leaf = node; node = pop(); }
-            else leaf = FM_TRAV_NONE;
+        if (h0) return cr.x;
+        if (h1) return cr.y;
+        return stack.pop();
+    } else if constexpr (WIDTH == 4) {
+        const float4* nd = reinterpret_cast<const float4*>(S.nodes4 + node);
+        const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
+        const int4 cr = reinterpret_cast<const int4*>(nd)[6];
+        /* all four tests unconditionally (children 0 and 1 always exist;
+         * an empty slot's box is masked after the test), so that the
+         * whole node is fetched at once */
+        float t0, t1, t2, t3;
+        const bool b0 = box_test<BM>(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, r, rp, tmax, t0);
+        const bool b1 = box_test<BM>(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, r, rp, tmax, t1);
+        const bool b2 = box_test<BM>(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, r, rp, tmax, t2) && cr.z != FM_NODE4_EMPTY;
+        const bool b3 = box_test<BM>(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, r, rp, tmax, t3) && cr.w != FM_NODE4_EMPTY;
+        uint32_t k0 = b0 ? (__float_as_uint(t0) & ~3u) | 0u : MISS;
+        uint32_t k1 = b1 ? (__float_as_uint(t1) & ~3u) | 1u : MISS;
+        uint32_t k2 = b2 ? (__float_as_uint(t2) & ~3u) | 2u : MISS;
+        uint32_t k3 = b3 ? (__float_as_uint(t3) & ~3u) | 3u : MISS;
+        sort2u(k0, k1); sort2u(k2, k3); sort2u(k0, k2); sort2u(k1, k3); sort2u(k1, k2);
+        if (k0 == MISS) return stack.pop();
+        if (k3 != MISS) stack.push((uint32_t)child_of4(cr, k3));
+        if (k2 != MISS) stack.push((uint32_t)child_of4(cr, k2));
+        if (k1 != MISS) stack.push((uint32_t)child_of4(cr, k1));
+        return child_of4(cr, k0);
+    } else {
+        const float4* nd = reinterpret_cast<const float4*>(S.nodes8 + node);
+        const int4 ca = reinterpret_cast<const int4*>(nd)[12], cb = reinterpret_cast<const int4*>(nd)[13];
+        uint32_t k[8];
+#pragma unroll
+        for (int g = 0; g < 2; g++) {
+            /* planes lo_x, hi_x, lo_y, hi_y, lo_z, hi_z: 8 floats each, lanes 4g..4g+3 */
+            const float4 lx = nd[0 + g], hx = nd[2 + g], ly = nd[4 + g], hy = nd[6 + g], lz = nd[8 + g], hz = nd[10 + g];
+            const int4 cr = g ? cb : ca;
+            float t0, t1, t2, t3;
+            const bool b0 = box_test<BM>(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, r, rp, tmax, t0) && cr.x != FM_NODE4_EMPTY;
+            const bool b1 = box_test<BM>(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, r, rp, tmax, t1) && cr.y != FM_NODE4_EMPTY;
+            const bool b2 = box_test<BM>(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, r, rp, tmax, t2) && cr.z != FM_NODE4_EMPTY;
+            const bool b3 = box_test<BM>(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, r, rp, tmax, t3) && cr.w != FM_NODE4_EMPTY;
+            k[4 * g + 0] = wide_key8(b0, t0, 4u * g + 0u);
+            k[4 * g + 1] = wide_key8(b1, t1, 4u * g + 1u);
+            k[4 * g + 2] = wide_key8(b2, t2, 4u * g + 2u);
+            k[4 * g + 3] = wide_key8(b3, t3, 4u * g + 3u);
         }
+        sort2u(k[0], k[2]); sort2u(k[1], k[3]); sort2u(k[4], k[6]); sort2u(k[5], k[7]);
+        sort2u(k[0], k[4]); sort2u(k[1], k[5]); sort2u(k[2], k[6]); sort2u(k[3], k[7]);
+        sort2u(k[0], k[1]); sort2u(k[2], k[3]); sort2u(k[4], k[5]); sort2u(k[6], k[7]);
+        sort2u(k[2], k[4]); sort2u(k[3], k[5]);
+        sort2u(k[1], k[4]); sort2u(k[3], k[6]);
+        sort2u(k[1], k[2]); sort2u(k[3], k[4]); sort2u(k[5], k[6]);
+        auto child = [&](uint32_t key) -> int32_t {
+            const uint32_t i = key & 7u;
+            const int4 c = i < 4u ? ca : cb;
+            const uint32_t j = i & 3u;
+            return j == 0u ? c.x : (j == 1u ? c.y : (j == 2u ? c.z : c.w));
+        };
+        if (k[0] == MISS) return stack.pop();
+#pragma unroll
+        for (int q = 7; q >= 1; q--)
+            if (k[q] != MISS) stack.push((uint32_t)child(k[q]));
+        return child(k[0]);
     }
-    return false;
+}
+
+/* Root box test: false if the ray misses the scene, else the traversal's
+ * first node (or leaf). */
+template <int BM, bool ANY>
+FMD bool trav_begin(const fm_dscene& S, const Ray& r, const RayPre& rp, const Hit& h, int32_t& node, int32_t& leaf) {
+    float tn;
+    node = FM_TRAV_NONE; leaf = FM_TRAV_NONE;
+    if (S.num_slots == 0) return false;
+    if (!box_test<BM>(S.root_box[0], S.root_box[1], S.root_box[2], S.root_box[3], S.root_box[4], S.root_box[5], r, rp,
+                      ANY ? r.tmax : h.t, tn))
+        return false;
+    node = S.root_ref;
+    if (node < 0) { leaf = node; node = FM_TRAV_NONE; }
+    return true;
+}
+
+/* One round of the speculative while-while loop: descend until every
+ * descending lane of the wave holds a leaf (trav_break), then test the
+ * leaves.  The ray is done when it returns true: ANY found a hit (`found`
+ * set), or node and leaf are both FM_TRAV_NONE. */
+template <int WIDTH, int BM, bool ANY, bool COUNT, bool SHORT>
+FMD bool trav_step(const fm_dscene& S, __amdgpu_buffer_rsrc_t nrs, const Ray& r, const RayPre& rp, Hit& h,
+                   TravStack<SHORT>& stack, int32_t& node, int32_t& leaf, bool& found, Cnt& cn) {
+    while (node != FM_TRAV_NONE && node >= 0) {
+        if (COUNT) cn.nodes++;
+        node = visit_node<WIDTH, BM, ANY>(S, nrs, node, r, rp, ANY ? r.tmax : h.t, stack);
+        if (node < 0 && leaf == FM_TRAV_NONE) { leaf = node; node = stack.pop(); }
+        if (trav_break(rp, leaf)) break;
+    }
+    while (leaf != FM_TRAV_NONE) {
+        const int32_t start = FM_LEAF_START(leaf), count = FM_LEAF_COUNT(leaf);
+        if (ANY) {
+            for (int k = 0; k < count; k++) {
+                count_slot<COUNT>(S, start + k, cn);
+                if (test_slot_any(S, r, rp, start + k)) { found = true; return true; }
+            }
+        } else {
+            for (int k = 0; k < count; k++) { count_slot<COUNT>(S, start + k, cn); test_slot_closest(S, r, rp, start + k, h); }
+        }
+        if (node < 0) { leaf = node; node = stack.pop(); }
+        else leaf = FM_TRAV_NONE;
+    }
+    return node == FM_TRAV_NONE && leaf == FM_TRAV_NONE;
+}
+
+/* A whole traversal of one ray (the closest hit into h, or whether any hit
+ * exists). */
+template <int WIDTH, int BM, bool ANY, bool COUNT, bool SHORT>
+FMD bool traverse(const fm_dscene& S, const Ray& r, const RayPre& rp, Hit& h, TravStack<SHORT>& stack, Cnt& cn) {
+    int32_t node, leaf;
+    if (!trav_begin<BM, ANY>(S, r, rp, h, node, leaf)) return false;
+    const __amdgpu_buffer_rsrc_t nrs = buf_rsrc(S.nodes, S.node_bytes);
+    bool found = false;
+    while (!trav_step<WIDTH, BM, ANY, COUNT, SHORT>(S, nrs, r, rp, h, stack, node, leaf, found, cn)) {
+    }
+    return found;
 }
 
 /* WIDTH = 2, 4 or 8 compiles one traversal only (the caller knows the
@@ -588,14 +564,14 @@ FMD Hit trace_closest_on(const fm_dscene& S, const Ray& r, TravStack<SHORT>& sta
     Cnt dummy;
     Cnt& c = COUNT ? *cn : dummy;
     if (WIDTH == 8 || (WIDTH == 0 && S.width == 8)) {
-        if (rp.fast) traverse8_impl<true, false, COUNT>(S, r, rp, h, stack, c);
-        else traverse8_impl<false, false, COUNT>(S, r, rp, h, stack, c);
+        if (rp.fast) traverse<8, BOX_FAST, false, COUNT>(S, r, rp, h, stack, c);
+        else traverse<8, BOX_EXACT, false, COUNT>(S, r, rp, h, stack, c);
     } else if (WIDTH == 4 || (WIDTH == 0 && S.width == 4)) {
-        if (rp.fast) traverse4_impl<true, false, COUNT>(S, r, rp, h, stack, c);
-        else traverse4_impl<false, false, COUNT>(S, r, rp, h, stack, c);
+        if (rp.fast) traverse<4, BOX_FAST, false, COUNT>(S, r, rp, h, stack, c);
+        else traverse<4, BOX_EXACT, false, COUNT>(S, r, rp, h, stack, c);
     } else {
-        if (rp.fast) traverse_impl<true, false, COUNT>(S, r, rp, h, stack, c);
-        else traverse_impl<false, false, COUNT>(S, r, rp, h, stack, c);
+        if (rp.fast) traverse<2, BOX_FAST, false, COUNT>(S, r, rp, h, stack, c);
+        else traverse<2, BOX_EXACT, false, COUNT>(S, r, rp, h, stack, c);
     }
     return h;
 }
@@ -609,13 +585,13 @@ FMD bool trace_any_on(const fm_dscene& S, const Ray& r, TravStack<SHORT>& stack,
     Hit h;
     h.t = r.tmax; h.slot = -1; h.kind = -1; h.tie = 0; h.half = 0; h.b0 = h.b1 = h.b2 = 0.0f; h.info = 0;
     if (WIDTH == 8 || (WIDTH == 0 && S.width == 8))
-        return rp.fast ? traverse8_impl<true, true, COUNT>(S, r, rp, h, stack, c)
-                       : traverse8_impl<false, true, COUNT>(S, r, rp, h, stack, c);
+        return rp.fast ? traverse<8, BOX_FAST, true, COUNT>(S, r, rp, h, stack, c)
+                       : traverse<8, BOX_EXACT, true, COUNT>(S, r, rp, h, stack, c);
     if (WIDTH == 4 || (WIDTH == 0 && S.width == 4))
-        return rp.fast ? traverse4_impl<true, true, COUNT>(S, r, rp, h, stack, c)
-                       : traverse4_impl<false, true, COUNT>(S, r, rp, h, stack, c);
-    return rp.fast ? traverse_impl<true, true, COUNT>(S, r, rp, h, stack, c)
-                   : traverse_impl<false, true, COUNT>(S, r, rp, h, stack, c);
+        return rp.fast ? traverse<4, BOX_FAST, true, COUNT>(S, r, rp, h, stack, c)
+                       : traverse<4, BOX_EXACT, true, COUNT>(S, r, rp, h, stack, c);
+    return rp.fast ? traverse<2, BOX_FAST, true, COUNT>(S, r, rp, h, stack, c)
+                   : traverse<2, BOX_EXACT, true, COUNT>(S, r, rp, h, stack, c);
 }
 
 /* The same with the full LDS stack at stk[i * stride]. */

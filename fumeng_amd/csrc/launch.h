@@ -90,6 +90,7 @@ struct WfArgs {
     int path_s0;
     uint32_t* spill;   /* short traversal stacks: spill_cap entries per trace lane (entry i of lane g at i * lanes + g) */
     int spill_cap;
+    unsigned char* cls;   /* [P (1 + D)]: shading class of the ray at each ray-queue position */
 };
 
 struct WfLaunch {
