@@ -135,6 +135,11 @@ FMD Hit load_hit(const WfArgs& A, int64_t r) {
     return h;
 }
 
+/* The parts of a ray slot that an environment-lit scene needs once the
+ * ray is traced: its direction, and whether it escaped (the hit slot). */
+FMD v3 load_dir(const WfArgs& A, int64_t r) { return xyz(reinterpret_cast<const float4*>(A.ray_d)[r]); }
+FMD int load_slot(const WfArgs& A, int64_t r) { return __float_as_int(reinterpret_cast<const float4*>(A.hit_a)[r].y); }
+
 FMD int64_t grid_id() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 FMD int64_t grid_stride() { return (int64_t)gridDim.x * blockDim.x; }
 /* one queue counter per 256 bytes, so that the atomics of different
@@ -414,6 +419,17 @@ FMD v3 resolve_direct(const fm_dscene& S, const WfArgs& A, int64_t p) {
         const v3 msf = xyz(f4v), beta = xyz(b4), ng = xyz(ng4);
         const float mpdf = f4v.w;
         const bool delta = b4.w != 0.0f;
+        if (LK == fm::LK_ENV) {
+            /* only an escaped ray adds light: its direction and slot */
+            if (load_slot(A, A.P + m) < 0 && S.env >= 0) {
+                const v3 d = load_dir(A, A.P + m);
+                float lp;
+                const v3 f = msf * env_radiance_pdf(S, S.envs[S.env], d, !delta, lp) * absdot(d, ng);
+                if (delta) direct += beta * (f / mpdf);
+                else direct += beta * (f / mpdf * ((mpdf * mpdf) / (mpdf * mpdf + lp * lp)));
+            }
+            continue;
+        }
         const Ray tr = load_ray(A, A.P + m);
         const Hit h = load_hit(A, A.P + m);
         if (h.slot < 0) {
@@ -424,7 +440,7 @@ FMD v3 resolve_direct(const fm_dscene& S, const WfArgs& A, int64_t p) {
                 if (delta) direct += beta * (f / mpdf);
                 else direct += beta * (f / mpdf * ((mpdf * mpdf) / (mpdf * mpdf + lp * lp)));
             }
-        } else if (LK != fm::LK_ENV) {
+        } else {
             const Surf ls = fill_hit(S, tr, h);
             if (ls.light >= 0) {
                 const fm_dlight& L = S.lights[ls.light];
@@ -533,7 +549,12 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
         uint64_t sb = 0;
         uint32_t mb = 0;
         if constexpr (C == fm::WC_END) {
-            if (fl & F_CONT) {
+            if (LK == fm::LK_ENV) {
+                /* an env-lit scene adds light here only for a camera ray that
+                 * escaped; deeper escapes are counted by the MisBSDF rays */
+                if ((fl & F_CONT) && depth == 1 && S.env >= 0 && load_slot(A, p) < 0)
+                    color = env_radiance(S, S.envs[S.env], -load_dir(A, p));
+            } else if (fl & F_CONT) {
                 const Ray tr = load_ray(A, p);
                 const Hit h = load_hit(A, p);
                 if (h.slot < 0) {
