@@ -320,8 +320,11 @@ FMD void trace_stream(const fm_dscene& S, const WfArgs& A, int64_t lo, int64_t h
  * slots for the closest hit, [n_ext, n_ext + n_sh) shadow records for the
  * any hit, so the two share the machine (and the latency of a short tail
  * iteration is paid once). */
+/* Six waves per SIMD (at most 80 VGPRs) in either stack form: the two-wide
+ * kernel then spills 20 bytes per lane, and its trace took 208 ms per frame
+ * against 217 ms at the five waves its 85 VGPRs allowed. */
 template <bool COUNT, int WIDTH, bool SHORT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SHORT ? 6 : 1))) k_wf_trace(fm_dscene S, WfArgs A) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_wf_trace(fm_dscene S, WfArgs A) {
     uint32_t* stk = wf_stack();
     const int stride = blockDim.x;
     /* SHORT: FM_SHORT_STACK entries per lane in LDS, the rest in this lane's
