@@ -337,17 +337,19 @@ def profiled(tag, kernel):
     1 GiB copy), bench-l2-hit = L2 hit rate (TCC_HIT / (TCC_HIT + TCC_MISS)),
     bench-valu-busy = share of cycles with VALU instructions issued,
     bench-launch-us = average launch duration in the counter run's trace.
-    None when no summary names the kernel."""
-    import glob
-    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
-        for line in open(path):
-            if line.startswith(tag + ":"):
-                name, _, value = line[len(tag) + 1:].strip().rpartition(" ")
-                if name == kernel:
-                    try:
-                        return float(value)
-                    except ValueError:
-                        return None
+    Read from the default workload's summary only (other workloads run the
+    same kernels at other sizes).  None when it does not name the kernel."""
+    path = os.path.join(ROOT, "profiles", "wavefront_icosphere_1080p_256spp.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(tag + ":"):
+            name, _, value = line[len(tag) + 1:].strip().rpartition(" ")
+            if name == kernel:
+                try:
+                    return float(value)
+                except ValueError:
+                    return None
     return None
 
 
