@@ -268,10 +268,12 @@ def stress_leg(fumeng_amd, ctx):
                             "reference_bvh_ms": info.build_ms_ref_bvh, "traversal_tree_ms": info.build_ms_trav_tree,
                             "tables_ms": info.build_ms_tables, "host_total_ms": info.build_ms_total,
                             "upload_ms": info.upload_ms, "host_gb": info.host_bytes / 1e9},
-           "trav_nodes": info.num_trav_nodes, "trav_stack": info.trav_stack, "device_builder": device,
+           "trav_width": info.trav_width, "trav_nodes": info.num_trav_nodes, "trav_stack": info.trav_stack,
+           "device_builder": device,
            "from_files": files}
-    log("stress: %.1f Mpaths/s, %.1f Mrays/s (%.0f ms per frame), built in %.1f s, %.2f GB in HBM"
-        % (res["mpaths_per_s"], res["mrays_per_s"], dt * 1e3, build, res["device_gb"]))
+    log("stress: %.1f Mpaths/s, %.1f Mrays/s (%.0f ms per frame, traversal %.0f ms), %d-wide tree, built in %.1f s, "
+        "%.2f GB in HBM" % (res["mpaths_per_s"], res["mrays_per_s"], dt * 1e3, st.trace_ms, info.trav_width, build,
+                           res["device_gb"]))
     log("stress build on %d threads: geometry %.0f ms, reference BVH %.0f ms, traversal tree %.0f ms, tables %.0f ms, "
         "upload %.0f ms; host copy %.2f GB" % (info.build_threads, info.build_ms_geometry, info.build_ms_ref_bvh,
                                                info.build_ms_trav_tree, info.build_ms_tables, info.upload_ms,
@@ -379,8 +381,9 @@ def main():
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--mode", choices=["megakernel", "wavefront"], default="wavefront")
-    ap.add_argument("--bvh-width", type=int, choices=[2, 4, 8], default=2,
-                    help="traversal tree width (the closest hit does not depend on it)")
+    ap.add_argument("--bvh-width", type=int, choices=[0, 2, 4, 8], default=0,
+                    help="traversal tree width; 0 (default): two-wide while the tree and primitive records fit in "
+                         "the 256 MB last-level cache, else four-wide")
     ap.add_argument("--bvh-builder", choices=["host", "device"], default="host",
                     help="traversal tree builder: binned SAH on the host threads or PLOC on the GPU")
     ap.add_argument("--no-stress", dest="stress", action="store_false",
@@ -559,7 +562,7 @@ def main():
            "config": {"workload": "displaced icosphere %d tris + ground, 64 Disney materials, 2048x1024 HDR env, "
                                   "thin lens, %dx%d, %d spp, depth 12, %s" % (20 * args.freq ** 2, W, H, spp, args.mode),
                       "mode": args.mode, "width": W, "height": H, "spp": spp, "depth": 12,
-                      "bvh_width": args.bvh_width, "bvh_builder": args.bvh_builder},
+                      "bvh_width": info.trav_width, "bvh_builder": args.bvh_builder},
            "mrays_per_s": mrays, "paths": paths, "rays": rays, "other_form": other, "box": box, "batch": batch, "adjoint": adjoint,
            "stress": stress, "roofline": roofline,
            "roofline_valu": box["roofline_valu"] if box else None, "cpu_baseline": cpu}

@@ -75,7 +75,9 @@ class Context:
             pass
 
     def set_bvh_width(self, width):
-        """Traversal tree width (2, the default, 4 or 8) of the scenes created afterwards."""
+        """Traversal tree width (2, 4 or 8) of the scenes created afterwards;
+        0 (the default) takes 2 while the two-wide tree and the primitive
+        records fit in the device's 256 MB last-level cache, else 4."""
         _check(lib().fm_set_bvh_width(self.h, int(width)), self.h)
 
     def set_bvh_builder(self, builder):
@@ -372,7 +374,7 @@ def host_bvh(desc):
     return boxes, leaf, a, b, order
 
 
-def host_scene_info(desc, bvh_width=2):
+def host_scene_info(desc, bvh_width=0):
     """fm_scene_info of the whole host build of a SceneDesc (no GPU needed)."""
     d, keep = desc.to_c()
     i = A.fm_scene_info()

@@ -36,6 +36,7 @@ FM_NOALLOC_MAX_PIXELS = 262144   # include/fm_gpu.h
 SHEAR_AXIS = {"truncated": 0, "float": 1}
 FM_RENDER_TIME_TRACE = 1
 FM_RENDER_GRAPH = 2
+FM_BVH_WIDTH_AUTO = 0
 FM_DPARAM_PER_MATERIAL = 6
 DPARAM_NAMES = ["base_r", "base_g", "base_b", "roughness", "metallic", "specular"]
 

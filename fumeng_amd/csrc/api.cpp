@@ -131,7 +131,7 @@ struct fm_ctx {
     int film_w = 0, film_h = 0;
     int64_t film_spp = 0;     /* samples accumulated into the device film */
     int counting = 0;
-    int bvh_width = 2;        /* traversal tree width of scenes created next */
+    int bvh_width = FM_BVH_WIDTH_AUTO;   /* traversal tree width of scenes created next */
     int bvh_builder = FM_BUILDER_HOST;   /* traversal tree builder of scenes created next */
     int trace_stack = FM_TRACE_STACK_AUTO;   /* wavefront trace kernel's stack form */
     int64_t path_budget = 0;  /* bytes of wavefront path state a pass may use (fixed per context) */
@@ -848,7 +848,8 @@ int fm_autotune(fm_ctx* ctx, fm_scene* s, const fm_render_params* base, int32_t 
 
 int fm_set_bvh_width(fm_ctx* ctx, int32_t width) {
     if (!ctx) return fail(nullptr, FM_ERR_INVALID, "NULL ctx");
-    if (width != 2 && width != 4 && width != 8) return fail(ctx, FM_ERR_INVALID, "BVH width must be 2, 4 or 8");
+    if (width != FM_BVH_WIDTH_AUTO && width != 2 && width != 4 && width != 8)
+        return fail(ctx, FM_ERR_INVALID, "BVH width must be 0 (automatic), 2, 4 or 8");
     ctx->bvh_width = width;
     return FM_OK;
 }
@@ -1585,7 +1586,8 @@ int fm_host_build_bvh(const fm_scene_desc* desc, int64_t* num_nodes, float* boxe
 
 int fm_host_scene_info(const fm_scene_desc* desc, int32_t bvh_width, fm_scene_info* out) {
     if (!desc || !out) return fail(nullptr, FM_ERR_INVALID, "NULL argument");
-    if (bvh_width != 2 && bvh_width != 4 && bvh_width != 8) return fail(nullptr, FM_ERR_INVALID, "BVH width must be 2, 4 or 8");
+    if (bvh_width != FM_BVH_WIDTH_AUTO && bvh_width != 2 && bvh_width != 4 && bvh_width != 8)
+        return fail(nullptr, FM_ERR_INVALID, "BVH width must be 0 (automatic), 2, 4 or 8");
     return guarded(nullptr, [&]() -> int {
         fmh::HostScene hs;
         hs.trav_width = bvh_width;

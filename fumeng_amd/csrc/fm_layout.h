@@ -38,6 +38,9 @@
 #define FM_STACK_DEPTH 32   /* depth limit of the binary traversal trees */
 #define FM_STACK_MAX 64     /* LDS traversal stack entries per lane, at most */
 #define FM_SHORT_STACK 16   /* LDS entries per lane of the short (spilling) traversal stack; a power of two */
+/* MI355X's Infinity Cache (MALL), the last level before HBM: traversal
+ * data that fits stays cache-resident across a frame's bounces. */
+#define FM_MALL_BYTES (256ull << 20)
 
 /* Traversal trees are two-wide (fm_dnode), four-wide (fm_dnode4) or
  * eight-wide (fm_dnode8). */

@@ -645,6 +645,11 @@ void make_camera(const fm_camera_desc& c, fm_dcamera& dc) {
  * pushes at most (children - 1) refs per node on the way down, so the stack
  * never holds more than the largest sum of (children - 1) over a
  * root-to-leaf path (one per binary level for W = 2). */
+int auto_width(size_t binary_nodes, size_t slots) {
+    const size_t bytes = binary_nodes * sizeof(fm_dnode) + slots * 12 * sizeof(float);
+    return bytes > (size_t)FM_MALL_BYTES ? 4 : 2;
+}
+
 void collapse_traversal(HostScene& hs, int W) {
     if (W != 2 && W != 4 && W != 8) throw BuildError(FM_ERR_INVALID, "traversal tree width must be 2, 4 or 8");
     hs.nodes4.clear();
@@ -941,8 +946,9 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
     std::vector<TNode> tn;
     hs.device_tree = hs.builder == FM_BUILDER_DEVICE && d.accelerator != FM_ACCEL_BVH_REFERENCE_TREE;
     if (hs.builder != FM_BUILDER_HOST && hs.builder != FM_BUILDER_DEVICE) throw BuildError(FM_ERR_INVALID, "unknown BVH builder");
-    if (hs.trav_width != 2 && hs.trav_width != 4 && hs.trav_width != 8)
+    if (hs.trav_width != FM_BVH_WIDTH_AUTO && hs.trav_width != 2 && hs.trav_width != 4 && hs.trav_width != 8)
         throw BuildError(FM_ERR_INVALID, "traversal tree width must be 2, 4 or 8");
+    if (hs.device_tree && hs.trav_width == FM_BVH_WIDTH_AUTO) hs.trav_width = 2;
     if (hs.device_tree && hs.trav_width != 2) throw BuildError(FM_ERR_UNSUPPORTED, "the device builder makes two-wide trees only");
     if (hs.device_tree) {
         /* slots in geometry order; the device builds the tree and permutes them */
@@ -1032,7 +1038,7 @@ void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh) {
             hs.root_ref = ref_of(0);   /* node 0 of every width when the root is inner */
             for (int k = 0; k < 6; k++) hs.root_box[k] = node_boxes[(size_t)k];
         }
-        collapse_traversal(hs, hs.trav_width);
+        collapse_traversal(hs, hs.trav_width == FM_BVH_WIDTH_AUTO ? auto_width(hs.nodes.size(), G) : hs.trav_width);
     }
 
     lap(hs.report.trav_tree_ms);

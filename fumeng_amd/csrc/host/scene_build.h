@@ -91,6 +91,10 @@ void make_camera(const fm_camera_desc& c, fm_dcamera& dc);
  * from the binary tree in hs.nodes: sets nodes4 / nodes8, trav_width and
  * trav_stack; the slots and their order do not change.  Throws BuildError. */
 void collapse_traversal(HostScene& hs, int W);
+/* The width FM_BVH_WIDTH_AUTO takes for a two-wide tree of binary_nodes
+ * nodes over `slots` primitive records: 2 while nodes and records fit in
+ * the device's last-level cache (FM_MALL_BYTES), else 4. */
+int auto_width(size_t binary_nodes, size_t slots);
 
 /* Throws BuildError.  only_bvh stops after the reference BVH (host tests). */
 void build_host_scene(const fm_scene_desc& d, HostScene& hs, bool only_bvh = false);

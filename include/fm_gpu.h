@@ -457,12 +457,16 @@ int fm_autotune(fm_ctx* ctx, fm_scene* scene, const fm_render_params* base, int3
 
 /* Enables the node / primitive counters in later renders (0 = off). */
 int fm_set_counting(fm_ctx* ctx, int enabled);
-/* Width (2, 4 or 8, default 2) of the traversal tree of the scenes created
- * on this context afterwards.  The four- and eight-wide trees are the binary
- * one with nodes merged (fewer node visits, more box tests per visit); the
- * closest hit does not depend on the width (ties are resolved by reference
- * slot).  No counterpart in the reference (whose BVHAggregate is binary,
- * bvh_aggregate.cpp:38-79): a device traversal parameter. */
+/* Width (2, 4 or 8) of the traversal tree of the scenes created on this
+ * context afterwards.  FM_BVH_WIDTH_AUTO (0, the default) takes 2 while the
+ * two-wide tree and the primitive records fit in the device's 256 MB
+ * last-level cache, else 4 (device-built trees are always two-wide).  The
+ * four- and eight-wide trees are the binary one with nodes merged (fewer
+ * node visits, more box tests per visit); the closest hit does not depend on
+ * the width (ties are resolved by reference slot).  No counterpart in the
+ * reference (whose BVHAggregate is binary, bvh_aggregate.cpp:38-79): a
+ * device traversal parameter. */
+enum { FM_BVH_WIDTH_AUTO = 0 };
 int fm_set_bvh_width(fm_ctx* ctx, int32_t width);
 /* Bytes of wavefront path state one pass may hold on this context (0 =
  * the default: 96 GB, or half the device memory free at fm_init if that is
@@ -593,7 +597,7 @@ int fm_host_build_bvh(const fm_scene_desc* desc, int64_t* num_nodes,
                       float* boxes, int32_t* is_leaf, int64_t* idx_a,
                       int64_t* idx_b, int64_t* num_geoms, int64_t* order);
 /* The whole host build of fm_scene_create (tables, traversal tree of the
- * given width, primitive records) without a device; fills the facts
+ * given width or FM_BVH_WIDTH_AUTO, primitive records) without a device; fills the facts
  * fm_scene_get_info reports, device_bytes = 0.  For tools and CPU tests of
  * the builder; no counterpart in the reference (SceneLoader::ParseScene,
  * scene_loader.h:293-302, builds and renders in one object). */

@@ -128,7 +128,7 @@ def test_renders_are_deterministic(gpu_ctx):
         try:
             g = gpu_ctx.scene(s)
         finally:
-            gpu_ctx.set_bvh_width(2)
+            gpu_ctx.set_bvh_width(0)
         for mode in (A.FM_MODE_WAVEFRONT, A.FM_MODE_MEGAKERNEL):
             ref = g.render(spp=256, seed=3, mode=mode)[0]
             for _ in range(8):

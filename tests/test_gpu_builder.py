@@ -204,7 +204,7 @@ def test_device_builder_arguments_and_empty_scene(gpu_ctx):
         with pytest.raises(FumengError):
             gpu_ctx.scene(s2)
     finally:
-        gpu_ctx.set_bvh_width(2)
+        gpu_ctx.set_bvh_width(0)
         gpu_ctx.set_bvh_builder("host")
     s2.accelerator = "reference_tree"      # the reference topology is kept as asked
     g = _scene_with(gpu_ctx, s2, "device")

@@ -126,7 +126,7 @@ def test_bvh_width_does_not_change_hits_or_images(gpu_ctx):
         with pytest.raises(FumengError):
             gpu_ctx.set_bvh_width(3)
     finally:
-        gpu_ctx.set_bvh_width(2)
+        gpu_ctx.set_bvh_width(0)
     i2, i4, i8 = g[2].info(), g[4].info(), g[8].info()
     assert (i2.trav_width, i2.trav_node_bytes, i4.trav_width, i4.trav_node_bytes) == (2, 64, 4, 128)
     assert (i8.trav_width, i8.trav_node_bytes) == (8, 256)

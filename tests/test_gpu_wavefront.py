@@ -160,7 +160,7 @@ def test_short_trace_stack_gives_the_same_films(gpu_ctx, width):
         gpu_ctx.set_bvh_width(width)
         g = gpu_ctx.scene(s)
     finally:
-        gpu_ctx.set_bvh_width(2)
+        gpu_ctx.set_bvh_width(0)
     assert g.info().trav_stack > 16
     out = {}
     try:

@@ -81,6 +81,31 @@ def test_host_builder_equals_reference_topology(freq):
         assert np.array_equal(a, b)
 
 
+def test_automatic_width_follows_the_last_level_cache():
+    """FM_BVH_WIDTH_AUTO (the default): a scene whose two-wide tree and
+    primitive records (64 B per node, 48 B per slot) fit in 256 MiB keeps
+    the two-wide tree; a 5.8 M-triangle grid, whose records alone take
+    277 MB, gets the four-wide one.  Either way the tree is the one that
+    width gives when asked for."""
+    small = mesh_scene(freq=20)
+    a, b = fumeng_amd.host_scene_info(small, 0), fumeng_amd.host_scene_info(small, 2)
+    assert a.trav_width == 2 and a.trav_hash == b.trav_hash
+    assert b.num_trav_nodes * 64 + b.num_geometries * 48 < (256 << 20)
+    n = 1700
+    g = np.arange(n + 1, dtype=np.float32)
+    x, y = np.meshgrid(g, g, indexing="ij")
+    pos = np.stack([x.ravel(), y.ravel(), np.zeros(x.size, np.float32)], 1).astype(np.float32)
+    k = (np.arange(n)[:, None] * (n + 1) + np.arange(n)[None, :]).ravel()
+    tris = np.concatenate([np.stack([k, k + n + 1, k + 1], 1), np.stack([k + 1, k + n + 1, k + n + 2], 1)]).astype(np.int32)
+    big = SceneDesc()
+    big.camera = dict(small.camera)
+    big.renderer = dict(small.renderer)
+    big.add_mesh_object(big.add_mesh(pos, tris), material=big.add_lambert(0.5))
+    c = fumeng_amd.host_scene_info(big, 0)
+    assert c.num_geometries == 2 * n * n and c.num_geometries * 48 > (256 << 20)
+    assert c.trav_width == 4 and c.trav_node_bytes == 128
+
+
 @pytest.mark.parametrize("name", ["mesh", "box", "stress_small"])
 def test_host_build_of_both_traversal_widths(name):
     """The whole host build (no device) for every tree width: a two-wide
