@@ -126,7 +126,9 @@ def batch_leg(fumeng_amd, ctx):
     from fumeng_amd import scenes
     jobs = scenes.batch_jobs(64, seed=4)
     box = ctx.scene(scenes.box_scene())
-    box.render_batch([dict(j, spp=1) for j in jobs[:2]], read=False)   # warm-up (code objects)
+    # warm-up: code objects, and the path-state buffers grown to this
+    # batch's pass size (a one-time allocation of up to the context's budget)
+    box.render_batch(jobs, read=False, graph=True)
     t0 = time.perf_counter()
     _, done, st = box.render_batch(jobs, read=False, graph=True)
     wall = time.perf_counter() - t0
@@ -460,7 +462,8 @@ def main():
         trace_ms += st.trace_ms
         rays += st.rays
         paths += st.paths
-        log("step %d: %.1f ms (traversal launches %.1f ms), %.1f Mrays" % (i, st.kernel_ms, st.trace_ms, st.rays / 1e6))
+        log("step %d: %.1f ms (traversal launches %.1f ms), %.1f Mrays, %d pass(es) of %d iterations"
+            % (i, st.kernel_ms, st.trace_ms, st.rays / 1e6, st.passes, st.iterations))
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed * 1e3 / args.steps
     mpaths = W * H * spp * args.steps / elapsed / 1e6
@@ -500,7 +503,7 @@ def main():
                      if mode == _abi.FM_MODE_WAVEFRONT else "k_render_mega<false>")
             traffic = profiled("bench-traffic", kname) if default_run else None
             traffic = int(traffic) if traffic is not None else None
-            launch_ms = trace_ms / args.steps / max(1, 2 * (12 + 1))   # 2 passes x 13 iterations per frame
+            launch_ms = trace_ms / args.steps / max(1, st.passes * st.iterations)   # trace launches per frame
             # `achieved` / `frac` rate the bytes the traversal requests; the
             # measured HBM rate (counter traffic per launch over the launch
             # time) and the L2 hit rate say how much of that reaches HBM
@@ -508,7 +511,8 @@ def main():
             if traffic is not None:
                 hbm_gbs = traffic / (launch_ms * 1e-3) / 1e9
                 measured = {"hbm_gbs": hbm_gbs, "hbm_frac": hbm_gbs / HBM_PEAK_GBS,
-                            "traffic_over_requested": traffic / (bytes_per_ray * rays / args.steps / 26.0),
+                            "traffic_over_requested": traffic / (bytes_per_ray * rays / args.steps /
+                                                                 max(1, st.passes * st.iterations)),
                             "l2_hit": profiled("bench-l2-hit", kname),
                             "valu_busy": profiled("bench-valu-busy", kname)}
             roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -148,7 +148,7 @@ class fm_render_stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("paths", C.c_int64), ("rays", C.c_int64),
                 ("node_visits", C.c_int64), ("prim_tests", C.c_int64), ("trace_ms", C.c_double),
                 ("trace_block", C.c_int32), ("trace_short_stack", C.c_int32), ("sphere_tests", C.c_int64),
-                ("rect_tests", C.c_int64)]
+                ("rect_tests", C.c_int64), ("passes", C.c_int32), ("iterations", C.c_int32)]
 
 
 class fm_job(C.Structure):

@@ -376,7 +376,7 @@ struct WfJob {
 
 struct WfRunOut {
     double kernel_ms = 0.0, trace_ms = 0.0;
-    int passes = 0, units = 0;
+    int passes = 0, units = 0, iterations = 0;
     int trace_block = 0, short_stack = 0;
     std::vector<double> job_done_ms;
 };
@@ -402,10 +402,11 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
     if (path_out && jobs.size() != 1) throw std::invalid_argument("per-path output needs exactly one job");
     const int NL = s->d.num_lights;
     const int64_t per_path = 100 + 65 * (1 + (int64_t)D) + 56 * (int64_t)D + 57 * (int64_t)D * std::max(NL, 1);
-    /* Passes as large as the context's path-state budget allows (96 GB on
-     * an idle device): fewer passes mean fewer nearly empty tail iterations
-     * (1920x1080 at 256 spp: 8 passes took 522 ms, 2 took 493).  The index
-     * bound keeps every queue entry inside int32. */
+    /* Passes as large as the context's path-state budget allows (192 GB on
+     * an idle 288 GB device): fewer passes mean fewer nearly empty tail
+     * iterations (1920x1080 at 256 spp: 8 passes took 522 ms, 2 took 493;
+     * later 2 passes 383-395 ms, 1 pass 378 ms).  The index bound keeps
+     * every queue entry inside int32. */
     const int64_t idx_cap = (((int64_t)1 << 31) - 1) / std::max<int64_t>(1 + D, (int64_t)D * (std::max(NL, 1) + 1));
     const int64_t budget = std::max<int64_t>(1, std::min<int64_t>(idx_cap, ctx->path_budget / per_path));
     std::vector<fm::WfUnit> units;
@@ -571,6 +572,7 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
             R.trace_ms += t;
         }
     R.passes = (int)npass;
+    R.iterations = W_.n_iter;
     R.units = (int)units.size();
     R.trace_block = L.block_trace;
     R.short_stack = L.short_stack ? 1 : 0;
@@ -582,14 +584,16 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
             R.job_done_ms[j] = (double)(clk[last_pass[j] + 1] - clk[0]) / (double)ctx->wall_khz;
 }
 
-/* Wavefront path state a pass may hold: 96 GB (a third of a 288 GB
- * device), or half the memory free when the context is created if that is
- * less.  It is fixed per context, so a frame's pass split, and with it the
- * film's FP32 sum order, does not change from call to call. */
+/* Wavefront path state a pass may hold: 192 GB (two thirds of a 288 GB
+ * device: a 1920x1080, 256-spp frame in one pass), or two thirds of the
+ * memory free when the context is created if that is less; the rest is
+ * left to scenes, films and other tenants.  It is fixed per context, so a
+ * frame's pass split, and with it the film's FP32 sum order, does not
+ * change from call to call. */
 int64_t default_path_budget() {
     size_t free_b = 0, total_b = 0;
     HIPCHECK(hipMemGetInfo(&free_b, &total_b));
-    return std::max<int64_t>((int64_t)1 << 26, std::min<int64_t>((int64_t)96e9, (int64_t)(free_b / 2)));
+    return std::max<int64_t>((int64_t)1 << 26, std::min<int64_t>((int64_t)192e9, (int64_t)(free_b / 3 * 2)));
 }
 
 /* Device address of fm_render_noalloc's static working array, looked up
@@ -909,7 +913,7 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
         const int64_t npix = (int64_t)W * H;
         const bool wave = p->mode == FM_MODE_WAVEFRONT;
         double trace_ms = 0.0;
-        int trace_block = 0, short_stack = 0;
+        int trace_block = 0, short_stack = 0, passes = 0, iterations = 0;
         /* megakernel: samples per work item (the wavefront sizes its passes
          * from a memory budget unless chunk caps the samples per pass) */
         A.chunk = p->chunk > 0 ? std::min(p->chunk, spp) : std::min(spp, 4);
@@ -962,6 +966,8 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
             trace_ms = R.trace_ms;
             trace_block = R.trace_block;
             short_stack = R.short_stack;
+            passes = R.passes;
+            iterations = R.iterations;
         }
         HIPCHECK(hipStreamSynchronize(ctx->stream));
         ctx->film_spp += spp;
@@ -980,6 +986,8 @@ int fm_render(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, fm_film* film
             stats->rect_tests = (int64_t)c[5];
             stats->trace_block = trace_block;
             stats->trace_short_stack = short_stack;
+            stats->passes = passes;
+            stats->iterations = iterations;
             if (p->flags & FM_RENDER_TIME_TRACE) stats->trace_ms = wave ? trace_ms : ms;
         }
         if (film && (film->color || film->albedo || film->normal || film->color_sum || film->sumsq)) return fm_film_read(ctx, film);

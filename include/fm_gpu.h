@@ -298,6 +298,8 @@ typedef struct fm_render_stats {
     int32_t trace_short_stack;  /* wavefront: 1 if it ran with the short stack */
     int64_t sphere_tests;    /* of prim_tests (counting only): spheres */
     int64_t rect_tests;      /* of prim_tests (counting only): rectangles (two triangle tests each) */
+    int32_t passes;          /* wavefront: passes the frame ran in (the path-state budget splits it) */
+    int32_t iterations;      /* wavefront: bounce iterations per pass (trace launches per pass) */
 } fm_render_stats;
 
 /* Renders into the context's device film (FP64 sums) and, when `film` has
@@ -469,8 +471,8 @@ int fm_set_counting(fm_ctx* ctx, int enabled);
 enum { FM_BVH_WIDTH_AUTO = 0 };
 int fm_set_bvh_width(fm_ctx* ctx, int32_t width);
 /* Bytes of wavefront path state one pass may hold on this context (0 =
- * the default: 96 GB, or half the device memory free at fm_init if that is
- * less).  A frame that needs more is rendered in several passes of equal
+ * the default: 192 GB, or two thirds of the device memory free at fm_init
+ * if that is less).  A frame that needs more is rendered in several passes of equal
  * sample counts; results do not depend on the split except for the order of
  * FP32 additions in the film.  No counterpart in the reference. */
 int fm_set_path_budget(fm_ctx* ctx, int64_t bytes);

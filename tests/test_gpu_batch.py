@@ -145,6 +145,7 @@ def test_lowered_path_budget_splits_a_frame_into_equal_passes(gpu_ctx):
         films, _, st = g.render_batch([{"camera": None, "seed": 3}], chunk=0)
         assert st.passes == 4 and st.units == 4
         wf = g.render(seed=3, mode=A.FM_MODE_WAVEFRONT, chunk=0)
+        assert wf[4].passes == 4 and wf[4].iterations == 9   # depth 8: max(depth, 1) + 1 iterations per pass
     finally:
         gpu_ctx.set_path_budget(0)
     ref, _, st6 = g.render_batch([{"camera": None, "seed": 3}], chunk=6)
