@@ -351,51 +351,74 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
 }
 
 /* The class queues of an iteration from the shading classes the streaming
- * trace wrote per ray-queue position, in queue order (the order the
- * round-based trace kernel pushes them in): one round of atomics per wave
- * for all four queues. */
+ * trace wrote per ray-queue position, in queue order (the order the pushes
+ * of a round-based trace would give).  A wave takes 1,024 consecutive
+ * positions, 16 per lane (one 16-byte load of classes and four of ray
+ * slots per lane), counts each class per lane, scans the four counts over
+ * the wave as one packed 64-bit value, reserves the four ranges with one
+ * atomic per class and writes each lane's entries after its predecessors'.
+ * (A 64-lane round of ballots per 64 positions took 2.0 ms on 265 M camera
+ * rays against 0.5 ms for the bytes alone.) */
 __global__ void __launch_bounds__(256) k_wf_classify(WfArgs A) {
+    constexpr int PER = 16, CHUNK = 64 * PER;
     const int it = A.iter;
-    const unsigned n = *cnt(A, it, fm::WQ_EXT);
+    const int64_t n = *cnt(A, it, fm::WQ_EXT);
     const int* q = A.q_ext[it & 1];
     int* qc = A.q_cls[it & 1];
     unsigned* const cc[fm::WQ_NCLS] = {cnt(A, it, fm::WQ_CLS), cnt(A, it, fm::WQ_CLS + 1), cnt(A, it, fm::WQ_CLS + 2),
                                        cnt(A, it, fm::WQ_CLS + 3)};
-    int* const qq[fm::WQ_NCLS] = {qc, qc + (size_t)A.P, qc + 2 * (size_t)A.P, qc + 3 * (size_t)A.P};
-    /* a wave takes CLS_CHUNK consecutive positions at a time: it counts
-     * each class over them, reserves the ranges with one atomic per class,
-     * then writes (whole waves iterate together) */
-    constexpr int CLS_CHUNK = 16 * 64;
     const int64_t waves = grid_stride() >> 6;
     const unsigned lane = lane_id();
-    for (int64_t b = (grid_id() >> 6) * CLS_CHUNK; b < (int64_t)n; b += waves * CLS_CHUNK) {
-        const int64_t e = min(b + CLS_CHUNK, (int64_t)n);
-        unsigned tot[fm::WQ_NCLS] = {0u, 0u, 0u, 0u};
-        for (int64_t j = b; j < e; j += 64) {
-            const int64_t i = j + lane;
-            const int c = i < e ? (int)(signed char)A.cls[i] : -1;
+    for (int64_t b = (grid_id() >> 6) * CHUNK; b < n; b += waves * CHUNK) {
+        const int64_t p0 = b + (int64_t)lane * PER;
+        const int nv = (int)max<int64_t>(0, min<int64_t>(PER, n - p0));   /* valid positions of this lane */
+        unsigned char c[PER];
+        int r[PER];
+        if (nv == PER) {   /* p0 is a multiple of 16: aligned 16- and 64-byte loads */
+            const uint4 cv = *reinterpret_cast<const uint4*>(A.cls + p0);
+            const unsigned w[4] = {cv.x, cv.y, cv.z, cv.w};
 #pragma unroll
-            for (int k = 0; k < fm::WQ_NCLS; k++) tot[k] += (unsigned)__popcll(__ballot(c == k));
+            for (int j = 0; j < PER; j++) c[j] = (unsigned char)(w[j >> 2] >> (8 * (j & 3)));
+#pragma unroll
+            for (int j = 0; j < PER; j += 4) {
+                const int4 rv = *reinterpret_cast<const int4*>(q + p0 + j);
+                r[j] = rv.x; r[j + 1] = rv.y; r[j + 2] = rv.z; r[j + 3] = rv.w;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < PER; j++) {
+                c[j] = j < nv ? A.cls[p0 + j] : (unsigned char)0xff;
+                r[j] = j < nv ? q[p0 + j] : 0;
+            }
         }
+        unsigned long long mine = 0ull;   /* four 16-bit class counts */
+#pragma unroll
+        for (int j = 0; j < PER; j++)
+            if (c[j] < fm::WQ_NCLS) mine += 1ull << (16 * c[j]);
+        unsigned long long incl = mine;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const unsigned long long v = __shfl_up(incl, d, 64);
+            if ((int)lane >= d) incl += v;
+        }
+        const unsigned long long tot = __shfl(incl, 63, 64);
+        unsigned long long excl = incl - mine;
         unsigned base[fm::WQ_NCLS] = {0u, 0u, 0u, 0u};
         if (lane == 0u) {
 #pragma unroll
-            for (int k = 0; k < fm::WQ_NCLS; k++)
-                if (tot[k]) base[k] = atomicAdd(cc[k], tot[k]);
+            for (int k = 0; k < fm::WQ_NCLS; k++) {
+                const unsigned t = (unsigned)(tot >> (16 * k)) & 0xffffu;
+                if (t) base[k] = atomicAdd(cc[k], t);
+            }
         }
 #pragma unroll
-        for (int k = 0; k < fm::WQ_NCLS; k++) base[k] = __builtin_amdgcn_readfirstlane(__shfl(base[k], 0, 64));
-        for (int64_t j = b; j < e; j += 64) {
-            const int64_t i = j + lane;
-            const int c = i < e ? (int)(signed char)A.cls[i] : -1;
+        for (int k = 0; k < fm::WQ_NCLS; k++) base[k] = __shfl(base[k], 0, 64) + ((unsigned)(excl >> (16 * k)) & 0xffffu);
 #pragma unroll
-            for (int k = 0; k < fm::WQ_NCLS; k++) {
-                const unsigned long long m = __ballot(c == k);
-                if (c == k) {
-                    const unsigned below = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-                    qq[k][base[k] + below] = q[i];
-                }
-                base[k] += (unsigned)__popcll(m);
+        for (int j = 0; j < PER; j++) {
+            const int k = c[j];
+            if (k < fm::WQ_NCLS) {
+                const unsigned slot = k == 0 ? base[0]++ : (k == 1 ? base[1]++ : (k == 2 ? base[2]++ : base[3]++));
+                qc[(size_t)k * A.P + slot] = r[j];
             }
         }
     }
