@@ -352,15 +352,55 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
 
 /* The class queues of an iteration from the shading classes the streaming
  * trace wrote per ray-queue position, in queue order (the order the pushes
- * of a round-based trace would give).  A wave takes 1,024 consecutive
- * positions, 16 per lane (one 16-byte load of classes and four of ray
- * slots per lane), counts each class per lane, scans the four counts over
- * the wave as one packed 64-bit value, reserves the four ranges with one
- * atomic per class and writes each lane's entries after its predecessors'.
- * (A 64-lane round of ballots per 64 positions took 2.0 ms on 265 M camera
- * rays against 0.5 ms for the bytes alone.) */
+ * of a round-based trace would give).  Each wave owns one contiguous range
+ * of positions: it counts the classes there first and reserves its four
+ * queue ranges with one atomic per class, then writes.  The range goes in
+ * chunks of 1,024 positions, 16 per lane (one 16-byte load of classes and
+ * four of ray slots per lane); a chunk's four class counts are scanned over
+ * the wave as one packed 64-bit value, and its entries are staged in LDS in
+ * queue order and written out by consecutive lanes to consecutive
+ * addresses.  (With four atomics per chunk, all waves contending for the
+ * same four counters, a launch over 531 M camera rays took 3.9 ms.) */
+constexpr int CLS_PER = 16, CLS_CHUNK = 64 * CLS_PER;
+
+/* Classes and ray slots of the lane's 16 positions of the chunk at b. */
+FMD int cls_load(const WfArgs& A, const int* q, int64_t b, int64_t n, unsigned lane, unsigned char (&c)[CLS_PER],
+                 int (&r)[CLS_PER], bool slots) {
+    const int64_t p0 = b + (int64_t)lane * CLS_PER;
+    const int nv = (int)max<int64_t>(0, min<int64_t>(CLS_PER, n - p0));
+    if (nv == CLS_PER) {   /* p0 is a multiple of 16: aligned 16- and 64-byte loads */
+        const uint4 cv = *reinterpret_cast<const uint4*>(A.cls + p0);
+        const unsigned w[4] = {cv.x, cv.y, cv.z, cv.w};
+#pragma unroll
+        for (int j = 0; j < CLS_PER; j++) c[j] = (unsigned char)(w[j >> 2] >> (8 * (j & 3)));
+        if (slots) {
+#pragma unroll
+            for (int j = 0; j < CLS_PER; j += 4) {
+                const int4 rv = *reinterpret_cast<const int4*>(q + p0 + j);
+                r[j] = rv.x; r[j + 1] = rv.y; r[j + 2] = rv.z; r[j + 3] = rv.w;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < CLS_PER; j++) {
+            c[j] = j < nv ? A.cls[p0 + j] : (unsigned char)0xff;
+            if (slots) r[j] = j < nv ? q[p0 + j] : 0;
+        }
+    }
+    return nv;
+}
+/* The lane's four class counts, 16 bits each. */
+FMD unsigned long long cls_counts(const unsigned char (&c)[CLS_PER]) {
+    unsigned long long m = 0ull;
+#pragma unroll
+    for (int j = 0; j < CLS_PER; j++)
+        if (c[j] < fm::WQ_NCLS) m += 1ull << (16 * c[j]);
+    return m;
+}
+
 __global__ void __launch_bounds__(256) k_wf_classify(WfArgs A) {
-    constexpr int PER = 16, CHUNK = 64 * PER;
+    __shared__ int stage[256 / 64][CLS_CHUNK];
+    int* const st = stage[threadIdx.x >> 6];
     const int it = A.iter;
     const int64_t n = *cnt(A, it, fm::WQ_EXT);
     const int* q = A.q_ext[it & 1];
@@ -369,58 +409,70 @@ __global__ void __launch_bounds__(256) k_wf_classify(WfArgs A) {
                                        cnt(A, it, fm::WQ_CLS + 3)};
     const int64_t waves = grid_stride() >> 6;
     const unsigned lane = lane_id();
-    for (int64_t b = (grid_id() >> 6) * CHUNK; b < n; b += waves * CHUNK) {
-        const int64_t p0 = b + (int64_t)lane * PER;
-        const int nv = (int)max<int64_t>(0, min<int64_t>(PER, n - p0));   /* valid positions of this lane */
-        unsigned char c[PER];
-        int r[PER];
-        if (nv == PER) {   /* p0 is a multiple of 16: aligned 16- and 64-byte loads */
-            const uint4 cv = *reinterpret_cast<const uint4*>(A.cls + p0);
-            const unsigned w[4] = {cv.x, cv.y, cv.z, cv.w};
+    /* the wave's range: whole chunks */
+    const int64_t per = ((n + waves - 1) / waves + CLS_CHUNK - 1) / CLS_CHUNK * CLS_CHUNK;
+    const int64_t lo = (grid_id() >> 6) * per, hi = min(n, lo + per);
+    if (lo >= hi) return;
+    unsigned char c[CLS_PER];
+    int r[CLS_PER];
+    /* counts over the range (four 16-bit fields per lane: at most 16 per chunk) */
+    unsigned tot[fm::WQ_NCLS] = {0u, 0u, 0u, 0u};
+    for (int64_t b = lo; b < hi; b += CLS_CHUNK) {
+        cls_load(A, q, b, hi, lane, c, r, false);
+        unsigned long long m = cls_counts(c);
 #pragma unroll
-            for (int j = 0; j < PER; j++) c[j] = (unsigned char)(w[j >> 2] >> (8 * (j & 3)));
+        for (int d = 32; d >= 1; d >>= 1) m += __shfl_xor(m, d, 64);
 #pragma unroll
-            for (int j = 0; j < PER; j += 4) {
-                const int4 rv = *reinterpret_cast<const int4*>(q + p0 + j);
-                r[j] = rv.x; r[j + 1] = rv.y; r[j + 2] = rv.z; r[j + 3] = rv.w;
-            }
-        } else {
+        for (int k = 0; k < fm::WQ_NCLS; k++) tot[k] += (unsigned)(m >> (16 * k)) & 0xffffu;
+    }
+    unsigned base[fm::WQ_NCLS] = {0u, 0u, 0u, 0u};
+    if (lane == 0u) {
 #pragma unroll
-            for (int j = 0; j < PER; j++) {
-                c[j] = j < nv ? A.cls[p0 + j] : (unsigned char)0xff;
-                r[j] = j < nv ? q[p0 + j] : 0;
-            }
-        }
-        unsigned long long mine = 0ull;   /* four 16-bit class counts */
+        for (int k = 0; k < fm::WQ_NCLS; k++)
+            if (tot[k]) base[k] = atomicAdd(cc[k], tot[k]);
+    }
 #pragma unroll
-        for (int j = 0; j < PER; j++)
-            if (c[j] < fm::WQ_NCLS) mine += 1ull << (16 * c[j]);
+    for (int k = 0; k < fm::WQ_NCLS; k++) base[k] = __shfl(base[k], 0, 64);
+    for (int64_t b = lo; b < hi; b += CLS_CHUNK) {
+        cls_load(A, q, b, hi, lane, c, r, true);
+        const unsigned long long mine = cls_counts(c);
         unsigned long long incl = mine;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
             const unsigned long long v = __shfl_up(incl, d, 64);
             if ((int)lane >= d) incl += v;
         }
-        const unsigned long long tot = __shfl(incl, 63, 64);
-        unsigned long long excl = incl - mine;
-        unsigned base[fm::WQ_NCLS] = {0u, 0u, 0u, 0u};
-        if (lane == 0u) {
+        const unsigned long long ctot = __shfl(incl, 63, 64);
+        const unsigned long long excl = incl - mine;
+        unsigned seg[fm::WQ_NCLS + 1];
+        seg[0] = 0u;
 #pragma unroll
-            for (int k = 0; k < fm::WQ_NCLS; k++) {
-                const unsigned t = (unsigned)(tot >> (16 * k)) & 0xffffu;
-                if (t) base[k] = atomicAdd(cc[k], t);
-            }
-        }
+        for (int k = 0; k < fm::WQ_NCLS; k++) seg[k + 1] = seg[k] + ((unsigned)(ctot >> (16 * k)) & 0xffffu);
+        /* stage: class k's entries of the chunk at seg[k] .. seg[k + 1] */
+        unsigned at[fm::WQ_NCLS];
 #pragma unroll
-        for (int k = 0; k < fm::WQ_NCLS; k++) base[k] = __shfl(base[k], 0, 64) + ((unsigned)(excl >> (16 * k)) & 0xffffu);
+        for (int k = 0; k < fm::WQ_NCLS; k++) at[k] = seg[k] + ((unsigned)(excl >> (16 * k)) & 0xffffu);
 #pragma unroll
-        for (int j = 0; j < PER; j++) {
+        for (int j = 0; j < CLS_PER; j++) {
             const int k = c[j];
             if (k < fm::WQ_NCLS) {
-                const unsigned slot = k == 0 ? base[0]++ : (k == 1 ? base[1]++ : (k == 2 ? base[2]++ : base[3]++));
-                qc[(size_t)k * A.P + slot] = r[j];
+                const unsigned o = k == 0 ? at[0]++ : (k == 1 ? at[1]++ : (k == 2 ? at[2]++ : at[3]++));
+                st[o] = r[j];
             }
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        /* write out: consecutive lanes, consecutive entries of each class */
+        for (unsigned e = lane; e < seg[fm::WQ_NCLS]; e += 64u) {
+            const int k = e < seg[1] ? 0 : (e < seg[2] ? 1 : (e < seg[3] ? 2 : 3));
+            qc[(size_t)k * A.P + base[k] + (e - seg[k])] = st[e];
+        }
+#pragma unroll
+        for (int k = 0; k < fm::WQ_NCLS; k++) base[k] += seg[k + 1] - seg[k];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
