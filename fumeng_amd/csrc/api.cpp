@@ -67,6 +67,7 @@ struct WfBuffers {
     DevBuf mis_f, mis_beta, mis_ng, sh_o, sh_d, sh_c, sh_vis;
     DevBuf q_ext[2], q_sh[2], q_cls[2], counts;
     DevBuf cls;     /* shading class per ray-queue position */
+    DevBuf push;    /* push record per class-queue position */
     DevBuf spill;   /* short traversal stacks' spill entries */
     /* capacities in elements: paths, ray slots, MisBSDF records, shadow records, any-hit queue entries,
      * iterations */
@@ -81,6 +82,7 @@ struct WfBuffers {
             cap_p = 0;
             for (DevBuf* b : {&st_beta, &st_color, &out_alb, &out_nrm}) b->alloc((size_t)P * 16);
             st_flags.alloc((size_t)P * 4);
+            push.alloc((size_t)P * 8);
             for (DevBuf& b : q_cls) b.alloc((size_t)P * fm::WQ_NCLS * 4);
             cap_p = P;
         }
@@ -114,7 +116,7 @@ struct WfBuffers {
         int64_t n = 0;
         for (const DevBuf* b : {&ray_o, &ray_d, &hit_a, &hit_b, &st_beta, &st_color, &st_flags, &out_alb, &out_nrm, &mis_f,
                                 &mis_beta, &mis_ng, &sh_o, &sh_d, &sh_c, &sh_vis, &q_ext[0], &q_ext[1], &q_sh[0], &q_sh[1],
-                                &q_cls[0], &q_cls[1], &counts, &spill, &cls})
+                                &q_cls[0], &q_cls[1], &counts, &spill, &cls, &push})
             n += (int64_t)b->n;
         return n;
     }
@@ -401,7 +403,7 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
             int grid, int chunk, int flags, WfRunOut& R, float* path_out = nullptr) {
     if (path_out && jobs.size() != 1) throw std::invalid_argument("per-path output needs exactly one job");
     const int NL = s->d.num_lights;
-    const int64_t per_path = 100 + 65 * (1 + (int64_t)D) + 56 * (int64_t)D + 57 * (int64_t)D * std::max(NL, 1);
+    const int64_t per_path = 108 + 65 * (1 + (int64_t)D) + 56 * (int64_t)D + 57 * (int64_t)D * std::max(NL, 1);
     /* Passes as large as the context's path-state budget allows (192 GB on
      * an idle 288 GB device): fewer passes mean fewer nearly empty tail
      * iterations (1920x1080 at 256 spp: 8 passes took 522 ms, 2 took 493;
@@ -491,6 +493,8 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
     }
     W_.counts = B.counts.as<unsigned>();
     W_.cls = B.cls.as<unsigned char>();
+    W_.push = B.push.as<uint2>();
+    W_.rec = (D * std::max(NL, 1) <= 32 && D <= 16) ? 1 : 0;
     W_.counters = ctx->counters.as<unsigned long long>();
     W_.path_out = path_out;
     W_.path_s0 = path_out ? jobs[0].sample_offset : 0;

@@ -91,6 +91,8 @@ struct WfArgs {
     uint32_t* spill;   /* short traversal stacks: spill_cap entries per trace lane (entry i of lane g at i * lanes + g) */
     int spill_cap;
     unsigned char* cls;   /* [P (1 + D)]: shading class of the ray at each ray-queue position */
+    uint2* push;       /* [P]: push record per class-queue position of an iteration (when rec) */
+    int rec;           /* shading kernels write push records (D * NL <= 32, D <= 16) instead of pushing */
 };
 
 struct WfLaunch {
@@ -98,6 +100,7 @@ struct WfLaunch {
     int block_trace, grid_trace;   /* trace kernel */
     int block_shade, grid_shade;   /* grid_shade > 0 overrides every class */
     int grid_class[WQ_NCLS];       /* persistent grid per shading class (its own occupancy) */
+    int grid_push, grid_cls;       /* push and classification kernels */
     int stack_levels;
     int stack_form;                /* 0: whichever keeps more trace waves resident, 1: full LDS stack, 2: short stack */
     bool short_stack;              /* the trace kernel's stack form (set by wavefront_grids) */

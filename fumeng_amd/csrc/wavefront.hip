@@ -45,6 +45,10 @@ enum { F_PENDING = 1, F_CONT = 2 };
 /* An any-hit queue entry with this bit is a MisBSDF ray slot (scenes lit by
  * the environment alone), without it a shadow record. */
 constexpr unsigned AH_RAY = 0x80000000u;
+/* Push record of one shaded path (WfArgs::rec): shadow-record bits
+ * (pass * NL + light) in x; MisBSDF-ray bits (pass) in the low 16 bits of y,
+ * with the continuation and path-end flags above them. */
+constexpr unsigned PR_CONT = 1u << 16, PR_END = 1u << 17;
 
 __device__ __forceinline__ uint32_t* wf_stack() {
     extern __shared__ uint32_t fm_wf_lds[];
@@ -609,6 +613,10 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
     const int it = A.iter, nx = it + 1;
     const unsigned n = *cnt(A, it, fm::WQ_CLS + C);
     const int* q = A.q_cls[it & 1] + (size_t)C * A.P;
+    /* this class's entries in the push records: after the lower classes' */
+    unsigned rbase = 0u;
+#pragma unroll
+    for (int c = 0; c < C; c++) rbase += *cnt(A, it, fm::WQ_CLS + c);
     for (int64_t i = grid_id(); i < (int64_t)n; i += grid_stride()) {
         const int p = q[i];
         const float4 b4 = reinterpret_cast<const float4*>(A.st_beta)[p];
@@ -623,7 +631,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
         /* the shadow and MisBSDF pushes are kept as lane bits (uniformly,
          * when they fit) and issued with the continuation and path-end
          * pushes in one round trip of atomics */
-        const bool bits = A.D * A.NL <= 64 && A.D <= 32;
+        const bool bits = A.rec || (A.D * A.NL <= 64 && A.D <= 32);
         uint64_t sb = 0;
         uint32_t mb = 0;
         if constexpr (C == fm::WC_END) {
@@ -718,10 +726,113 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
             }
             dim_out = rng.dim;
         }
-        push_step<LK == fm::LK_ENV>(A, nx, p, sb, mb, !term, term && any);
+        if (A.rec)
+            A.push[rbase + i] = make_uint2((unsigned)sb, mb | (term ? 0u : PR_CONT) | (term && any ? PR_END : 0u));
+        else
+            push_step<LK == fm::LK_ENV>(A, nx, p, sb, mb, !term, term && any);
         reinterpret_cast<float4*>(A.st_beta)[p] = f4(beta, __int_as_float(depth));
         reinterpret_cast<float4*>(A.st_color)[p] = f4(color, __uint_as_float(dim_out));
         A.st_flags[p] = (fl & ~0xff) | (any ? F_PENDING : 0) | (term ? 0 : F_CONT);
+    }
+}
+
+/* The next iteration's queues from the push records the shading kernels
+ * wrote (WfArgs::rec), in class-queue order: shadow records, MisBSDF rays
+ * (tagged into the any-hit queue in env-lit scenes, else the ray queue),
+ * continuations and path ends.  Each wave owns one contiguous range of the
+ * records: it counts the entries there, reserves its three queue ranges
+ * with one atomic per queue, then writes them (per round of 64 records, a
+ * wave scan of the lanes' counts places each lane's entries).  Pushing from
+ * the shading kernels instead, one round of atomics per wave and step with
+ * every wave on the same three counters, left the first Lambert launch of
+ * the 1 M-triangle frame waiting on the counters (36.6 ms; 212 ms with the
+ * atomics doubled). */
+__global__ void __launch_bounds__(256) k_wf_push(WfArgs A, int mis_any) {
+    const int it = A.iter, nx = it + 1;
+    unsigned nc[fm::WQ_NCLS + 1];
+    nc[0] = 0u;
+#pragma unroll
+    for (int c = 0; c < fm::WQ_NCLS; c++) nc[c + 1] = nc[c] + *cnt(A, it, fm::WQ_CLS + c);
+    const int64_t n = nc[fm::WQ_NCLS];
+    const int nsb = A.D * A.NL, nmb = A.D;
+    const int64_t waves = grid_stride() >> 6;
+    const unsigned lane = lane_id();
+    const int64_t per = ((n + waves - 1) / waves + 63) / 64 * 64;
+    const int64_t lo = (grid_id() >> 6) * per, hi = min(n, lo + per);
+    if (lo >= hi) return;
+    auto counts = [&](uint2 r, unsigned& ns, unsigned& ne, unsigned& nd) {
+        const unsigned nm = (unsigned)__popc(r.y & 0xffffu);
+        ns = (unsigned)__popc(r.x) + (mis_any ? nm : 0u);
+        ne = (mis_any ? 0u : nm) + ((r.y & PR_CONT) ? 1u : 0u);
+        nd = (r.y & PR_END) ? 1u : 0u;
+    };
+    unsigned ts = 0u, te = 0u, td = 0u;
+#pragma unroll 4
+    for (int64_t g = lo + lane; g < hi; g += 64) {
+        unsigned ns, ne, nd;
+        counts(A.push[g], ns, ne, nd);
+        ts += ns; te += ne; td += nd;
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        ts += __shfl_xor(ts, d, 64); te += __shfl_xor(te, d, 64); td += __shfl_xor(td, d, 64);
+    }
+    unsigned bs = 0u, be = 0u, bd = 0u;
+    if (lane == 0u) {
+        if (ts) bs = atomicAdd(cnt(A, nx, fm::WQ_SH), ts);
+        if (te) be = atomicAdd(cnt(A, nx, fm::WQ_EXT), te);
+        if (td) bd = atomicAdd(cnt(A, nx, fm::WQ_CLS + fm::WC_END), td);
+    }
+    bs = __shfl(bs, 0, 64); be = __shfl(be, 0, 64); bd = __shfl(bd, 0, 64);
+    int* const qs = A.q_sh[nx & 1];
+    int* const qe = A.q_ext[nx & 1];
+    int* const qd = A.q_cls[nx & 1] + (size_t)fm::WC_END * A.P;
+    /* four rounds of 64 records per step: their loads go out together */
+    constexpr int U = 4;
+    for (int64_t g0 = lo; g0 < hi; g0 += 64 * U) {
+        uint2 rr[U];
+        int64_t pp[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t g = g0 + 64 * u + lane;
+            const bool valid = g < hi;
+            /* the path: position g of the concatenated class queues */
+            int c = 0;
+#pragma unroll
+            for (int k = 1; k < fm::WQ_NCLS; k++) c += (unsigned)g >= nc[k] ? 1 : 0;
+            pp[u] = valid ? A.q_cls[it & 1][(size_t)c * A.P + ((unsigned)g - nc[c])] : 0;
+            rr[u] = valid ? A.push[g] : make_uint2(0u, 0u);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint2 r = rr[u];
+            const int64_t p = pp[u];
+            unsigned ns, ne, nd;
+            counts(r, ns, ne, nd);
+            /* exclusive scan of the three counts (each at most 64 + 33, packed in 21 bits) */
+            const unsigned long long mine = (unsigned long long)ns | ((unsigned long long)ne << 21) | ((unsigned long long)nd << 42);
+            unsigned long long incl = mine;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const unsigned long long v = __shfl_up(incl, d, 64);
+                if ((int)lane >= d) incl += v;
+            }
+            const unsigned long long tot = __shfl(incl, 63, 64), ex = incl - mine;
+            if (ns | ne | nd) {
+                unsigned os = bs + (unsigned)(ex & 0x1fffffull), oe = be + (unsigned)((ex >> 21) & 0x1fffffull);
+                const unsigned od = bd + (unsigned)((ex >> 42) & 0x1fffffull);
+                for (unsigned m = r.x; m; m &= m - 1u) qs[os++] = (int)(p * nsb + (__ffs(m) - 1));
+                for (unsigned m = r.y & 0xffffu; m; m &= m - 1u) {
+                    const unsigned slot = (unsigned)(A.P + p * nmb + (__ffs(m) - 1));
+                    if (mis_any) qs[os++] = (int)(slot | AH_RAY);
+                    else qe[oe++] = (int)slot;
+                }
+                if (r.y & PR_CONT) qe[oe++] = (int)p;
+                if (nd) qd[od] = (int)p;
+            }
+            bs += (unsigned)(tot & 0x1fffffull);
+            be += (unsigned)((tot >> 21) & 0x1fffffull);
+            bd += (unsigned)((tot >> 42) & 0x1fffffull);
+        }
     }
 }
 
@@ -832,6 +943,10 @@ void wavefront_grids(int num_cus, int stack_levels, WfLaunch& L) {
     else if (L.lights == LK_AREA) occ4(k_wf_shade<WC_END, LK_AREA>, k_wf_shade<WC_LAMBERT, LK_AREA>, k_wf_shade<WC_DISNEY, LK_AREA>, k_wf_shade<WC_SPEC, LK_AREA>);
     else occ4(k_wf_shade<WC_END, LK_ANY>, k_wf_shade<WC_LAMBERT, LK_ANY>, k_wf_shade<WC_DISNEY, LK_ANY>, k_wf_shade<WC_SPEC, LK_ANY>);
     if (L.grid_shade <= 0) L.grid_shade = L.grid_class[WC_END];
+    /* the classification and push kernels: as many waves as fit (both are
+     * latency-bound streams of short rounds) */
+    L.grid_push = num_cus * occupancy(k_wf_push, 256, 0);
+    L.grid_cls = num_cus * occupancy(k_wf_classify, 256, 0);
 }
 
 template <int LK>
@@ -886,8 +1001,9 @@ hipError_t launch_wavefront_pass(const fm_dscene& S, WfArgs A, const WfLaunch& L
             }
         }
         if (ev) hipEventRecord(ev[2 * it + 1], st);
-        hipLaunchKernelGGL(k_wf_classify, dim3(L.grid_class[WC_END]), dim3(256), 0, st, A);
+        hipLaunchKernelGGL(k_wf_classify, dim3(L.grid_cls), dim3(256), 0, st, A);
         launch_shades(S, A, L, st);
+        if (A.rec) hipLaunchKernelGGL(k_wf_push, dim3(L.grid_push), dim3(256), 0, st, A, L.lights == LK_ENV ? 1 : 0);
     }
     const int film_grid = (int)std::min<int64_t>((L.max_pix + 255) / 256, (int64_t)L.grid_shade * 4);
     hipLaunchKernelGGL(k_wf_film, dim3(std::max(film_grid, 1)), dim3(256), 0, st, A);

@@ -66,6 +66,22 @@ def test_wavefront_matches_megakernel_disney_env_two_lights(gpu_ctx):
     assert abs(wf[4].rays - mk[4].rays) <= 5e-3 * mk[4].rays
 
 
+def test_wavefront_matches_megakernel_with_many_shadow_records(gpu_ctx):
+    """12 area lights (an emissive mesh's faces) and three direct-light
+    passes per vertex: 36 shadow records per vertex, more than a push record
+    holds, so the shading kernels push their queue entries themselves (the
+    other tests take the push-record path and its compaction kernel)."""
+    from helpers import mesh_light_scene
+    s = mesh_light_scene(accelerator="bvh", width=24, height=24, spp=4, depth=4)
+    s.renderer.update(direct_loop=3)
+    g = gpu_ctx.scene(s)
+    assert g.info().num_lights == 12
+    mk, wf = _both(g, seed=17, chunk=2)
+    _same_film(mk, wf, "36 shadow records per vertex")
+    assert wf[4].paths == mk[4].paths == 24 * 24 * 4
+    assert abs(wf[4].rays - mk[4].rays) <= 5e-3 * mk[4].rays
+
+
 def test_wavefront_uneven_passes_and_shallow_depths(gpu_ctx):
     """spp not a multiple of the pass size (passes of 2, 2, 1) and depths 1
     and 2 (the deferred direct light of the last vertex still lands)."""
