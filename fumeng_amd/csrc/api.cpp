@@ -145,6 +145,7 @@ struct fm_ctx {
     DevBuf adj_in, adj_gmat, adj_genv;                  /* adjoint: image gradient, FP64 parameter gradients */
     DevBuf adj_tot;                                     /* adjoint: path radiance per work item */
     DevBuf adj_cell;                                    /* adjoint: env table derivatives per cell (2 x texels), rows (2H + 1) */
+    DevBuf adj_priv;                                    /* adjoint: the walk's copies of the env texel and cell sums */
     DevBuf adj_film;                                    /* adjoint: FP64 sums of its forward run (unused) */
     double* noalloc = nullptr;         /* this device's fm_render_noalloc working array */
     std::vector<hipEvent_t> trace_ev;  /* pairs around wavefront trace launches */
@@ -1373,6 +1374,23 @@ int render_adjoint(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, const fl
             HIPCHECK(fm::launch_zero_f64(ctx->adj_cell.as<double>(), 2 * ncell, st));
             A.env_cell = ctx->adj_cell.as<double>();
         }
+        /* the walk adds env gradients into AD_ENV_COPIES zeroed copies (by
+         * block), summed into the arrays above after it */
+        double* const dst_env = nenv > 0 ? A.g_env : nullptr;
+        double* const dst_cell = ncell > 0 ? A.env_cell : nullptr;
+        A.ncopy = 1; A.env_stride = 0; A.cell_stride = 0;
+        if (nenv > 0 || ncell > 0) {
+            constexpr int AD_ENV_COPIES = 8;
+            const int64_t per = nenv + 2 * ncell;
+            if (ctx->adj_priv.n < (size_t)(AD_ENV_COPIES * per) * sizeof(double))
+                ctx->adj_priv.alloc((size_t)(AD_ENV_COPIES * per) * sizeof(double));
+            double* priv = ctx->adj_priv.as<double>();
+            HIPCHECK(fm::launch_zero_f64(priv, AD_ENV_COPIES * per, st));
+            A.ncopy = AD_ENV_COPIES;
+            A.env_stride = nenv; A.cell_stride = 2 * ncell;
+            A.g_env = nenv > 0 ? priv : nullptr;
+            A.env_cell = ncell > 0 ? priv + (size_t)AD_ENV_COPIES * nenv : nullptr;
+        }
         HIPCHECK(fm::launch_zero_f64((double*)ctx->counters.p, 8, st));
         A.env_w = im ? im->w : 0; A.env_h = im ? im->h : 0;
         A.lds_grad = (int64_t)nm * FM_DPARAM_PER_MATERIAL <= 4096 ? 1 : 0;
@@ -1407,15 +1425,17 @@ int render_adjoint(fm_ctx* ctx, fm_scene* s, const fm_render_params* p, const fl
         HIPCHECK(fm::launch_zero_f64((double*)ctx->counters.p, 8, st));
         HIPCHECK(hipEventRecord(ctx->ev0, st));
         HIPCHECK(fm::launch_adjoint(s->d, A, block, std::max(grid, 1), levels, st));
+        if (dst_env) HIPCHECK(fm::launch_sum_copies(A.g_env, A.ncopy, nenv, dst_env, st));
+        if (dst_cell) HIPCHECK(fm::launch_sum_copies(A.env_cell, A.ncopy, 2 * ncell, dst_cell, st));
         if (ncell > 0)
-            HIPCHECK(fm::launch_env_table_grad(A.env_cell, A.env_cell + 2 * ncell,
+            HIPCHECK(fm::launch_env_table_grad(dst_cell, dst_cell + 2 * ncell,
                                                reinterpret_cast<const float*>((const char*)s->texels.p + dim->offset), dim->w, dim->h,
-                                               s->env_aux.as<double>() + s->hs.envs[(size_t)s->hs.env].off_aux, A.g_env, st));
+                                               s->env_aux.as<double>() + s->hs.envs[(size_t)s->hs.env].off_aux, dst_env, st));
         HIPCHECK(hipEventRecord(ctx->ev1, st));
         HIPCHECK(hipStreamSynchronize(st));
         if (!device && g_material)
             HIPCHECK(hipMemcpy(g_material, A.g_mat, (size_t)nm * FM_DPARAM_PER_MATERIAL * sizeof(double), hipMemcpyDeviceToHost));
-        if (!device && g_env && nenv > 0) HIPCHECK(hipMemcpy(g_env, A.g_env, (size_t)nenv * sizeof(double), hipMemcpyDeviceToHost));
+        if (!device && g_env && nenv > 0) HIPCHECK(hipMemcpy(g_env, dst_env, (size_t)nenv * sizeof(double), hipMemcpyDeviceToHost));
         if (stats) {
             memset(stats, 0, sizeof(*stats));
             float ms = 0.0f;

@@ -130,8 +130,10 @@ struct AdjointArgs {
     int64_t total_items;
     const float* dl_dimg;     /* H x W x 3, row 0 = top */
     double* g_mat;            /* num_materials x 6 */
-    double* g_env;            /* env texels x 3, or null */
-    double* env_cell;         /* 2 x env texels, zeroed (table derivatives per env cell), or null */
+    double* g_env;            /* ncopy copies of env texels x 3 (zeroed), or null */
+    double* env_cell;         /* ncopy copies of 2 x env texels (zeroed; table derivatives per env cell), or null */
+    int ncopy;                /* block b adds to copy b mod ncopy of g_env and env_cell */
+    int64_t env_stride, cell_stride;   /* doubles per copy */
     int env_w, env_h;
     int lds_grad;             /* accumulate material gradients in LDS first */
     unsigned long long* counters;
@@ -140,6 +142,8 @@ struct AdjointArgs {
 /* The adjoint walk of every work item (A.totals filled beforehand). */
 hipError_t launch_adjoint(const fm_dscene& S, const AdjointArgs& A, int block, int grid, int stack_levels, hipStream_t st);
 int adjoint_blocks_per_cu(int block, size_t lds);
+/* dst[k] = sum over c < ncopy of src[c n + k] (copies in order) */
+hipError_t launch_sum_copies(const double* src, int ncopy, int64_t n, double* dst, hipStream_t st);
 
 /* Device FP64 rebuild of one float env image's sampling tables (envtab.hip),
  * bit-identical to the host build. */
