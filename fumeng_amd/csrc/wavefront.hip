@@ -128,15 +128,16 @@ FMD void store_ray(const WfArgs& A, int64_t r, const Ray& ray) {
     reinterpret_cast<float4*>(A.ray_o)[r] = f4(ray.o, ray.tmin);
     reinterpret_cast<float4*>(A.ray_d)[r] = f4(ray.d, ray.tmax);
 }
-FMD Hit load_hit(const WfArgs& A, int64_t r) {
-    const float4 a = reinterpret_cast<const float4*>(A.hit_a)[r];
-    const float2 b = reinterpret_cast<const float2*>(A.hit_b)[r];
+FMD Hit hit_of(float4 a, float2 b) {
     Hit h;
     h.t = a.x; h.slot = __float_as_int(a.y);
     const int kh = __float_as_int(a.z);
     h.kind = (kh << 24) >> 24; h.half = kh >> 8; h.tie = 0; h.info = 0;
     h.b0 = a.w; h.b1 = b.x; h.b2 = b.y;
     return h;
+}
+FMD Hit load_hit(const WfArgs& A, int64_t r) {
+    return hit_of(reinterpret_cast<const float4*>(A.hit_a)[r], reinterpret_cast<const float2*>(A.hit_b)[r]);
 }
 
 /* The parts of a ray slot that an environment-lit scene needs once the
@@ -617,11 +618,43 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
     unsigned rbase = 0u;
 #pragma unroll
     for (int c = 0; c < C; c++) rbase += *cnt(A, it, fm::WQ_CLS + c);
-    for (int64_t i = grid_id(); i < (int64_t)n; i += grid_stride()) {
-        const int p = q[i];
-        const float4 b4 = reinterpret_cast<const float4*>(A.st_beta)[p];
-        const float4 c4 = reinterpret_cast<const float4*>(A.st_color)[p];
-        const int fl = A.st_flags[p];
+    /* Software pipeline over the lane's items i, i + gs, ...: the path
+     * index of item i + 2 gs and the state and hit record of item i + gs
+     * load while item i is shaded, so a shading step does not start with
+     * two dependent round trips.  Only the surface classes of env-lit
+     * scenes have the registers for it below their wave count's limit
+     * (Lambert 147 -> 165 of 168 VGPRs, Disney 233 -> 251 of 256); the
+     * others would lose a wave and load item by item. */
+    constexpr bool PF = C != fm::WC_END && LK == fm::LK_ENV;
+    const int64_t gs = grid_stride();
+    int64_t i = grid_id();
+    int p1 = PF && i < (int64_t)n ? q[i] : 0;
+    int p2 = PF && i + gs < (int64_t)n ? q[i + gs] : 0;
+    float4 b1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c1 = b1, h1 = b1;
+    int f1 = 0;
+    auto state = [&](int pp, float4& b, float4& c, int& f, float4& h) {
+        b = reinterpret_cast<const float4*>(A.st_beta)[pp];
+        c = reinterpret_cast<const float4*>(A.st_color)[pp];
+        f = A.st_flags[pp];
+        h = reinterpret_cast<const float4*>(A.hit_a)[pp];
+    };
+    if (PF && i < (int64_t)n) state(p1, b1, c1, f1, h1);
+    for (; i < (int64_t)n; i += gs) {
+        int p;
+        float4 b4, c4, ha;
+        int fl;
+        if (PF) {
+            p = p1; b4 = b1; c4 = c1; fl = f1; ha = h1;
+            p1 = p2;
+            if (i + gs < (int64_t)n) state(p1, b1, c1, f1, h1);
+            p2 = i + 2 * gs < (int64_t)n ? q[i + 2 * gs] : 0;
+        } else {
+            p = q[i];
+            b4 = reinterpret_cast<const float4*>(A.st_beta)[p];
+            c4 = reinterpret_cast<const float4*>(A.st_color)[p];
+            fl = A.st_flags[p];
+            if (C != fm::WC_END) ha = reinterpret_cast<const float4*>(A.hit_a)[p];
+        }
         v3 beta = xyz(b4), color = xyz(c4);
         int depth = __float_as_int(b4.w);
         const uint32_t dim = __float_as_uint(c4.w);
@@ -652,7 +685,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
             }
         } else {
             const Ray tr = load_ray(A, p);
-            const Hit h = load_hit(A, p);
+            const Hit h = hit_of(ha, reinterpret_cast<const float2*>(A.hit_b)[p]);
             const Surf sf = fill_hit(S, tr, h);
             const Bsdf b = make_bsdf<C == fm::WC_SPEC ? FM_BSDF_DELTA : C - 1>(S, sf.material, sf.ns, sf.ss, sf.wo, sf.u, sf.v);
             if (depth == 1) {
@@ -726,10 +759,13 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
             }
             dim_out = rng.dim;
         }
-        if (A.rec)
-            A.push[rbase + i] = make_uint2((unsigned)sb, mb | (term ? 0u : PR_CONT) | (term && any ? PR_END : 0u));
-        else
-            push_step<LK == fm::LK_ENV>(A, nx, p, sb, mb, !term, term && any);
+        /* the path-end class pushes nothing (no continuation, no samples) */
+        if constexpr (C != fm::WC_END) {
+            if (A.rec)
+                A.push[rbase + i] = make_uint2((unsigned)sb, mb | (term ? 0u : PR_CONT) | (term && any ? PR_END : 0u));
+            else
+                push_step<LK == fm::LK_ENV>(A, nx, p, sb, mb, !term, term && any);
+        }
         reinterpret_cast<float4*>(A.st_beta)[p] = f4(beta, __int_as_float(depth));
         reinterpret_cast<float4*>(A.st_color)[p] = f4(color, __uint_as_float(dim_out));
         A.st_flags[p] = (fl & ~0xff) | (any ? F_PENDING : 0) | (term ? 0 : F_CONT);
@@ -737,7 +773,8 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
 }
 
 /* The next iteration's queues from the push records the shading kernels
- * wrote (WfArgs::rec), in class-queue order: shadow records, MisBSDF rays
+ * wrote (WfArgs::rec; the path-end class writes none, its positions are
+ * skipped), in class-queue order: shadow records, MisBSDF rays
  * (tagged into the any-hit queue in env-lit scenes, else the ray queue),
  * continuations and path ends.  Each wave owns one contiguous range of the
  * records: it counts the entries there, reserves its three queue ranges
@@ -753,12 +790,12 @@ __global__ void __launch_bounds__(256) k_wf_push(WfArgs A, int mis_any) {
     nc[0] = 0u;
 #pragma unroll
     for (int c = 0; c < fm::WQ_NCLS; c++) nc[c + 1] = nc[c] + *cnt(A, it, fm::WQ_CLS + c);
-    const int64_t n = nc[fm::WQ_NCLS];
+    const int64_t n = nc[fm::WQ_NCLS], g_first = nc[1];   /* records of the classes after the path end */
     const int nsb = A.D * A.NL, nmb = A.D;
     const int64_t waves = grid_stride() >> 6;
     const unsigned lane = lane_id();
-    const int64_t per = ((n + waves - 1) / waves + 63) / 64 * 64;
-    const int64_t lo = (grid_id() >> 6) * per, hi = min(n, lo + per);
+    const int64_t per = ((n - g_first + waves - 1) / waves + 63) / 64 * 64;
+    const int64_t lo = g_first + (grid_id() >> 6) * per, hi = min(n, lo + per);
     if (lo >= hi) return;
     auto counts = [&](uint2 r, unsigned& ns, unsigned& ne, unsigned& nd) {
         const unsigned nm = (unsigned)__popc(r.y & 0xffffu);
