@@ -41,7 +41,9 @@ using fm::WfArgs;
 
 namespace {
 
-enum { F_PENDING = 1, F_CONT = 2 };
+/* F_CAM: the path's ray is still its camera ray (depth 1; cleared when the
+ * first vertex is shaded) */
+enum { F_PENDING = 1, F_CONT = 2, F_CAM = 4 };
 /* An any-hit queue entry with this bit is a MisBSDF ray slot (scenes lit by
  * the environment alone), without it a shadow record. */
 constexpr unsigned AH_RAY = 0x80000000u;
@@ -196,7 +198,7 @@ __global__ void __launch_bounds__(256) k_wf_camera(fm_dscene S, WfArgs A) {
         store_ray(A, p, cr);
         reinterpret_cast<float4*>(A.st_beta)[p] = make_float4(1.0f, 1.0f, 1.0f, __int_as_float(1));
         reinterpret_cast<float4*>(A.st_color)[p] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(rng.dim));
-        A.st_flags[p] = F_CONT | (ui << 8);
+        A.st_flags[p] = F_CONT | F_CAM | (ui << 8);
         reinterpret_cast<float4*>(A.out_alb)[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         reinterpret_cast<float4*>(A.out_nrm)[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         A.q_ext[0][p] = (int)p;
@@ -650,7 +652,10 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
             p2 = i + 2 * gs < (int64_t)n ? q[i + 2 * gs] : 0;
         } else {
             p = q[i];
-            b4 = reinterpret_cast<const float4*>(A.st_beta)[p];
+            /* an env-lit path end needs no throughput (its depth-1 case is
+             * the F_CAM flag) */
+            if (C == fm::WC_END && LK == fm::LK_ENV) b4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            else b4 = reinterpret_cast<const float4*>(A.st_beta)[p];
             c4 = reinterpret_cast<const float4*>(A.st_color)[p];
             fl = A.st_flags[p];
             if (C != fm::WC_END) ha = reinterpret_cast<const float4*>(A.hit_a)[p];
@@ -671,7 +676,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
             if (LK == fm::LK_ENV) {
                 /* an env-lit scene adds light here only for a camera ray that
                  * escaped; deeper escapes are counted by the MisBSDF rays */
-                if ((fl & F_CONT) && depth == 1 && S.env >= 0 && load_slot(A, p) < 0)
+                if ((fl & F_CONT) && (fl & F_CAM) && S.env >= 0 && load_slot(A, p) < 0)
                     color = env_radiance(S, S.envs[S.env], -load_dir(A, p));
             } else if (fl & F_CONT) {
                 const Ray tr = load_ray(A, p);
@@ -766,9 +771,13 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
             else
                 push_step<LK == fm::LK_ENV>(A, nx, p, sb, mb, !term, term && any);
         }
-        reinterpret_cast<float4*>(A.st_beta)[p] = f4(beta, __int_as_float(depth));
+        /* a path end's throughput and flags are read by no later kernel
+         * (the film reads the radiance) */
+        if constexpr (C != fm::WC_END) {
+            reinterpret_cast<float4*>(A.st_beta)[p] = f4(beta, __int_as_float(depth));
+            A.st_flags[p] = (fl & ~0xff) | (any ? F_PENDING : 0) | (term ? 0 : F_CONT);
+        }
         reinterpret_cast<float4*>(A.st_color)[p] = f4(color, __uint_as_float(dim_out));
-        A.st_flags[p] = (fl & ~0xff) | (any ? F_PENDING : 0) | (term ? 0 : F_CONT);
     }
 }
 
