@@ -64,7 +64,7 @@ struct DevBuf {
 /* Wavefront path state and queues, sized for the largest pass seen so far. */
 struct WfBuffers {
     DevBuf ray_o, ray_d, hit_a, hit_b, st_beta, st_color, st_flags, out_alb, out_nrm;
-    DevBuf mis_f, mis_beta, mis_ng, sh_o, sh_d, sh_c, sh_vis;
+    DevBuf mis, sh_o, sh_d, sh_c, sh_vis;
     DevBuf q_ext[2], q_sh[2], q_cls[2], counts;
     DevBuf cls;     /* shading class per ray-queue position */
     DevBuf push;    /* push record per class-queue position */
@@ -96,7 +96,7 @@ struct WfBuffers {
         }
         if (PD > cap_pd) {
             cap_pd = 0;
-            for (DevBuf* b : {&mis_f, &mis_beta, &mis_ng}) b->alloc((size_t)PD * 16);
+            mis.alloc((size_t)PD * 48);
             cap_pd = PD;
         }
         if (PDL > cap_pdl) {
@@ -114,8 +114,8 @@ struct WfBuffers {
     }
     int64_t bytes() const {
         int64_t n = 0;
-        for (const DevBuf* b : {&ray_o, &ray_d, &hit_a, &hit_b, &st_beta, &st_color, &st_flags, &out_alb, &out_nrm, &mis_f,
-                                &mis_beta, &mis_ng, &sh_o, &sh_d, &sh_c, &sh_vis, &q_ext[0], &q_ext[1], &q_sh[0], &q_sh[1],
+        for (const DevBuf* b : {&ray_o, &ray_d, &hit_a, &hit_b, &st_beta, &st_color, &st_flags, &out_alb, &out_nrm, &mis,
+                                &sh_o, &sh_d, &sh_c, &sh_vis, &q_ext[0], &q_ext[1], &q_sh[0], &q_sh[1],
                                 &q_cls[0], &q_cls[1], &counts, &spill, &cls, &push})
             n += (int64_t)b->n;
         return n;
@@ -486,7 +486,7 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
     W_.hit_a = B.hit_a.as<float>(); W_.hit_b = B.hit_b.as<float>();
     W_.st_beta = B.st_beta.as<float>(); W_.st_color = B.st_color.as<float>(); W_.st_flags = B.st_flags.as<int>();
     W_.out_alb = B.out_alb.as<float>(); W_.out_nrm = B.out_nrm.as<float>();
-    W_.mis_f = B.mis_f.as<float>(); W_.mis_beta = B.mis_beta.as<float>(); W_.mis_ng = B.mis_ng.as<float>();
+    W_.mis = B.mis.as<float4>();
     W_.sh_o = B.sh_o.as<float>(); W_.sh_d = B.sh_d.as<float>(); W_.sh_c = B.sh_c.as<float>();
     W_.sh_vis = B.sh_vis.as<unsigned char>();
     for (int b = 0; b < 2; b++) {

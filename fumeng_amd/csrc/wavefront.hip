@@ -497,10 +497,10 @@ FMD v3 resolve_direct(const fm_dscene& S, const WfArgs& A, int64_t p) {
             if (A.sh_vis[k] == 2) direct += xyz(reinterpret_cast<const float4*>(A.sh_c)[k]);
         }
         const int64_t m = (int64_t)p * A.D + pass;
-        const float4 ng4 = reinterpret_cast<const float4*>(A.mis_ng)[m];
+        const float4 ng4 = A.mis[3 * m + 2];
         if (ng4.w == 0.0f) continue;
-        const float4 f4v = reinterpret_cast<const float4*>(A.mis_f)[m];
-        const float4 b4 = reinterpret_cast<const float4*>(A.mis_beta)[m];
+        const float4 f4v = A.mis[3 * m];
+        const float4 b4 = A.mis[3 * m + 1];
         const v3 msf = xyz(f4v), beta = xyz(b4), ng = xyz(ng4);
         const float mpdf = f4v.w;
         const bool delta = b4.w != 0.0f;
@@ -735,11 +735,11 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
                 if (ok) {
                     ms.wi = normalize(ms.wi);
                     store_ray(A, A.P + m, gen_ray(sf.pos, sf.ng, ms.wi));
-                    reinterpret_cast<float4*>(A.mis_f)[m] = f4(ms.f, ms.pdf);
-                    reinterpret_cast<float4*>(A.mis_beta)[m] = f4(beta, ms.delta ? 1.0f : 0.0f);
-                    reinterpret_cast<float4*>(A.mis_ng)[m] = f4(sf.ng, 1.0f);
+                    A.mis[3 * m] = f4(ms.f, ms.pdf);
+                    A.mis[3 * m + 1] = f4(beta, ms.delta ? 1.0f : 0.0f);
+                    A.mis[3 * m + 2] = f4(sf.ng, 1.0f);
                 } else {
-                    reinterpret_cast<float4*>(A.mis_ng)[m] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    A.mis[3 * m + 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 }
                 if (bits) mb |= (uint32_t)(ok ? 1 : 0) << pass;
                 else if (LK == fm::LK_ENV) wave_push(cnt(A, nx, fm::WQ_SH), A.q_sh[nx & 1], ok, (int)((unsigned)(A.P + m) | AH_RAY));
