@@ -199,9 +199,10 @@ __global__ void __launch_bounds__(256) k_wf_camera(fm_dscene S, WfArgs A) {
         reinterpret_cast<float4*>(A.st_beta)[p] = make_float4(1.0f, 1.0f, 1.0f, __int_as_float(1));
         reinterpret_cast<float4*>(A.st_color)[p] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(rng.dim));
         A.st_flags[p] = F_CONT | F_CAM | (ui << 8);
-        reinterpret_cast<float4*>(A.out_alb)[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        reinterpret_cast<float4*>(A.out_nrm)[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        A.q_ext[0][p] = (int)p;
+        /* the first ray queue is every path in order (no entries are
+         * written: the trace and classification read item i as path i);
+         * albedo and normal are written at the first vertex, or as zeros by
+         * the path end when the camera ray finds no surface to shade */
     }
 }
 
@@ -301,7 +302,7 @@ FMD void trace_stream(const fm_dscene& S, const WfArgs& A, int64_t lo, int64_t h
                 o = reinterpret_cast<const float4*>(mis ? A.ray_o : A.sh_o)[slot];
                 d = reinterpret_cast<const float4*>(mis ? A.ray_d : A.sh_d)[slot];
             } else {
-                slot = (unsigned)q[i];
+                slot = q ? (unsigned)q[i] : (unsigned)i;
                 o = reinterpret_cast<const float4*>(A.ray_o)[slot];
                 d = reinterpret_cast<const float4*>(A.ray_d)[slot];
             }
@@ -342,7 +343,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
     const unsigned n_ext = *cnt(A, it, fm::WQ_EXT);
     const unsigned n_sh = *cnt(A, it, fm::WQ_SH);
     const int64_t total = (int64_t)n_ext + n_sh;
-    const int* q = A.q_ext[it & 1];
+    const int* q = it == 0 ? nullptr : A.q_ext[it & 1];   /* iteration 0: item i is path i */
     const int* qs = A.q_sh[it & 1];
     Cnt cn;
     TravStack<SHORT> ts = make_stack<SHORT>(stk, stride, spill, lanes, A.spill_cap);
@@ -380,18 +381,21 @@ FMD int cls_load(const WfArgs& A, const int* q, int64_t b, int64_t n, unsigned l
         const unsigned w[4] = {cv.x, cv.y, cv.z, cv.w};
 #pragma unroll
         for (int j = 0; j < CLS_PER; j++) c[j] = (unsigned char)(w[j >> 2] >> (8 * (j & 3)));
-        if (slots) {
+        if (slots && q) {
 #pragma unroll
             for (int j = 0; j < CLS_PER; j += 4) {
                 const int4 rv = *reinterpret_cast<const int4*>(q + p0 + j);
                 r[j] = rv.x; r[j + 1] = rv.y; r[j + 2] = rv.z; r[j + 3] = rv.w;
             }
+        } else if (slots) {
+#pragma unroll
+            for (int j = 0; j < CLS_PER; j++) r[j] = (int)(p0 + j);
         }
     } else {
 #pragma unroll
         for (int j = 0; j < CLS_PER; j++) {
             c[j] = j < nv ? A.cls[p0 + j] : (unsigned char)0xff;
-            if (slots) r[j] = j < nv ? q[p0 + j] : 0;
+            if (slots) r[j] = j < nv ? (q ? q[p0 + j] : (int)(p0 + j)) : 0;
         }
     }
     return nv;
@@ -410,7 +414,7 @@ __global__ void __launch_bounds__(256) k_wf_classify(WfArgs A) {
     int* const st = stage[threadIdx.x >> 6];
     const int it = A.iter;
     const int64_t n = *cnt(A, it, fm::WQ_EXT);
-    const int* q = A.q_ext[it & 1];
+    const int* q = it == 0 ? nullptr : A.q_ext[it & 1];   /* iteration 0: position i is path i */
     int* qc = A.q_cls[it & 1];
     unsigned* const cc[fm::WQ_NCLS] = {cnt(A, it, fm::WQ_CLS), cnt(A, it, fm::WQ_CLS + 1), cnt(A, it, fm::WQ_CLS + 2),
                                        cnt(A, it, fm::WQ_CLS + 3)};
@@ -673,6 +677,10 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
         uint64_t sb = 0;
         uint32_t mb = 0;
         if constexpr (C == fm::WC_END) {
+            if (fl & F_CAM) {   /* no surface shaded at the first vertex */
+                reinterpret_cast<float4*>(A.out_alb)[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                reinterpret_cast<float4*>(A.out_nrm)[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
             if (LK == fm::LK_ENV) {
                 /* an env-lit scene adds light here only for a camera ray that
                  * escaped; deeper escapes are counted by the MisBSDF rays */
