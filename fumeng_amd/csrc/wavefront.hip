@@ -51,6 +51,8 @@ constexpr unsigned AH_RAY = 0x80000000u;
  * (pass * NL + light) in x; MisBSDF-ray bits (pass) in the low 16 bits of y,
  * with the continuation and path-end flags above them. */
 constexpr unsigned PR_CONT = 1u << 16, PR_END = 1u << 17;
+/* Random dimensions a camera ray draws (film x, y; lens u, v). */
+constexpr uint32_t CAM_DIMS = 4;
 
 __device__ __forceinline__ uint32_t* wf_stack() {
     extern __shared__ uint32_t fm_wf_lds[];
@@ -196,8 +198,9 @@ __global__ void __launch_bounds__(256) k_wf_camera(fm_dscene S, WfArgs A) {
         const float px = ((float)i + fx) / (float)U.W, py = ((float)jj + fy) / (float)U.H;
         const Ray cr = camera_ray(A.cams[U.cam], px, py, lx, ly);
         store_ray(A, p, cr);
-        reinterpret_cast<float4*>(A.st_beta)[p] = make_float4(1.0f, 1.0f, 1.0f, __int_as_float(1));
-        reinterpret_cast<float4*>(A.st_color)[p] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(rng.dim));
+        /* no throughput or radiance record: every path the first iteration
+         * shades has unit throughput, depth 1, zero radiance and CAM_DIMS
+         * random dimensions used (the four draws above) */
         A.st_flags[p] = F_CONT | F_CAM | (ui << 8);
         /* the first ray queue is every path in order (no entries are
          * written: the trace and classification read item i as path i);
@@ -221,7 +224,9 @@ FMD int hit_class(const WfArgs& A, int r, const Hit& h) {
     /* emitters seen directly from the camera end the path */
     if (h.slot < 0) return fm::WC_END;
     const bool emissive = (h.info >> 8) & 1;
-    if (emissive && __float_as_int(reinterpret_cast<const float4*>(A.st_beta)[r].w) == 1) return fm::WC_END;
+    /* paths advance one vertex per iteration: the first iteration's
+     * continuation rays are the camera rays (depth 1) */
+    if (emissive && A.iter == 0) return fm::WC_END;
     return h.info & 0xff;
 }
 
@@ -638,9 +643,14 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
     int p2 = PF && i + gs < (int64_t)n ? q[i + gs] : 0;
     float4 b1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c1 = b1, h1 = b1;
     int f1 = 0;
+    /* the first iteration's paths are camera paths: unit throughput, depth
+     * 1, no radiance yet, CAM_DIMS dimensions drawn (the camera kernel writes
+     * neither record) */
+    const bool first = it == 0;
+    const float4 b0 = make_float4(1.0f, 1.0f, 1.0f, __int_as_float(1)), c0 = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(CAM_DIMS));
     auto state = [&](int pp, float4& b, float4& c, int& f, float4& h) {
-        b = reinterpret_cast<const float4*>(A.st_beta)[pp];
-        c = reinterpret_cast<const float4*>(A.st_color)[pp];
+        b = first ? b0 : reinterpret_cast<const float4*>(A.st_beta)[pp];
+        c = first ? c0 : reinterpret_cast<const float4*>(A.st_color)[pp];
         f = A.st_flags[pp];
         h = reinterpret_cast<const float4*>(A.hit_a)[pp];
     };
@@ -659,8 +669,8 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
             /* an env-lit path end needs no throughput (its depth-1 case is
              * the F_CAM flag) */
             if (C == fm::WC_END && LK == fm::LK_ENV) b4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            else b4 = reinterpret_cast<const float4*>(A.st_beta)[p];
-            c4 = reinterpret_cast<const float4*>(A.st_color)[p];
+            else b4 = first ? b0 : reinterpret_cast<const float4*>(A.st_beta)[p];
+            c4 = first ? c0 : reinterpret_cast<const float4*>(A.st_color)[p];
             fl = A.st_flags[p];
             if (C != fm::WC_END) ha = reinterpret_cast<const float4*>(A.hit_a)[p];
         }
