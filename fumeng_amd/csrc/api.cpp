@@ -96,7 +96,7 @@ struct WfBuffers {
         }
         if (PD > cap_pd) {
             cap_pd = 0;
-            mis.alloc((size_t)PD * 48);
+            mis.alloc((size_t)PD * 16);
             cap_pd = PD;
         }
         if (PDL > cap_pdl) {
@@ -404,7 +404,7 @@ void wf_run(fm_ctx* ctx, fm_scene* s, const std::vector<WfJob>& jobs, int D, int
             int grid, int chunk, int flags, WfRunOut& R, float* path_out = nullptr) {
     if (path_out && jobs.size() != 1) throw std::invalid_argument("per-path output needs exactly one job");
     const int NL = s->d.num_lights;
-    const int64_t per_path = 108 + 65 * (1 + (int64_t)D) + 56 * (int64_t)D + 57 * (int64_t)D * std::max(NL, 1);
+    const int64_t per_path = 108 + 65 * (1 + (int64_t)D) + 24 * (int64_t)D + 57 * (int64_t)D * std::max(NL, 1);
     /* Passes as large as the context's path-state budget allows (192 GB on
      * an idle 288 GB device): fewer passes mean fewer nearly empty tail
      * iterations (1920x1080 at 256 spp: 8 passes took 522 ms, 2 took 493;

@@ -243,7 +243,7 @@ FMD void adjoint_walk(const fm_dscene& S, const PathCfg& cfg, Ray ray, Rng& rng,
                 const v3 f = ms.f * le * absdot(mr.d, sf.ng);
                 const float w = ms.delta ? 1.0f : [&] { const float lp = env_pdf(S, E, mr.d);
                                                          return (ms.pdf * ms.pdf) / (ms.pdf * ms.pdf + lp * lp); }();
-                direct += ms.delta ? beta * (f / ms.pdf) : beta * (f / ms.pdf * w);
+                direct += ms.delta ? mis_factor(beta, ms, sf.ng) * le : mis_factor(beta, ms, sf.ng) * le * w;
                 const float s = absdot(mr.d, sf.ng) / ms.pdf * w / dl;
                 if (dis && !ms.delta) {
                     const float lp = env_pdf(S, E, mr.d);
@@ -266,7 +266,7 @@ FMD void adjoint_walk(const fm_dscene& S, const PathCfg& cfg, Ray ray, Rng& rng,
                 const v3 f = ms.f * rad * absdot(sf.ng, mr.d);
                 const float w = ms.delta ? 1.0f : [&] { const float lp = area_pdf(S, L, mr.o, ls.pos, ls.ng, l2s);
                                                          return (ms.pdf * ms.pdf) / (ms.pdf * ms.pdf + lp * lp); }();
-                direct += ms.delta ? beta * (f / ms.pdf) : beta * (f / ms.pdf * w);
+                direct += ms.delta ? mis_factor(beta, ms, sf.ng) * rad : mis_factor(beta, ms, sf.ng) * rad * w;
                 const float s = absdot(sf.ng, mr.d) / ms.pdf * w / dl;
                 if (dis && !ms.delta) {
                     const float lp = area_pdf(S, L, mr.o, ls.pos, ls.ng, l2s);

@@ -266,6 +266,18 @@ FMD Ray gen_ray(v3 pos, v3 ng, v3 dir) {
     return r;
 }
 
+/* PathTracer::MisBSDF (path_tracing.cpp:166-229) split at its ray: the
+ * factor beta f |cos| / pdf is formed when the sample is drawn, and the
+ * light the ray finds (radiance le, light pdf lp) adds factor * le, weighted
+ * by the power heuristic unless the sample is delta.  This is the
+ * reference's beta * (f le |cos| / pdf * w) multiplied in another order (a
+ * few ulp apart), so that a sample waiting for its ray is one RGB factor and
+ * its pdf. */
+FMD v3 mis_factor(v3 beta, const BsdfSample& ms, v3 ng) { return beta * (ms.f * (absdot(ms.wi, ng) / ms.pdf)); }
+FMD v3 mis_add(v3 mf, v3 le, float pdf, float lp, bool delta) {
+    return delta ? mf * le : mf * le * ((pdf * pdf) / (pdf * pdf + lp * lp));
+}
+
 struct PathOut { v3 color, albedo, normal; uint32_t rays; };
 
 struct PathCfg { int depth, direct, rr_depth; float rr_coef; };
@@ -294,7 +306,9 @@ FMD PathOut render_path(const fm_dscene& S, const PathCfg& cfg, Ray ray, Rng& rn
     int phase = PH_EXTEND;
     int depth = 1, li = 0, pass = 0;
     Surf sf;
-    BsdfSample ms;                   /* MisBSDF sample waiting for its ray */
+    v3 mf = splat(0.0f);             /* MisBSDF sample waiting for its ray: factor, pdf, delta */
+    float mpdf = 0.0f;
+    bool mdelta = false;
     for (;;) {
         Hit h;
         bool occluded = false;
@@ -320,10 +334,9 @@ FMD PathOut render_path(const fm_dscene& S, const PathCfg& cfg, Ray ray, Rng& rn
             if (h.slot < 0) {
                 if (S.env >= 0) {
                     const fm_denv& E = S.envs[S.env];
-                    float lp;
-                    const v3 f = ms.f * env_radiance_pdf(S, E, tr.d, !ms.delta, lp) * absdot(tr.d, sf.ng);
-                    if (ms.delta) direct += beta * (f / ms.pdf);
-                    else direct += beta * (f / ms.pdf * ((ms.pdf * ms.pdf) / (ms.pdf * ms.pdf + lp * lp)));
+                    float lp = 0.0f;
+                    const v3 le = env_radiance_pdf(S, E, tr.d, !mdelta, lp);
+                    direct += mis_add(mf, le, mpdf, lp, mdelta);
                 }
             } else {
                 const Surf ls = fill_hit(S, tr, h);
@@ -332,12 +345,8 @@ FMD PathOut render_path(const fm_dscene& S, const PathCfg& cfg, Ray ray, Rng& rn
                     const v3 l2s = normalize(tr.o - ls.pos);
                     const v3 rad = dot(ls.ng, l2s) > 0 ? ld3(L.radiance) : splat(0.0f);
                     if (!(length(rad) < FM_EPS_PDF)) {
-                        const v3 f = ms.f * rad * absdot(sf.ng, tr.d);
-                        if (ms.delta) direct += beta * (f / ms.pdf);
-                        else {
-                            const float lp = area_pdf(S, L, tr.o, ls.pos, ls.ng, l2s);
-                            direct += beta * (f / ms.pdf * ((ms.pdf * ms.pdf) / (ms.pdf * ms.pdf + lp * lp)));
-                        }
+                        const float lp = mdelta ? 0.0f : area_pdf(S, L, tr.o, ls.pos, ls.ng, l2s);
+                        direct += mis_add(mf, rad, mpdf, lp, mdelta);
                     }
                 }
             }
@@ -371,10 +380,11 @@ FMD PathOut render_path(const fm_dscene& S, const PathCfg& cfg, Ray ray, Rng& rn
                 li++;
             }
             if (traced) break;
-            ms = b.sample(sf.wo, rng.next3());
+            BsdfSample ms = b.sample(sf.wo, rng.next3());
             if (!(length(ms.f) < FM_EPS_PDF || ms.pdf < FM_EPS_PDF)) {
                 ms.wi = normalize(ms.wi);
                 tr = gen_ray(sf.pos, sf.ng, ms.wi);
+                mf = mis_factor(beta, ms, sf.ng); mpdf = ms.pdf; mdelta = ms.delta;
                 phase = PH_MIS;
                 traced = true;
                 break;

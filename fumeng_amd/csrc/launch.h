@@ -74,8 +74,8 @@ struct WfArgs {
     int* st_flags;     /* [P]: F_* bits | unit index << 8 */
     float* out_alb;    /* float4 [P] */
     float* out_nrm;    /* float4 [P] */
-    float4* mis;       /* [P D] records of three float4: BSDF value + pdf, throughput at the vertex + delta,
-                          geometric normal + valid (one record, so a gather of it touches one or two lines) */
+    float4* mis;       /* [P D] MisBSDF records: factor beta f |cos| / pdf, and the pdf (negative for a
+                          delta sample, 0 for no sample) */
     float* sh_o;       /* float4 [P D NL]: origin, t_min */
     float* sh_d;       /* float4 [P D NL]: direction, t_max */
     float* sh_c;       /* float4 [P D NL]: contribution if unoccluded */

@@ -505,22 +505,20 @@ FMD v3 resolve_direct(const fm_dscene& S, const WfArgs& A, int64_t p) {
             const int64_t k = ((int64_t)p * A.D + pass) * A.NL + li;
             if (A.sh_vis[k] == 2) direct += xyz(reinterpret_cast<const float4*>(A.sh_c)[k]);
         }
+        /* the MisBSDF record: factor beta f |cos| / pdf, and the pdf (negative
+         * for a delta sample, 0 for none) */
         const int64_t m = (int64_t)p * A.D + pass;
-        const float4 ng4 = A.mis[3 * m + 2];
-        if (ng4.w == 0.0f) continue;
-        const float4 f4v = A.mis[3 * m];
-        const float4 b4 = A.mis[3 * m + 1];
-        const v3 msf = xyz(f4v), beta = xyz(b4), ng = xyz(ng4);
-        const float mpdf = f4v.w;
-        const bool delta = b4.w != 0.0f;
+        const float4 r4 = A.mis[m];
+        if (r4.w == 0.0f) continue;
+        const v3 mf = xyz(r4);
+        const bool delta = r4.w < 0.0f;
+        const float mpdf = fabsf(r4.w);
         if (LK == fm::LK_ENV) {
             /* only an escaped ray adds light: its direction and slot */
             if (load_slot(A, A.P + m) < 0 && S.env >= 0) {
-                const v3 d = load_dir(A, A.P + m);
-                float lp;
-                const v3 f = msf * env_radiance_pdf(S, S.envs[S.env], d, !delta, lp) * absdot(d, ng);
-                if (delta) direct += beta * (f / mpdf);
-                else direct += beta * (f / mpdf * ((mpdf * mpdf) / (mpdf * mpdf + lp * lp)));
+                float lp = 0.0f;
+                const v3 le = env_radiance_pdf(S, S.envs[S.env], load_dir(A, A.P + m), !delta, lp);
+                direct += mis_add(mf, le, mpdf, lp, delta);
             }
             continue;
         }
@@ -528,11 +526,9 @@ FMD v3 resolve_direct(const fm_dscene& S, const WfArgs& A, int64_t p) {
         const Hit h = load_hit(A, A.P + m);
         if (h.slot < 0) {
             if (LK != fm::LK_AREA && S.env >= 0) {
-                const fm_denv& E = S.envs[S.env];
-                float lp;
-                const v3 f = msf * env_radiance_pdf(S, E, tr.d, !delta, lp) * absdot(tr.d, ng);
-                if (delta) direct += beta * (f / mpdf);
-                else direct += beta * (f / mpdf * ((mpdf * mpdf) / (mpdf * mpdf + lp * lp)));
+                float lp = 0.0f;
+                const v3 le = env_radiance_pdf(S, S.envs[S.env], tr.d, !delta, lp);
+                direct += mis_add(mf, le, mpdf, lp, delta);
             }
         } else {
             const Surf ls = fill_hit(S, tr, h);
@@ -541,12 +537,8 @@ FMD v3 resolve_direct(const fm_dscene& S, const WfArgs& A, int64_t p) {
                 const v3 l2s = normalize(tr.o - ls.pos);
                 const v3 rad = dot(ls.ng, l2s) > 0 ? ld3(L.radiance) : splat(0.0f);
                 if (!(length(rad) < FM_EPS_PDF)) {
-                    const v3 f = msf * rad * absdot(ng, tr.d);
-                    if (delta) direct += beta * (f / mpdf);
-                    else {
-                        const float lp = area_pdf(S, L, tr.o, ls.pos, ls.ng, l2s);
-                        direct += beta * (f / mpdf * ((mpdf * mpdf) / (mpdf * mpdf + lp * lp)));
-                    }
+                    const float lp = delta ? 0.0f : area_pdf(S, L, tr.o, ls.pos, ls.ng, l2s);
+                    direct += mis_add(mf, rad, mpdf, lp, delta);
                 }
             }
         }
@@ -620,8 +612,13 @@ FMD LightSample light_sample_k(const fm_dscene& S, const fm_dlight& L, v3 pos, v
     return light_sample(S, L, pos, s);
 }
 
+/* Lambert and mirror shading need about 140-165 VGPRs: held to 3 waves per
+ * SIMD (168) so that a few more registers never cost a wave. */
+template <int C>
+constexpr int shade_waves() { return C == fm::WC_LAMBERT || C == fm::WC_SPEC ? 3 : 1; }
+
 template <int C, int LK>
-__global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(shade_waves<C>()))) k_wf_shade(fm_dscene S, WfArgs A) {
     const int it = A.iter, nx = it + 1;
     const unsigned n = *cnt(A, it, fm::WQ_CLS + C);
     const int* q = A.q_cls[it & 1] + (size_t)C * A.P;
@@ -753,11 +750,9 @@ __global__ void __launch_bounds__(256) k_wf_shade(fm_dscene S, WfArgs A) {
                 if (ok) {
                     ms.wi = normalize(ms.wi);
                     store_ray(A, A.P + m, gen_ray(sf.pos, sf.ng, ms.wi));
-                    A.mis[3 * m] = f4(ms.f, ms.pdf);
-                    A.mis[3 * m + 1] = f4(beta, ms.delta ? 1.0f : 0.0f);
-                    A.mis[3 * m + 2] = f4(sf.ng, 1.0f);
+                    A.mis[m] = f4(mis_factor(beta, ms, sf.ng), ms.delta ? -ms.pdf : ms.pdf);
                 } else {
-                    A.mis[3 * m + 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    A.mis[m] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 }
                 if (bits) mb |= (uint32_t)(ok ? 1 : 0) << pass;
                 else if (LK == fm::LK_ENV) wave_push(cnt(A, nx, fm::WQ_SH), A.q_sh[nx & 1], ok, (int)((unsigned)(A.P + m) | AH_RAY));

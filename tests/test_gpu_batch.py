@@ -137,9 +137,9 @@ def test_lowered_path_budget_splits_a_frame_into_equal_passes(gpu_ctx):
     """With chunk 0 a frame larger than the path-state budget runs in passes
     of equal sample counts: 24 spp under a budget of 7 samples' worth of
     paths gives 4 passes of 6, bit for bit the film of an explicit chunk of
-    6.  (Box scene: one light, one direct loop, 351 B of path state per path.)"""
+    6.  (Box scene: one light, one direct loop, 319 B of path state per path.)"""
     g = gpu_ctx.scene(scenes.box_scene(width=64, height=48, spp=24))
-    per_path = 108 + 65 * 2 + 56 + 57
+    per_path = 108 + 65 * 2 + 24 + 57
     try:
         gpu_ctx.set_path_budget(7 * 64 * 48 * per_path)
         films, _, st = g.render_batch([{"camera": None, "seed": 3}], chunk=0)
