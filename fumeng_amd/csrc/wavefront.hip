@@ -176,6 +176,30 @@ FMD int find_unit_pix(const WfArgs& A, const fm::WfPass& ps, int64_t q) {
     return lo;
 }
 
+/* The paths of one sample of a unit are stored in 8x8 pixel tiles, so that
+ * the 64 consecutive paths of a wave are a square of the frame: its camera
+ * rays are coherent, and the later rays of its paths start close together.
+ * Bands of 8 rows (the last one shorter), in a band blocks of 8 columns (the
+ * last one narrower), rows in order within a block.  tile_pixel maps a
+ * path's index within its sample to its pixel, pixel_tile back. */
+FMD int64_t tile_pixel(int W, int H, int64_t r) {
+    /* 32-bit divisions: r < W H <= 2^31, and W >= 2^28 leaves one band */
+    const uint32_t u = (uint32_t)r;
+    const uint32_t band = W >= (1 << 28) ? 0u : u / (8u * (uint32_t)W);
+    const uint32_t rb = u - band * 8u * (uint32_t)W;
+    const uint32_t hb = (uint32_t)min(8, H - 8 * (int)band);
+    const uint32_t c = rb / (8u * hb);
+    const uint32_t off = rb - c * 8u * hb;
+    const uint32_t wc = (uint32_t)min(8, W - 8 * (int)c);
+    const uint32_t row = off / wc;
+    return (int64_t)(band * 8u + row) * W + 8 * (int64_t)c + (off - row * wc);
+}
+FMD int64_t pixel_tile(int W, int H, int64_t pix) {
+    const int y = (int)(pix / W), x = (int)(pix - (int64_t)y * W);
+    const int hb = min(8, H - (y & ~7)), wc = min(8, W - (x & ~7));
+    return (int64_t)(y >> 3) * 8 * W + (int64_t)(x >> 3) * 8 * hb + (y & 7) * wc + (x & 7);
+}
+
 __global__ void __launch_bounds__(256) k_wf_camera(fm_dscene S, WfArgs A) {
     const fm::WfPass ps = A.passes[*A.pass_idx];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -188,7 +212,7 @@ __global__ void __launch_bounds__(256) k_wf_camera(fm_dscene S, WfArgs A) {
         const fm::WfUnit& U = A.units[ui];
         const int64_t rel = p - U.path_base;
         const int64_t j = rel / U.npix;
-        const int64_t pix = rel - j * U.npix;
+        const int64_t pix = tile_pixel(U.W, U.H, rel - j * U.npix);
         const int row = (int)(pix / U.W), i = (int)(pix - (int64_t)row * U.W);
         const int jj = U.H - 1 - row;
         Rng rng;
@@ -716,7 +740,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(shade_
             const int64_t rel = p - U.path_base;
             const int64_t j = rel / U.npix;
             Rng rng;
-            rng.init(U.seed, (uint32_t)(rel - j * U.npix), (uint32_t)(U.sample_base + j), 0u);
+            rng.init(U.seed, (uint32_t)tile_pixel(U.W, U.H, rel - j * U.npix), (uint32_t)(U.sample_base + j), 0u);
             rng.seek(dim);
             for (int pass = 0; pass < A.D; pass++) {
                 for (int li = 0; li < A.NL; li++) {
@@ -901,11 +925,11 @@ __global__ void __launch_bounds__(256) k_wf_film(WfArgs A) {
     const fm::WfPass ps = A.passes[*A.pass_idx];
     for (int64_t q = grid_id(); q < ps.total_pix; q += grid_stride()) {
         const fm::WfUnit& U = A.units[find_unit_pix(A, ps, q)];
-        const int64_t pix = q - U.pix_base;
+        const int64_t pix = q - U.pix_base, tp = pixel_tile(U.W, U.H, pix);
         v3 cs = splat(0.0f), as = splat(0.0f), ns = splat(0.0f);
         float q0 = 0.0f, q1 = 0.0f, q2 = 0.0f;
         for (int j = 0; j < U.k; j++) {
-            const int64_t p = U.path_base + (int64_t)j * U.npix + pix;
+            const int64_t p = U.path_base + (int64_t)j * U.npix + tp;
             const v3 c = xyz(reinterpret_cast<const float4*>(A.st_color)[p]);
             if (A.path_out) {
                 float* o = A.path_out + 3 * ((int64_t)(U.sample_base + j - A.path_s0) * U.npix + pix);

@@ -93,6 +93,19 @@ def test_wavefront_uneven_passes_and_shallow_depths(gpu_ctx):
         assert wf[4].paths == 24 * 20 * 5
 
 
+def test_wavefront_odd_frame_sizes(gpu_ctx):
+    """The wavefront form stores each sample's paths in 8x8 pixel tiles
+    (shorter bands and narrower blocks at the bottom and right edges): frames
+    whose sides are not multiples of 8 render pixel for pixel like the
+    megakernel."""
+    s = scenes.box_scene(width=8, height=8, spp=3)
+    g = gpu_ctx.scene(s)
+    for w, h in ((13, 11), (5, 19), (1, 7), (17, 1)):
+        mk, wf = _both(g, seed=19, width=w, height=h, chunk=3)
+        _same_film(mk, wf, "%dx%d" % (w, h), frac=0.99)
+        assert wf[4].paths == w * h * 3
+
+
 def test_wavefront_matches_oracle_with_common_random_numbers(gpu_ctx):
     s = scenes.box_scene(width=32, height=32, spp=16)
     g = gpu_ctx.scene(s)
