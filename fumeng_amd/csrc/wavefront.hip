@@ -195,7 +195,7 @@ FMD int64_t tile_pixel(int W, int H, int64_t r) {
     return (int64_t)(band * 8u + row) * W + 8 * (int64_t)c + (off - row * wc);
 }
 FMD int64_t pixel_tile(int W, int H, int64_t pix) {
-    const int y = (int)(pix / W), x = (int)(pix - (int64_t)y * W);
+    const int y = (int)((uint32_t)pix / (uint32_t)W), x = (int)(pix - (int64_t)y * W);
     const int hb = min(8, H - (y & ~7)), wc = min(8, W - (x & ~7));
     return (int64_t)(y >> 3) * 8 * W + (int64_t)(x >> 3) * 8 * hb + (y & 7) * wc + (x & 7);
 }
@@ -210,10 +210,11 @@ __global__ void __launch_bounds__(256) k_wf_camera(fm_dscene S, WfArgs A) {
     for (int64_t p = grid_id(); p < ps.P; p += grid_stride()) {
         const int ui = find_unit_path(A, ps, p);
         const fm::WfUnit& U = A.units[ui];
-        const int64_t rel = p - U.path_base;
-        const int64_t j = rel / U.npix;
-        const int64_t pix = tile_pixel(U.W, U.H, rel - j * U.npix);
-        const int row = (int)(pix / U.W), i = (int)(pix - (int64_t)row * U.W);
+        /* 32-bit divisions: a pass holds fewer than 2^31 paths */
+        const uint32_t rel = (uint32_t)(p - U.path_base), np = (uint32_t)U.npix;
+        const uint32_t j = rel / np;
+        const uint32_t pix = (uint32_t)tile_pixel(U.W, U.H, rel - j * np);
+        const int row = (int)(pix / (uint32_t)U.W), i = (int)(pix - (uint32_t)row * (uint32_t)U.W);
         const int jj = U.H - 1 - row;
         Rng rng;
         rng.init(U.seed, (uint32_t)pix, (uint32_t)(U.sample_base + j), 0u);
@@ -737,10 +738,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(shade_
                 reinterpret_cast<float4*>(A.out_nrm)[p] = f4(sf.ng, 0.0f);
             }
             const fm::WfUnit& U = A.units[fl >> 8];
-            const int64_t rel = p - U.path_base;
-            const int64_t j = rel / U.npix;
+            const uint32_t rel = (uint32_t)(p - U.path_base), np = (uint32_t)U.npix;
+            const uint32_t j = rel / np;
             Rng rng;
-            rng.init(U.seed, (uint32_t)tile_pixel(U.W, U.H, rel - j * U.npix), (uint32_t)(U.sample_base + j), 0u);
+            rng.init(U.seed, (uint32_t)tile_pixel(U.W, U.H, rel - j * np), (uint32_t)(U.sample_base + j), 0u);
             rng.seek(dim);
             for (int pass = 0; pass < A.D; pass++) {
                 for (int li = 0; li < A.NL; li++) {
