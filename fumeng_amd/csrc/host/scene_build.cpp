@@ -302,12 +302,23 @@ float srgb_to_linear(float c) {
 }
 
 /* Binned SAH builder over true bounds (16 bins, three axes).  Leaves hold at
- * most 7 primitives; when the remaining depth budget gets tight it switches
- * to median splits so that the tree always fits FM_STACK_DEPTH levels.
- * Each primitive's centroid and bounds travel with it through the
- * partitions (40-byte records), so every pass over a node's range streams
- * through memory; the three axes are binned in one pass, and the passes of
- * large nodes run on the build threads. */
+ * most 7 primitives, and a node stays a leaf when splitting it would not
+ * save more than half a primitive test per ray reaching it (a node visit
+ * priced at half a primitive test: the GPU traversal reads a 64-byte node
+ * with one load and tests its two boxes in a few dozen instructions, and a
+ * primitive's 48 bytes in two dependent loads).  Against a visit priced
+ * like a primitive test: the 1 M-triangle icosphere frame 334 -> 330 ms
+ * (the shading kernels, which read the slot records in the tree's leaf
+ * order, gained too), the 9.8 M-triangle stress frame (four-wide tree)
+ * 1402 -> 1390 ms; a visit priced at 2 or 3 tests: icosphere 350 and
+ * 370 ms.  32 bins took the icosphere frame to 327.5 ms but the stress
+ * frame to 1482 ms (its four-wide stack bound went from 51 to 55 entries);
+ * 64 bins measured like 32 on the icosphere.  When the remaining depth
+ * budget gets tight the builder switches to median splits so that the tree
+ * always fits FM_STACK_DEPTH levels.  Each primitive's centroid and bounds
+ * travel with it through the partitions (40-byte records), so every pass
+ * over a node's range streams through memory; the three axes are binned in
+ * one pass, and the passes of large nodes run on the build threads. */
 struct SahBuilder {
     struct N { bool leaf; int64_t a, b; };
     struct Ref { float c[3], lo[3], hi[3]; int32_t id; };
@@ -453,8 +464,8 @@ struct SahBuilder {
             }
         }
         const float parent_area = area(bd.pmn, bd.pmx);
-        /* SAH with traversal cost 1 and intersection cost 1 per primitive */
-        if (!force_median && cnt <= 7 && (best_axis < 0 || 1.0f + best_cost / std::max(parent_area, 1e-30f) >= (float)cnt)) return id;
+        /* SAH with traversal cost 1/2 and intersection cost 1 per primitive */
+        if (!force_median && cnt <= 7 && (best_axis < 0 || 0.5f + best_cost / std::max(parent_area, 1e-30f) >= (float)cnt)) return id;
         int64_t mid;
         if (force_median || best_axis < 0) {
             int ax = 0;
