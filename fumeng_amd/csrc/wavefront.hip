@@ -146,8 +146,15 @@ FMD Hit load_hit(const WfArgs& A, int64_t r) {
 
 /* The parts of a ray slot that an environment-lit scene needs once the
  * ray is traced: its direction, and whether it escaped (the hit slot). */
-FMD v3 load_dir(const WfArgs& A, int64_t r) { return xyz(reinterpret_cast<const float4*>(A.ray_d)[r]); }
 FMD int load_slot(const WfArgs& A, int64_t r) { return __float_as_int(reinterpret_cast<const float4*>(A.hit_a)[r].y); }
+/* Direction of a ray that found nothing, from its hit record (the trace
+ * leaves it there as (d.x, -1, d.y, d.z)), or a zero-length vector if the
+ * ray found something. */
+FMD bool load_escape(const WfArgs& A, int64_t r, v3& d) {
+    const float4 e = reinterpret_cast<const float4*>(A.hit_a)[r];
+    d = mk(e.x, e.z, e.w);
+    return __float_as_int(e.y) < 0;
+}
 
 FMD int64_t grid_id() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 FMD int64_t grid_stride() { return (int64_t)gridDim.x * blockDim.x; }
@@ -292,14 +299,21 @@ FMD void trace_stream(const fm_dscene& S, const WfArgs& A, int64_t lo, int64_t h
     int32_t node = FM_TRAV_NONE, leaf = FM_TRAV_NONE;
     unsigned slot = 0u;
     bool mis = false, live = false;
+    /* A ray that found nothing (a closest-hit miss, an unoccluded MisBSDF
+     * ray) leaves its direction in its hit record, (d.x, slot -1, d.y, d.z),
+     * so that the env lookup after it reads one record instead of two. */
     auto finish = [&](bool occluded) {
         if (ANY) {
-            if (mis) reinterpret_cast<float4*>(A.hit_a)[slot] = make_float4(0.0f, __int_as_float(occluded ? 0 : -1), 0.0f, 0.0f);
+            if (mis) reinterpret_cast<float4*>(A.hit_a)[slot] = make_float4(tr.d.x, __int_as_float(occluded ? 0 : -1), tr.d.y, tr.d.z);
             else A.sh_vis[slot] = occluded ? 1 : 2;
         } else {
-            reinterpret_cast<float4*>(A.hit_a)[slot] =
-                make_float4(h.t, __int_as_float(h.slot), __int_as_float((h.kind & 0xff) | (h.half << 8)), h.b0);
-            reinterpret_cast<float2*>(A.hit_b)[slot] = make_float2(h.b1, h.b2);
+            if (h.slot < 0) {
+                reinterpret_cast<float4*>(A.hit_a)[slot] = make_float4(tr.d.x, __int_as_float(-1), tr.d.y, tr.d.z);
+            } else {
+                reinterpret_cast<float4*>(A.hit_a)[slot] =
+                    make_float4(h.t, __int_as_float(h.slot), __int_as_float((h.kind & 0xff) | (h.half << 8)), h.b0);
+                reinterpret_cast<float2*>(A.hit_b)[slot] = make_float2(h.b1, h.b2);
+            }
             A.cls[i] = (unsigned char)hit_class(A, (int)slot, h);
         }
     };
@@ -540,9 +554,10 @@ FMD v3 resolve_direct(const fm_dscene& S, const WfArgs& A, int64_t p) {
         const float mpdf = fabsf(r4.w);
         if (LK == fm::LK_ENV) {
             /* only an escaped ray adds light: its direction and slot */
-            if (load_slot(A, A.P + m) < 0 && S.env >= 0) {
+            v3 d;
+            if (load_escape(A, A.P + m, d) && S.env >= 0) {
                 float lp = 0.0f;
-                const v3 le = env_radiance_pdf(S, S.envs[S.env], load_dir(A, A.P + m), !delta, lp);
+                const v3 le = env_radiance_pdf(S, S.envs[S.env], d, !delta, lp);
                 direct += mis_add(mf, le, mpdf, lp, delta);
             }
             continue;
@@ -716,8 +731,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(shade_
             if (LK == fm::LK_ENV) {
                 /* an env-lit scene adds light here only for a camera ray that
                  * escaped; deeper escapes are counted by the MisBSDF rays */
-                if ((fl & F_CONT) && (fl & F_CAM) && S.env >= 0 && load_slot(A, p) < 0)
-                    color = env_radiance(S, S.envs[S.env], -load_dir(A, p));
+                v3 d;
+                if ((fl & F_CONT) && (fl & F_CAM) && S.env >= 0 && load_escape(A, p, d))
+                    color = env_radiance(S, S.envs[S.env], -d);
             } else if (fl & F_CONT) {
                 const Ray tr = load_ray(A, p);
                 const Hit h = load_hit(A, p);
