@@ -206,9 +206,9 @@ FMD void adjoint_walk(const fm_dscene& S, const PathCfg& cfg, Ray ray, Rng& rng,
                 const fm_dlight& L = S.lights[li];
                 const LightSample ls = light_sample(S, L, sf.pos, rng.next3());
                 if (ls.pdf < FM_EPS_PDF || length(ls.radiance) < FM_EPS_PDF) continue;
-                const v3 f = b.eval(sf.wo, ls.wi);
+                float bp;
+                const v3 f = b.eval_pdf(sf.wo, ls.wi, bp);
                 if (length(f) < FM_EPS_PDF) continue;
-                const float bp = b.pdf(sf.wo, ls.wi);
                 if (bp < FM_EPS_PDF) continue;
                 Ray sr;
                 sr.o = ls.ref_pos; sr.d = ls.wi; sr.tmin = FM_EPS; sr.tmax = ls.dist - FM_EPS;

@@ -190,29 +190,41 @@ struct Bsdf {
     }
     FMD float fr(float c) const { return dielectric_fr(1.0f, ior, c); }
 
-    /* ---- Disney lobes ---- */
-    FMD v3 spec_eval(v3 wo, v3 wi) const {
+    /* ---- Disney lobes ----
+     * The reflection lobes' values and densities share the half vector
+     * normalize(wo + wi) and its GTR2 / GTR1 densities (RTerms), computed
+     * once per (wo, wi) pair: the same expressions, so the same bits as
+     * each lobe computing them itself. */
+    struct RTerms { v3 wh; float d2, d1; };
+    FMD RTerms rterms(v3 wo, v3 wi, bool cc) const {
+        RTerms t;
+        t.wh = normalize(wo + wi);
+        t.d2 = gtr2(t.wh, ax, ay);
+        t.d1 = cc ? gtr1(t.wh, cc_rough) : 0.0f;
+        return t;
+    }
+    FMD v3 spec_eval(v3 wo, v3 wi, const RTerms& t) const {
         if (!is_refl(wo, wi)) return splat(0.0f);
-        const v3 wh = normalize(wo + wi);
+        const v3 wh = t.wh;
         const float co = wo.z, ci = wi.z;   /* CosDir of unit vectors */
         const float cd = dot(wi, wh.z > 0 ? wh : -wh);
         const v3 fd = cspec0 * fr(cd);
         const float p5 = pow5(clampf(1.0f - fabsf(cd), 0.0f, 1.0f));
         const v3 fc = cspec0 + (splat(1.0f) - cspec0) * p5;
         const v3 f = lerp3(specular * fd, fc, metallic);
-        const float d = gtr2(wh, ax, ay);
+        const float d = t.d2;
         const float g = smith_g(wo, wi, ay);
         return f * d * g / fabsf(4.0f * ci * co);
     }
-    FMD float spec_pdf(v3 wo, v3 wi) const {
+    FMD float spec_pdf(v3 wo, v3 wi, const RTerms& t) const {
         if (!is_refl(wo, wi)) return 0.0f;
-        const v3 wh = normalize(wo + wi);
+        const v3 wh = t.wh;
         const float cd = dot(wi, wh.z > 0 ? wh : -wh);
-        const float d = gtr2(wh, ax, ay);
+        const float d = t.d2;
         return fabsf(fabsf(wh.z) * d / (4.0f * cd));
     }
-    FMD v3 diff_eval(v3 wo, v3 wi) const {
-        const v3 wh = normalize(wo + wi);
+    FMD v3 diff_eval(v3 wo, v3 wi, const RTerms& t) const {
+        const v3 wh = t.wh;
         const float co = fabsf(wo.z), ci = fabsf(wi.z), cd = fabsf(dot(wi, wh));
         const v3 fl_ = base / FM_PI;
         const float fl = pow5(1.0f - ci), fv = pow5(1.0f - co);
@@ -225,20 +237,20 @@ struct Bsdf {
         return wt * fl_ * (1.0f - 0.5f * fl) * (1.0f - 0.5f * fv) + retro + fsh;
     }
     FMD float diff_pdf(v3 wo, v3 wi) const { return is_refl(wo, wi) ? fabsf(wi.z) * FM_INV_PI : 0.0f; }
-    FMD v3 cc_eval(v3 wo, v3 wi) const {
+    FMD v3 cc_eval(v3 wo, v3 wi, const RTerms& t) const {
         if (!is_refl(wo, wi)) return splat(0.0f);
-        const v3 wh = normalize(wo + wi);
+        const v3 wh = t.wh;
         const float co = fabsf(wo.z), ci = fabsf(wi.z), cd = fabsf(dot(wi, wh));
         const float f = 0.04f + (1.0f - 0.04f) * pow5(clampf(1.0f - cd, 0.0f, 1.0f));
-        const float d = gtr1(wh, cc_rough);
+        const float d = t.d1;
         const float g = smith_sep(wi, 0.25f) * smith_sep(wo, 0.25f);
         return splat(clearcoat * f * d * g / fabsf(4.0f * ci * co));
     }
-    FMD float cc_pdf(v3 wo, v3 wi) const {
+    FMD float cc_pdf(v3 wo, v3 wi, const RTerms& t) const {
         if (!is_refl(wo, wi)) return 0.0f;
-        const v3 wh = normalize(wo + wi);
+        const v3 wh = t.wh;
         const float cd = dot(wi, wh);
-        return fabsf(fabsf(wh.z) * gtr1(wh, cc_rough) / (4.0f * cd));
+        return fabsf(fabsf(wh.z) * t.d1 / (4.0f * cd));
     }
     FMD v3 mt_eval(v3 wo, v3 wi, float eax, float eay, v3 color) const {
         if (!is_refr(wo, wi)) return splat(0.0f);
@@ -273,9 +285,9 @@ struct Bsdf {
         if (!refract_dir(wo, n, eta_r, wt)) return splat(0.0f);
         return wt;
     }
-    FMD v3 fakess_eval(v3 wo, v3 wi) const {
+    FMD v3 fakess_eval(v3 wo, v3 wi, const RTerms& t) const {
         if (!is_refl(wo, wi)) return splat(0.0f);
-        const v3 wh = normalize(wo + wi);
+        const v3 wh = t.wh;
         const float co = fabsf(wo.z), ci = fabsf(wi.z), cd = fabsf(dot(wi, wh));
         const float f90 = cd * cd * roughness;
         const float fo = pow5(1.0f - co), fi = pow5(1.0f - ci);
@@ -286,44 +298,66 @@ struct Bsdf {
     }
 
     /* ---- BSDF interface ---- */
+    /* Disney value of a pair with wo.z * wi.z >= 0 (t: its RTerms) */
+    FMD v3 eval_refl(v3 wo, v3 wi, const RTerms& t) const {
+        v3 ret = splat(0.0f);
+        if (metallic < 1.0f && specTrans < 1.0f) {
+            if (thin && flatness > 0.0f) ret += fakess_eval(wo, wi, t);
+            ret += diff_eval(wo, wi, t);
+        }
+        ret += spec_eval(wo, wi, t);
+        if (clearcoat > 0.0f) ret += cc_eval(wo, wi, t);
+        return ret;
+    }
+    FMD v3 eval_trans(v3 wo, v3 wi) const {
+        v3 ret = splat(0.0f);
+        if (thin) {
+            ret += mt_eval(wo, wi, rax, ray, mk(sqrtf(base.x), sqrtf(base.y), sqrtf(base.z)));
+            if (is_refr(wo, wi)) ret += diffTrans * base;
+        } else {
+            ret += mt_eval(wo, wi, tax, tay, base);
+        }
+        return ret;
+    }
     FMD v3 eval_local(v3 wo, v3 wi) const {
         if (type == FM_MAT_LAMBERT) return base * FM_INV_PI;
         if (type == FM_MAT_SPECULAR_TRANSMISSION || type == FM_MAT_SPECULAR_REFLECTION) return splat(0.0f);
-        v3 ret = splat(0.0f);
-        if (wo.z * wi.z < 0) {
-            if (thin) {
-                ret += mt_eval(wo, wi, rax, ray, mk(sqrtf(base.x), sqrtf(base.y), sqrtf(base.z)));
-                if (is_refr(wo, wi)) ret += diffTrans * base;
-            } else {
-                ret += mt_eval(wo, wi, tax, tay, base);
-            }
-            return ret;
-        }
-        if (metallic < 1.0f && specTrans < 1.0f) {
-            if (thin && flatness > 0.0f) ret += fakess_eval(wo, wi);
-            ret += diff_eval(wo, wi);
-        }
-        ret += spec_eval(wo, wi);
-        if (clearcoat > 0.0f) ret += cc_eval(wo, wi);
-        return ret;
+        if (wo.z * wi.z < 0) return eval_trans(wo, wi);
+        return eval_refl(wo, wi, rterms(wo, wi, clearcoat > 0.0f));
     }
     FMD float rt_fr(v3 wo) const { const float f = fr(wo.z); return 1.0f - ((1.0f - f) * specTrans * (1.0f - metallic)); }
-    FMD float pdf_local(v3 wo, v3 wi) const {
-        if (type == FM_MAT_LAMBERT) return wo.z * wi.z < 0.0f ? 0.0f : fabsf(wi.z) * FM_INV_PI;
-        if (type == FM_MAT_SPECULAR_TRANSMISSION || type == FM_MAT_SPECULAR_REFLECTION) return 0.0f;
+    /* Disney density (t: the pair's RTerms, read only when wo.z * wi.z > 0) */
+    FMD float pdf_disney(v3 wo, v3 wi, const RTerms& t) const {
         float p = 0.0f;
         if (w_dr > 0.0f) p += w_dr * diff_pdf(wo, wi);
-        if (w_cc > 0.0f) p += w_cc * cc_pdf(wo, wi);
+        if (w_cc > 0.0f) p += w_cc * cc_pdf(wo, wi, t);
         if (w_st + w_sr > 0.0f) {
             const float f = rt_fr(wo);
-            if (wo.z * wi.z > 0.0f) p += (w_st + w_sr) * f * spec_pdf(wo, wi);
+            if (wo.z * wi.z > 0.0f) p += (w_st + w_sr) * f * spec_pdf(wo, wi, t);
             else p += (w_st + w_sr) * (1.0f - f) * mt_pdf(wo, wi);
         }
         if (w_dt > 0.0f) p += w_dt * (is_refr(wo, wi) ? fabsf(wi.z) * FM_INV_PI : 0.0f);
         return p;
     }
+    FMD float pdf_local(v3 wo, v3 wi) const {
+        if (type == FM_MAT_LAMBERT) return wo.z * wi.z < 0.0f ? 0.0f : fabsf(wi.z) * FM_INV_PI;
+        if (type == FM_MAT_SPECULAR_TRANSMISSION || type == FM_MAT_SPECULAR_REFLECTION) return 0.0f;
+        RTerms t;
+        if (is_refl(wo, wi)) t = rterms(wo, wi, w_cc > 0.0f);
+        else { t.wh = splat(0.0f); t.d2 = t.d1 = 0.0f; }
+        return pdf_disney(wo, wi, t);
+    }
+    /* value and density of one pair, their shared terms computed once */
+    FMD v3 eval_pdf_local(v3 wo, v3 wi, float& pdf) const {
+        if (type != FM_MAT_DISNEY) { pdf = pdf_local(wo, wi); return eval_local(wo, wi); }
+        if (wo.z * wi.z < 0) { pdf = pdf_local(wo, wi); return eval_trans(wo, wi); }
+        const RTerms t = rterms(wo, wi, clearcoat > 0.0f || w_cc > 0.0f);
+        pdf = pdf_disney(wo, wi, t);
+        return eval_refl(wo, wi, t);
+    }
     FMD v3 eval(v3 wo_w, v3 wi_w) const { return eval_local(local(wo_w), local(wi_w)); }
     FMD float pdf(v3 wo_w, v3 wi_w) const { return pdf_local(local(wo_w), local(wi_w)); }
+    FMD v3 eval_pdf(v3 wo_w, v3 wi_w, float& pdf) const { return eval_pdf_local(local(wo_w), local(wi_w), pdf); }
 
     FMD BsdfSample sample(v3 wo_w, v3 s) const {
         BsdfSample o;
@@ -392,8 +426,7 @@ struct Bsdf {
         }
         o.delta = false;
         if (length(wi) < FM_EPS_PDF) { o.f = splat(0.0f); o.wi = splat(0.0f); o.pdf = 0.0f; return o; }
-        o.pdf = pdf_local(wo, wi);
-        o.f = eval_local(wo, wi);
+        o.f = eval_pdf_local(wo, wi, o.pdf);
         o.wi = world(wi);
         return o;
     }

@@ -363,9 +363,9 @@ FMD PathOut render_path(const fm_dscene& S, const PathCfg& cfg, Ray ray, Rng& rn
                 const fm_dlight& L = S.lights[li];
                 const LightSample ls = light_sample(S, L, sf.pos, rng.next3());
                 if (!(ls.pdf < FM_EPS_PDF || length(ls.radiance) < FM_EPS_PDF)) {
-                    const v3 f = b.eval(sf.wo, ls.wi);
+                    float bp;
+                    const v3 f = b.eval_pdf(sf.wo, ls.wi, bp);
                     if (!(length(f) < FM_EPS_PDF)) {
-                        const float bp = b.pdf(sf.wo, ls.wi);
                         if (!(bp < FM_EPS_PDF)) {
                             const v3 ff = f * ls.radiance * absdot(ls.wi, sf.ng);
                             pend = L.kind == FM_DLIGHT_DIR ? beta * (ff / ls.pdf)

@@ -766,9 +766,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(shade_
                     const LightSample ls = light_sample_k<LK>(S, L, sf.pos, rng.next3());
                     bool ok = false;
                     if (!(ls.pdf < FM_EPS_PDF || length(ls.radiance) < FM_EPS_PDF)) {
-                        const v3 f = b.eval(sf.wo, ls.wi);
+                        float bp;
+                        const v3 f = b.eval_pdf(sf.wo, ls.wi, bp);
                         if (!(length(f) < FM_EPS_PDF)) {
-                            const float bp = b.pdf(sf.wo, ls.wi);
                             if (!(bp < FM_EPS_PDF)) {
                                 const v3 ff = f * ls.radiance * absdot(ls.wi, sf.ng);
                                 const v3 pend = L.kind == FM_DLIGHT_DIR ? beta * (ff / ls.pdf)
