@@ -672,13 +672,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(shade_
      * two dependent round trips.  Only the surface classes of env-lit
      * scenes have the registers for it below their wave count's limit
      * (Lambert 147 -> 165 of 168 VGPRs, Disney 233 -> 251 of 256); the
-     * others would lose a wave and load item by item. */
+     * others would lose a wave and load item by item.  Lambert and mirror
+     * shading also prefetch the next item's ray and second hit record
+     * (PF2; 159 VGPRs, the rest of the frame after the traversal 131.5-135.5
+     * -> 131.1 ms); Disney has no registers left for them. */
     constexpr bool PF = C != fm::WC_END && LK == fm::LK_ENV;
     const int64_t gs = grid_stride();
     int64_t i = grid_id();
     int p1 = PF && i < (int64_t)n ? q[i] : 0;
     int p2 = PF && i + gs < (int64_t)n ? q[i + gs] : 0;
-    float4 b1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c1 = b1, h1 = b1;
+    float4 b1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c1 = b1, h1 = b1, o1 = b1, d1 = b1;
+    float2 hb1 = make_float2(0.0f, 0.0f);
+    constexpr bool PF2 = PF && C != fm::WC_DISNEY;
     int f1 = 0;
     /* the first iteration's paths are camera paths: unit throughput, depth
      * 1, no radiance yet, CAM_DIMS dimensions drawn (the camera kernel writes
@@ -690,14 +695,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(shade_
         c = first ? c0 : reinterpret_cast<const float4*>(A.st_color)[pp];
         f = A.st_flags[pp];
         h = reinterpret_cast<const float4*>(A.hit_a)[pp];
+        if constexpr (PF2) {
+            o1 = reinterpret_cast<const float4*>(A.ray_o)[pp];
+            d1 = reinterpret_cast<const float4*>(A.ray_d)[pp];
+            hb1 = reinterpret_cast<const float2*>(A.hit_b)[pp];
+        }
     };
     if (PF && i < (int64_t)n) state(p1, b1, c1, f1, h1);
     for (; i < (int64_t)n; i += gs) {
         int p;
-        float4 b4, c4, ha;
+        float4 b4, c4, ha, ro, rd;
+        float2 hb;
         int fl;
         if (PF) {
-            p = p1; b4 = b1; c4 = c1; fl = f1; ha = h1;
+            p = p1; b4 = b1; c4 = c1; fl = f1; ha = h1; ro = o1; rd = d1; hb = hb1;
             p1 = p2;
             if (i + gs < (int64_t)n) state(p1, b1, c1, f1, h1);
             p2 = i + 2 * gs < (int64_t)n ? q[i + 2 * gs] : 0;
@@ -745,8 +756,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(shade_
                 }
             }
         } else {
-            const Ray tr = load_ray(A, p);
-            const Hit h = hit_of(ha, reinterpret_cast<const float2*>(A.hit_b)[p]);
+            Ray tr;
+            Hit h;
+            if constexpr (PF2) {
+                tr.o = xyz(ro); tr.tmin = ro.w; tr.d = xyz(rd); tr.tmax = rd.w;
+                h = hit_of(ha, hb);
+            } else {
+                tr = load_ray(A, p);
+                h = hit_of(ha, reinterpret_cast<const float2*>(A.hit_b)[p]);
+            }
             const Surf sf = fill_hit(S, tr, h);
             const Bsdf b = make_bsdf<C == fm::WC_SPEC ? FM_BSDF_DELTA : C - 1>(S, sf.material, sf.ns, sf.ss, sf.wo, sf.u, sf.v);
             if (depth == 1) {
