@@ -201,7 +201,7 @@ def _no_worse_than_fp32(g, a32, a64, floor, name):
 
 
 def _within_1e5_where_well_conditioned(g, a32, a64, floor, name):
-    """The north-star bar: BSDF values within 1e-5 relative of the FP64
+    """The BSDF bar: values within 1e-5 relative of the FP64
     oracle on at least 99.9 % of the well-conditioned queries (those where
     the FP32 oracle itself is within 1e-6 of FP64); the rest are logged."""
     r32 = _rel(a32, a64, floor).ravel()
