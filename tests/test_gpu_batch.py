@@ -193,7 +193,7 @@ def test_config4_uneven_box_batch(gpu_ctx):
     jobs = scenes.batch_jobs(64)
     films, done, st = g.render_batch(jobs, chunk=16, graph=True)
     total = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
-    print("config 4: %d paths, %d passes, %d units, %.1f ms" % (st.paths, st.passes, st.units, st.kernel_ms))
+    print("64 uneven box jobs: %d paths, %d passes, %d units, %.1f ms" % (st.paths, st.passes, st.units, st.kernel_ms))
     assert st.paths == total
     assert np.all(done > 0) and np.all(done <= st.kernel_ms * 1.001 + 0.05), (done.max(), st.kernel_ms)
     # the last job ends with the batch's last pass
