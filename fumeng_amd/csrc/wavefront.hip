@@ -542,20 +542,25 @@ FMD v3 resolve_direct(const fm_dscene& S, const WfArgs& A, int64_t p) {
     for (int pass = 0; pass < A.D; pass++) {
         for (int li = 0; li < A.NL; li++) {
             const int64_t k = ((int64_t)p * A.D + pass) * A.NL + li;
-            if (A.sh_vis[k] == 2) direct += xyz(reinterpret_cast<const float4*>(A.sh_c)[k]);
+            /* the contribution and the MisBSDF ray's record load with the
+             * states that select them, not after them: one round trip fewer
+             * (the rest of the frame after the traversal 133-136 -> 130 ms) */
+            const float4 c = reinterpret_cast<const float4*>(A.sh_c)[k];
+            direct += A.sh_vis[k] == 2 ? xyz(c) : splat(0.0f);
         }
         /* the MisBSDF record: factor beta f |cos| / pdf, and the pdf (negative
          * for a delta sample, 0 for none) */
         const int64_t m = (int64_t)p * A.D + pass;
         const float4 r4 = A.mis[m];
+        v3 d;
+        const bool esc = LK == fm::LK_ENV && load_escape(A, A.P + m, d);
         if (r4.w == 0.0f) continue;
         const v3 mf = xyz(r4);
         const bool delta = r4.w < 0.0f;
         const float mpdf = fabsf(r4.w);
         if (LK == fm::LK_ENV) {
             /* only an escaped ray adds light: its direction and slot */
-            v3 d;
-            if (load_escape(A, A.P + m, d) && S.env >= 0) {
+            if (esc && S.env >= 0) {
                 float lp = 0.0f;
                 const v3 le = env_radiance_pdf(S, S.envs[S.env], d, !delta, lp);
                 direct += mis_add(mf, le, mpdf, lp, delta);
