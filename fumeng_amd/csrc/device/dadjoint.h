@@ -89,10 +89,11 @@ FMD void disney_param_grads(const Bsdf& b, v3 wo_w, v3 wo, v3 wi, bool want_pdf,
             else if (k == 4) q.metallic += h;
             else q.specular += h;
             disney_derive(q, wo_w);
-            f[s] = q.eval_local(wo, wi);
             if (want_pdf) {
-                p[s] = q.pdf_local(wo, wi);
+                f[s] = q.eval_pdf_local(wo, wi, p[s]);
                 if (fold && p[s] > 0.0f) lq[s] = log1pf(fold_pdf(q, wo, wi) / p[s]);
+            } else {
+                f[s] = q.eval_local(wo, wi);
             }
         }
         G.df[k] = (f[0] - f[1]) * (0.5f / FM_AD_STEP);
