@@ -278,8 +278,10 @@ FMD int hit_class(const WfArgs& A, int r, const Hit& h) {
  * then fills the class queues in queue order.  Measured on that frame:
  * chunks of 128 / 256 / 1024 / 4096 items gave 257 / 252 / 274 / 400 ms of
  * trace (larger chunks leave waves idle at the end of a launch), refill
- * thresholds of 1 to 32 lanes within 1 %. */
-constexpr int TRACE_CHUNK = 256, TRACE_REFILL = 16;
+ * thresholds of 1 to 32 lanes within 1 %; with the later kernels (pixel
+ * tiles, the current tree) 128 / 256 / 512 items gave 198.6 / 193.9 /
+ * 192.9 ms. */
+constexpr int TRACE_CHUNK = 512, TRACE_REFILL = 16;
 template <bool COUNT, int WIDTH, bool SHORT, bool ANY>
 FMD void trace_stream(const fm_dscene& S, const WfArgs& A, int64_t lo, int64_t hi, const int* q, TravStack<SHORT>& ts,
                       Cnt& cn) {
